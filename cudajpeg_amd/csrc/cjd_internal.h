@@ -1,0 +1,107 @@
+// Internal structures shared by the host engine and the HIP kernels of the
+// MI355X baseline-JPEG decoder.  Nothing here crosses the public C ABI.
+#pragma once
+#include <stdint.h>
+
+namespace cjd {
+
+constexpr int kLutBits = 9;                 // fast Huffman lookup width
+constexpr int kLutSize = 1 << kLutBits;
+constexpr int kMaxSlots = 10;               // blocks per MCU (T.81 B.2.3 limit)
+constexpr int kListLen = 8;                 // block starts remembered per hypothesis
+constexpr int kWG = 256;                    // entropy workgroup size (4 wave64)
+
+// Device Huffman table: a 2^kLutBits direct table for codes up to kLutBits
+// long plus canonical (maxcode, value offset) arrays for longer codes
+// (T.81 F.2.2.3).  LUT entry: bits 0-4 code length (0 = longer code), bits
+// 8-15 the symbol.
+struct HuffTable {
+  uint32_t lut[kLutSize];
+  int32_t maxcode[18];      // maxcode[l] for l in 1..16, -1 when no code of length l; [17] sentinel
+  int32_t valoff[18];       // index of the first symbol of length l minus the first code of length l
+  uint8_t vals[256];
+};
+
+// Per-image constants used by both entropy and pixel kernels.
+struct ImageDesc {
+  int32_t width, height, ncomp;
+  int32_t mcus_x, mcus_y;
+  int32_t nslots;                 // blocks per MCU
+  int32_t mcu_w, mcu_h;           // MCU size in pixels
+  int32_t table_base;             // index of this image's 4 HuffTables
+  int32_t qtab_base;              // index of this image's 4 quant tables (64 uint16, zigzag order)
+  uint8_t slot_comp[kMaxSlots];   // component of each slot
+  uint8_t slot_dc[kMaxSlots];     // DC table id (0..1) of each slot
+  uint8_t slot_ac[kMaxSlots];     // AC table id (2..3) of each slot
+  uint8_t comp_dq[3];             // quant table id per component
+  uint8_t comp_sx[3], comp_sy[3]; // sampling factors
+  uint8_t comp_xshift[3], comp_yshift[3]; // nearest-upsample shifts (CPUonly/src/pixelTransformCPU.c:110-127)
+  int32_t comp_first_slot[3];
+  int32_t comp_stride[3];         // planar stride (src/format.cu:295)
+  int32_t comp_plane_h[3];
+  int64_t coef_base;              // first block of this image in the coefficient buffer
+  int64_t nblocks;
+  uint64_t out_offset;            // byte offset of the interleaved output in the output buffer
+  uint64_t plane_offset[3];       // byte offsets of optional planar outputs (~0 = none)
+  int32_t error_index;            // index into the per-image status array
+  int32_t pad_;
+};
+
+// A segment is a run of the unstuffed entropy-coded stream whose starting
+// decoder state is known: the start of the scan or the byte after an RSTn
+// marker (DC predictors reset, first slot of an MCU).
+struct SegmentDesc {
+  uint64_t word_base;   // first 32-bit word of the segment in the scan buffer
+  uint32_t nbits;       // data bits in the segment
+  uint32_t nwords;      // readable words from word_base (data + 0xFF padding)
+  uint32_t image;       // ImageDesc index
+  uint32_t tile_bits;   // tile length S in bits
+  uint32_t ntiles;
+  uint32_t nblocks;     // blocks the segment must contain
+  int64_t first_block;  // global coefficient-buffer index of its first block
+  uint32_t first_item;  // index of its first work item
+  uint32_t pad_;
+};
+
+// One entropy workgroup: tiles [t0, t0 + ntiles) of a segment.  When t0 > 0 the
+// workgroup also decodes tile t0-1 (the lead-in) to link to its predecessor.
+struct WorkItem {
+  uint32_t segment;
+  uint32_t t0;
+  uint32_t ntiles;
+  uint32_t prev_item;   // predecessor in the same segment, ~0 for the first
+};
+
+// Transfer of the decoder state across one workgroup: for each hypothesis h
+// of the lead-in tile, the hypothesis followed in the last tile and the
+// blocks / DC sums added on the way.  `inclusive` is the concrete state at the
+// workgroup's last tile once the true path through it is known.
+struct LookbackRec {
+  int32_t status;         // 0 none, 1 aggregate, 2 inclusive  (written last)
+  int32_t incl_h;
+  int32_t incl_c[4];      // blocks, dc0, dc1, dc2
+  int8_t agg_h[kMaxSlots];
+  int8_t pad_[6];
+  int32_t agg_c[kMaxSlots][4];
+};
+
+// Per-image status bits (written by kernels with atomicOr).
+enum : uint32_t {
+  kStatusSyntax = 1u,        // invalid Huffman code / coefficient index (reference SYNTAX_ERROR)
+  kStatusTruncated = 2u,     // scan ended before all blocks were decoded
+  kStatusUnresolved = 4u,    // a tile link could not be resolved; rerun with the serial decoder
+  kStatusInconsistent = 8u,  // final decode disagreed with the resolved path; rerun serially
+};
+
+enum IdctMode : int { kIdctInt = 0, kIdctFloat = 1, kIdctMfma = 2 };
+
+// One pixel workgroup: `count` consecutive MCUs of one MCU row of an image.
+struct PixItem {
+  uint32_t image;
+  uint32_t mcu_first;
+  uint32_t count;
+  uint32_t pad_;
+};
+constexpr int kPixBlocks = 64;   // max 8x8 blocks per pixel workgroup
+
+}  // namespace cjd

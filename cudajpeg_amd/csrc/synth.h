@@ -1,0 +1,36 @@
+/* Seeded synthetic-image generator and baseline JPEG encoder (C ABI). */
+#ifndef CUDAJPEG_AMD_SYNTH_H
+#define CUDAJPEG_AMD_SYNTH_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  uint64_t seed;
+  int32_t width, height;
+  int32_t ncomp;            /* 1 (grayscale) or 3 (YCbCr) */
+  int32_t h_samp, v_samp;   /* luma sampling factors; chroma is 1x1 */
+  int32_t quality;          /* IJG quality 1..100 */
+  int32_t restart_interval; /* MCUs per restart interval, 0 = none */
+  double sigma;             /* Gaussian noise std */
+  double amp_total;         /* sum of the 6 sinusoid amplitudes */
+  double random_tile_frac;  /* fraction of 64x64 tiles replaced by uniform noise */
+} synth_params;
+
+/* W*H*ncomp interleaved pixels; returns 0 on success. */
+int synth_pixels(const synth_params *p, uint8_t *out);
+/* Encodes pixels; returns the file size, or <0 on error (-3 = cap too small).
+ * coefs (optional): int16[num_blocks*64], quantised, natural order, scan order.
+ * block_bits (optional): uint32[num_blocks], start bit of every block in the
+ * unstuffed entropy-coded segment. */
+long synth_encode(const synth_params *p, const uint8_t *pixels, uint8_t *out, long cap,
+                  int16_t *coefs, uint32_t *block_bits);
+long synth_make(const synth_params *p, uint8_t *out, long cap, int16_t *coefs, uint32_t *block_bits);
+long synth_num_blocks(const synth_params *p);
+void synth_quant_table(int quality, int chroma, uint8_t out_natural[64]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,536 @@
+/*
+ * CPU oracle for the baseline-JPEG scan-decode path -- TEST INFRASTRUCTURE.
+ *
+ * Restates the reference algorithm (jndean/CUDAjpeg) in plain C so that the
+ * HIP decoder can be checked against it.  Only tests, smoke() and the bench's
+ * CPU-baseline leg use this file; the product decoder never does.
+ *
+ * Pinned by: (1) the encoder's own emitted coefficients and block offsets,
+ * (2) the reference sources compiled from /root/reference into oracle/_ref
+ * (CPUonly pixels; src/decodeScanCPU.cu dequantised coefficients) on the
+ * golden files, (3) closed forms (iDCT shortcut algebra, FP64 iDCT vs the
+ * integer iDCT bound).
+ */
+#include "jpeg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* include/format.h:117-120 */
+static const uint8_t kDeZigZag[64] = {
+    0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48,
+    41, 34, 27, 20, 13, 6, 7, 14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15,
+    23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+
+/* CPUonly/include/pixelTransformCPU.h:5-10 */
+#define W1 2841
+#define W2 2676
+#define W3 2408
+#define W5 1609
+#define W6 1108
+#define W7 565
+
+#define FAIL(e) do { j->error = (e); j->error_line = __LINE__; return (e); } while (0)
+
+static unsigned read16(const uint8_t *p) { return (unsigned)(p[0] << 8 | p[1]); }
+
+/* ---------------------------------------------------------------- parse -- */
+/* src/format.cu:242-354 */
+static int parse_sof(orc_jpeg *j, const uint8_t *block, const uint8_t *end) {
+  unsigned block_len = read16(block);
+  if (block_len < 9 || block + block_len >= end) FAIL(ORC_SYNTAX_ERROR);
+  if (block[2] != 8) FAIL(ORC_UNSUPPORTED_ERROR);
+  j->height = (int)read16(block + 3);
+  j->width = (int)read16(block + 5);
+  if (!j->width || !j->height) FAIL(ORC_SYNTAX_ERROR);
+  j->num_channels = block[7];
+  if (j->num_channels != 1 && j->num_channels != 3) FAIL(ORC_UNSUPPORTED_ERROR);
+  if (block_len < 8u + (unsigned)(j->num_channels * 3)) FAIL(ORC_SYNTAX_ERROR);
+  block += 8;
+  int sxm = 0, sym = 0;
+  for (int i = 0; i < j->num_channels; i++, block += 3) {
+    orc_channel *c = &j->ch[i];
+    c->id = block[0];
+    c->samples_x = block[1] >> 4;
+    c->samples_y = block[1] & 0xF;
+    c->dq_id = block[2];
+    if (!c->samples_x || !c->samples_y || c->dq_id > 3) FAIL(ORC_SYNTAX_ERROR);
+    if ((c->samples_x & (c->samples_x - 1)) || (c->samples_y & (c->samples_y - 1)))
+      FAIL(ORC_UNSUPPORTED_ERROR);
+    if (c->samples_x > sxm) sxm = c->samples_x;
+    if (c->samples_y > sym) sym = c->samples_y;
+  }
+  if (j->num_channels == 1) { /* src/format.cu:281-284 */
+    j->ch[0].samples_x = sxm = 1;
+    j->ch[0].samples_y = sym = 1;
+  }
+  j->block_size_x = sxm << 3;
+  j->block_size_y = sym << 3;
+  j->num_blocks_x = (j->width + j->block_size_x - 1) / j->block_size_x;
+  j->num_blocks_y = (j->height + j->block_size_y - 1) / j->block_size_y;
+  j->blocks_per_mcu = 0;
+  for (int i = 0; i < j->num_channels; i++) {
+    orc_channel *c = &j->ch[i];
+    c->width = (j->width * c->samples_x + sxm - 1) / sxm;
+    c->height = (j->height * c->samples_y + sym - 1) / sym;
+    c->stride = j->num_blocks_x * (c->samples_x << 3);
+    c->plane_height = j->num_blocks_y * (c->samples_y << 3);
+    if ((c->width < 3 && c->samples_x != sxm) || (c->height < 3 && c->samples_y != sym))
+      FAIL(ORC_UNSUPPORTED_ERROR);
+    j->blocks_per_mcu += c->samples_x * c->samples_y;
+  }
+  if (j->num_channels == 3) { /* src/format.cu:346-350 */
+    if (j->ch[1].width != j->ch[2].width || j->ch[1].height != j->ch[2].height ||
+        j->ch[0].width != j->width || j->ch[0].height != j->height)
+      FAIL(ORC_UNSUPPORTED_ERROR);
+  }
+  j->num_blocks = (long)j->num_blocks_x * j->num_blocks_y * j->blocks_per_mcu;
+  return 0;
+}
+
+/* src/format.cu:357-400 (table_id = Tc*2 + Th; the 65536-entry LUT itself is
+ * built at decode time from the stored BITS/HUFFVAL, see build_lut) */
+static int parse_dht(orc_jpeg *j, const uint8_t *pos, const uint8_t *end) {
+  unsigned block_len = read16(pos);
+  const uint8_t *block_end = pos + block_len;
+  if (block_end > end) FAIL(ORC_SYNTAX_ERROR);
+  pos += 2;
+  while (pos < block_end) {
+    uint8_t val = pos[0];
+    if (val & 0xEC) FAIL(ORC_SYNTAX_ERROR);
+    if (val & 0x02) FAIL(ORC_UNSUPPORTED_ERROR);
+    int id = (val | (val >> 3)) & 3;
+    const uint8_t *tuple = pos + 17;
+    int remain = 65536, n = 0;
+    for (int l = 1; l <= 16; l++) {
+      int count = pos[l];
+      j->dht_bits[id][l - 1] = (uint8_t)count;
+      if (!count) continue;
+      if (tuple + count > block_end) FAIL(ORC_SYNTAX_ERROR);
+      remain -= count << (16 - l);
+      if (remain < 0) FAIL(ORC_SYNTAX_ERROR);
+      for (int i = 0; i < count; i++) j->dht_vals[id][n++] = *tuple++;
+    }
+    j->dht_present[id] = 1;
+    pos = tuple;
+  }
+  if (pos != block_end) FAIL(ORC_SYNTAX_ERROR);
+  return 0;
+}
+
+/* src/format.cu:424-442 */
+static int parse_dqt(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const uint8_t **out_end) {
+  unsigned block_len = read16(p);
+  const uint8_t *block_end = p + block_len;
+  if (block_end > end) FAIL(ORC_SYNTAX_ERROR);
+  const uint8_t *pos = p + 2;
+  while (pos + 65 <= block_end) {
+    uint8_t id = pos[0];
+    if (id & 0xFC) FAIL(ORC_SYNTAX_ERROR);
+    memcpy(j->dq[id], pos + 1, 64);
+    pos += 65;
+  }
+  if (pos != block_end) FAIL(ORC_SYNTAX_ERROR);
+  *out_end = block_end;
+  return 0;
+}
+
+/* src/decodeScanGPU.cu:124-141 */
+static int parse_sos(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const uint8_t **after) {
+  unsigned header_len = read16(p);
+  if (p + header_len > end) FAIL(ORC_SYNTAX_ERROR);
+  const uint8_t *pos = p + 2;
+  if (header_len < 4u + 2u * (unsigned)j->num_channels) FAIL(ORC_SYNTAX_ERROR);
+  if (*(pos++) != j->num_channels) FAIL(ORC_UNSUPPORTED_ERROR);
+  for (int i = 0; i < j->num_channels; i++, pos += 2) {
+    if (pos[0] != j->ch[i].id) FAIL(ORC_SYNTAX_ERROR);
+    if (pos[1] & 0xEE) FAIL(ORC_SYNTAX_ERROR);
+    j->ch[i].dc_id = pos[1] >> 4;
+    j->ch[i].ac_id = (pos[1] & 1) | 2;
+  }
+  if (pos[0] || pos[1] != 63 || pos[2]) FAIL(ORC_UNSUPPORTED_ERROR);
+  *after = p + header_len;
+  return 0;
+}
+
+/* src/format.cu:113-215 (file read replaced by an in-memory buffer) */
+int orc_parse(const uint8_t *buf, size_t size, orc_jpeg *j) {
+  memset(j, 0, sizeof(*j));
+  if (size < 4 || buf[0] != 0xFF || buf[1] != 0xD8 || buf[size - 2] != 0xFF || buf[size - 1] != 0xD9)
+    FAIL(ORC_FILE_ERROR);
+  if (size < 6) FAIL(ORC_SYNTAX_ERROR);
+  const uint8_t *end = buf + size - 2;
+  const uint8_t *pos = buf + 2;
+  for (;;) {
+    if (pos > end) FAIL(ORC_SYNTAX_ERROR);
+    if (pos[0] != 0xFF) FAIL(ORC_SYNTAX_ERROR);
+    pos += 2;
+    uint8_t m = pos[-1];
+    int e = 0;
+    switch (m) {
+      case 0xC0:
+        e = parse_sof(j, pos, end);
+        if (!e) pos += read16(pos);
+        break;
+      case 0xC4:
+        e = parse_dht(j, pos, end);
+        if (!e) pos += read16(pos);
+        break;
+      case 0xDB: {
+        const uint8_t *ne;
+        e = parse_dqt(j, pos, end, &ne);
+        if (!e) pos = ne;
+        break;
+      }
+      case 0xDD: { /* src/format.cu:404-421 */
+        unsigned bl = read16(pos);
+        if (bl < 2 || pos + bl > end) FAIL(ORC_SYNTAX_ERROR);
+        j->restart_interval = (int)read16(pos + 2);
+        pos += bl;
+        break;
+      }
+      case 0xDA: {
+        const uint8_t *after;
+        e = parse_sos(j, pos, end, &after);
+        if (!e) {
+          /* the scan runs to the EOI (src/decodeScanGPU.cu:13 unstuffs up to jpg->end) */
+          j->scan = after;
+          j->scan_len = (size_t)(end - after);
+          j->have_scan = 1;
+          pos = end;
+        }
+        break;
+      }
+      case 0xFE:
+        pos += read16(pos);
+        break;
+      case 0xD9:
+        break;
+      default:
+        if ((m & 0xF0) == 0xE0) pos += read16(pos);
+        else FAIL(ORC_SYNTAX_ERROR);
+    }
+    if (e) return e;
+    if (m == 0xD9) break;
+  }
+  return 0;
+}
+
+/* src/decodeScanGPU.cu:11-24 */
+size_t orc_unstuff(const uint8_t *src, size_t len, uint8_t *dst) {
+  size_t s = 0, d = 0;
+  while (s < len) {
+    uint8_t b = src[s++];
+    dst[d++] = b;
+    if (b != 0xFF) continue;
+    uint8_t nx = s < len ? src[s] : 0xFF; /* past the scan: the EOI's 0xFF */
+    s++;
+    if (nx == 0x00) continue;
+    if ((nx & 0xF8) == 0xD0) dst[d++] = nx;
+  }
+  return d;
+}
+
+/* ----------------------------------------------------------- scan decode -- */
+typedef struct { uint8_t tuple, num_bits; } vlc_t; /* include/format.h:30-33 */
+
+/* The reference's 65536-entry direct LUT (src/format.cu:371-391): each code of
+ * length L is spread over 2^(16-L) consecutive entries; the rest are invalid. */
+static void build_lut(const uint8_t bits[16], const uint8_t *vals, vlc_t *vlc) {
+  int remain = 65536, spread = 65536, k = 0;
+  vlc_t *v = vlc;
+  for (int l = 1; l <= 16; l++) {
+    spread >>= 1;
+    int count = bits[l - 1];
+    remain -= count << (16 - l);
+    for (int i = 0; i < count; i++, k++)
+      for (int s = spread; s; s--, v++) { v->num_bits = (uint8_t)l; v->tuple = vals[k]; }
+  }
+  while (remain-- > 0) { v->num_bits = 0; v->tuple = 0; v++; }
+}
+
+typedef struct {
+  const uint8_t *buf, *pos, *end;
+  uint32_t bufbits;
+  int num_bufbits;
+  int error;
+} reader_t;
+
+/* src/decodeScanGPU.cu:28-37: 0xFF beyond the end */
+static int show_bits(reader_t *r, int n) {
+  if (!n) return 0;
+  while (r->num_bufbits < n) {
+    uint8_t b = (r->pos >= r->end) ? 0xFF : *r->pos++;
+    r->bufbits = (r->bufbits << 8) | b;
+    r->num_bufbits += 8;
+  }
+  return (int)((r->bufbits >> (r->num_bufbits - n)) & ((1u << n) - 1));
+}
+static int get_bits(reader_t *r, int n) {
+  int v = show_bits(r, n);
+  r->num_bufbits -= n;
+  return v;
+}
+/* src/decodeScanGPU.cu:50-75 (EXTEND, T.81 F.2.2.1) */
+static int get_vlc(reader_t *r, const vlc_t *tab, uint8_t *code) {
+  int sym = show_bits(r, 16);
+  vlc_t v = tab[sym];
+  if (!v.num_bits) { r->error = ORC_SYNTAX_ERROR; return 0; }
+  r->num_bufbits -= v.num_bits;
+  if (code) *code = v.tuple;
+  int nb = v.tuple & 0x0F;
+  if (!nb) return 0;
+  int value = get_bits(r, nb);
+  if (value < (1 << (nb - 1))) value += ((-1) << nb) + 1;
+  return value;
+}
+
+int orc_decode_scan(const orc_jpeg *jc, int16_t *qcoef, int32_t *work[3], uint32_t *bitpos) {
+  orc_jpeg *j = (orc_jpeg *)jc; /* only error fields are written */
+  if (!j->have_scan) FAIL(ORC_SYNTAX_ERROR);
+  vlc_t *lut = (vlc_t *)malloc(sizeof(vlc_t) * 65536 * 4);
+  uint8_t *ub = (uint8_t *)malloc(j->scan_len + 2);
+  if (!lut || !ub) { free(lut); free(ub); FAIL(ORC_OOM_ERROR); }
+  for (int t = 0; t < 4; t++) build_lut(j->dht_bits[t], j->dht_vals[t], lut + 65536 * t);
+  size_t ulen = orc_unstuff(j->scan, j->scan_len, ub);
+  reader_t r = {ub, ub, ub + ulen, 0, 0, 0};
+  int dc[3] = {0, 0, 0};
+  int32_t *wpos[3] = {0, 0, 0};
+  for (int c = 0; c < j->num_channels; c++) {
+    if (work && work[c]) {
+      wpos[c] = work[c];
+      memset(work[c], 0, sizeof(int32_t) * (size_t)j->ch[c].stride * j->ch[c].plane_height);
+    }
+  }
+  int ri = j->restart_interval, rcount = ri, next_rst = 0;
+  long blk = 0;
+  int err = 0;
+  for (int by = 0; by < j->num_blocks_y && !err; by++)
+    for (int bx = 0; bx < j->num_blocks_x && !err; bx++) {
+      for (int c = 0; c < j->num_channels && !err; c++) {
+        const orc_channel *ch = &j->ch[c];
+        const vlc_t *dct = lut + 65536 * ch->dc_id, *act = lut + 65536 * ch->ac_id;
+        const uint8_t *dq = j->dq[ch->dq_id];
+        for (int sy = 0; sy < ch->samples_y && !err; sy++)
+          for (int sx = 0; sx < ch->samples_x && !err; sx++) {
+            /* src/decodeScanGPU.cu:78-100 (decodeBlock) */
+            if (bitpos) bitpos[blk] = (uint32_t)((r.pos - r.buf) * 8 - r.num_bufbits);
+            int16_t q[64];
+            memset(q, 0, sizeof(q));
+            dc[c] += get_vlc(&r, dct, NULL);
+            q[0] = (int16_t)dc[c];
+            int32_t *w = wpos[c];
+            if (w) { memset(w, 0, 64 * sizeof(int32_t)); w[0] = dc[c] * dq[0]; }
+            int coef = 0;
+            uint8_t code = 0;
+            do {
+              int value = get_vlc(&r, act, &code);
+              if (!code) break;
+              if (!(code & 0x0F) && code != 0xF0) { err = ORC_SYNTAX_ERROR; break; }
+              coef += (code >> 4) + 1;
+              if (coef > 63) { err = ORC_SYNTAX_ERROR; break; }
+              q[kDeZigZag[coef]] = (int16_t)value;
+              if (w) w[kDeZigZag[coef]] = value * dq[coef];
+            } while (coef < 63);
+            if (r.error) err = r.error;
+            if (qcoef) memcpy(qcoef + blk * 64, q, sizeof(q));
+            if (w) wpos[c] += 64;
+            blk++;
+          }
+      }
+      /* src/decodeScanGPU.cu:476-486 */
+      if (!err && ri && !(--rcount) && r.pos < r.end) {
+        r.num_bufbits &= 0xF8;
+        int m = get_bits(&r, 16);
+        if ((m & 0xFFF8) != 0xFFD0 || (m & 7) != next_rst) err = ORC_SYNTAX_ERROR;
+        next_rst = (next_rst + 1) & 7;
+        rcount = ri;
+        dc[0] = dc[1] = dc[2] = 0;
+      }
+    }
+  free(lut);
+  free(ub);
+  if (err) FAIL(err);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ iDCT -- */
+static inline uint8_t clip8(int x) { return (uint8_t)(x < 0 ? 0 : (x > 255 ? 255 : x)); }
+
+/* CPUonly/src/pixelTransformCPU.c:14-57 */
+static void idct_row(int *D, int shortcut) {
+  int x0, x1, x2, x3, x4, x5, x6, x7, x8;
+  x1 = D[4] << 11; x2 = D[6]; x3 = D[2]; x4 = D[1]; x5 = D[7]; x6 = D[5]; x7 = D[3];
+  if (shortcut && !(x1 | x2 | x3 | x4 | x5 | x6 | x7)) {
+    D[0] = D[1] = D[2] = D[3] = D[4] = D[5] = D[6] = D[7] = D[0] << 3;
+    return;
+  }
+  x0 = (D[0] << 11) + 128;
+  x8 = W7 * (x4 + x5);
+  x4 = x8 + (W1 - W7) * x4;
+  x5 = x8 - (W1 + W7) * x5;
+  x8 = W3 * (x6 + x7);
+  x6 = x8 - (W3 - W5) * x6;
+  x7 = x8 - (W3 + W5) * x7;
+  x8 = x0 + x1; x0 -= x1;
+  x1 = W6 * (x3 + x2);
+  x2 = x1 - (W2 + W6) * x2;
+  x3 = x1 + (W2 - W6) * x3;
+  x1 = x4 + x6; x4 -= x6; x6 = x5 + x7; x5 -= x7;
+  x7 = x8 + x3; x8 -= x3; x3 = x0 + x2; x0 -= x2;
+  x2 = (181 * (x4 + x5) + 128) >> 8;
+  x4 = (181 * (x4 - x5) + 128) >> 8;
+  D[0] = (x7 + x1) >> 8; D[1] = (x3 + x2) >> 8; D[2] = (x0 + x4) >> 8; D[3] = (x8 + x6) >> 8;
+  D[4] = (x8 - x6) >> 8; D[5] = (x0 - x4) >> 8; D[6] = (x3 - x2) >> 8; D[7] = (x7 - x1) >> 8;
+}
+
+/* CPUonly/src/pixelTransformCPU.c:60-107 */
+static void idct_col(const int *D, uint8_t *out, int stride, int shortcut) {
+  int x0, x1, x2, x3, x4, x5, x6, x7, x8;
+  x1 = D[8 * 4] << 8; x2 = D[8 * 6]; x3 = D[8 * 2]; x4 = D[8 * 1];
+  x5 = D[8 * 7]; x6 = D[8 * 5]; x7 = D[8 * 3];
+  if (shortcut && !(x1 | x2 | x3 | x4 | x5 | x6 | x7)) {
+    uint8_t v = clip8(((D[0] + 32) >> 6) + 128);
+    for (int i = 0; i < 8; i++) out[i * stride] = v;
+    return;
+  }
+  x0 = (D[0] << 8) + 8192;
+  x8 = W7 * (x4 + x5) + 4;
+  x4 = (x8 + (W1 - W7) * x4) >> 3;
+  x5 = (x8 - (W1 + W7) * x5) >> 3;
+  x8 = W3 * (x6 + x7) + 4;
+  x6 = (x8 - (W3 - W5) * x6) >> 3;
+  x7 = (x8 - (W3 + W5) * x7) >> 3;
+  x8 = x0 + x1; x0 -= x1;
+  x1 = W6 * (x3 + x2) + 4;
+  x2 = (x1 - (W2 + W6) * x2) >> 3;
+  x3 = (x1 + (W2 - W6) * x3) >> 3;
+  x1 = x4 + x6; x4 -= x6; x6 = x5 + x7; x5 -= x7;
+  x7 = x8 + x3; x8 -= x3; x3 = x0 + x2; x0 -= x2;
+  x2 = (181 * (x4 + x5) + 128) >> 8;
+  x4 = (181 * (x4 - x5) + 128) >> 8;
+  out[0 * stride] = clip8(((x7 + x1) >> 14) + 128);
+  out[1 * stride] = clip8(((x3 + x2) >> 14) + 128);
+  out[2 * stride] = clip8(((x0 + x4) >> 14) + 128);
+  out[3 * stride] = clip8(((x8 + x6) >> 14) + 128);
+  out[4 * stride] = clip8(((x8 - x6) >> 14) + 128);
+  out[5 * stride] = clip8(((x0 - x4) >> 14) + 128);
+  out[6 * stride] = clip8(((x3 - x2) >> 14) + 128);
+  out[7 * stride] = clip8(((x7 - x1) >> 14) + 128);
+}
+
+static void idct_int_impl(const int32_t in[64], uint8_t *out, int stride, int shortcut) {
+  int D[64];
+  for (int i = 0; i < 64; i++) D[i] = in[i];
+  for (int r = 0; r < 64; r += 8) idct_row(D + r, shortcut);
+  for (int c = 0; c < 8; c++) idct_col(D + c, out + c, stride, shortcut);
+}
+void orc_idct_int(const int32_t in[64], uint8_t *out, int stride) { idct_int_impl(in, out, stride, 1); }
+void orc_idct_int_noshortcut(const int32_t in[64], uint8_t *out, int stride) {
+  idct_int_impl(in, out, stride, 0);
+}
+
+/* T.81 A.3.3: s(y,x) = 1/4 sum_v sum_u C(u)C(v) S(v,u) cos((2x+1)u pi/16) cos((2y+1)v pi/16) */
+static double g_c[8][8];
+static int g_c_ready = 0;
+void orc_idct_f64_raw(const int32_t in[64], double out[64]) {
+  if (!g_c_ready) {
+    for (int u = 0; u < 8; u++)
+      for (int x = 0; x < 8; x++)
+        g_c[u][x] = (u == 0 ? sqrt(0.125) : 0.5) * cos((2 * x + 1) * u * M_PI / 16.0);
+    g_c_ready = 1;
+  }
+  double t[64];
+  for (int v = 0; v < 8; v++)
+    for (int x = 0; x < 8; x++) {
+      double s = 0;
+      for (int u = 0; u < 8; u++) s += g_c[u][x] * in[v * 8 + u];
+      t[v * 8 + x] = s;
+    }
+  for (int y = 0; y < 8; y++)
+    for (int x = 0; x < 8; x++) {
+      double s = 0;
+      for (int v = 0; v < 8; v++) s += g_c[v][y] * t[v * 8 + x];
+      out[y * 8 + x] = s + 128.0;
+    }
+}
+void orc_idct_f64(const int32_t in[64], uint8_t *out, int stride) {
+  double o[64];
+  orc_idct_f64_raw(in, o);
+  for (int y = 0; y < 8; y++)
+    for (int x = 0; x < 8; x++) out[y * stride + x] = clip8((int)floor(o[y * 8 + x] + 0.5));
+}
+
+/* src/pixelTransformGPU.cu:120-131 block placement; decode order is MCU raster,
+ * then component, then sample row, then sample column (src/decodeScanCPU.cu:118-137). */
+int orc_planes(const orc_jpeg *j, const int16_t *qcoef, uint8_t *planes[3], int mode) {
+  long blk = 0;
+  for (int by = 0; by < j->num_blocks_y; by++)
+    for (int bx = 0; bx < j->num_blocks_x; bx++)
+      for (int c = 0; c < j->num_channels; c++) {
+        const orc_channel *ch = &j->ch[c];
+        const uint8_t *dq = j->dq[ch->dq_id];
+        for (int sy = 0; sy < ch->samples_y; sy++)
+          for (int sx = 0; sx < ch->samples_x; sx++, blk++) {
+            int32_t deq[64];
+            for (int k = 0; k < 64; k++) deq[kDeZigZag[k]] = qcoef[blk * 64 + kDeZigZag[k]] * dq[k];
+            size_t off = ((size_t)(by * ch->samples_y + sy) * ch->stride + (size_t)(bx * ch->samples_x + sx)) << 3;
+            if (mode == 0) orc_idct_int(deq, planes[c] + off, ch->stride);
+            else orc_idct_f64(deq, planes[c] + off, ch->stride);
+          }
+      }
+  return 0;
+}
+
+/* CPUonly/src/pixelTransformCPU.c:110-172 */
+int orc_colour(const orc_jpeg *j, uint8_t *const planes[3], uint8_t *out) {
+  const int W = j->width, H = j->height;
+  if (j->num_channels == 1) {
+    const orc_channel *c = &j->ch[0];
+    for (int y = 0; y < H; y++) memcpy(out + (size_t)y * W, planes[0] + (size_t)y * c->stride, (size_t)W);
+    return 0;
+  }
+  int xs[3], ys[3];
+  for (int c = 0; c < 3; c++) {
+    int w = j->ch[c].width, h = j->ch[c].height;
+    xs[c] = ys[c] = 0;
+    while (w < W) { w <<= 1; xs[c]++; }
+    while (h < H) { h <<= 1; ys[c]++; }
+  }
+  for (int y = 0; y < H; y++) {
+    const uint8_t *py = planes[0] + (size_t)(y >> ys[0]) * j->ch[0].stride;
+    const uint8_t *pb = planes[1] + (size_t)(y >> ys[1]) * j->ch[1].stride;
+    const uint8_t *pr = planes[2] + (size_t)(y >> ys[2]) * j->ch[2].stride;
+    uint8_t *o = out + (size_t)y * W * 3;
+    for (int x = 0; x < W; x++) {
+      int Y = py[x >> xs[0]] << 8, cb = pb[x >> xs[1]] - 128, cr = pr[x >> xs[2]] - 128;
+      o[3 * x + 0] = clip8((Y + 359 * cr + 128) >> 8);
+      o[3 * x + 1] = clip8((Y - 88 * cb - 183 * cr + 128) >> 8);
+      o[3 * x + 2] = clip8((Y + 454 * cb + 128) >> 8);
+    }
+  }
+  return 0;
+}
+
+int orc_decode_file(const uint8_t *file, size_t size, int idct_mode, uint8_t *out, size_t out_cap,
+                    int *width, int *height, int *ncomp) {
+  orc_jpeg j;
+  int e = orc_parse(file, size, &j);
+  if (e) return e;
+  *width = j.width; *height = j.height; *ncomp = j.num_channels;
+  if ((size_t)j.width * j.height * j.num_channels > out_cap) return ORC_OOM_ERROR;
+  int16_t *q = (int16_t *)malloc(sizeof(int16_t) * 64 * (size_t)j.num_blocks);
+  uint8_t *pl[3] = {0, 0, 0};
+  if (!q) return ORC_OOM_ERROR;
+  e = orc_decode_scan(&j, q, NULL, NULL);
+  for (int c = 0; c < j.num_channels && !e; c++) {
+    pl[c] = (uint8_t *)malloc((size_t)j.ch[c].stride * j.ch[c].plane_height);
+    if (!pl[c]) e = ORC_OOM_ERROR;
+  }
+  if (!e) e = orc_planes(&j, q, pl, idct_mode);
+  if (!e) e = orc_colour(&j, pl, out);
+  for (int c = 0; c < 3; c++) free(pl[c]);
+  free(q);
+  return e;
+}
