@@ -29,7 +29,7 @@ struct ImageDesc {
   int32_t nslots;                 // blocks per MCU
   int32_t mcu_w, mcu_h;           // MCU size in pixels
   int32_t table_base;             // index of this image's 4 HuffTables
-  int32_t qtab_base;              // index of this image's 4 quant tables (64 uint16, zigzag order)
+  int32_t qtab_base;              // index of this image's 4 quant tables (64 uint16, natural order)
   uint8_t slot_comp[kMaxSlots];   // component of each slot
   uint8_t slot_dc[kMaxSlots];     // DC table id (0..1) of each slot
   uint8_t slot_ac[kMaxSlots];     // AC table id (2..3) of each slot
@@ -73,12 +73,15 @@ struct WorkItem {
 };
 
 // Transfer of the decoder state across one workgroup: for each hypothesis h
-// of the lead-in tile, the hypothesis followed in the last tile and the
-// blocks / DC sums added on the way.  `inclusive` is the concrete state at the
-// workgroup's last tile once the true path through it is known.
+// of the lead-in tile, the hypothesis followed in the last tile (-1 when a
+// link on the way did not resolve) and the blocks / DC sums added on the way.
+// `inclusive` is the true state at the exit of the workgroup's last tile: the
+// hypothesis it lies on (-1 if none), its bit and slot, and the cumulative
+// blocks / DC sums of the segment.
 struct LookbackRec {
   int32_t status;         // 0 none, 1 aggregate, 2 inclusive  (written last)
   int32_t incl_h;
+  uint32_t incl_p, incl_s;
   int32_t incl_c[4];      // blocks, dc0, dc1, dc2
   int8_t agg_h[kMaxSlots];
   int8_t pad_[6];
@@ -91,6 +94,7 @@ enum : uint32_t {
   kStatusTruncated = 2u,     // scan ended before all blocks were decoded
   kStatusUnresolved = 4u,    // a tile link could not be resolved; rerun with the serial decoder
   kStatusInconsistent = 8u,  // final decode disagreed with the resolved path; rerun serially
+  kStatusFallback = 0x100u,  // host: the image was decoded again by the serial decoder
 };
 
 enum IdctMode : int { kIdctInt = 0, kIdctFloat = 1, kIdctMfma = 2 };

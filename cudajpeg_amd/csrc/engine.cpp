@@ -1,0 +1,355 @@
+// Batch engine of the MI355X baseline-JPEG decoder: host staging (parse,
+// unstuff, compact Huffman tables, work lists), device-resident buffers and
+// kernel orchestration behind the C ABI of include/cudajpeg_amd.h.
+//
+// Host side of what the reference does in decodeScanGPU (src/decodeScanGPU.cu:
+// 104-151: table reshuffle, unstuffing, scan upload) and in
+// iDCT_resample_colourTransform (src/pixelTransformGPU.cu:621-677), with the
+// grow-only buffer policy of ensureMemSize (src/format.cu:81-110).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/cudajpeg_amd.h"
+#include "cjd_internal.h"
+#include "launch.h"
+#include "parse_internal.h"
+#include "plan.h"
+
+using namespace cjd;
+
+struct cjd_context {
+  int device = 0;
+};
+
+namespace {
+
+// Grow-only device buffer (src/format.cu:81-110 policy).
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t n = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&p, n) != hipSuccess) {
+      p = nullptr;
+      return CJD_DEVICE_MEM_ERROR;
+    }
+    cap = n;
+    return 0;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+}  // namespace
+
+struct cjd_batch {
+  cjd_context* ctx = nullptr;
+  int idct_mode = kIdctInt;
+  int entropy_mode = CJD_ENTROPY_TILED;
+  Plan plan;
+
+  DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_qtabs, d_coefs, d_out, d_scratch;
+  size_t scratch_words = 0, status_words = 0, ticket_word = 0;
+  std::vector<uint32_t> h_status;
+  int64_t fallbacks = 0;
+
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  bool timed = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
+  ~cjd_batch() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+extern "C" int cjd_context_create(int device, cjd_context** out) {
+  if (!out) return CJD_PROGRAMMER_ERROR;
+  if (hipSetDevice(device) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+  auto* c = new (std::nothrow) cjd_context;
+  if (!c) return CJD_OOM_ERROR;
+  c->device = device;
+  *out = c;
+  return 0;
+}
+
+extern "C" void cjd_context_destroy(cjd_context* ctx) { delete ctx; }
+
+extern "C" int cjd_batch_create(cjd_context* ctx, cjd_batch** out) {
+  if (!ctx || !out) return CJD_PROGRAMMER_ERROR;
+  auto* b = new (std::nothrow) cjd_batch;
+  if (!b) return CJD_OOM_ERROR;
+  b->ctx = ctx;
+  for (auto& e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete b;
+      return CJD_DEVICE_MEM_ERROR;
+    }
+  *out = b;
+  return 0;
+}
+
+extern "C" void cjd_batch_destroy(cjd_batch* b) { delete b; }
+
+extern "C" int cjd_batch_set_options(cjd_batch* b, int idct_mode, int entropy_mode, int tile_bits, int want_planes) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  if (idct_mode < 0 || idct_mode > 2 || entropy_mode < 0 || entropy_mode > 1) return CJD_PROGRAMMER_ERROR;
+  if (tile_bits && (tile_bits < 2048 || (tile_bits & 31))) return CJD_PROGRAMMER_ERROR;
+  b->idct_mode = idct_mode;
+  b->entropy_mode = entropy_mode;
+  if (tile_bits) b->plan.tile_bits = (uint32_t)tile_bits;
+  b->plan.want_planes = want_planes ? 1 : 0;
+  return 0;
+}
+
+template <class T>
+static int upload(DevBuf& d, const std::vector<T>& v) {
+  const size_t bytes = v.size() * sizeof(T);
+  if (int e = d.ensure(bytes)) return e;
+  if (bytes && hipMemcpy(d.p, v.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+  return 0;
+}
+
+extern "C" int cjd_batch_stage(cjd_batch* b, int n, const uint8_t* const* files, const size_t* sizes, int* errors) {
+  if (!b || n < 0 || (n && (!files || !sizes))) return CJD_PROGRAMMER_ERROR;
+  if (hipSetDevice(b->ctx->device) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  plan_batch(b->plan, n, files, sizes, errors);
+  int e = 0;
+  if (!e) e = upload(b->d_words, b->plan.words);
+  if (!e) e = upload(b->d_images, b->plan.images);
+  if (!e) e = upload(b->d_segs, b->plan.segs);
+  if (!e) e = upload(b->d_items, b->plan.items);
+  if (!e) e = upload(b->d_pix, b->plan.pix);
+  if (!e) e = upload(b->d_tables, b->plan.tables);
+  if (!e) e = upload(b->d_qtabs, b->plan.qtabs);
+  if (!e) e = b->d_coefs.ensure((size_t)b->plan.total_blocks * 128);
+  if (!e) e = b->d_out.ensure(b->plan.out_bytes);
+  // scratch: per-image status, ticket, lookback records (cleared per decode)
+  b->status_words = ((size_t)n + 3) & ~(size_t)3;
+  b->ticket_word = b->status_words;
+  b->scratch_words = b->status_words + 4 + b->plan.items.size() * (sizeof(LookbackRec) / 4);
+  if (!e) e = b->d_scratch.ensure(b->scratch_words * 4);
+  b->h_status.assign(b->status_words, 0);
+  return e;
+}
+
+static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
+  uint32_t* scratch = b->d_scratch.as<uint32_t>();
+  uint32_t* status = scratch;
+  uint32_t* ticket = scratch + b->ticket_word;
+  LookbackRec* lb = reinterpret_cast<LookbackRec*>(scratch + b->ticket_word + 4);
+  if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  if (launch_clear(scratch, b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  hipError_t he;
+  if (b->entropy_mode == CJD_ENTROPY_SERIAL)
+    he = launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
+                               (uint32_t)b->plan.segs.size(), b->d_tables.as<HuffTable>(), b->d_coefs.as<int16_t>(),
+                               status, s);
+  else
+    he = launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
+                        b->d_items.as<WorkItem>(), (uint32_t)b->plan.items.size(), b->d_tables.as<HuffTable>(),
+                        b->d_coefs.as<int16_t>(), status, lb, ticket, s);
+  if (he != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  if (timed && hipEventRecord(b->ev[1], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  return 0;
+}
+
+static int enqueue_pixel(cjd_batch* b, hipStream_t s, bool timed) {
+  if (launch_pixel(b->idct_mode, b->d_images.as<ImageDesc>(), b->d_pix.as<PixItem>(), (uint32_t)b->plan.pix.size(),
+                   b->d_coefs.as<int16_t>(), b->d_qtabs.as<uint16_t>(), b->d_out.as<uint8_t>(), s) != hipSuccess)
+    return CJD_KERNEL_LAUNCH_ERROR;
+  if (timed && hipEventRecord(b->ev[2], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  return 0;
+}
+
+static int enqueue(cjd_batch* b, hipStream_t s, bool timed) {
+  int e = enqueue_entropy(b, s, timed);
+  return e ? e : enqueue_pixel(b, s, timed);
+}
+
+// Wait for the entropy stage; decode again with the serial decoder every image
+// whose tile links did not resolve; map status bits to reference error codes.
+// `redo` receives the images that were decoded again.
+static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo) {
+  if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  if (b->status_words &&
+      hipMemcpy(b->h_status.data(), b->d_scratch.p, b->status_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return CJD_DEVICE_MEM_ERROR;
+  for (size_t i = 0; i < b->plan.recs.size(); ++i) {
+    ImageRec& r = b->plan.recs[i];
+    if (r.error) continue;
+    const uint32_t st = b->h_status[i];
+    if (st & (kStatusUnresolved | kStatusInconsistent)) {
+      uint32_t zero = 0;
+      uint32_t* status = b->d_scratch.as<uint32_t>() + i;
+      if (hipMemcpy(status, &zero, 4, hipMemcpyHostToDevice) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+      if (launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(),
+                                b->d_segs.as<SegmentDesc>() + r.seg_first, r.seg_count, b->d_tables.as<HuffTable>(),
+                                b->d_coefs.as<int16_t>(), b->d_scratch.as<uint32_t>(), s) != hipSuccess)
+        return CJD_KERNEL_LAUNCH_ERROR;
+      if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+      if (hipMemcpy(&b->h_status[i], status, 4, hipMemcpyDeviceToHost) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+      b->h_status[i] |= kStatusFallback;
+      ++b->fallbacks;
+      if (redo) redo->push_back(i);
+    }
+    if (b->h_status[i] & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
+  }
+  return 0;
+}
+
+extern "C" int cjd_batch_decode_scan(cjd_batch* b, void* stream) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  b->timed = false;
+  int e = enqueue_entropy(b, (hipStream_t)stream, false);
+  if (!e) e = check_entropy(b, (hipStream_t)stream, nullptr);
+  return e;
+}
+
+extern "C" int cjd_batch_transform(cjd_batch* b, void* stream) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  return enqueue_pixel(b, (hipStream_t)stream, false);
+}
+
+extern "C" int cjd_batch_decode(cjd_batch* b, void* stream) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  b->timed = true;
+  return enqueue(b, (hipStream_t)stream, true);
+}
+
+extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  hipStream_t s = (hipStream_t)stream;
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  int e = enqueue(b, s, false);
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(s, &g) != hipSuccess || e) {
+    if (g) (void)hipGraphDestroy(g);
+    return e ? e : CJD_KERNEL_LAUNCH_ERROR;
+  }
+  b->graph = g;
+  if (hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  return 0;
+}
+
+extern "C" int cjd_batch_replay(cjd_batch* b, void* stream) {
+  if (!b || !b->graph_exec) return CJD_PROGRAMMER_ERROR;
+  b->timed = false;
+  return hipGraphLaunch(b->graph_exec, (hipStream_t)stream) == hipSuccess ? 0 : CJD_KERNEL_LAUNCH_ERROR;
+}
+
+extern "C" int cjd_batch_finish(cjd_batch* b, void* stream, int* errors) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<size_t> redo;
+  int e = check_entropy(b, s, &redo);
+  if (e) return e;
+  for (size_t i : redo) {
+    const ImageRec& r = b->plan.recs[i];
+    if (launch_pixel(b->idct_mode, b->d_images.as<ImageDesc>(), b->d_pix.as<PixItem>() + r.pix_first, r.pix_count,
+                     b->d_coefs.as<int16_t>(), b->d_qtabs.as<uint16_t>(), b->d_out.as<uint8_t>(), s) != hipSuccess)
+      return CJD_KERNEL_LAUNCH_ERROR;
+  }
+  if (!redo.empty() && hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  if (errors)
+    for (size_t i = 0; i < b->plan.recs.size(); ++i) errors[i] = b->plan.recs[i].error;
+  return 0;
+}
+
+extern "C" int cjd_batch_fallbacks(cjd_batch* b) { return b ? (int)b->fallbacks : -1; }
+
+extern "C" int cjd_batch_timings(cjd_batch* b, float* ms_entropy, float* ms_pixel, float* ms_total) {
+  if (!b || !b->timed) return CJD_PROGRAMMER_ERROR;
+  if (hipEventSynchronize(b->ev[2]) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  float a = 0, c = 0, t = 0;
+  (void)hipEventElapsedTime(&a, b->ev[0], b->ev[1]);
+  (void)hipEventElapsedTime(&c, b->ev[1], b->ev[2]);
+  (void)hipEventElapsedTime(&t, b->ev[0], b->ev[2]);
+  if (ms_entropy) *ms_entropy = a;
+  if (ms_pixel) *ms_pixel = c;
+  if (ms_total) *ms_total = t;
+  return 0;
+}
+
+extern "C" int cjd_batch_image_info(cjd_batch* b, int i, cjd_frame_info* info) {
+  if (!b || i < 0 || (size_t)i >= b->plan.recs.size() || !info) return CJD_PROGRAMMER_ERROR;
+  *info = b->plan.recs[(size_t)i].f;
+  return b->plan.recs[(size_t)i].error;
+}
+
+extern "C" int cjd_batch_device_ptr(cjd_batch* b, int i, int what, void** ptr, size_t* bytes) {
+  if (!b || i < 0 || (size_t)i >= b->plan.recs.size()) return CJD_PROGRAMMER_ERROR;
+  const ImageRec& r = b->plan.recs[(size_t)i];
+  if (r.error) return r.error;
+  size_t n = 0;
+  void* p = nullptr;
+  if (what == CJD_OUT_PIXELS) {
+    p = b->d_out.as<uint8_t>() + r.out_offset;
+    n = r.out_bytes;
+  } else if (what == CJD_OUT_COEFFICIENTS) {
+    p = b->d_coefs.as<int16_t>() + r.coef_base * 64;
+    n = (size_t)r.f.num_blocks * 128;
+  } else if (what >= CJD_OUT_PLANE0 && what < CJD_OUT_PLANE0 + r.f.num_channels) {
+    const int c = what - CJD_OUT_PLANE0;
+    if (r.plane_offset[c] == ~0ull) return CJD_PROGRAMMER_ERROR;
+    p = b->d_out.as<uint8_t>() + r.plane_offset[c];
+    n = (size_t)r.f.comp_stride[c] * r.f.comp_plane_height[c];
+  } else {
+    return CJD_PROGRAMMER_ERROR;
+  }
+  if (ptr) *ptr = p;
+  if (bytes) *bytes = n;
+  return 0;
+}
+
+extern "C" int cjd_batch_download(cjd_batch* b, int i, int what, void* host, size_t cap) {
+  void* p = nullptr;
+  size_t n = 0;
+  int e = cjd_batch_device_ptr(b, i, what, &p, &n);
+  if (e) return e;
+  if (n > cap) return CJD_PROGRAMMER_ERROR;
+  return hipMemcpy(host, p, n, hipMemcpyDeviceToHost) == hipSuccess ? 0 : CJD_DEVICE_MEM_ERROR;
+}
+
+extern "C" int cjd_batch_totals(cjd_batch* b, int64_t* images, int64_t* pixels, int64_t* scan_bytes, int64_t* blocks,
+                                int64_t* out_bytes) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  if (images) *images = (int64_t)b->plan.images.size();
+  if (pixels) *pixels = b->plan.total_pixels;
+  if (scan_bytes) *scan_bytes = b->plan.scan_bytes;
+  if (blocks) *blocks = b->plan.total_blocks;
+  if (out_bytes) {
+    int64_t ob = 0;
+    for (auto& r : b->plan.recs)
+      if (!r.error) ob += (int64_t)r.out_bytes;
+    *out_bytes = ob;
+  }
+  return 0;
+}

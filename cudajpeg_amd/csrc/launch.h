@@ -1,0 +1,19 @@
+// Host-side launchers of the HIP kernels (defined beside the kernels).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cjd_internal.h"
+
+namespace cjd {
+
+hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s);
+hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
+                          const WorkItem* items, uint32_t nitems, const HuffTable* tables, int16_t* coefs,
+                          uint32_t* status, LookbackRec* lb, uint32_t* ticket, hipStream_t s);
+hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
+                                 uint32_t nsegs, const HuffTable* tables, int16_t* coefs, uint32_t* status,
+                                 hipStream_t s);
+hipError_t launch_pixel(int mode, const ImageDesc* images, const PixItem* items, uint32_t nitems,
+                        const int16_t* coefs, const uint16_t* qtabs, uint8_t* out, hipStream_t s);
+
+}  // namespace cjd

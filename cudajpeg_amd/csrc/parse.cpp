@@ -1,0 +1,238 @@
+// Host marker / table parser of the MI355X baseline-JPEG decoder.
+//
+// Same checks, same order and same error codes as the reference parser
+// (src/format.cu:113-442 and the SOS header of src/decodeScanGPU.cu:124-141),
+// operating on an in-memory file.  The 65536-entry Huffman LUTs of the
+// reference (src/format.cu:371-391) are not built here: DHT data is kept as
+// BITS/HUFFVAL and turned into compact device tables by the engine.
+#include <string.h>
+
+#include "../../include/cudajpeg_amd.h"
+#include "parse_internal.h"
+
+namespace cjd {
+
+static inline unsigned read16(const uint8_t* p) { return (unsigned)((p[0] << 8) | p[1]); }
+
+#define PFAIL(e)              \
+  do {                        \
+    st.error = (e);           \
+    st.error_line = __LINE__; \
+    return;                   \
+  } while (0)
+
+// src/format.cu:242-354
+void parse_sof(ParseState& st) {
+  cjd_frame_info& f = *st.f;
+  const uint8_t* block = st.pos;
+  const unsigned block_len = read16(block);
+  if (block_len < 9 || block + block_len >= st.end) PFAIL(CJD_SYNTAX_ERROR);
+  if (block[2] != 8) PFAIL(CJD_UNSUPPORTED_ERROR);
+  f.height = (int)read16(block + 3);
+  f.width = (int)read16(block + 5);
+  if (!f.width || !f.height) PFAIL(CJD_SYNTAX_ERROR);
+  f.num_channels = block[7];
+  if (f.num_channels != 1 && f.num_channels != 3) PFAIL(CJD_UNSUPPORTED_ERROR);
+  if (block_len < 8u + (unsigned)(f.num_channels * 3)) PFAIL(CJD_SYNTAX_ERROR);
+  block += 8;
+  int sxm = 0, sym = 0;
+  for (int i = 0; i < f.num_channels; ++i, block += 3) {
+    f.comp_id[i] = block[0];
+    f.samples_x[i] = block[1] >> 4;
+    f.samples_y[i] = block[1] & 0xF;
+    f.dq_id[i] = block[2];
+    if (!f.samples_x[i] || !f.samples_y[i] || f.dq_id[i] > 3) PFAIL(CJD_SYNTAX_ERROR);
+    if ((f.samples_x[i] & (f.samples_x[i] - 1)) || (f.samples_y[i] & (f.samples_y[i] - 1)))
+      PFAIL(CJD_UNSUPPORTED_ERROR);
+    if (f.samples_x[i] > sxm) sxm = f.samples_x[i];
+    if (f.samples_y[i] > sym) sym = f.samples_y[i];
+  }
+  if (f.num_channels == 1) {  // src/format.cu:281-284
+    f.samples_x[0] = sxm = 1;
+    f.samples_y[0] = sym = 1;
+  }
+  f.mcu_width = sxm << 3;
+  f.mcu_height = sym << 3;
+  f.mcus_x = (f.width + f.mcu_width - 1) / f.mcu_width;
+  f.mcus_y = (f.height + f.mcu_height - 1) / f.mcu_height;
+  f.blocks_per_mcu = 0;
+  for (int i = 0; i < f.num_channels; ++i) {
+    f.comp_width[i] = (f.width * f.samples_x[i] + sxm - 1) / sxm;
+    f.comp_height[i] = (f.height * f.samples_y[i] + sym - 1) / sym;
+    f.comp_stride[i] = f.mcus_x * (f.samples_x[i] << 3);
+    f.comp_plane_height[i] = f.mcus_y * (f.samples_y[i] << 3);
+    if ((f.comp_width[i] < 3 && f.samples_x[i] != sxm) || (f.comp_height[i] < 3 && f.samples_y[i] != sym))
+      PFAIL(CJD_UNSUPPORTED_ERROR);
+    f.blocks_per_mcu += f.samples_x[i] * f.samples_y[i];
+  }
+  if (f.num_channels == 3) {  // src/format.cu:346-350
+    if (f.comp_width[1] != f.comp_width[2] || f.comp_height[1] != f.comp_height[2] ||
+        f.comp_width[0] != f.width || f.comp_height[0] != f.height)
+      PFAIL(CJD_UNSUPPORTED_ERROR);
+  }
+  f.num_blocks = (int64_t)f.mcus_x * f.mcus_y * f.blocks_per_mcu;
+  st.have_sof = true;
+  st.pos += block_len;
+}
+
+// src/format.cu:357-400 (table id = Tc*2 + Th)
+void parse_dht(ParseState& st) {
+  cjd_frame_info& f = *st.f;
+  const uint8_t* pos = st.pos;
+  const unsigned block_len = read16(pos);
+  const uint8_t* block_end = pos + block_len;
+  if (block_end > st.end) PFAIL(CJD_SYNTAX_ERROR);
+  pos += 2;
+  while (pos < block_end) {
+    const uint8_t val = pos[0];
+    if (val & 0xEC) PFAIL(CJD_SYNTAX_ERROR);
+    if (val & 0x02) PFAIL(CJD_UNSUPPORTED_ERROR);
+    const int id = (val | (val >> 3)) & 3;
+    const uint8_t* tuple = pos + 17;
+    int remain = 65536, n = 0;
+    uint8_t bits[16];
+    uint8_t vals[256];
+    for (int l = 1; l <= 16; ++l) {
+      const int count = pos[l];
+      bits[l - 1] = (uint8_t)count;
+      if (!count) continue;
+      if (tuple + count > block_end) PFAIL(CJD_SYNTAX_ERROR);
+      remain -= count << (16 - l);
+      if (remain < 0) PFAIL(CJD_SYNTAX_ERROR);
+      for (int i = 0; i < count; ++i) vals[n++] = *tuple++;
+    }
+    memcpy(f.huff_bits[id], bits, 16);
+    memset(f.huff_vals[id], 0, 256);
+    memcpy(f.huff_vals[id], vals, (size_t)n);
+    pos = tuple;
+  }
+  if (pos != block_end) PFAIL(CJD_SYNTAX_ERROR);
+  st.pos = block_end;
+}
+
+// src/format.cu:404-421
+void parse_dri(ParseState& st) {
+  const unsigned block_len = read16(st.pos);
+  const uint8_t* block_end = st.pos + block_len;
+  if (block_len < 2 || block_end > st.end) PFAIL(CJD_SYNTAX_ERROR);
+  st.f->restart_interval = (int)read16(st.pos + 2);
+  st.pos = block_end;
+}
+
+// src/format.cu:424-442
+void parse_dqt(ParseState& st) {
+  const unsigned block_len = read16(st.pos);
+  const uint8_t* block_end = st.pos + block_len;
+  if (block_end > st.end) PFAIL(CJD_SYNTAX_ERROR);
+  const uint8_t* pos = st.pos + 2;
+  while (pos + 65 <= block_end) {
+    const uint8_t id = pos[0];
+    if (id & 0xFC) PFAIL(CJD_SYNTAX_ERROR);
+    memcpy(st.f->dq_tables[id], pos + 1, 64);
+    pos += 65;
+  }
+  if (pos != block_end) PFAIL(CJD_SYNTAX_ERROR);
+  st.pos = block_end;
+}
+
+// src/decodeScanGPU.cu:124-141
+void parse_sos(ParseState& st) {
+  cjd_frame_info& f = *st.f;
+  const uint8_t* pos = st.pos;
+  const unsigned header_len = read16(pos);
+  if (pos + header_len > st.end) PFAIL(CJD_SYNTAX_ERROR);
+  pos += 2;
+  if (header_len < 4u + 2u * (unsigned)f.num_channels) PFAIL(CJD_SYNTAX_ERROR);
+  if (*(pos++) != f.num_channels) PFAIL(CJD_UNSUPPORTED_ERROR);
+  for (int i = 0; i < f.num_channels; ++i, pos += 2) {
+    if (pos[0] != f.comp_id[i]) PFAIL(CJD_SYNTAX_ERROR);
+    if (pos[1] & 0xEE) PFAIL(CJD_SYNTAX_ERROR);
+    f.dc_id[i] = pos[1] >> 4;
+    f.ac_id[i] = (pos[1] & 1) | 2;
+  }
+  if (pos[0] || pos[1] != 63 || pos[2]) PFAIL(CJD_UNSUPPORTED_ERROR);
+  st.pos += header_len;
+  // The scan runs to the EOI: the reference unstuffs everything up to
+  // jpg->end (src/decodeScanGPU.cu:13).
+  f.scan_offset = (uint64_t)(st.pos - st.buf);
+  f.scan_length = (uint64_t)(st.end - st.pos);
+}
+
+// src/format.cu:113-215 without the file read; stops after the SOS header
+// (or at the EOI) and leaves st.pos there.
+int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame_info* f) {
+  memset(f, 0, sizeof(*f));
+  st.f = f;
+  st.buf = data;
+  st.error = 0;
+  st.error_line = 0;
+  st.have_sof = false;
+  st.at_scan = false;
+  if (size < 4 || data[0] != 0xFF || data[1] != 0xD8 || data[size - 2] != 0xFF || data[size - 1] != 0xD9) {
+    st.error = CJD_FILE_ERROR;
+    st.error_line = __LINE__;
+    return st.error;
+  }
+  if (size < 6) {
+    st.error = CJD_SYNTAX_ERROR;
+    st.error_line = __LINE__;
+    return st.error;
+  }
+  st.end = data + size - 2;
+  st.pos = data + 2;
+  while (!st.error) {
+    if (st.pos > st.end || st.pos[0] != 0xFF) {
+      st.error = CJD_SYNTAX_ERROR;
+      st.error_line = __LINE__;
+      break;
+    }
+    st.pos += 2;
+    const uint8_t m = st.pos[-1];
+    switch (m) {
+      case 0xC0: parse_sof(st); break;
+      case 0xC4: parse_dht(st); break;
+      case 0xDB: parse_dqt(st); break;
+      case 0xDD: parse_dri(st); break;
+      case 0xDA:
+        parse_sos(st);
+        if (!st.error) st.at_scan = true;
+        break;
+      case 0xFE: st.pos += read16(st.pos); break;
+      case 0xD9: break;
+      default:
+        if ((m & 0xF0) == 0xE0) st.pos += read16(st.pos);
+        else {
+          st.error = CJD_SYNTAX_ERROR;
+          st.error_line = __LINE__;
+        }
+    }
+    if (st.error || st.at_scan || m == 0xD9) break;
+  }
+  f->error_line = st.error_line;
+  return st.error;
+}
+
+}  // namespace cjd
+
+extern "C" int cjd_parse(const uint8_t* data, size_t size, cjd_frame_info* out) {
+  cjd::ParseState st;
+  int e = cjd::parse_until_scan(st, data, size, out);
+  if (!e && !st.at_scan) e = CJD_SYNTAX_ERROR;  // EOI before any scan
+  if (!e && !st.have_sof) e = CJD_SYNTAX_ERROR;
+  out->error_line = st.error_line;
+  return e;
+}
+
+extern "C" const char* cjd_error_name(int code) {
+  switch (code) {  // src/format.cu:62-78
+    case CJD_NO_ERROR: return "NO_ERROR";
+    case CJD_SYNTAX_ERROR: return "SYNTAX_ERROR";
+    case CJD_UNSUPPORTED_ERROR: return "UNSUPPORTED_ERROR";
+    case CJD_OOM_ERROR: return "OOM_ERROR";
+    case CJD_DEVICE_MEM_ERROR: return "DEVICE_MEM_ERROR";
+    case CJD_FILE_ERROR: return "FILE_ERROR";
+    case CJD_KERNEL_LAUNCH_ERROR: return "KERNEL_LAUNCH_ERROR";
+    case CJD_PROGRAMMER_ERROR: return "PROGRAMMER_ERROR";
+    default: return "UNRECOGNISED_ERROR";
+  }
+}

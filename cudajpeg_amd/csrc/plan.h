@@ -1,0 +1,47 @@
+// Host planning of a decode batch: parse, unstuff, compact Huffman tables,
+// quant tables, image / segment descriptors and the entropy and pixel work
+// lists.  Pure host code (no HIP calls), shared by the engine and the CPU
+// emulator of the kernels.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/cudajpeg_amd.h"
+#include "cjd_internal.h"
+
+namespace cjd {
+
+struct ImageRec {
+  int error = 0;
+  cjd_frame_info f{};
+  int64_t pixels = 0;
+  uint64_t out_offset = 0, out_bytes = 0;
+  uint64_t plane_offset[3] = {~0ull, ~0ull, ~0ull};
+  int64_t coef_base = 0;
+  uint32_t desc = 0;  // ImageDesc index (valid when error == 0)
+  uint32_t seg_first = 0, seg_count = 0;
+  uint32_t pix_first = 0, pix_count = 0;
+};
+
+struct Plan {
+  uint32_t tile_bits = 4096;
+  int want_planes = 0;
+  std::vector<ImageRec> recs;
+  std::vector<ImageDesc> images;
+  std::vector<SegmentDesc> segs;
+  std::vector<WorkItem> items;
+  std::vector<PixItem> pix;
+  std::vector<HuffTable> tables;
+  std::vector<uint16_t> qtabs;   // natural order, 64 per table
+  std::vector<uint32_t> words;   // unstuffed segments, word aligned, 0xFF padded
+  int64_t total_blocks = 0, total_pixels = 0, scan_bytes = 0;
+  uint64_t out_bytes = 0;
+};
+
+// Build the plan for n files; errors[i] gets each file's parse/stage result.
+void plan_batch(Plan& p, int n, const uint8_t* const* files, const size_t* sizes, int* errors);
+void build_table(const uint8_t bits[16], const uint8_t vals[256], HuffTable& t);
+
+}  // namespace cjd

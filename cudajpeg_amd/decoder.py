@@ -1,0 +1,273 @@
+"""Python face of the decoder: a batch decoder over the C ABI and the
+reference-style JPGReader (include/format.h).
+
+Everything runs on the GPU through libcudajpeg_amd.so; this module only moves
+arguments and copies results out.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(getattr(stream, "cuda_stream"))
+
+
+class DecodeError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        super().__init__(f"{what}{L.error_name(code)} ({code})")
+        self.code = code
+
+
+def _check(code: int, what: str = "") -> None:
+    if code:
+        raise DecodeError(code, what)
+
+
+def parse(data: bytes) -> tuple[int, L.FrameInfo]:
+    """Host marker/table parse of one file (cjd_parse)."""
+    info = L.FrameInfo()
+    e = L.lib().cjd_parse(data, len(data), ctypes.byref(info))
+    return e, info
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = ctypes.c_void_p()
+        _check(L.lib().cjd_context_create(device, ctypes.byref(self.h)), "context: ")
+
+    def close(self):
+        if self.h:
+            L.lib().cjd_context_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class ImageOut:
+    width: int
+    height: int
+    channels: int
+    num_blocks: int
+    error: int
+
+
+class BatchDecoder:
+    """Decodes a batch of JPEG byte strings together on one GPU.
+
+    stage() parses, unstuffs and uploads; decode() enqueues the entropy and
+    pixel kernels; finish() waits and reports per-image reference error codes.
+    """
+
+    def __init__(self, idct: int = L.IDCT_INT, entropy: int = L.ENTROPY_TILED, tile_bits: int = 0,
+                 planes: bool = False, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.h = ctypes.c_void_p()
+        _check(L.lib().cjd_batch_create(self.ctx.h, ctypes.byref(self.h)), "batch: ")
+        self.set_options(idct, entropy, tile_bits, planes)
+        self.n = 0
+        self.errors = np.zeros(0, np.int32)
+
+    def set_options(self, idct: int, entropy: int = L.ENTROPY_TILED, tile_bits: int = 0, planes: bool = False):
+        _check(L.lib().cjd_batch_set_options(self.h, idct, entropy, tile_bits, int(planes)), "options: ")
+
+    def stage(self, files: list[bytes]) -> np.ndarray:
+        n = len(files)
+        self._bufs = [ctypes.create_string_buffer(f, len(f)) for f in files]
+        ptrs = (ctypes.c_void_p * n)(*[ctypes.cast(b, ctypes.c_void_p) for b in self._bufs])
+        sizes = (ctypes.c_size_t * n)(*[len(f) for f in files])
+        self.errors = np.zeros(n, np.int32)
+        _check(L.lib().cjd_batch_stage(self.h, n, ptrs, sizes, self.errors.ctypes.data), "stage: ")
+        self.n = n
+        self._bufs = None
+        return self.errors.copy()
+
+    def decode(self, stream=None) -> None:
+        _check(L.lib().cjd_batch_decode(self.h, _stream_handle(stream)), "decode: ")
+
+    def decode_scan(self, stream=None) -> None:
+        _check(L.lib().cjd_batch_decode_scan(self.h, _stream_handle(stream)), "decode_scan: ")
+
+    def transform(self, stream=None) -> None:
+        _check(L.lib().cjd_batch_transform(self.h, _stream_handle(stream)), "transform: ")
+
+    def finish(self, stream=None) -> np.ndarray:
+        errs = np.zeros(max(self.n, 1), np.int32)
+        _check(L.lib().cjd_batch_finish(self.h, _stream_handle(stream), errs.ctypes.data), "finish: ")
+        self.errors = errs[: self.n]
+        return self.errors.copy()
+
+    def capture(self, stream) -> None:
+        _check(L.lib().cjd_batch_capture(self.h, _stream_handle(stream)), "capture: ")
+
+    def replay(self, stream) -> None:
+        _check(L.lib().cjd_batch_replay(self.h, _stream_handle(stream)), "replay: ")
+
+    def fallbacks(self) -> int:
+        return int(L.lib().cjd_batch_fallbacks(self.h))
+
+    def timings(self) -> tuple[float, float, float]:
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        _check(L.lib().cjd_batch_timings(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "timings: ")
+        return a.value, b.value, c.value
+
+    def info(self, i: int) -> L.FrameInfo:
+        f = L.FrameInfo()
+        L.lib().cjd_batch_image_info(self.h, i, ctypes.byref(f))
+        return f
+
+    def totals(self) -> dict:
+        v = [ctypes.c_int64() for _ in range(5)]
+        _check(L.lib().cjd_batch_totals(self.h, *[ctypes.byref(x) for x in v]), "totals: ")
+        return dict(zip(["images", "pixels", "scan_bytes", "blocks", "out_bytes"], [x.value for x in v]))
+
+    def device_ptr(self, i: int, what: int = L.OUT_PIXELS) -> tuple[int, int]:
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(L.lib().cjd_batch_device_ptr(self.h, i, what, ctypes.byref(p), ctypes.byref(n)), "device_ptr: ")
+        return p.value, n.value
+
+    def _download(self, i: int, what: int, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        _check(L.lib().cjd_batch_download(self.h, i, what, out.ctypes.data, out.nbytes), "download: ")
+        return out
+
+    def pixels(self, i: int) -> np.ndarray:
+        f = self.info(i)
+        c = 3 if f.num_channels == 3 else 1
+        return self._download(i, L.OUT_PIXELS, (f.height, f.width, c), np.uint8)
+
+    def coefficients(self, i: int) -> np.ndarray:
+        f = self.info(i)
+        return self._download(i, L.OUT_COEFFICIENTS, (f.num_blocks, 64), np.int16)
+
+    def plane(self, i: int, comp: int) -> np.ndarray:
+        f = self.info(i)
+        return self._download(i, L.OUT_PLANE0 + comp, (f.comp_plane_height[comp], f.comp_stride[comp]), np.uint8)
+
+    def close(self):
+        if self.h:
+            L.lib().cjd_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def decode(data: bytes, idct: int = L.IDCT_INT) -> np.ndarray:
+    """Decode one JPEG to an HxWx{1,3} uint8 array (raises DecodeError)."""
+    b = BatchDecoder(idct)
+    err = b.stage([data])
+    _check(int(err[0]), "parse: ")
+    b.decode()
+    err = b.finish()
+    _check(int(err[0]), "decode: ")
+    out = b.pixels(0)
+    b.close()
+    return out
+
+
+# ------------------------------------------------ reference-style reader --
+class _ColourChannel(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int), ("dq_id", ctypes.c_int), ("ac_id", ctypes.c_int), ("dc_id", ctypes.c_int),
+                ("width", ctypes.c_int), ("height", ctypes.c_int), ("samples_x", ctypes.c_int),
+                ("samples_y", ctypes.c_int), ("stride", ctypes.c_int), ("plane_height", ctypes.c_int),
+                ("working_space", ctypes.POINTER(ctypes.c_int)), ("working_space_size", ctypes.c_uint),
+                ("raw_pixels", ctypes.POINTER(ctypes.c_uint8)), ("raw_pixels_size", ctypes.c_uint),
+                ("pixels", ctypes.POINTER(ctypes.c_uint8)), ("pixels_size", ctypes.c_uint)]
+
+
+class _JPGReader(ctypes.Structure):
+    _fields_ = [("buf", ctypes.c_void_p), ("pos", ctypes.c_void_p), ("end", ctypes.c_void_p),
+                ("width", ctypes.c_ushort), ("height", ctypes.c_ushort),
+                ("num_blocks_x", ctypes.c_ushort), ("num_blocks_y", ctypes.c_ushort),
+                ("block_size_x", ctypes.c_ushort), ("block_size_y", ctypes.c_ushort),
+                ("num_channels", ctypes.c_ubyte), ("restart_interval", ctypes.c_int),
+                ("channels", _ColourChannel * 3),
+                ("pixels", ctypes.POINTER(ctypes.c_uint8)), ("max_pixels_size", ctypes.c_int),
+                ("dq_tables", (ctypes.c_uint8 * 64) * 4),
+                ("file_buf", ctypes.c_void_p), ("file_buf_size", ctypes.c_uint),
+                ("idct_mode", ctypes.c_int), ("entropy_mode", ctypes.c_int),
+                ("keep_coefficients", ctypes.c_int), ("keep_planes", ctypes.c_int),
+                ("engine", ctypes.c_void_p),
+                ("error", ctypes.c_int), ("error_line", ctypes.c_int),
+                ("error_file", ctypes.c_char_p), ("error_func", ctypes.c_char_p)]
+
+
+class JPGReader:
+    """The reference's reader (newJPGReader / openJPG / delJPGReader,
+    src/format.cu:12-215): reusable across images; after open() the RGB or
+    gray frame is in `pixels`."""
+
+    def __init__(self, idct: int = L.IDCT_INT, entropy: int = L.ENTROPY_TILED, keep_coefficients: bool = False,
+                 keep_planes: bool = False):
+        p = L.lib().newJPGReader()
+        if not p:
+            raise DecodeError(L.OOM_ERROR, "newJPGReader: ")
+        self._p = ctypes.cast(p, ctypes.POINTER(_JPGReader))
+        r = self._p.contents
+        r.idct_mode = idct
+        r.entropy_mode = entropy
+        r.keep_coefficients = int(keep_coefficients)
+        r.keep_planes = int(keep_planes)
+
+    @property
+    def raw(self) -> _JPGReader:
+        return self._p.contents
+
+    def open(self, src) -> int:
+        """openJPG(path) or openJPGMem(bytes); returns the reference error code."""
+        if isinstance(src, (bytes, bytearray)):
+            return int(L.lib().openJPGMem(self._p, bytes(src), len(src)))
+        return int(L.lib().openJPG(self._p, str(src).encode()))
+
+    @property
+    def error(self) -> tuple[int, int, str, str]:
+        r = self.raw
+        return (r.error, r.error_line, (r.error_file or b"").decode(), (r.error_func or b"").decode())
+
+    @property
+    def pixels(self) -> np.ndarray:
+        r = self.raw
+        if r.num_channels == 3:
+            return np.ctypeslib.as_array(r.pixels, (r.height, r.width, 3)).copy()
+        return np.ctypeslib.as_array(r.channels[0].pixels, (r.height, r.width, 1)).copy()
+
+    def working_space(self, comp: int) -> np.ndarray:
+        ch = self.raw.channels[comp]
+        return np.ctypeslib.as_array(ch.working_space, (ch.stride * ch.plane_height,)).copy()
+
+    def raw_pixels(self, comp: int) -> np.ndarray:
+        ch = self.raw.channels[comp]
+        return np.ctypeslib.as_array(ch.raw_pixels, (ch.plane_height, ch.stride)).copy()
+
+    def write(self, path: str) -> None:
+        L.lib().writeJPG(self._p, path.encode())
+
+    def close(self):
+        if getattr(self, "_p", None):
+            L.lib().delJPGReader(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()

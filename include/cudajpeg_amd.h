@@ -1,0 +1,142 @@
+/*
+ * cudajpeg_amd -- C ABI of the MI355X-native baseline-JPEG decoder.
+ *
+ * Plain pointers and sizes only; no C++ types, no exceptions, no torch types.
+ * Return values use the error numbering of the reference (include/format.h:7-14).
+ * Device memory is owned by the context/batch; host buffers by the caller.
+ * Streams cross as opaque `void*` (a hipStream_t; NULL = the null stream).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef CUDAJPEG_AMD_H
+#define CUDAJPEG_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/format.h:7-14 */
+#define CJD_NO_ERROR 0
+#define CJD_SYNTAX_ERROR 1
+#define CJD_UNSUPPORTED_ERROR 2
+#define CJD_OOM_ERROR 3
+#define CJD_DEVICE_MEM_ERROR 4      /* reference CUDA_MEM_ERROR */
+#define CJD_FILE_ERROR 5
+#define CJD_KERNEL_LAUNCH_ERROR 6   /* reference CUDA_KERNEL_LAUNCH_ERROR */
+#define CJD_PROGRAMMER_ERROR (-1)
+
+/* iDCT variants of the fused pixel stage */
+#define CJD_IDCT_INT 0    /* reference integer Chen-Wang iDCT, bit-exact pixels */
+#define CJD_IDCT_FLOAT 1  /* FP32 VALU separable iDCT */
+#define CJD_IDCT_MFMA 2   /* FP16 hi/lo 64x64 contraction on the matrix cores, FP32 accumulate */
+
+/* entropy decoders */
+#define CJD_ENTROPY_TILED 0   /* self-synchronising tiles + lookback scan (default) */
+#define CJD_ENTROPY_SERIAL 1  /* one lane per restart segment (reference order) */
+
+/* what cjd_batch_download copies */
+#define CJD_OUT_PIXELS 0        /* interleaved RGB (W*H*3) or gray (W*H) */
+#define CJD_OUT_COEFFICIENTS 1  /* int16 [num_blocks][64], quantised, natural order, scan order */
+#define CJD_OUT_PLANE0 2        /* planar component c = CJD_OUT_PLANE0 + c, stride x plane height */
+
+/* Frame description produced by the host parser: what decodeSOF / decodeDHT /
+ * decodeDQT / decodeDRI and the SOS header leave in the reference JPGReader
+ * (src/format.cu:242-442, src/decodeScanGPU.cu:124-141). */
+typedef struct {
+  int32_t width, height, num_channels;
+  int32_t mcus_x, mcus_y;               /* reference num_blocks_x / num_blocks_y */
+  int32_t mcu_width, mcu_height;        /* reference block_size_x / block_size_y */
+  int32_t restart_interval;
+  int32_t blocks_per_mcu;
+  int64_t num_blocks;
+  int32_t comp_id[3];
+  int32_t samples_x[3], samples_y[3];
+  int32_t comp_width[3], comp_height[3];
+  int32_t comp_stride[3], comp_plane_height[3];
+  int32_t dq_id[3], dc_id[3], ac_id[3]; /* ac_id is 2|Ta as in the reference */
+  uint8_t dq_tables[4][64];             /* as stored in the file: zigzag order */
+  uint8_t huff_bits[4][16];             /* table id = Tc*2 + Th */
+  uint8_t huff_vals[4][256];
+  uint64_t scan_offset;                 /* entropy-coded segment: file[scan_offset, +scan_length) */
+  uint64_t scan_length;
+  int32_t error_line;                   /* source line of the failed check, 0 if none */
+  int32_t pad_;
+} cjd_frame_info;
+
+typedef struct cjd_context cjd_context;
+typedef struct cjd_batch cjd_batch;
+
+/* Marker, quant-table and Huffman-table parsing of an in-memory file
+ * (replaces openJPG's marker loop, src/format.cu:113-215, with decodeSOF
+ * :242-354, decodeDHT :357-400, decodeDRI :404-421, decodeDQT :424-442 and the
+ * SOS header check of src/decodeScanGPU.cu:124-141).  Stops at the scan. */
+int cjd_parse(const uint8_t *data, size_t size, cjd_frame_info *out);
+
+/* Device context (replaces the device half of newJPGReader / delJPGReader,
+ * src/format.cu:12-59). */
+int cjd_context_create(int device, cjd_context **out);
+void cjd_context_destroy(cjd_context *ctx);
+
+/* A batch of images decoded together: device-resident scans, tables, work
+ * lists, coefficients and frames (replaces the per-reader device buffers
+ * managed by ensureMemSize, src/format.cu:81-110). */
+int cjd_batch_create(cjd_context *ctx, cjd_batch **out);
+void cjd_batch_destroy(cjd_batch *b);
+
+/* Options: iDCT variant, entropy decoder, tile length in bits (>= 2048,
+ * 0 = default), whether planar component planes are also written. */
+int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int tile_bits, int want_planes);
+
+/* Parse, unstuff (src/decodeScanGPU.cu:11-24) and upload n files; plans the
+ * entropy and pixel work lists.  errors[i] gets each file's parse result; a
+ * failed file is skipped by the decode.  Returns the first device error. */
+int cjd_batch_stage(cjd_batch *b, int n, const uint8_t *const *files, const size_t *sizes, int *errors);
+
+/* Enqueue the decode of the staged batch on `stream` (asynchronous):
+ * entropy decode (replaces decodeScanGPU, src/decodeScanGPU.cu:123) and the
+ * fused pixel stage (replaces iDCT_resample_colourTransform,
+ * src/pixelTransformGPU.cu:621). */
+int cjd_batch_decode(cjd_batch *b, void *stream);
+
+/* The two halves of cjd_batch_decode.  decode_scan replaces decodeScanGPU
+ * (src/decodeScanGPU.cu:123) and waits for it (running the serial decoder for
+ * images whose tile links did not resolve); transform replaces
+ * iDCT_resample_colourTransform (src/pixelTransformGPU.cu:621) and is
+ * asynchronous. */
+int cjd_batch_decode_scan(cjd_batch *b, void *stream);
+int cjd_batch_transform(cjd_batch *b, void *stream);
+
+/* Number of images the serial decoder has redone since the batch was created. */
+int cjd_batch_fallbacks(cjd_batch *b);
+
+/* Wait for the decode, run the serial entropy decoder for any image whose
+ * tile links did not resolve, and report per-image reference error codes. */
+int cjd_batch_finish(cjd_batch *b, void *stream, int *errors);
+
+/* Capture the decode of the staged batch into a HIP graph / replay it. */
+int cjd_batch_capture(cjd_batch *b, void *stream);
+int cjd_batch_replay(cjd_batch *b, void *stream);
+
+/* Device time of the last cjd_batch_decode (ms), measured with HIP events on
+ * the decode stream: entropy kernel, pixel kernel, whole decode. */
+int cjd_batch_timings(cjd_batch *b, float *ms_entropy, float *ms_pixel, float *ms_total);
+
+/* Geometry and device pointers of image i's outputs. */
+int cjd_batch_image_info(cjd_batch *b, int i, cjd_frame_info *info);
+int cjd_batch_device_ptr(cjd_batch *b, int i, int what, void **ptr, size_t *bytes);
+/* Copy image i's output `what` to host memory (synchronous). */
+int cjd_batch_download(cjd_batch *b, int i, int what, void *host, size_t cap);
+/* Totals of the staged batch: images, luma pixels, scan bytes (unstuffed),
+ * 8x8 blocks, output bytes. */
+int cjd_batch_totals(cjd_batch *b, int64_t *images, int64_t *pixels, int64_t *scan_bytes, int64_t *blocks,
+                     int64_t *out_bytes);
+
+/* Human-readable name of a reference error code (src/format.cu:62-78). */
+const char *cjd_error_name(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

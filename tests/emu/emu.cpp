@@ -1,0 +1,215 @@
+// CPU emulator of the decoder's HIP kernels -- test infrastructure.
+//
+// Compiles the product's host planner (plan.cpp, parse.cpp) and the
+// __host__ __device__ kernel phases (entropy_core.h, pixel_core.h) with g++ and
+// runs every workgroup lane by lane, with the barriers between phases, so the
+// kernels' indexing and the tile-link / lookback logic are checked against the
+// oracle without a GPU.  `order` = 1 runs the decoupled lookback with every
+// predecessor holding only its aggregate, which exercises the composition path.
+#include <stdint.h>
+#include <string.h>
+
+#include <memory>
+#include <vector>
+
+#include "entropy_core.h"
+#include "pixel_core.h"
+#include "plan.h"
+
+using namespace cjd;
+
+namespace {
+
+struct Emu {
+  Plan plan;
+  std::vector<int16_t> coefs;
+  std::vector<uint8_t> out;
+  std::vector<uint32_t> status;
+  int unresolved_items = 0;
+  int fallbacks = 0;
+};
+
+struct EntRun {
+  Plan* p;
+  std::vector<LookbackRec> lb;
+  std::vector<std::unique_ptr<EntShared>> shs;
+  std::vector<EntCtx> ctx;
+  std::vector<char> done;
+  int walks = 0;
+};
+
+// Lookback, state resolution and inclusive publication of item i, resolving
+// predecessors first only when the kernel would have waited for them.
+void resolve(EntRun& R, size_t i) {
+  if (R.done[i]) return;
+  EntShared& sh = *R.shs[i];
+  const EntCtx& c = R.ctx[i];
+  if (c.lead) {
+    LookAcc acc;
+    acc.identity(c.nh);
+    uint32_t q = c.it.prev_item;
+    int eh = -1;
+    uint32_t ep = 0, es = 0;
+    int32_t ec[4] = {0, 0, 0, 0};
+    for (;;) {
+      if (R.lb[q].status == 0) resolve(R, q);  // the kernel spins until it is published
+      const LookbackRec& r = R.lb[q];
+      if (r.status == 2) {
+        if (!acc.apply(r, sh, eh, ep, es, ec)) {
+          resolve(R, c.it.prev_item);
+          acc.identity(c.nh);
+          acc.apply(R.lb[c.it.prev_item], sh, eh, ep, es, ec);
+        }
+        break;
+      }
+      acc.absorb(r, c.nh);
+      q = R.p->items[q].prev_item;
+    }
+    sh.entry_h = eh;
+    sh.entry_p = ep;
+    sh.entry_s = es;
+    for (int k = 0; k < 4; ++k) sh.entry_c[k] = ec[k];
+  }
+  sh.need_walk = 0;
+  for (int t = 0; t < kWG; ++t) ent_states(sh, t, c);
+  if (sh.need_walk) {
+    ent_walk(sh, c);
+    ++R.walks;
+  }
+  ent_inclusive(sh, c, &R.lb[i]);
+  R.lb[i].status = 2;
+  R.done[i] = 1;
+}
+
+void run_entropy(Emu& E, int order) {
+  Plan& p = E.plan;
+  const size_t ni = p.items.size();
+  EntRun R;
+  R.p = &p;
+  R.lb.resize(ni);
+  memset(R.lb.data(), 0, ni * sizeof(LookbackRec));
+  R.shs.resize(ni);
+  R.ctx.resize(ni);
+  R.done.assign(ni, 0);
+  // phases 1-3 and aggregate publication for every item
+  for (size_t i = 0; i < ni; ++i) {
+    R.shs[i].reset(new EntShared);
+    EntShared& sh = *R.shs[i];
+    memset(&sh, 0, sizeof(sh));
+    EntCtx& c = R.ctx[i];
+    c.init(p.words.data(), p.images.data(), p.segs.data(), p.items[i]);
+    memcpy(sh.tab, &p.tables[c.im->table_base], sizeof(sh.tab));
+    for (int t = 0; t < kWG; ++t) ent_phase1(sh, t, c);
+    for (int t = 0; t < kWG; ++t) ent_phase2(sh, t, c);
+    for (int t = 0; t < kWG; ++t) ent_phase3_init(sh, t, c);
+    int buf = 1;
+    for (int d = 1; d < c.J; d <<= 1) {
+      for (int t = 0; t < kWG; ++t) ent_phase3_step(sh, t, c, d, buf);
+      buf ^= 1;
+    }
+    sh.buf = buf;
+    if (!ent_publish_aggregate(sh, c, &R.lb[i])) R.lb[i].status = 1;
+  }
+  // segment-first items resolve from the known start; then the rest in
+  // order (predecessors inclusive) or backwards (predecessors aggregate-only)
+  for (size_t i = 0; i < ni; ++i)
+    if (!R.ctx[i].lead) resolve(R, i);
+  for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
+  E.unresolved_items = R.walks;
+  for (size_t i = 0; i < ni; ++i)
+    for (int t = 0; t < kWG; ++t) ent_phase5(*R.shs[i], t, R.ctx[i], E.coefs.data(), E.status.data());
+}
+
+void run_pixel(Emu& E, int mode, size_t first, size_t count) {
+  Plan& p = E.plan;
+  std::unique_ptr<PixShared> sh(new PixShared);
+  for (size_t i = first; i < first + count; ++i) {
+    PixCtx c;
+    c.init(p.images.data(), p.pix[i]);
+    memset(sh.get(), 0, sizeof(PixShared));
+    for (int t = 0; t < kPixThreads; ++t) pix_stage(*sh, t, c, E.coefs.data(), p.qtabs.data());
+    if (mode == kIdctInt) {
+      for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctInt>(*sh, t, c);
+      for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctInt>(*sh, t, c);
+    } else {
+      for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctFloat>(*sh, t, c);
+      for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctFloat>(*sh, t, c);
+    }
+    for (int t = 0; t < kPixThreads; ++t) pix_planes(*sh, t, c, E.out.data());
+    for (int t = 0; t < kPixThreads; ++t) pix_colour(*sh, t, c, E.out.data());
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_mode, int tile_bits,
+              int want_planes, int order, int serial, int* errors) {
+  Emu* E = new Emu;
+  E->plan.tile_bits = tile_bits ? (uint32_t)tile_bits : 4096u;
+  E->plan.want_planes = want_planes;
+  plan_batch(E->plan, n, files, sizes, errors);
+  E->coefs.assign((size_t)E->plan.total_blocks * 64, (int16_t)0x5555);
+  E->out.assign(E->plan.out_bytes + 256, 0xAB);
+  E->status.assign((size_t)n + 4, 0);
+  if (serial) {
+    for (auto& s : E->plan.segs)
+      ent_serial_segment(E->plan.words.data(), E->plan.images.data(), s, E->plan.tables.data(), E->coefs.data(),
+                         E->status.data());
+  } else {
+    run_entropy(*E, order);
+  }
+  for (int i = 0; i < n; ++i) {
+    ImageRec& r = E->plan.recs[(size_t)i];
+    if (r.error) continue;
+    if (E->status[(size_t)i] & (kStatusUnresolved | kStatusInconsistent)) {
+      E->status[(size_t)i] = kStatusFallback;
+      ++E->fallbacks;
+      for (uint32_t s = 0; s < r.seg_count; ++s)
+        ent_serial_segment(E->plan.words.data(), E->plan.images.data(), E->plan.segs[r.seg_first + s],
+                           E->plan.tables.data(), E->coefs.data(), E->status.data());
+    }
+    if (E->status[(size_t)i] & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
+    if (errors) errors[i] = r.error;
+  }
+  run_pixel(*E, idct_mode, 0, E->plan.pix.size());
+  return E;
+}
+
+// what: 0 pixels, 1 coefficients, 2+c plane c; returns bytes copied or -1
+long emu_get(void* h, int i, int what, void* dst, size_t cap) {
+  Emu* E = static_cast<Emu*>(h);
+  const ImageRec& r = E->plan.recs[(size_t)i];
+  if (r.error) return -1;
+  const uint8_t* src;
+  size_t n;
+  if (what == 0) {
+    src = E->out.data() + r.out_offset;
+    n = r.out_bytes;
+  } else if (what == 1) {
+    src = reinterpret_cast<const uint8_t*>(E->coefs.data() + r.coef_base * 64);
+    n = (size_t)r.f.num_blocks * 128;
+  } else {
+    const int c = what - 2;
+    if (c >= r.f.num_channels || r.plane_offset[c] == ~0ull) return -1;
+    src = E->out.data() + r.plane_offset[c];
+    n = (size_t)r.f.comp_stride[c] * r.f.comp_plane_height[c];
+  }
+  if (n > cap) return -1;
+  memcpy(dst, src, n);
+  return (long)n;
+}
+
+// [items, items that needed a serial walk, fallbacks, segments]
+void emu_stats(void* h, long* out4) {
+  Emu* E = static_cast<Emu*>(h);
+  out4[0] = (long)E->plan.items.size();
+  out4[1] = E->unresolved_items;
+  out4[2] = E->fallbacks;
+  out4[3] = (long)E->plan.segs.size();
+}
+
+void emu_free(void* h) { delete static_cast<Emu*>(h); }
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+"""The product's kernel code (entropy_core.h, pixel_core.h) and host planner,
+compiled for the CPU and run lane by lane, against the oracle: coefficient
+parity of the tile/link/lookback entropy decoder and pixel parity of the fused
+pixel stage, without a GPU."""
+import numpy as np
+import pytest
+
+import oracle
+from cudajpeg_amd import synth
+from tests.emu import EmuResult
+
+SPECS = [
+    synth.ImageSpec(64, 48, "420", 75, 6.0, 1),
+    synth.ImageSpec(33, 17, "444", 95, 8.0, 2),
+    synth.ImageSpec(40, 24, "gray", 50, 8.0, 3),
+    synth.ImageSpec(37, 29, "422", 80, 5.0, 7),
+    synth.ImageSpec(100, 60, "420", 70, 6.0, 9, restart_interval=3),
+    synth.ImageSpec(300, 200, "420", 95, 12.0, 11, random_tile_frac=0.3),
+    synth.ImageSpec(512, 512, "420", 75, 6.0, 1000),
+    synth.ImageSpec(640, 480, "444", 95, 8.0, 15),
+    synth.ImageSpec(17, 9, "420", 90, 4.0, 12),
+]
+
+
+@pytest.fixture(scope="module")
+def encoded():
+    return [synth.encode(s, truth=True) for s in SPECS]
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("tile_bits", [2048, 4096])
+def test_emulated_kernels_bit_exact(encoded, order, tile_bits):
+    r = EmuResult([e.data for e in encoded], 0, tile_bits, True, order)
+    assert not r.errors.any()
+    assert r.fallbacks == 0
+    for i, (s, e) in enumerate(zip(SPECS, encoded)):
+        p = oracle.parse(e.data)
+        assert np.array_equal(r.get(i, 1, (p.num_blocks, 64), np.int16), e.coefs), s
+        pl = oracle.planes(p, e.coefs, 0)
+        for c in range(p.num_channels):
+            assert np.array_equal(r.get(i, 2 + c, pl[c].shape, np.uint8), pl[c]), (s, c)
+        ref = oracle.colour(p, pl)
+        assert np.array_equal(r.get(i, 0, ref.shape, np.uint8), ref), s
+
+
+def test_emulated_float_idct_within_one(encoded):
+    r = EmuResult([e.data for e in encoded], 1, 4096, True)
+    for i, (s, e) in enumerate(zip(SPECS, encoded)):
+        p = oracle.parse(e.data)
+        pl = oracle.planes(p, e.coefs, 1)
+        for c in range(p.num_channels):
+            got = r.get(i, 2 + c, pl[c].shape, np.uint8).astype(int)
+            assert np.abs(got - pl[c]).max() <= 1, s
+
+
+def test_emulated_serial_decoder(encoded):
+    r = EmuResult([e.data for e in encoded], 0, 4096, False, 0, serial=True)
+    for i, (s, e) in enumerate(zip(SPECS, encoded)):
+        p = oracle.parse(e.data)
+        assert np.array_equal(r.get(i, 1, (p.num_blocks, 64), np.int16), e.coefs), s
+
+
+def test_emulated_large_high_entropy():
+    # 4:2:0 q90 with 10% random tiles: long blocks stress tile links; the
+    # lookback must still resolve without the serial fallback
+    spec = synth.ImageSpec(1024, 576, "420", 90, 12.0, 2001, random_tile_frac=0.10)
+    e = synth.encode(spec, truth=True)
+    r = EmuResult([e.data], 0, 2048, False, 1)
+    p = oracle.parse(e.data)
+    assert r.fallbacks == 0
+    assert np.array_equal(r.get(0, 1, (p.num_blocks, 64), np.int16), e.coefs)
+
+
+def test_emulated_corrupt_scan_reports_syntax():
+    enc = synth.encode(synth.ImageSpec(64, 64, "420", 75, 6.0, 21))
+    data = bytearray(enc.data)
+    p = oracle.parse(bytes(data))
+    start = len(data) - 2 - p._raw.scan_len
+    mid = start + p._raw.scan_len // 2
+    data[mid:mid + 6] = b"\xff\x00\xff\x00\xff\x00"
+    assert oracle.decode_scan(oracle.parse(bytes(data))).error == 1
+    r = EmuResult([bytes(data)])
+    assert r.errors[0] == 1
