@@ -10,14 +10,17 @@ constexpr int kLutSize = 1 << kLutBits;
 constexpr int kMaxSlots = 10;               // blocks per MCU (T.81 B.2.3 limit)
 constexpr int kListLen = 8;                 // block starts remembered per hypothesis
 constexpr int kWG = 256;                    // entropy workgroup size (4 wave64)
+constexpr int kSegPadWords = 96;            // 0xFF words after every segment's data
 
 // Device Huffman table: a 2^kLutBits direct table for codes up to kLutBits
-// long plus canonical (maxcode, value offset) arrays for longer codes
-// (T.81 F.2.2.3).  LUT entry: bits 0-4 code length (0 = longer code), bits
-// 8-15 the symbol.
+// long plus canonical limits for longer codes (T.81 F.2.2.3).  LUT entry:
+// bits 0-4 code length (0 = longer code), bits 8-15 the symbol.  lim[l] is the
+// first left-justified 16-bit value past all codes of length <= l (non-
+// decreasing in l), so the length of a long code is the first l with
+// peek16 < lim[l]; its symbol is vals[valoff[l] + (peek16 >> (16 - l))].
 struct HuffTable {
   uint32_t lut[kLutSize];
-  int32_t maxcode[18];      // maxcode[l] for l in 1..16, -1 when no code of length l; [17] sentinel
+  uint32_t lim[18];         // lim[l] for l in 1..16
   int32_t valoff[18];       // index of the first symbol of length l minus the first code of length l
   uint8_t vals[256];
 };

@@ -46,6 +46,7 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
     const uint32_t* src = reinterpret_cast<const uint32_t*>(tables + c.im->table_base);
     uint32_t* dst = reinterpret_cast<uint32_t*>(sh.tab);
     for (int i = tid; i < (int)(sizeof(HuffTable) * 4 / 4); i += kWG) dst[i] = src[i];
+    if (tid < 64) sh.dz[tid] = kDeZigZag[tid];
   }
   __syncthreads();
   ent_phase1(sh, tid, c);

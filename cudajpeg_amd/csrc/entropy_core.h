@@ -47,33 +47,101 @@ CJD_HD inline uint32_t bswap32(uint32_t v) {
   return (v >> 24) | ((v >> 8) & 0xFF00u) | ((v << 8) & 0xFF0000u) | (v << 24);
 }
 
-// 32 bits of the unstuffed segment from bit p, MSB first; bytes beyond the
-// segment read as 0xFF (src/decodeScanGPU.cu:32).
-CJD_HD inline uint32_t load_word(const uint32_t* w, uint32_t nwords, uint32_t i) {
-  return i < nwords ? bswap32(w[i]) : 0xFFFFFFFFu;
-}
-CJD_HD inline uint32_t peek32(const uint32_t* w, uint32_t nwords, uint32_t p) {
-  const uint32_t i = p >> 5, o = p & 31;
-  const uint64_t v = ((uint64_t)load_word(w, nwords, i) << 32) | load_word(w, nwords, i + 1);
-  return (uint32_t)((v << o) >> 32);
-}
+// Sequential reader of an unstuffed segment, MSB first.  The next bits sit in
+// a 64-bit accumulator fed one 32-bit word at a time from a 4-word chunk; the
+// following chunk is already in flight (a 16-byte load issued one chunk ahead),
+// so a lane decoding symbols does not wait on memory.  Words past the
+// segment read as 0xFFFFFFFF (the reference pads with 0xFF beyond the end,
+// src/decodeScanGPU.cu:32).
+struct BitReader {
+  const uint32_t* w;
+  uint32_t nwords;
+  uint64_t acc;   // next bits, MSB-aligned; nacc of them valid (> 32 after refill)
+  int nacc;
+  uint32_t q0, q1, q2, q3;  // current chunk (byte-swapped)
+  uint32_t n0, n1, n2, n3;  // next chunk
+  uint32_t qi;    // next word of the current chunk to append
+  uint32_t chunk; // word index of q0
+  uint32_t pos;   // stream bit of the top of acc
+
+  // Words past the data are 0xFF padding (>= kSegPadWords of them, enough for
+  // the longest block a speculative decoder can read past the end plus the
+  // prefetch); the chunk index is clamped so a read can never leave the
+  // segment's buffer.
+  CJD_HD inline void load(uint32_t wi, uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) const {
+    wi = wi < nwords - 4 ? wi : nwords - 4;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4 v = *reinterpret_cast<const uint4*>(w + wi);
+    a = bswap32(v.x);
+    b = bswap32(v.y);
+    c = bswap32(v.z);
+    d = bswap32(v.w);
+#else
+    a = bswap32(w[wi]);
+    b = bswap32(w[wi + 1]);
+    c = bswap32(w[wi + 2]);
+    d = bswap32(w[wi + 3]);
+#endif
+  }
+  CJD_HD inline void refill() {
+    while (nacc <= 32) {
+      const uint32_t word = qi == 0 ? q0 : qi == 1 ? q1 : qi == 2 ? q2 : q3;
+      acc |= (uint64_t)word << (32 - nacc);
+      nacc += 32;
+      if (++qi == 4) {
+        q0 = n0;
+        q1 = n1;
+        q2 = n2;
+        q3 = n3;
+        chunk += 4;
+        qi = 0;
+        load(chunk + 4, n0, n1, n2, n3);
+      }
+    }
+  }
+  CJD_HD inline void seek(const uint32_t* words, uint32_t n, uint32_t p) {
+    w = words;
+    nwords = n;
+    const uint32_t wi = p >> 5;
+    chunk = wi & ~3u;
+    load(chunk, q0, q1, q2, q3);
+    load(chunk + 4, n0, n1, n2, n3);
+    qi = wi - chunk;
+    acc = 0;
+    nacc = 0;
+    pos = p;
+    refill();
+    const int o = (int)(p & 31);
+    acc <<= o;
+    nacc -= o;
+    refill();
+  }
+  CJD_HD inline uint32_t peek() const { return (uint32_t)(acc >> 32); }
+  CJD_HD inline void skip(int n) {
+    acc <<= n;
+    nacc -= n;
+    pos += (uint32_t)n;
+    refill();
+  }
+};
 
 // Huffman symbol of the code at the top of `bits`; len = code length, 0 for an
-// invalid code.  Codes longer than the LUT use the canonical (maxcode, value
-// offset) decode of T.81 F.2.2.3.
+// invalid code.  Codes longer than the LUT: the length is found by comparing
+// against all canonical limits at once (T.81 F.2.2.3).
 CJD_HD inline int huff_sym(const HuffTable& T, uint32_t bits, int& len) {
   const uint32_t e = T.lut[bits >> (32 - kLutBits)];
   len = (int)(e & 31);
   if (len) return (int)((e >> 8) & 0xFF);
-  for (int l = kLutBits + 1; l <= 16; ++l) {
-    const int32_t code = (int32_t)(bits >> (32 - l));
-    if (code <= T.maxcode[l]) {
-      len = l;
-      return T.vals[(T.valoff[l] + code) & 0xFF];
-    }
+  const uint32_t v = bits >> 16;
+  int l = kLutBits + 1;
+#pragma unroll
+  for (int k = kLutBits + 1; k <= 16; ++k) l += v >= T.lim[k] ? 1 : 0;
+  if (l > 16) {
+    len = 0;
+    return 0;
   }
-  len = 0;
-  return 0;
+  len = l;
+  return T.vals[(T.valoff[l] + (int)(v >> (16 - l))) & 0xFF];
 }
 
 // EXTEND (T.81 F.2.2.1) of the `size` bits following a `len`-bit code.
@@ -84,47 +152,48 @@ CJD_HD inline int extend_value(uint32_t bits, int len, int size) {
   return v;
 }
 
-// Decode one block at bit p (src/decodeScanGPU.cu:78-100).  err = 1 where the
-// reference raises SYNTAX_ERROR.  kWrite: AC coefficients into blk (natural
-// order); the caller zeroes blk and stores the DC.
+// Decode one block at the reader's position (src/decodeScanGPU.cu:78-100).
+// err = 1 where the reference raises SYNTAX_ERROR; an invalid DC code
+// consumes one bit so that speculative decoders always advance.  kWrite: AC
+// coefficients into blk (natural order); the caller zeroes blk and stores DC.
 template <bool kWrite>
-CJD_HD inline uint32_t decode_block(const uint32_t* w, uint32_t nwords, uint32_t p, const HuffTable& dct,
-                                    const HuffTable& act, int& dcdiff, int& err, int16_t* blk) {
-  uint32_t bits = peek32(w, nwords, p);
+CJD_HD inline void decode_block(BitReader& br, const HuffTable& dct, const HuffTable& act, int& dcdiff, int& err,
+                                int16_t* blk) {
+  uint32_t bits = br.peek();
   int len;
   int sym = huff_sym(dct, bits, len);
   if (!len) {
     err = 1;
     dcdiff = 0;
-    return p + 1;
+    br.skip(1);
+    return;
   }
   int size = sym & 15;
   dcdiff = extend_value(bits, len, size);
-  p += len + size;
+  br.skip(len + size);
   int coef = 0;
   while (coef < 63) {
-    bits = peek32(w, nwords, p);
+    bits = br.peek();
     sym = huff_sym(act, bits, len);
     if (!len) {
       err = 1;
-      return p;
+      return;
     }
     size = sym & 15;
     const int v = extend_value(bits, len, size);
-    p += len + size;
+    br.skip(len + size);
     if (!sym) break;  // EOB
     if (!size && sym != 0xF0) {
       err = 1;
-      return p;
+      return;
     }
     coef += (sym >> 4) + 1;
     if (coef > 63) {
       err = 1;
-      return p;
+      return;
     }
     if (kWrite) blk[kDeZigZag[coef]] = (int16_t)v;
   }
-  return p;
 }
 
 CJD_HD inline void status_or(uint32_t* p, uint32_t v) {
@@ -138,6 +207,7 @@ CJD_HD inline void status_or(uint32_t* p, uint32_t v) {
 // ------------------------------------------------------------ LDS state --
 struct EntShared {
   HuffTable tab[4];
+  uint8_t dz[64];                     // zigzag index -> natural index
   uint32_t exit_p[kWG];
   uint32_t exit_s[kWG];
   int32_t tot[kWG][4];   // blocks, dc0, dc1, dc2 of each hypothesis
@@ -178,6 +248,7 @@ struct EntCtx {
   int lead;     // 1 when local tile 0 is the lead-in (last tile of the previous item)
   int J;        // local tiles including the lead-in
   uint32_t tfirst;
+  uint64_t slotinfo;  // 4 bits per MCU slot: DC table (bit 0), AC table - 2 (bit 1), component (bits 2-3)
 
   CJD_HD void init(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs, const WorkItem& item) {
     it = item;
@@ -190,10 +261,19 @@ struct EntCtx {
     lead = item.t0 > 0 ? 1 : 0;
     J = (int)item.ntiles + lead;
     tfirst = item.t0 - (uint32_t)lead;
+    slotinfo = 0;
+    for (int k = 0; k < nh; ++k)
+      slotinfo |= (uint64_t)((im->slot_dc[k] & 1) | ((im->slot_ac[k] & 1) << 1) | (im->slot_comp[k] << 2))
+                  << (4 * k);
   }
+  CJD_HD int comp(int slot) const { return (int)((slotinfo >> (4 * slot + 2)) & 3); }
   CJD_HD uint32_t tile_end(uint32_t t) const { return (t + 1 >= sg.ntiles) ? sg.nbits : (t + 1) * S; }
-  CJD_HD const HuffTable& dct(const EntShared& sh, int slot) const { return sh.tab[im->slot_dc[slot]]; }
-  CJD_HD const HuffTable& act(const EntShared& sh, int slot) const { return sh.tab[im->slot_ac[slot]]; }
+  CJD_HD const HuffTable& dct(const EntShared& sh, int slot) const {
+    return sh.tab[(slotinfo >> (4 * slot)) & 1];
+  }
+  CJD_HD const HuffTable& act(const EntShared& sh, int slot) const {
+    return sh.tab[2 + ((slotinfo >> (4 * slot + 1)) & 1)];
+  }
 };
 
 CJD_HD inline void add_dc(int32_t* c3, int comp, int d) {
@@ -202,28 +282,67 @@ CJD_HD inline void add_dc(int32_t* c3, int comp, int d) {
   c3[2] += comp == 2 ? d : 0;
 }
 
+// One Huffman symbol of the block-structured stream (the body of
+// decode_block, one symbol per call, so that the lanes of a wave stay in step
+// symbol by symbol instead of waiting for each other's longest block).
+// State: slot of the current block; coef = -1 when the next symbol is a DC
+// (block start), else the index of the last coefficient.  Out: DC difference
+// (0 for AC symbols), natural position and value of a coefficient to store
+// (wpos = -1 for none).  Returns true when the block ended with this symbol.
+CJD_HD inline bool sym_step(BitReader& br, const EntShared& sh, uint64_t slotinfo, int nh, int& slot, int& coef,
+                            int& dcdiff, int& wpos, int& wval, int& err) {
+  const uint32_t bits = br.peek();
+  const bool isdc = coef < 0;
+  const int sinfo = (int)((slotinfo >> (4 * slot)) & 15);
+  const int tidx = isdc ? (sinfo & 1) : 2 + ((sinfo >> 1) & 1);
+  int len;
+  const int sym = huff_sym(sh.tab[tidx], bits, len);
+  const int size = sym & 15;
+  const int v = extend_value(bits, len, size);
+  // an invalid DC code consumes one bit, an invalid AC code none
+  br.skip(len ? len + size : (isdc ? 1 : 0));
+  const int nc = coef + (sym >> 4) + 1;
+  const bool bad = !len || (!isdc && sym && !size && sym != 0xF0) || (!isdc && sym && nc > 63);
+  const bool end = isdc ? !len : (bad || !sym || nc >= 63);
+  err |= bad ? 1 : 0;
+  dcdiff = isdc ? v : 0;
+  wpos = (!isdc && !bad && sym) ? (int)sh.dz[nc] : -1;
+  wval = v;
+  coef = isdc ? 0 : nc;
+  if (end) {
+    coef = -1;
+    if (++slot == nh) slot = 0;
+  }
+  return end;
+}
+
 // Phase 1: speculative decode of (local tile j, hypothesis h) = lane tid.
 CJD_HD inline void ent_phase1(EntShared& sh, int tid, const EntCtx& c) {
   const int j = tid / c.nh, h = tid - j * c.nh;
   if (j >= c.J) return;
   const uint32_t t = c.tfirst + (uint32_t)j;
-  uint32_t p = t * c.S;
-  int slot = (t == 0) ? 0 : h;
+  BitReader br;
+  br.seek(c.w, c.nwords, t * c.S);
+  int slot = (t == 0) ? 0 : h, coef = -1, err = 0;
   const uint32_t end = c.tile_end(t);
   int32_t nb = 0, dc[3] = {0, 0, 0};
-  while (p < end) {
-    if (nb < kListLen) {
-      sh.u.l.p[tid][nb] = p;
-      sh.u.l.s_dc0[tid][nb] = (uint32_t)slot | ((uint32_t)(uint16_t)(int16_t)dc[0] << 16);
-      sh.u.l.dc12[tid][nb] = (uint32_t)(uint16_t)(int16_t)dc[1] | ((uint32_t)(uint16_t)(int16_t)dc[2] << 16);
+  for (;;) {
+    if (coef < 0) {
+      const uint32_t p = br.pos;
+      if (p >= end) break;
+      if (nb < kListLen) {
+        sh.u.l.p[tid][nb] = p;
+        sh.u.l.s_dc0[tid][nb] = (uint32_t)slot | ((uint32_t)(uint16_t)(int16_t)dc[0] << 16);
+        sh.u.l.dc12[tid][nb] = (uint32_t)(uint16_t)(int16_t)dc[1] | ((uint32_t)(uint16_t)(int16_t)dc[2] << 16);
+      }
+      ++nb;
     }
-    int d, e = 0;
-    p = decode_block<false>(c.w, c.nwords, p, c.dct(sh, slot), c.act(sh, slot), d, e, nullptr);
-    add_dc(dc, c.im->slot_comp[slot], d);
-    ++nb;
-    if (++slot == c.nh) slot = 0;
+    const int s0 = slot;
+    int d, wp, wv;
+    sym_step(br, sh, c.slotinfo, c.nh, slot, coef, d, wp, wv, err);
+    add_dc(dc, c.comp(s0), d);
   }
-  sh.exit_p[tid] = p;
+  sh.exit_p[tid] = br.pos;
   sh.exit_s[tid] = (uint32_t)slot;
   sh.tot[tid][0] = nb;
   sh.tot[tid][1] = dc[0];
@@ -242,28 +361,38 @@ CJD_HD inline int ent_resync(const EntShared& sh, const EntCtx& c, int j, uint32
   const int nh = c.nh;
   const uint32_t end = c.tile_end(c.tfirst + (uint32_t)j);
   int32_t rb = 0, r[3] = {0, 0, 0};
-  int cur[kMaxSlots];
-  for (int k = 0; k < kMaxSlots; ++k) cur[k] = 0;
+  uint64_t cur = 0;  // 4-bit list cursor per hypothesis (kept in a register)
   int hit = -1, hit_i = 0;
   const int base = j * nh;
+  BitReader br;
+  br.seek(c.w, c.nwords, p);
+  int coef = -1, err = 0;
   for (;;) {
+    if (coef >= 0) {  // inside a block: one more symbol
+      const int s0 = slot;
+      int d, wp, wv;
+      sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
+      add_dc(r, c.comp(s0), d);
+      continue;
+    }
+    p = br.pos;
     for (int k = 0; k < nh && hit < 0; ++k) {
       const int L = base + k;
       const int n = (int)sh.nlist[L];
-      int q = cur[k];
+      int q = (int)((cur >> (4 * k)) & 15);
       while (q < n && sh.u.l.p[L][q] < p) ++q;
-      cur[k] = q;
+      cur = (cur & ~(15ull << (4 * k))) | ((uint64_t)q << (4 * k));
       if (q < n && sh.u.l.p[L][q] == p && (int)(sh.u.l.s_dc0[L][q] & 0xFF) == slot) {
         hit = k;
         hit_i = q;
       }
     }
     if (hit >= 0 || p >= end) break;
-    int d, e = 0;
-    p = decode_block<false>(c.w, c.nwords, p, c.dct(sh, slot), c.act(sh, slot), d, e, nullptr);
-    add_dc(r, c.im->slot_comp[slot], d);
     ++rb;
-    if (++slot == nh) slot = 0;
+    const int s0 = slot;
+    int d, wp, wv;
+    sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
+    add_dc(r, c.comp(s0), d);
   }
   cnt[0] = rb;
   cnt[1] = r[0];
@@ -501,31 +630,42 @@ CJD_HD inline void ent_phase5(EntShared& sh, int tid, const EntCtx& c, int16_t* 
   const int32_t* cx = sh.ex_c[fj];
   const uint32_t end = c.tile_end(t);
   int64_t bidx = cp[0];
-  int dc[3] = {cp[1], cp[2], cp[3]};
+  int dc0 = cp[1], dc1 = cp[2], dc2 = cp[3];
   int16_t* blk = sh.u.coef[tid];
   int err = 0;
   bool stopped = false;
-  while (p < end) {
-    if (bidx >= (int64_t)c.sg.nblocks) {
-      stopped = true;
-      break;
+  BitReader br;
+  br.seek(c.w, c.nwords, p);
+  int coef = -1;
+  for (;;) {
+    if (coef < 0) {
+      if (br.pos >= end) break;
+      if (bidx >= (int64_t)c.sg.nblocks) {
+        stopped = true;
+        break;
+      }
+      __builtin_memset(blk, 0, 128);
     }
-    __builtin_memset(blk, 0, 128);
-    int d;
-    p = decode_block<true>(c.w, c.nwords, p, c.dct(sh, slot), c.act(sh, slot), d, err, blk);
+    const int s0 = slot, comp = c.comp(s0);
+    int d, wp, wv;
+    const bool ended = sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
     if (err) break;
-    const int comp = c.im->slot_comp[slot];
-    dc[comp] += d;
-    blk[0] = (int16_t)dc[comp];
-    __builtin_memcpy(coefs + (c.sg.first_block + bidx) * 64, blk, 128);
-    ++bidx;
-    if (++slot == nh) slot = 0;
+    dc0 += comp == 0 ? d : 0;
+    dc1 += comp == 1 ? d : 0;
+    dc2 += comp == 2 ? d : 0;
+    if (wp >= 0) blk[wp] = (int16_t)wv;
+    if (ended) {
+      blk[0] = (int16_t)(comp == 0 ? dc0 : comp == 1 ? dc1 : dc2);
+      __builtin_memcpy(coefs + (c.sg.first_block + bidx) * 64, blk, 128);
+      ++bidx;
+    }
   }
+  p = br.pos;
   if (err) {
     status_or(st, kStatusSyntax);
   } else if (!stopped) {
-    if (p != sh.ex_p[fj] || (uint32_t)slot != sh.ex_s[fj] || bidx != (int64_t)cx[0] || dc[0] != cx[1] ||
-        dc[1] != cx[2] || dc[2] != cx[3])
+    if (p != sh.ex_p[fj] || (uint32_t)slot != sh.ex_s[fj] || bidx != (int64_t)cx[0] || dc0 != cx[1] ||
+        dc1 != cx[2] || dc2 != cx[3])
       status_or(st, kStatusInconsistent);
     if (t + 1 == c.sg.ntiles && bidx < (int64_t)c.sg.nblocks) status_or(st, kStatusTruncated);
   }
@@ -537,14 +677,14 @@ CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* im
                                       const HuffTable* tables, int16_t* coefs, uint32_t* status) {
   const ImageDesc& im = images[sg.image];
   const HuffTable* tab = tables + im.table_base;
-  const uint32_t* w = words + sg.word_base;
-  uint32_t p = 0;
+  BitReader br;
+  br.seek(words + sg.word_base, sg.nwords, 0);
   int slot = 0, dc[3] = {0, 0, 0};
   int16_t blk[64];
   for (uint32_t b = 0; b < sg.nblocks; ++b) {
     for (int k = 0; k < 64; ++k) blk[k] = 0;
     int d, err = 0;
-    p = decode_block<true>(w, sg.nwords, p, tab[im.slot_dc[slot]], tab[im.slot_ac[slot]], d, err, blk);
+    decode_block<true>(br, tab[im.slot_dc[slot]], tab[im.slot_ac[slot]], d, err, blk);
     if (err) {
       status_or(status + im.error_index, kStatusSyntax);
       return;

@@ -29,7 +29,6 @@ void build_table(const uint8_t bits[16], const uint8_t vals[256], HuffTable& t) 
   int code = 0, k = 0;
   for (int l = 1; l <= 16; ++l) {
     const int n = bits[l - 1];
-    t.maxcode[l] = n ? code + n - 1 : -1;
     t.valoff[l] = k - code;
     if (l <= kLutBits) {
       for (int i = 0; i < n; ++i) {
@@ -39,10 +38,9 @@ void build_table(const uint8_t bits[16], const uint8_t vals[256], HuffTable& t) 
     }
     code += n;
     k += n;
+    t.lim[l] = (uint32_t)code << (16 - l);
     code <<= 1;
   }
-  t.maxcode[0] = -1;
-  t.maxcode[17] = -1;
   memcpy(t.vals, vals, 256);
 }
 
@@ -170,7 +168,11 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
     SegmentDesc sd{};
     sd.word_base = b->words.size();
     sd.nbits = (uint32_t)(data.size() * 8);
-    const size_t nw = (data.size() + 3) / 4 + 2;
+    // 16-byte aligned start (the decoder reads 4-word chunks) and
+    // kSegPadWords words of 0xFF padding past the data
+    b->words.resize((b->words.size() + 3) & ~(size_t)3, 0xFFFFFFFFu);
+    sd.word_base = b->words.size();
+    const size_t nw = (((data.size() + 3) / 4 + kSegPadWords) + 3) & ~(size_t)3;
     sd.nwords = (uint32_t)nw;
     const size_t w0 = b->words.size();
     b->words.resize(w0 + nw, 0xFFFFFFFFu);

@@ -99,6 +99,7 @@ void run_entropy(Emu& E, int order) {
     EntCtx& c = R.ctx[i];
     c.init(p.words.data(), p.images.data(), p.segs.data(), p.items[i]);
     memcpy(sh.tab, &p.tables[c.im->table_base], sizeof(sh.tab));
+    memcpy(sh.dz, kDeZigZag, 64);
     for (int t = 0; t < kWG; ++t) ent_phase1(sh, t, c);
     for (int t = 0; t < kWG; ++t) ent_phase2(sh, t, c);
     for (int t = 0; t < kWG; ++t) ent_phase3_init(sh, t, c);
