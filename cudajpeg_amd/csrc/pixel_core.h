@@ -253,3 +253,83 @@ CJD_HD inline void pix_colour(const PixShared& sh, int tid, const PixCtx& c, uin
 }
 
 }  // namespace cjd
+
+namespace cjd {
+
+// ----------------------------------------------------------- MFMA iDCT --
+// The inverse DCT of a block as one 64x64 contraction,
+//   s[p] = 128 + sum_k c[k] * M_t[k][p],   M_t[k][p] = q_t[k] * Cu[u][x] * Cv[v][y]
+// (k = 8v + u natural coefficient index, p = 8y + x pixel index, q_t the
+// quant table), evaluated on the matrix cores: the quantised coefficients
+// (|c| <= 2047) are exact in FP16, M_t is split into FP16 hi + lo parts and
+// the products accumulate in FP32.  Tiles of 16 blocks x 64 pixels; one pass
+// per distinct quant table of the image, rows of other tables masked to zero.
+//
+// Lane layout of v_mfma_f32_16x16x32_f16 (checked on the device): A[row][k]
+// lane l holds row l%16, k = 8*(l/16) + i; B[k][col] lane l holds col l%16,
+// the same k; C lane l holds col l%16, rows 4*(l/16) + i.
+
+// Round a float to the nearest FP16 value (ties to even), as a float.
+CJD_HD inline float round_to_half(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (float)(_Float16)x;
+#else
+  if (x == 0.f) return 0.f;
+  int e;
+  frexpf(x, &e);                   // x = m * 2^e, 0.5 <= |m| < 1
+  int qe = e - 11;                 // quantum of an 11-bit significand
+  if (qe < -24) qe = -24;          // subnormal quantum
+  return ldexpf(nearbyintf(ldexpf(x, -qe)), qe);
+#endif
+}
+
+// M_t split for lane l of the B operand: hi/lo[kk][n][i] for K-step kk,
+// N-tile n, element i.
+CJD_HD inline void mfma_basis(const uint16_t* q, int l, float hi[2][4][8], float lo[2][4][8]) {
+  const int x = l & 7, ysub = (l & 15) >> 3, vsub = l >> 4;
+  for (int kk = 0; kk < 2; ++kk)
+    for (int n = 0; n < 4; ++n) {
+      const float cv = kCosF[kk * 4 + vsub][2 * n + ysub];
+      for (int i = 0; i < 8; ++i) {
+        const float m = (float)q[kk * 32 + vsub * 8 + i] * kCosF[i][x] * cv;
+        const float h = round_to_half(m);
+        hi[kk][n][i] = h;
+        lo[kk][n][i] = round_to_half(m - h);
+      }
+    }
+}
+
+// Write one contraction result (pixel p of block b) into the sample tile.
+CJD_HD inline void mfma_store(PixShared& sh, const PixCtx& c, int b, int p, float v) {
+  const int m = b / c.nslots, slot = b - m * c.nslots;
+  const int comp = c.im->slot_comp[slot];
+  const int k = slot - c.im->comp_first_slot[comp];
+  const int sx = k % c.im->comp_sx[comp], sy = k / c.im->comp_sx[comp];
+  const int tw = c.tw[comp];
+  const int y = p >> 3, x = p & 7;
+  sh.samp[c.toff[comp] + (sy * 8 + y) * tw + (m * c.im->comp_sx[comp] + sx) * 8 + x] =
+      clip8((int)floorf(v + 128.5f));
+}
+
+// Host restatement of the same contraction for the CPU emulator: identical
+// hi/lo split and FP32 products, accumulated in a fixed order.
+inline void pix_idct_hilo_host(PixShared& sh, const PixCtx& c) {
+  for (int b = 0; b < c.nblk; ++b) {
+    const int comp = c.im->slot_comp[b % c.nslots];
+    const uint16_t* q = sh.q[comp];
+    for (int p = 0; p < 64; ++p) {
+      const int y = p >> 3, x = p & 7;
+      float s = 0.f;
+      for (int k = 0; k < 64; ++k) {
+        const float m = (float)q[k] * kCosF[k & 7][x] * kCosF[k >> 3][y];
+        const float h = round_to_half(m);
+        const float lo = round_to_half(m - h);
+        s += (float)sh.coef[b][k] * h;
+        s += (float)sh.coef[b][k] * lo;
+      }
+      mfma_store(sh, c, b, p, s);
+    }
+  }
+}
+
+}  // namespace cjd

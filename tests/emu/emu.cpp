@@ -132,6 +132,8 @@ void run_pixel(Emu& E, int mode, size_t first, size_t count) {
     if (mode == kIdctInt) {
       for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctInt>(*sh, t, c);
       for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctInt>(*sh, t, c);
+    } else if (mode == kIdctMfma) {
+      pix_idct_hilo_host(*sh, c);
     } else {
       for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctFloat>(*sh, t, c);
       for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctFloat>(*sh, t, c);

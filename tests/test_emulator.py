@@ -81,3 +81,18 @@ def test_emulated_corrupt_scan_reports_syntax():
     assert oracle.decode_scan(oracle.parse(bytes(data))).error == 1
     r = EmuResult([bytes(data)])
     assert r.errors[0] == 1
+
+
+def test_emulated_mfma_contraction_matches_fp64(encoded):
+    # host restatement of the MFMA iDCT (FP16 hi/lo basis, FP32 sums) is within
+    # 1 of the FP64 iDCT and within 1 of the reference's integer iDCT
+    r = EmuResult([e.data for e in encoded], 2, 4096, True)
+    for i, (s, e) in enumerate(zip(SPECS, encoded)):
+        p = oracle.parse(e.data)
+        f64 = oracle.planes(p, e.coefs, 1)
+        i32 = oracle.planes(p, e.coefs, 0)
+        for c in range(p.num_channels):
+            got = r.get(i, 2 + c, f64[c].shape, np.uint8).astype(int)
+            assert np.abs(got - f64[c]).max() <= 1, s
+            assert (got == f64[c]).mean() > 0.999, s
+            assert np.abs(got - i32[c]).max() <= 1, s

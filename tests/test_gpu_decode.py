@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import oracle
-from cudajpeg_amd import (ENTROPY_SERIAL, ENTROPY_TILED, IDCT_FLOAT, IDCT_INT, BatchDecoder, JPGReader, synth)
+from cudajpeg_amd import (ENTROPY_SERIAL, ENTROPY_TILED, IDCT_FLOAT, IDCT_INT, IDCT_MFMA, BatchDecoder, JPGReader,
+                          synth)
 
 pytestmark = pytest.mark.gpu
 
@@ -80,14 +81,15 @@ def test_large_configs(gpu, tile_bits):
     assert b.fallbacks() == 0
 
 
-def test_float_idct_within_one(gpu, small_encoded):
+@pytest.mark.parametrize("idct", [IDCT_FLOAT, IDCT_MFMA])
+def test_float_idct_within_one(gpu, small_encoded, idct):
     # The reference's iDCT is integer (CPUonly/src/pixelTransformCPU.c:14-107);
     # an exact iDCT rounded to nearest differs from it by at most 1 per planar
     # sample.  Tolerance: planar |d| <= 1 against the integer iDCT and against
     # the FP64 iDCT; RGB is reported, not asserted beyond the colour gains
     # (359/256, 454/256 amplify a 1-LSB chroma step to <= 3 LSB).
     for spec, enc in zip(SMALL, small_encoded):
-        b = BatchDecoder(IDCT_FLOAT, planes=True)
+        b = BatchDecoder(idct, planes=True)
         assert b.stage([enc.data])[0] == 0
         b.decode()
         assert b.finish()[0] == 0
@@ -128,3 +130,27 @@ def test_corrupt_scan_reports_syntax(gpu):
     assert b.stage([bytes(data)])[0] == 0
     b.decode()
     assert b.finish()[0] == e_ref
+
+
+def test_mfma_idct_error_vs_fp64(gpu):
+    # FP16 hi/lo basis with FP32 accumulation: planar samples within 1 of the
+    # FP64 iDCT (T.81 A.3.3) and >= 99.9% equal to it; within 1 of the
+    # reference's integer iDCT
+    specs = [synth.config_plain_512(), synth.config_4k(), synth.ImageSpec(640, 480, "444", 95, 8.0, 5),
+             synth.ImageSpec(512, 384, "gray", 60, 8.0, 6)]
+    for spec in specs:
+        enc = synth.encode(spec, truth=True)
+        b = BatchDecoder(IDCT_MFMA, planes=True)
+        assert b.stage([enc.data])[0] == 0
+        b.decode()
+        assert b.finish()[0] == 0
+        p = oracle.parse(enc.data)
+        f64 = oracle.planes(p, enc.coefs, 1)
+        i32 = oracle.planes(p, enc.coefs, 0)
+        for c in range(p.num_channels):
+            got = b.plane(0, c).astype(int)
+            d = np.abs(got - f64[c])
+            print(spec.width, spec.height, spec.sampling, c, "max", d.max(), "exact", (d == 0).mean(), flush=True)
+            assert d.max() <= 1 and (d == 0).mean() > 0.999
+            assert np.abs(got - i32[c]).max() <= 1
+        b.close()
