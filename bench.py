@@ -11,7 +11,9 @@ Reported:
   plain_mpix_s       plain form: one image per launch, FP32 VALU iDCT
   ratio              batched / plain
   p50/p99_ms         per-image decode latency of the plain form
-  roofline           dominant kernel, achieved algorithmic GB/s vs 8 TB/s
+  roofline           dominant kernel, achieved algorithmic GB/s vs 8 TB/s;
+                     traffic = its HBM bytes per launch from the committed
+                     FETCH_SIZE/WRITE_SIZE summary of this workload
   cpu_baseline       the reference CPUonly decoder (built from its sources
                      into oracle/_ref) on a bounded sample, one host core
 """
@@ -73,6 +75,18 @@ def cpu_baseline(encs, budget_s: float):
                       f"file in page cache, {done_t:.1f} s total, {done_px / 1e6:.0f} Mpix"}
 
 
+def measured_traffic(kernel: str, args) -> int | None:
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    summary of this same workload (profiles/hbm_traffic_mixed256.md, counters
+    scaled by a copy kernel of known size); None for any other workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed256.md")
+    if args.images != 256 or args.seed != 4000 or args.tile_bits or not os.path.exists(path):
+        return None
+    text = open(path).read()
+    blob = text[text.rfind("```json") + 7:text.rfind("```")]
+    return json.loads(blob)["traffic_bytes"].get(kernel)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,10 +95,13 @@ def main():
     ap.add_argument("--images", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4000)
     ap.add_argument("--tile-bits", type=int, default=0)
-    ap.add_argument("--batched-idct", default="int", choices=["int", "float", "mfma"])
+    ap.add_argument("--batched-idct", default="mfma", choices=["int", "float", "mfma"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain read+write elementwise kernel over MIB MiB (reads MIB, writes MIB) "
+                         "three times: the reference point for scaling FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every image decodes on one device, and the multi-GPU "
@@ -97,6 +114,15 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration << 18  # int32 elements
+        x = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.bitwise_not(x, out=y)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written per launch")
+        return
     dev = torch.cuda.current_device()
     stream = torch.cuda.Stream()
 
@@ -147,9 +173,9 @@ def main():
     ent_bytes = tot["scan_bytes"] + tot["blocks"] * 128
     pix_bytes = tot["blocks"] * 128 + tot["out_bytes"]
     if ms_ent >= ms_pix:
-        dom, dom_bytes, dom_ms = "k_entropy", ent_bytes, ms_ent
+        dom, dom_bytes, dom_ms = "cjd::k_entropy", ent_bytes, ms_ent
     else:
-        dom, dom_bytes, dom_ms = "k_pixel", pix_bytes, ms_pix
+        dom, dom_bytes, dom_ms = f"cjd::k_pixel<{idct}>", pix_bytes, ms_pix
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     b.close()
 
@@ -201,7 +227,7 @@ def main():
         "vs_baseline": None,
         "dtype": {"int": "int32", "float": "fp32", "mfma": "fp16x2+fp32acc"}[args.batched_idct],
         "data": "synthetic (seeded sinusoid+noise images, project baseline encoder)",
-        "config": {"workload": f"mixed batch of {args.images} baseline JPEGs (config 4)",
+        "config": {"workload": f"mixed batch of {args.images} baseline JPEGs, 256x256..4096x3072",
                    "images": args.images, "mpix": round(total_px / 1e6, 1),
                    "scan_mb": round(tot["scan_bytes"] / 1e6, 1), "blocks": tot["blocks"],
                    "batched_idct": args.batched_idct, "plain_idct": "float", "parallelism": "1 GPU"},
@@ -211,7 +237,7 @@ def main():
         "p99_ms": None if not lat else round(float(np.percentile(lat, 99)), 3),
         "kernel_ms": {"k_entropy": round(ms_ent, 3), "k_pixel": round(ms_pix, 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
                      "algorithmic_bytes": int(dom_bytes)},
         "cpu_baseline": cpu,
     }
