@@ -39,6 +39,9 @@ struct ImageDesc {
   uint8_t comp_dq[3];             // quant table id per component
   uint8_t comp_sx[3], comp_sy[3]; // sampling factors
   uint8_t comp_xshift[3], comp_yshift[3]; // nearest-upsample shifts (CPUonly/src/pixelTransformCPU.c:110-127)
+  uint8_t ntab;                   // distinct quant tables among the components (1..3)
+  uint8_t comp_tix[3];            // index of each component's table among them
+  uint8_t tab_dq[3];              // quant table id of each distinct table
   int32_t comp_first_slot[3];
   int32_t comp_stride[3];         // planar stride (src/format.cu:295)
   int32_t comp_plane_h[3];
@@ -109,6 +112,6 @@ struct PixItem {
   uint32_t count;
   uint32_t pad_;
 };
-constexpr int kPixBlocks = 64;   // max 8x8 blocks per pixel workgroup
+constexpr int kPixBlocks = 128;  // max 8x8 blocks per pixel workgroup
 
 }  // namespace cjd

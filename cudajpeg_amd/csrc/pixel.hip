@@ -3,6 +3,8 @@
 // upsampleAndYUVtoRGB_kernel (src/pixelTransformGPU.cu:25-142, :475-511).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "launch.h"
 #include "pixel_core.h"
 
@@ -11,96 +13,127 @@ namespace cjd {
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-// Batched-form iDCT on the matrix cores: wave w takes 16-block tile w of the
-// workgroup and runs one pass per distinct quant table (see pixel_core.h).
-__device__ __forceinline__ void pix_idct_mfma(PixShared& sh, int tid, const PixCtx& c) {
-  const int wave = tid >> 6, l = tid & 63;
+constexpr int kPixItemsPerWG = 4;  // consecutive items per workgroup
+
+// B operands of one wave: [table slot][K half], hi and lo parts.
+struct MfmaB {
+  half8_t hi[2][2], lo[2][2];
+};
+
+// Batched-form iDCT of one item on the matrix cores (see the MFMA section of
+// pixel_core.h).  Wave w computes N tile w (pixels 16w .. 16w+15) of every
+// 16-block tile, so its B operands depend only on the image's quant tables:
+// they are built in registers and kept while consecutive items belong to the
+// same image (`b_image`, ~0 = none).  Tiles go two at a time; all LDS reads
+// of a pair (A operands, table indices, output descriptors) are issued
+// together from clamped, always-valid rows, and both table slots always run
+// (rows of other tables are masked to zero), so the loop has no per-lane
+// branches around loads or MFMAs.
+__device__ __forceinline__ void pix_idct_mfma(PixSharedMfma& sh, int tid, const PixCtx& c, MfmaB& B,
+                                              uint32_t& b_image) {
+  const int n = tid >> 6, l = tid & 63;
   const int ntiles = (c.nblk + 15) >> 4;
-  if (wave >= ntiles) return;
-  const int b0 = wave * 16;
-  const int brow = b0 + (l & 15);              // block of this lane's A row
-  const int bcol0 = b0 + 4 * (l >> 4);         // first block of this lane's C rows
-  int tab_row = -1;
-  if (brow < c.nblk) tab_row = c.im->comp_dq[c.im->slot_comp[brow % c.nslots]];
-  int tab_out[4];
-  for (int i = 0; i < 4; ++i) {
-    const int b = bcol0 + i;
-    tab_out[i] = b < c.nblk ? c.im->comp_dq[c.im->slot_comp[b % c.nslots]] : -1;
-  }
-  // A operands (coefficients, exact in FP16), loaded once
-  half8_t a[2];
-  for (int kk = 0; kk < 2; ++kk) {
-    int16_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (brow < c.nblk) __builtin_memcpy(v, &sh.coef[brow][kk * 32 + 8 * (l >> 4)], 16);
-    for (int i = 0; i < 8; ++i) a[kk][i] = (_Float16)v[i];
-  }
-  // one pass per distinct quant table among the components
-  for (int comp = 0; comp < c.ncomp; ++comp) {
-    const int t = c.im->comp_dq[comp];
-    bool seen = false;
-    for (int e = 0; e < comp; ++e) seen |= c.im->comp_dq[e] == t;
-    if (seen) continue;
-    float hi[2][4][8], lo[2][4][8];
-    mfma_basis(sh.q[comp], l, hi, lo);
-    half8_t am[2];
-    for (int kk = 0; kk < 2; ++kk)
-      for (int i = 0; i < 8; ++i) am[kk][i] = tab_row == t ? a[kk][i] : (_Float16)0.0f;
-    for (int n = 0; n < 4; ++n) {
-      float4_t acc = {0.f, 0.f, 0.f, 0.f};
-      for (int kk = 0; kk < 2; ++kk) {
-        half8_t bh, bl;
-        for (int i = 0; i < 8; ++i) {
-          bh[i] = (_Float16)hi[kk][n][i];
-          bl[i] = (_Float16)lo[kk][n][i];
+  const int last = c.nblk - 1;
+  const half8_t zero = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int y = 2 * n + ((l & 15) >> 3), x = l & 7, kofs = 8 * (l >> 4);
+  for (int t0 = 0; t0 < c.ntab; t0 += 2) {
+    const int nt = c.ntab - t0 < 2 ? c.ntab - t0 : 2;
+    if (t0 > 0 || b_image != c.it.image) {
+      for (int tl = 0; tl < 2; ++tl)
+        for (int h = 0; h < 2; ++h) {
+          // slot 1 repeats slot 0 when there is one table: its rows are all masked
+          uint16_t hb[8], lb[8];
+          mfma_b_lane(sh.q[t0 + (tl < nt ? tl : 0)], sh.cosf, n, h, l, hb, lb);
+          __builtin_memcpy(&B.hi[tl][h], hb, 16);
+          __builtin_memcpy(&B.lo[tl][h], lb, 16);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(am[kk], bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(am[kk], bl, acc, 0, 0, 0);
+    }
+    for (int tile = 0; tile < ntiles; tile += 2) {
+      half8_t a[2][2];
+      int tix[2];
+      uint32_t ob[2][4];
+      for (int jj = 0; jj < 2; ++jj) {
+        const int brow = (tile + jj) * 16 + (l & 15);
+        const int r = brow <= last ? brow : last;
+        __builtin_memcpy(&a[jj][0], &sh.coef[r][kofs], 16);
+        __builtin_memcpy(&a[jj][1], &sh.coef[r][32 + kofs], 16);
+        const int t = binfo_tix(sh.binfo[r]);
+        tix[jj] = brow <= last ? t : -1;
+        for (int i = 0; i < 4; ++i) {
+          const int b = (tile + jj) * 16 + 4 * (l >> 4) + i;
+          ob[jj][i] = sh.binfo[b <= last ? b : last];
+        }
       }
-      const int p = n * 16 + (l & 15);
-      for (int i = 0; i < 4; ++i)
-        if (tab_out[i] == t) mfma_store(sh, c, bcol0 + i, p, acc[i]);
+      float4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+      for (int tl = 0; tl < 2; ++tl)
+        for (int h = 0; h < 2; ++h)
+          for (int jj = 0; jj < 2; ++jj) {
+            const half8_t am = tix[jj] == t0 + tl ? a[jj][h] : zero;
+            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, B.hi[tl][h], acc[jj], 0, 0, 0);
+            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, B.lo[tl][h], acc[jj], 0, 0, 0);
+          }
+      for (int jj = 0; jj < 2; ++jj)
+        for (int i = 0; i < 4; ++i) {
+          const int b = (tile + jj) * 16 + 4 * (l >> 4) + i;
+          const uint32_t info = ob[jj][i];
+          const int tb = binfo_tix(info);
+          if (b <= last && tb >= t0 && tb < t0 + nt)
+            sh.samp[binfo_base(info) + y * binfo_tw(info) + x] = clip8((int)floorf(acc[jj][i] + 128.5f));
+        }
     }
   }
+  b_image = c.ntab <= 2 ? c.it.image : ~0u;
 }
 
+// One workgroup decodes kPixItemsPerWG consecutive items.  Barriers: staging
+// -> iDCT (coefficients, descriptors, quant tables); iDCT -> colour
+// (samples); colour -> next item's staging.
 template <int kMode>
 __global__ void __launch_bounds__(kPixThreads) k_pixel(const ImageDesc* __restrict__ images,
-                                                       const PixItem* __restrict__ items,
+                                                       const PixItem* __restrict__ items, uint32_t nitems,
                                                        const int16_t* __restrict__ coefs,
                                                        const uint16_t* __restrict__ qtabs,
                                                        uint8_t* __restrict__ out) {
-  __shared__ PixShared sh;
+  using SH = typename std::conditional<kMode == kIdctMfma, PixSharedMfma, PixShared>::type;
+  __shared__ SH sh;
   const int tid = threadIdx.x;
-  PixCtx c;
-  c.init(images, items[blockIdx.x]);
-  pix_stage(sh, tid, c, coefs, qtabs);
-  __syncthreads();
-  if (kMode == kIdctMfma) {
-    pix_idct_mfma(sh, tid, c);
-  } else {
-    pix_rows<kMode>(sh, tid, c);
+  pix_stage_cos(sh, tid);
+  MfmaB B;
+  uint32_t b_image = ~0u;
+  for (int k = 0; k < kPixItemsPerWG; ++k) {
+    const uint32_t idx = blockIdx.x * kPixItemsPerWG + k;
+    if (idx >= nitems) break;
+    PixCtx c;
+    c.init(images, items[idx]);
+    __syncthreads();  // the previous item is done with coef, samp, q and binfo
+    pix_stage(sh, tid, c, coefs, qtabs);
     __syncthreads();
-    pix_cols<kMode>(sh, tid, c);
+    if constexpr (kMode == kIdctMfma) {
+      pix_idct_mfma(sh, tid, c, B, b_image);
+    } else {
+      pix_rows<kMode>(sh, tid, c);
+      __syncthreads();
+      pix_cols<kMode>(sh, tid, c);
+    }
+    __syncthreads();
+    pix_planes(sh, tid, c, out);
+    pix_colour(sh, tid, c, out);
   }
-  __syncthreads();
-  pix_planes(sh, tid, c, out);
-  pix_colour(sh, tid, c, out);
 }
 
 hipError_t launch_pixel(int mode, const ImageDesc* images, const PixItem* items, uint32_t nitems,
                         const int16_t* coefs, const uint16_t* qtabs, uint8_t* out, hipStream_t s) {
   if (!nitems) return hipSuccess;
+  const dim3 grid((nitems + kPixItemsPerWG - 1) / kPixItemsPerWG);
   switch (mode) {
     case kIdctInt:
-      hipLaunchKernelGGL(k_pixel<kIdctInt>, dim3(nitems), dim3(kPixThreads), 0, s, images, items, coefs, qtabs, out);
+      hipLaunchKernelGGL(k_pixel<kIdctInt>, grid, dim3(kPixThreads), 0, s, images, items, nitems, coefs, qtabs, out);
       break;
     case kIdctFloat:
-      hipLaunchKernelGGL(k_pixel<kIdctFloat>, dim3(nitems), dim3(kPixThreads), 0, s, images, items, coefs, qtabs,
-                         out);
+      hipLaunchKernelGGL(k_pixel<kIdctFloat>, grid, dim3(kPixThreads), 0, s, images, items, nitems, coefs, qtabs, out);
       break;
     case kIdctMfma:
-      hipLaunchKernelGGL(k_pixel<kIdctMfma>, dim3(nitems), dim3(kPixThreads), 0, s, images, items, coefs, qtabs,
-                         out);
+      hipLaunchKernelGGL(k_pixel<kIdctMfma>, grid, dim3(kPixThreads), 0, s, images, items, nitems, coefs, qtabs, out);
       break;
     default:
       return hipErrorInvalidValue;

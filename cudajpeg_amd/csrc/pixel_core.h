@@ -1,24 +1,32 @@
 // Fused pixel stage of the MI355X baseline-JPEG decoder: dequant + 8x8 iDCT +
 // nearest chroma upsample + YCbCr->RGB (or gray copy), one write per output
-// pixel.  The phases are __host__ __device__ functions over the workgroup's
+// byte.  The phases are __host__ __device__ functions over the workgroup's
 // LDS state, run by k_pixel (pixel.hip) and lane by lane by the CPU emulator.
 //
 // Replaces src/pixelTransformGPU.cu:25-142 (iDCT_GPU), :475-511
 // (upsampleAndYUVtoRGB_kernel) and :621-677 (iDCT_resample_colourTransform),
 // with the edge semantics of CPUonly/src/pixelTransformCPU.c:110-172.
 //
-// One workgroup takes a run of consecutive MCUs of one MCU row.  Their
-// quantised coefficients (contiguous in scan order) are staged in LDS, each
-// block is inverse-transformed by 8 lanes (rows, then columns) into per-
-// component sample tiles, and the tiles are colour-converted straight to the
-// interleaved output.  iDCT variants:
+// One workgroup takes a run of up to kPixBlocks blocks: consecutive MCUs of
+// one MCU row.  Their quantised coefficients (contiguous in scan order) are
+// staged in LDS with a per-block descriptor (where the block's 8x8 samples go
+// in the component tiles, and which quant table it uses), the blocks are
+// inverse-transformed into the component sample tiles, and the tiles are
+// colour-converted straight to the interleaved output with aligned 4-byte
+// stores.  iDCT variants:
 //   kIdctInt   : the reference's integer Chen-Wang iDCT, bit-exact
-//                (CPUonly/src/pixelTransformCPU.c:14-107)
+//                (CPUonly/src/pixelTransformCPU.c:14-107), 8 lanes per block
 //   kIdctFloat : separable FP32 iDCT of the dequantised coefficients
-//                (T.81 A.3.3), rounded to nearest
+//                (T.81 A.3.3), rounded to nearest, 8 lanes per block
+//   kIdctMfma  : the same transform as one 64x64 contraction per block on the
+//                matrix cores (see the MFMA section below)
 #pragma once
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
+#if !defined(__HIPCC__)
+#include <vector>
+#endif
 
 #include "cjd_internal.h"
 #include "entropy_core.h"
@@ -26,6 +34,7 @@
 namespace cjd {
 
 constexpr int kPixThreads = 256;
+constexpr int kPixWaves = kPixThreads / 64;
 
 // CPUonly/include/pixelTransformCPU.h:5-10
 #define W1 2841
@@ -101,174 +110,7 @@ constexpr float kCosF[8][8] = {
     {0.19134172f, -0.46193977f, 0.46193977f, -0.19134172f, -0.19134172f, 0.46193977f, -0.46193977f, 0.19134172f},
     {0.09754516f, -0.27778512f, 0.41573481f, -0.49039264f, 0.49039264f, -0.41573481f, 0.27778512f, -0.09754516f}};
 
-
-struct PixShared {
-  int16_t coef[kPixBlocks][64];  // quantised coefficients, natural order
-  union {
-    int32_t ti[kPixBlocks][64];  // row-pass output (integer path)
-    float tf[kPixBlocks][64];    // row-pass output (float path)
-  } t;
-  uint8_t samp[kPixBlocks * 64];  // component sample tiles
-  uint16_t q[3][64];              // natural-order quant table per component
-};
-
-struct PixCtx {
-  const ImageDesc* im;
-  PixItem it;
-  int nslots, nblk, ncomp;
-  int toff[3], tw[3];  // component c's tile: offset in samp, width (count*sx*8); height sy*8
-  int mcu_x0, mcu_y;
-  CJD_HD void init(const ImageDesc* images, const PixItem& item) {
-    it = item;
-    im = images + item.image;
-    nslots = im->nslots;
-    nblk = (int)it.count * nslots;
-    ncomp = im->ncomp;
-    int off = 0;
-    for (int c = 0; c < 3; ++c) {
-      toff[c] = off;
-      tw[c] = c < ncomp ? (int)it.count * im->comp_sx[c] * 8 : 0;
-      if (c < ncomp) off += tw[c] * im->comp_sy[c] * 8;
-    }
-    mcu_y = (int)(it.mcu_first / (uint32_t)im->mcus_x);
-    mcu_x0 = (int)(it.mcu_first % (uint32_t)im->mcus_x);
-  }
-};
-
-// Stage coefficients (contiguous: MCUs in scan order) and quant tables.
-CJD_HD inline void pix_stage(PixShared& sh, int tid, const PixCtx& c, const int16_t* coefs, const uint16_t* qtabs) {
-  const int64_t first = c.im->coef_base + (int64_t)c.it.mcu_first * c.nslots;
-  // 16-byte pieces, copied with memcpy (no type punning of the int16 data)
-  const int16_t* src = coefs + first * 64;
-  int16_t* dst = &sh.coef[0][0];
-  for (int i = tid; i < c.nblk * 8; i += kPixThreads) __builtin_memcpy(dst + i * 8, src + i * 8, 16);
-  for (int i = tid; i < c.ncomp * 64; i += kPixThreads) {
-    const int comp = i >> 6;
-    sh.q[comp][i & 63] = qtabs[(c.im->qtab_base + c.im->comp_dq[comp]) * 64 + (i & 63)];
-  }
-}
-
-// Row pass: lane -> (block, row).
-template <int kMode>
-CJD_HD inline void pix_rows(PixShared& sh, int tid, const PixCtx& c) {
-  for (int task = tid; task < c.nblk * 8; task += kPixThreads) {
-    const int b = task >> 3, r = task & 7;
-    const int comp = c.im->slot_comp[b % c.nslots];
-    const int16_t* cb = &sh.coef[b][r * 8];
-    const uint16_t* qq = &sh.q[comp][r * 8];
-    if (kMode == kIdctInt) {
-      int D[8], O[8];
-      for (int k = 0; k < 8; ++k) D[k] = (int)cb[k] * (int)qq[k];
-      idct_row_int(D, O);
-      for (int k = 0; k < 8; ++k) sh.t.ti[b][r * 8 + k] = O[k];
-    } else {
-      float D[8];
-      for (int k = 0; k < 8; ++k) D[k] = (float)((int)cb[k] * (int)qq[k]);
-      for (int x = 0; x < 8; ++x) {
-        float s = 0.f;
-        for (int u = 0; u < 8; ++u) s = fmaf(kCosF[u][x], D[u], s);
-        sh.t.tf[b][r * 8 + x] = s;
-      }
-    }
-  }
-}
-
-// Column pass: lane -> (block, column); samples into the component tile.
-template <int kMode>
-CJD_HD inline void pix_cols(PixShared& sh, int tid, const PixCtx& c) {
-  for (int task = tid; task < c.nblk * 8; task += kPixThreads) {
-    const int b = task >> 3, x = task & 7;
-    const int m = b / c.nslots, slot = b - m * c.nslots;
-    const int comp = c.im->slot_comp[slot];
-    const int k = slot - c.im->comp_first_slot[comp];
-    const int sx = k % c.im->comp_sx[comp], sy = k / c.im->comp_sx[comp];
-    const int tw = c.tw[comp];
-    uint8_t* dst = &sh.samp[c.toff[comp] + (sy * 8) * tw + (m * c.im->comp_sx[comp] + sx) * 8 + x];
-    if (kMode == kIdctInt) {
-      int D[8];
-      uint8_t o[8];
-      for (int v = 0; v < 8; ++v) D[v] = sh.t.ti[b][v * 8 + x];
-      idct_col_int(D, o);
-      for (int y = 0; y < 8; ++y) dst[y * tw] = o[y];
-    } else {
-      float D[8];
-      for (int v = 0; v < 8; ++v) D[v] = sh.t.tf[b][v * 8 + x];
-      for (int y = 0; y < 8; ++y) {
-        float s = 128.f;
-        for (int v = 0; v < 8; ++v) s = fmaf(kCosF[v][y], D[v], s);
-        dst[y * tw] = clip8((int)floorf(s + 0.5f));
-      }
-    }
-  }
-}
-
-// Optional planar outputs (reference plane layout, src/format.cu:295-303).
-CJD_HD inline void pix_planes(const PixShared& sh, int tid, const PixCtx& c, uint8_t* out) {
-  for (int comp = 0; comp < c.ncomp; ++comp) {
-    if (c.im->plane_offset[comp] == ~0ull) continue;
-    uint8_t* pl = out + c.im->plane_offset[comp];
-    const int rows = c.im->comp_sy[comp] * 8, w = c.tw[comp];
-    const int px0 = c.mcu_x0 * c.im->comp_sx[comp] * 8, py0 = c.mcu_y * rows;
-    for (int i = tid; i < rows * w; i += kPixThreads) {
-      const int yy = i / w, xx = i - yy * w;
-      pl[(size_t)(py0 + yy) * c.im->comp_stride[comp] + px0 + xx] = sh.samp[c.toff[comp] + yy * w + xx];
-    }
-  }
-}
-
-// Colour conversion / gray copy, one output byte per lane per step
-// (CPUonly/src/pixelTransformCPU.c:130-172).
-CJD_HD inline void pix_colour(const PixShared& sh, int tid, const PixCtx& c, uint8_t* out) {
-  const ImageDesc& im = *c.im;
-  const int x0 = c.mcu_x0 * im.mcu_w, y0 = c.mcu_y * im.mcu_h;
-  int x1 = x0 + (int)c.it.count * im.mcu_w;
-  if (x1 > im.width) x1 = im.width;
-  int y1 = y0 + im.mcu_h;
-  if (y1 > im.height) y1 = im.height;
-  const int rw = x1 - x0, rows = y1 - y0;
-  uint8_t* o = out + im.out_offset;
-  if (c.ncomp == 1) {
-    for (int i = tid; i < rw * rows; i += kPixThreads) {
-      const int yy = i / rw, xx = i - yy * rw;
-      o[(size_t)(y0 + yy) * im.width + x0 + xx] = sh.samp[yy * c.tw[0] + xx];
-    }
-    return;
-  }
-  const int xs0 = im.comp_xshift[0], ys0 = im.comp_yshift[0];
-  const int xs1 = im.comp_xshift[1], ys1 = im.comp_yshift[1];
-  const int xs2 = im.comp_xshift[2], ys2 = im.comp_yshift[2];
-  const int nbytes = rw * 3;
-  for (int i = tid; i < nbytes * rows; i += kPixThreads) {
-    const int yy = i / nbytes, bx = i - yy * nbytes;
-    const int xx = bx / 3, ch = bx - xx * 3;
-    const int Y = (int)sh.samp[c.toff[0] + (yy >> ys0) * c.tw[0] + (xx >> xs0)] << 8;
-    const int cb = (int)sh.samp[c.toff[1] + (yy >> ys1) * c.tw[1] + (xx >> xs1)] - 128;
-    const int cr = (int)sh.samp[c.toff[2] + (yy >> ys2) * c.tw[2] + (xx >> xs2)] - 128;
-    int v;
-    if (ch == 0) v = (Y + 359 * cr + 128) >> 8;
-    else if (ch == 1) v = (Y - 88 * cb - 183 * cr + 128) >> 8;
-    else v = (Y + 454 * cb + 128) >> 8;
-    o[((size_t)(y0 + yy) * im.width + x0) * 3 + bx] = clip8(v);
-  }
-}
-
-}  // namespace cjd
-
-namespace cjd {
-
-// ----------------------------------------------------------- MFMA iDCT --
-// The inverse DCT of a block as one 64x64 contraction,
-//   s[p] = 128 + sum_k c[k] * M_t[k][p],   M_t[k][p] = q_t[k] * Cu[u][x] * Cv[v][y]
-// (k = 8v + u natural coefficient index, p = 8y + x pixel index, q_t the
-// quant table), evaluated on the matrix cores: the quantised coefficients
-// (|c| <= 2047) are exact in FP16, M_t is split into FP16 hi + lo parts and
-// the products accumulate in FP32.  Tiles of 16 blocks x 64 pixels; one pass
-// per distinct quant table of the image, rows of other tables masked to zero.
-//
-// Lane layout of v_mfma_f32_16x16x32_f16 (checked on the device): A[row][k]
-// lane l holds row l%16, k = 8*(l/16) + i; B[k][col] lane l holds col l%16,
-// the same k; C lane l holds col l%16, rows 4*(l/16) + i.
-
+// ------------------------------------------------------- FP16 helpers --
 // Round a float to the nearest FP16 value (ties to even), as a float.
 CJD_HD inline float round_to_half(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -283,53 +125,410 @@ CJD_HD inline float round_to_half(float x) {
 #endif
 }
 
-// M_t split for lane l of the B operand: hi/lo[kk][n][i] for K-step kk,
-// N-tile n, element i.
-CJD_HD inline void mfma_basis(const uint16_t* q, int l, float hi[2][4][8], float lo[2][4][8]) {
-  const int x = l & 7, ysub = (l & 15) >> 3, vsub = l >> 4;
-  for (int kk = 0; kk < 2; ++kk)
-    for (int n = 0; n < 4; ++n) {
-      const float cv = kCosF[kk * 4 + vsub][2 * n + ysub];
-      for (int i = 0; i < 8; ++i) {
-        const float m = (float)q[kk * 32 + vsub * 8 + i] * kCosF[i][x] * cv;
-        const float h = round_to_half(m);
-        hi[kk][n][i] = h;
-        lo[kk][n][i] = round_to_half(m - h);
-      }
-    }
+// FP16 bit pattern of a value that is already exactly representable in FP16.
+CJD_HD inline uint16_t half_bits(float h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(uint16_t, (_Float16)h);
+#else
+  uint32_t u;
+  memcpy(&u, &h, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const float a = fabsf(h);
+  if (a == 0.f) return (uint16_t)sign;
+  int e;
+  frexpf(a, &e);  // a = m * 2^e
+  if (e <= -14) return (uint16_t)(sign | (uint32_t)ldexpf(a, 24));  // subnormal: a / 2^-24
+  const uint32_t mant = (uint32_t)ldexpf(a, 11 - e) - 1024u;         // 10 stored bits
+  return (uint16_t)(sign | ((uint32_t)(e + 14) << 10) | mant);
+#endif
 }
 
-// Write one contraction result (pixel p of block b) into the sample tile.
-CJD_HD inline void mfma_store(PixShared& sh, const PixCtx& c, int b, int p, float v) {
-  const int m = b / c.nslots, slot = b - m * c.nslots;
-  const int comp = c.im->slot_comp[slot];
-  const int k = slot - c.im->comp_first_slot[comp];
-  const int sx = k % c.im->comp_sx[comp], sy = k / c.im->comp_sx[comp];
-  const int tw = c.tw[comp];
-  const int y = p >> 3, x = p & 7;
-  sh.samp[c.toff[comp] + (sy * 8 + y) * tw + (m * c.im->comp_sx[comp] + sx) * 8 + x] =
-      clip8((int)floorf(v + 128.5f));
+// Value of an FP16 bit pattern.
+CJD_HD inline float half_value(uint16_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (float)__builtin_bit_cast(_Float16, b);
+#else
+  const int e = (b >> 10) & 31, m = b & 1023;
+  const float a = e == 0 ? ldexpf((float)m, -24) : ldexpf((float)(m + 1024), e - 25);
+  return (b & 0x8000u) ? -a : a;
+#endif
 }
 
-// Host restatement of the same contraction for the CPU emulator: identical
-// hi/lo split and FP32 products, accumulated in a fixed order.
-inline void pix_idct_hilo_host(PixShared& sh, const PixCtx& c) {
-  for (int b = 0; b < c.nblk; ++b) {
-    const int comp = c.im->slot_comp[b % c.nslots];
-    const uint16_t* q = sh.q[comp];
-    for (int p = 0; p < 64; ++p) {
-      const int y = p >> 3, x = p & 7;
-      float s = 0.f;
-      for (int k = 0; k < 64; ++k) {
-        const float m = (float)q[k] * kCosF[k & 7][x] * kCosF[k >> 3][y];
-        const float h = round_to_half(m);
-        const float lo = round_to_half(m - h);
-        s += (float)sh.coef[b][k] * h;
-        s += (float)sh.coef[b][k] * lo;
+// Per-block descriptor: offset of the block's top-left sample in `samp`
+// (bits 0-13), width of its component tile (bits 14-25), index of its quant
+// table among the image's distinct tables (bits 26-27).
+CJD_HD inline uint32_t binfo_pack(int base, int tw, int tix) {
+  return (uint32_t)base | ((uint32_t)tw << 14) | ((uint32_t)tix << 26);
+}
+CJD_HD inline int binfo_base(uint32_t v) { return (int)(v & 0x3FFFu); }
+CJD_HD inline int binfo_tw(uint32_t v) { return (int)((v >> 14) & 0xFFFu); }
+CJD_HD inline int binfo_tix(uint32_t v) { return (int)(v >> 26); }
+
+// LDS of the integer and FP32 forms.  The sample tiles overlay the
+// coefficients: the row pass has read every coefficient (behind a barrier)
+// before the column pass writes the first sample.
+struct PixShared {
+  union {
+    alignas(16) int16_t coef[kPixBlocks][64];  // quantised coefficients, natural order
+    uint8_t samp[kPixBlocks * 64];  // component sample tiles
+  };
+  union {
+    int32_t ti[kPixBlocks][64];  // row-pass output (integer path)
+    float tf[kPixBlocks][64];    // row-pass output (float path)
+  } t;
+  uint16_t q[3][64];              // natural-order quant table per distinct table index
+  uint32_t binfo[kPixBlocks];     // per-block descriptor, see binfo_pack
+  float cosf[64];                 // kCosF, row-major
+};
+
+// LDS of the MFMA form: the coefficients as FP16 bit patterns (the A
+// operands), and the sample tiles beside them (waves read A operands and
+// write samples concurrently).
+struct PixSharedMfma {
+  alignas(16) uint16_t coef[kPixBlocks][64];
+  uint8_t samp[kPixBlocks * 64];
+  uint16_t q[3][64];
+  uint32_t binfo[kPixBlocks];
+  float cosf[64];
+};
+
+// 16-byte piece i (8 coefficients) into LDS: as int16, or as FP16 (exact for
+// |c| <= 2048, which holds for the coefficients of any 8-bit-sample image).
+typedef uint32_t u32x4 __attribute__((vector_size(16)));
+CJD_HD inline void stage_piece(PixShared& sh, int i, const u32x4& v) { __builtin_memcpy(&sh.coef[0][0] + i * 8, &v, 16); }
+CJD_HD inline void stage_piece(PixSharedMfma& sh, int i, const u32x4& v) {
+  int16_t c[8];
+  uint16_t h[8];
+  __builtin_memcpy(c, &v, 16);
+  for (int k = 0; k < 8; ++k) h[k] = half_bits(round_to_half((float)c[k]));
+  __builtin_memcpy(&sh.coef[0][0] + i * 8, h, 16);
+}
+
+struct PixCtx {
+  const ImageDesc* im;
+  PixItem it;
+  int nslots, nblk, ncomp, ntab;
+  // component c's tile: offset in samp, width (count*sx*8); height sy*8.
+  // Scalars for the fixed-component uses; tw(c) / toff(c) for a variable
+  // component recompute from the image descriptor, which keeps the context
+  // out of private memory.
+  int toff0, toff1, toff2, tw0, tw1, tw2;
+  int mcu_x0, mcu_y;
+  CJD_HD void init(const ImageDesc* images, const PixItem& item) {
+    it = item;
+    im = images + item.image;
+    nslots = im->nslots;
+    nblk = (int)it.count * nslots;
+    ncomp = im->ncomp;
+    ntab = im->ntab;
+    tw0 = (int)it.count * im->comp_sx[0] * 8;
+    tw1 = ncomp > 1 ? (int)it.count * im->comp_sx[1] * 8 : 0;
+    tw2 = ncomp > 2 ? (int)it.count * im->comp_sx[2] * 8 : 0;
+    toff0 = 0;
+    toff1 = tw0 * im->comp_sy[0] * 8;
+    toff2 = toff1 + tw1 * (ncomp > 1 ? im->comp_sy[1] * 8 : 0);
+    mcu_y = (int)(it.mcu_first / (uint32_t)im->mcus_x);
+    mcu_x0 = (int)(it.mcu_first % (uint32_t)im->mcus_x);
+  }
+  CJD_HD int tw(int comp) const { return (int)it.count * im->comp_sx[comp] * 8; }
+  CJD_HD int toff(int comp) const {
+    int off = 0;
+    for (int e = 0; e < comp; ++e) off += tw(e) * im->comp_sy[e] * 8;
+    return off;
+  }
+};
+
+// The cosine table, once per workgroup.
+template <class SH>
+CJD_HD inline void pix_stage_cos(SH& sh, int tid) {
+  if (tid < 64) sh.cosf[tid] = kCosF[tid >> 3][tid & 7];
+}
+
+// Stage one item: coefficients (contiguous: MCUs in scan order; all of a
+// lane's 16-byte loads are issued before its LDS writes), quant tables and
+// block descriptors.
+template <class SH>
+CJD_HD inline void pix_stage(SH& sh, int tid, const PixCtx& c, const int16_t* coefs, const uint16_t* qtabs) {
+  // four named vectors, not an array: a private array of this size would be
+  // moved into LDS by the compiler
+  static_assert(kPixBlocks * 8 == 4 * kPixThreads, "four 16-byte pieces per lane");
+  const int64_t first = c.im->coef_base + (int64_t)c.it.mcu_first * c.nslots;
+  const int16_t* src = coefs + first * 64;
+  const int lim = c.nblk * 8;
+  const int i0 = tid, i1 = tid + kPixThreads, i2 = tid + 2 * kPixThreads, i3 = tid + 3 * kPixThreads;
+  u32x4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0}, v2 = {0, 0, 0, 0}, v3 = {0, 0, 0, 0};
+  if (i0 < lim) __builtin_memcpy(&v0, src + i0 * 8, 16);
+  if (i1 < lim) __builtin_memcpy(&v1, src + i1 * 8, 16);
+  if (i2 < lim) __builtin_memcpy(&v2, src + i2 * 8, 16);
+  if (i3 < lim) __builtin_memcpy(&v3, src + i3 * 8, 16);
+  if (i0 < lim) stage_piece(sh, i0, v0);
+  if (i1 < lim) stage_piece(sh, i1, v1);
+  if (i2 < lim) stage_piece(sh, i2, v2);
+  if (i3 < lim) stage_piece(sh, i3, v3);
+  for (int i = tid; i < c.ntab * 64; i += kPixThreads) {
+    const int t = i >> 6;
+    sh.q[t][i & 63] = qtabs[(c.im->qtab_base + c.im->tab_dq[t]) * 64 + (i & 63)];
+  }
+  for (int b = tid; b < c.nblk; b += kPixThreads) {
+    const int m = b / c.nslots, slot = b - m * c.nslots;
+    const int comp = c.im->slot_comp[slot];
+    const int sxf = c.im->comp_sx[comp];
+    const int k = slot - c.im->comp_first_slot[comp];
+    const int sx = k % sxf, sy = k / sxf;
+    const int tw = c.tw(comp);
+    sh.binfo[b] = binfo_pack(c.toff(comp) + sy * 8 * tw + (m * sxf + sx) * 8, tw, c.im->comp_tix[comp]);
+  }
+}
+
+// Row pass: lane -> (block, row).
+template <int kMode>
+CJD_HD inline void pix_rows(PixShared& sh, int tid, const PixCtx& c) {
+  for (int task = tid; task < c.nblk * 8; task += kPixThreads) {
+    const int b = task >> 3, r = task & 7;
+    const int16_t* cb = &sh.coef[b][r * 8];
+    const uint16_t* qq = &sh.q[binfo_tix(sh.binfo[b])][r * 8];
+    if (kMode == kIdctInt) {
+      int D[8], O[8];
+      for (int k = 0; k < 8; ++k) D[k] = (int)cb[k] * (int)qq[k];
+      idct_row_int(D, O);
+      for (int k = 0; k < 8; ++k) sh.t.ti[b][r * 8 + k] = O[k];
+    } else {
+      float D[8];
+      for (int k = 0; k < 8; ++k) D[k] = (float)((int)cb[k] * (int)qq[k]);
+      for (int x = 0; x < 8; ++x) {
+        float s = 0.f;
+        for (int u = 0; u < 8; ++u) s = fmaf(sh.cosf[u * 8 + x], D[u], s);
+        sh.t.tf[b][r * 8 + x] = s;
       }
-      mfma_store(sh, c, b, p, s);
     }
   }
 }
+
+// Column pass: lane -> (block, column); samples into the component tile.
+template <int kMode>
+CJD_HD inline void pix_cols(PixShared& sh, int tid, const PixCtx& c) {
+  for (int task = tid; task < c.nblk * 8; task += kPixThreads) {
+    const int b = task >> 3, x = task & 7;
+    const uint32_t info = sh.binfo[b];
+    const int tw = binfo_tw(info);
+    uint8_t* dst = &sh.samp[binfo_base(info) + x];
+    if (kMode == kIdctInt) {
+      int D[8];
+      uint8_t o[8];
+      for (int v = 0; v < 8; ++v) D[v] = sh.t.ti[b][v * 8 + x];
+      idct_col_int(D, o);
+      for (int y = 0; y < 8; ++y) dst[y * tw] = o[y];
+    } else {
+      float D[8];
+      for (int v = 0; v < 8; ++v) D[v] = sh.t.tf[b][v * 8 + x];
+      for (int y = 0; y < 8; ++y) {
+        float s = 128.f;
+        for (int v = 0; v < 8; ++v) s = fmaf(sh.cosf[v * 8 + y], D[v], s);
+        dst[y * tw] = clip8((int)floorf(s + 0.5f));
+      }
+    }
+  }
+}
+
+// Optional planar outputs (reference plane layout, src/format.cu:295-303):
+// wave -> row, lane -> column.
+template <class SH>
+CJD_HD inline void pix_planes(const SH& sh, int tid, const PixCtx& c, uint8_t* out) {
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int comp = 0; comp < c.ncomp; ++comp) {
+    if (c.im->plane_offset[comp] == ~0ull) continue;
+    uint8_t* pl = out + c.im->plane_offset[comp];
+    const int rows = c.im->comp_sy[comp] * 8;
+    const int w = c.tw(comp), toff = c.toff(comp);
+    const int px0 = c.mcu_x0 * c.im->comp_sx[comp] * 8, py0 = c.mcu_y * rows;
+    for (int yy = wave; yy < rows; yy += kPixWaves)
+      for (int xx = lane; xx < w; xx += 64)
+        pl[(size_t)(py0 + yy) * c.im->comp_stride[comp] + px0 + xx] = sh.samp[toff + yy * w + xx];
+  }
+}
+
+// clip8(v >> 8) written as a clamp of v before the shift.  The two forms are
+// equal for every int v; this one keeps the device compiler from fusing two
+// adjacent clip-and-shift results into one packed 16-bit operation whose
+// upper half it then assumes to be zero (on gfx950 it is not: the bits come
+// back from the register's previous value).
+CJD_HD inline uint32_t clip_shr8(int v) { return (uint32_t)(v < 0 ? 0 : (v > 65535 ? 65535 : v)) >> 8; }
+
+// Aligned 4-byte store (device: one dword store; host: memcpy).
+CJD_HD inline void store_u32(uint8_t* p, uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *reinterpret_cast<uint32_t*>(p) = v;
+#else
+  memcpy(p, &v, 4);
+#endif
+}
+
+// Colour conversion / gray copy (CPUonly/src/pixelTransformCPU.c:130-172).
+// Each output row segment of the run is written as aligned 4-byte words:
+// wave -> row, lane -> word.  A word of interleaved RGB covers bytes of at
+// most two pixels, so a lane converts those two pixels and shifts the word
+// out of their six bytes.  The partial words at the two ends of a segment
+// (shared with neighbouring runs or images) are written byte by byte.
+template <class SH>
+CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* out) {
+  const ImageDesc& im = *c.im;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int x0 = c.mcu_x0 * im.mcu_w, y0 = c.mcu_y * im.mcu_h;
+  int x1 = x0 + (int)c.it.count * im.mcu_w;
+  if (x1 > im.width) x1 = im.width;
+  int y1 = y0 + im.mcu_h;
+  if (y1 > im.height) y1 = im.height;
+  const int rows = y1 - y0, rw = x1 - x0;
+  const int bpp = c.ncomp == 1 ? 1 : 3;
+  const int n = rw * bpp;  // bytes per row segment
+  const int xs0 = im.comp_xshift[0], ys0 = im.comp_yshift[0];
+  const int xs1 = im.comp_xshift[1], ys1 = im.comp_yshift[1];
+  const int xs2 = im.comp_xshift[2], ys2 = im.comp_yshift[2];
+  for (int yy = wave; yy < rows; yy += kPixWaves) {
+    const uint64_t a0 = im.out_offset + ((uint64_t)(y0 + yy) * (uint64_t)im.width + (uint64_t)x0) * (uint64_t)bpp;
+    const int mis = (int)(a0 & 3u);
+    const int nw = (mis + n + 3) >> 2;
+    uint8_t* base = out + (a0 - (uint64_t)mis);
+    const uint8_t* srow0 = &sh.samp[c.toff0 + (yy >> ys0) * c.tw0];
+    const uint8_t* srow1 = &sh.samp[c.toff1 + (yy >> ys1) * c.tw1];
+    const uint8_t* srow2 = &sh.samp[c.toff2 + (yy >> ys2) * c.tw2];
+    for (int w = lane; w < nw; w += 64) {
+      const int pos = 4 * w - mis;  // segment byte of the word's first byte (-3..)
+      uint32_t word;
+      if (bpp == 1) {
+        word = 0;
+        for (int j = 0; j < 4; ++j) {
+          int q = pos + j;
+          q = q < 0 ? 0 : (q >= rw ? rw - 1 : q);
+          word |= (uint32_t)srow0[q] << (8 * j);
+        }
+      } else {
+        const int q0 = (pos + 3) / 3 - 1;  // pixel of byte pos (floor, pos >= -3)
+        uint64_t rgb = 0;
+        for (int e = 0; e < 2; ++e) {
+          int xx = q0 + e;
+          xx = xx < 0 ? 0 : (xx >= rw ? rw - 1 : xx);
+          const int Y = (int)srow0[xx >> xs0] << 8;
+          const int cb = (int)srow1[xx >> xs1] - 128;
+          const int cr = (int)srow2[xx >> xs2] - 128;
+          const uint32_t r = clip_shr8(Y + 359 * cr + 128);
+          const uint32_t g = clip_shr8(Y - 88 * cb - 183 * cr + 128);
+          const uint32_t b = clip_shr8(Y + 454 * cb + 128);
+          rgb |= (uint64_t)(r | (g << 8) | (b << 16)) << (24 * e);
+        }
+        word = (uint32_t)(rgb >> (8 * (pos - 3 * q0)));
+      }
+      uint8_t* dst = base + 4 * w;
+      if (pos >= 0 && pos + 4 <= n) {
+        store_u32(dst, word);
+      } else {
+        for (int j = 0; j < 4; ++j)
+          if (pos + j >= 0 && pos + j < n) dst[j] = (uint8_t)(word >> (8 * j));
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------- MFMA iDCT --
+// The inverse DCT of a block as one 64x64 contraction,
+//   s[p] = 128 + sum_k c[k] * M_t[k][p],   M_t[k][p] = q_t[k] * Cu[u][x] * Cv[v][y]
+// (k = 8v + u natural coefficient index, p = 8y + x pixel index, q_t the
+// quant table), evaluated on the matrix cores with v_mfma_f32_16x16x32_f16:
+// the quantised coefficients (|c| <= 2047) are exact in FP16, M_t is split
+// into FP16 hi + lo parts and the products accumulate in FP32.  A tile is 16
+// blocks x 64 pixels; the image's distinct quant tables are stacked along K
+// (a block's coefficients are zero in the K range of the other tables), so a
+// tile with mixed luma/chroma blocks is one pass.
+//
+// Lane layout of v_mfma_f32_16x16x32_f16 (checked on the device): A[row][k]
+// lane l holds row l%16, k = 8*(l/16) + i; B[k][col] lane l holds col l%16,
+// the same k; C lane l holds col l%16, rows 4*(l/16) + i.  With K half h
+// (coefficients 32h..32h+31) and N tile n (pixels 16n..16n+15), lane l's B
+// element i is M_t[32h + 8(l/16) + i][16n + l%16].
+
+// B operand of lane l for quant table q (natural order), N tile n and K half
+// h: its 8 elements of M = (q[k] * Cu[u][x]) * Cv[v][y] in FP32, split into
+// FP16 hi and lo bit patterns.
+CJD_HD inline void mfma_b_lane(const uint16_t* q, const float* cosf, int n, int h, int l, uint16_t hb[8],
+                               uint16_t lb[8]) {
+  const int p = n * 16 + (l & 15), x = p & 7, y = p >> 3;
+  const int kb = h * 32 + 8 * (l >> 4);  // k = kb + i: v = kb / 8, u = i
+  const float cv = cosf[(kb >> 3) * 8 + y];
+  for (int i = 0; i < 8; ++i) {
+    const float m = (float)q[kb + i] * cosf[i * 8 + x] * cv;
+    const float hi = round_to_half(m);
+    hb[i] = half_bits(hi);
+    lb[i] = half_bits(round_to_half(m - hi));
+  }
+}
+
+// Write one contraction result (pixel p of block b) into its sample tile.
+CJD_HD inline void pix_put(PixSharedMfma& sh, int b, int p, float v) {
+  const uint32_t info = sh.binfo[b];
+  sh.samp[binfo_base(info) + (p >> 3) * binfo_tw(info) + (p & 7)] = clip8((int)floorf(v + 128.5f));
+}
+
+// Host emulation of the device MFMA phase, all table passes: the same
+// operands (A from the FP16 coefficients in LDS, B from mfma_b_lane) held by
+// the same lanes, with each 16x16x32 product summed in k order in FP32 (the
+// device's summation order inside one MFMA is the hardware's).  Checks the
+// operand layout, masking and output addressing on the CPU.
+#if !defined(__HIPCC__)
+inline void pix_idct_mfma_emul(PixSharedMfma& sh, const PixCtx& c) {
+  const int ntiles = (c.nblk + 15) >> 4;
+  for (int t0 = 0; t0 < c.ntab; t0 += 2) {
+    const int nt = c.ntab - t0 < 2 ? c.ntab - t0 : 2;
+    for (int n = 0; n < 4; ++n) {  // the wave that owns N tile n
+      float B[2][2][2][64][8];     // [table][h][hi, lo][lane][element]
+      for (int tl = 0; tl < nt; ++tl)
+        for (int h = 0; h < 2; ++h)
+          for (int l = 0; l < 64; ++l) {
+            uint16_t hb[8], lb[8];
+            mfma_b_lane(sh.q[t0 + tl], sh.cosf, n, h, l, hb, lb);
+            for (int i = 0; i < 8; ++i) {
+              B[tl][h][0][l][i] = half_value(hb[i]);
+              B[tl][h][1][l][i] = half_value(lb[i]);
+            }
+          }
+      for (int tile = 0; tile < ntiles; ++tile) {
+        float a[64][2][8];
+        int tix[64];
+        for (int l = 0; l < 64; ++l) {
+          const int brow = tile * 16 + (l & 15);
+          tix[l] = brow < c.nblk ? binfo_tix(sh.binfo[brow]) : -1;
+          for (int h = 0; h < 2; ++h)
+            for (int i = 0; i < 8; ++i)
+              a[l][h][i] = brow < c.nblk ? half_value(sh.coef[brow][h * 32 + 8 * (l >> 4) + i]) : 0.f;
+        }
+        float acc[64][4] = {};
+        for (int tl = 0; tl < nt; ++tl)
+          for (int h = 0; h < 2; ++h)
+            for (int hl = 0; hl < 2; ++hl) {
+              float prod[16][16];
+              for (int row = 0; row < 16; ++row)
+                for (int col = 0; col < 16; ++col) {
+                  float s = 0.f;
+                  for (int k = 0; k < 32; ++k) {
+                    const int la = row + 16 * (k >> 3), lb = col + 16 * (k >> 3);
+                    const float av = tix[la] == t0 + tl ? a[la][h][k & 7] : 0.f;
+                    s += av * B[tl][h][hl][lb][k & 7];
+                  }
+                  prod[row][col] = s;
+                }
+              for (int l = 0; l < 64; ++l)
+                for (int i = 0; i < 4; ++i) acc[l][i] += prod[4 * (l >> 4) + i][l & 15];
+            }
+        for (int l = 0; l < 64; ++l)
+          for (int i = 0; i < 4; ++i) {
+            const int b = tile * 16 + 4 * (l >> 4) + i;
+            if (b >= c.nblk) continue;
+            const int tb = binfo_tix(sh.binfo[b]);
+            if (tb >= t0 && tb < t0 + nt) pix_put(sh, b, n * 16 + (l & 15), acc[l][i]);
+          }
+      }
+    }
+  }
+}
+#endif
 
 }  // namespace cjd

@@ -127,6 +127,10 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
       d.slot_dc[slot] = (uint8_t)f.dc_id[c];
       d.slot_ac[slot] = (uint8_t)f.ac_id[c];
     }
+    int t = 0;
+    while (t < d.ntab && d.tab_dq[t] != d.comp_dq[c]) ++t;
+    if (t == d.ntab) d.tab_dq[d.ntab++] = d.comp_dq[c];
+    d.comp_tix[c] = (uint8_t)t;
   }
   for (int t = 0; t < 4; ++t) {
     HuffTable ht;

@@ -302,7 +302,10 @@ static void fdct_quant(const float blk[64], const uint8_t qnat[64], int16_t out[
     }
 }
 
-static void put_header(bw_t *w, const synth_params *p, uint8_t q[2][64]) {
+/* quant table of component c: Y 0, Cb 1, Cr 1 (or 2 with three tables) */
+static int comp_qtab(const synth_params *p, int c) { return c == 0 ? 0 : (c == 2 && p->qtables == 3 ? 2 : 1); }
+
+static void put_header(bw_t *w, const synth_params *p, uint8_t q[3][64]) {
   const int C = p->ncomp == 1 ? 1 : 3;
   raw(w, 0xFF); raw(w, 0xD8);
   /* APP0 JFIF */
@@ -310,7 +313,7 @@ static void put_header(bw_t *w, const synth_params *p, uint8_t q[2][64]) {
   raw(w, 'J'); raw(w, 'F'); raw(w, 'I'); raw(w, 'F'); raw(w, 0);
   raw(w, 1); raw(w, 1); raw(w, 0); raw16(w, 1); raw16(w, 1); raw(w, 0); raw(w, 0);
   /* DQT, zigzag order */
-  for (int t = 0; t < (C == 1 ? 1 : 2); t++) {
+  for (int t = 0; t < (C == 1 ? 1 : (p->qtables == 3 ? 3 : 2)); t++) {
     raw(w, 0xFF); raw(w, 0xDB); raw16(w, 67); raw(w, (uint8_t)t);
     for (int k = 0; k < 64; k++) raw(w, q[t][k_zz[k]]);
   }
@@ -321,7 +324,7 @@ static void put_header(bw_t *w, const synth_params *p, uint8_t q[2][64]) {
     raw(w, (uint8_t)(c + 1));
     if (c == 0) raw(w, (uint8_t)((C == 1 ? 1 : p->h_samp) << 4 | (C == 1 ? 1 : p->v_samp)));
     else raw(w, 0x11);
-    raw(w, (uint8_t)(c == 0 ? 0 : 1));
+    raw(w, (uint8_t)comp_qtab(p, c));
   }
   /* DHT */
   const uint8_t *bits[4] = {k_dc_lum_bits, k_ac_lum_bits, k_dc_chr_bits, k_ac_chr_bits};
@@ -358,9 +361,10 @@ long synth_encode(const synth_params *p, const uint8_t *pix, uint8_t *out, long 
   cos_init();
   const int hmax = hs[0], vmax = vs[0];
   const int nbx = (W + 8 * hmax - 1) / (8 * hmax), nby = (H + 8 * vmax - 1) / (8 * vmax);
-  uint8_t q[2][64];
+  uint8_t q[3][64];
   synth_quant_table(p->quality, 0, q[0]);
   synth_quant_table(p->quality, 1, q[1]);
+  synth_quant_table(p->quality > 20 ? p->quality - 15 : p->quality + 15, 1, q[2]);
 
   /* component planes, padded to whole MCUs by edge replication */
   uint8_t *plane[3] = {0, 0, 0};
@@ -423,7 +427,7 @@ long synth_encode(const synth_params *p, const uint8_t *pix, uint8_t *out, long 
   for (long m = 0; m < n_mcu; m++) {
     const int mx = (int)(m % nbx), my = (int)(m / nbx);
     for (int c = 0; c < C; c++) {
-      const int t = c == 0 ? 0 : 1;
+      const int t = c == 0 ? 0 : 1, tq = comp_qtab(p, c);
       for (int sy = 0; sy < vs[c]; sy++)
         for (int sx = 0; sx < hs[c]; sx++) {
           float blk[64];
@@ -432,7 +436,7 @@ long synth_encode(const synth_params *p, const uint8_t *pix, uint8_t *out, long 
             for (int x = 0; x < 8; x++)
               blk[y * 8 + x] = (float)plane[c][(size_t)(y0 + y) * pw[c] + x0 + x] - 128.0f;
           int16_t qc[64];
-          fdct_quant(blk, q[t], qc);
+          fdct_quant(blk, q[tq], qc);
           int dcd = qc[0] - pred[c];
           if (dcd > 2047 || dcd < -2047) { /* keep the DC difference encodable */
             qc[0] = (int16_t)(pred[c] + (dcd > 0 ? 2047 : -2047));

@@ -16,6 +16,8 @@ typedef struct {
   double sigma;             /* Gaussian noise std */
   double amp_total;         /* sum of the 6 sinusoid amplitudes */
   double random_tile_frac;  /* fraction of 64x64 tiles replaced by uniform noise */
+  int32_t qtables;          /* quant tables for YCbCr: 2 (Y | CbCr) or 3 (Y | Cb | Cr) */
+  int32_t pad_;
 } synth_params;
 
 /* W*H*ncomp interleaved pixels; returns 0 on success. */

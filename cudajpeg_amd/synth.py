@@ -32,6 +32,8 @@ class _Params(ctypes.Structure):
         ("sigma", ctypes.c_double),
         ("amp_total", ctypes.c_double),
         ("random_tile_frac", ctypes.c_double),
+        ("qtables", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
     ]
 
 
@@ -71,11 +73,13 @@ class ImageSpec:
     random_tile_frac: float = 0.0
     amp_total: float = 80.0
     restart_interval: int = 0
+    qtables: int = 2               # YCbCr: 2 (Y | CbCr) or 3 (Y | Cb | Cr) quant tables
 
     def _params(self) -> _Params:
         ncomp, h, v = SAMPLING[self.sampling]
         return _Params(self.seed, self.width, self.height, ncomp, h, v, self.quality,
-                       self.restart_interval, self.sigma, self.amp_total, self.random_tile_frac)
+                       self.restart_interval, self.sigma, self.amp_total, self.random_tile_frac,
+                       self.qtables, 0)
 
     @property
     def ncomp(self) -> int:

@@ -6,6 +6,7 @@
 // kernels' indexing and the tile-link / lookback logic are checked against the
 // oracle without a GPU.  `order` = 1 runs the decoupled lookback with every
 // predecessor holding only its aggregate, which exercises the composition path.
+#include <type_traits>
 #include <stdint.h>
 #include <string.h>
 
@@ -121,25 +122,34 @@ void run_entropy(Emu& E, int order) {
     for (int t = 0; t < kWG; ++t) ent_phase5(*R.shs[i], t, R.ctx[i], E.coefs.data(), E.status.data());
 }
 
+template <class SH>
+void pixel_item(Emu& E, SH& sh, int mode, const PixCtx& c) {
+  Plan& p = E.plan;
+  memset(&sh, 0, sizeof(SH));
+  for (int t = 0; t < kPixThreads; ++t) pix_stage_cos(sh, t);
+  for (int t = 0; t < kPixThreads; ++t) pix_stage(sh, t, c, E.coefs.data(), p.qtabs.data());
+  if constexpr (std::is_same<SH, PixSharedMfma>::value) {
+    pix_idct_mfma_emul(sh, c);
+  } else if (mode == kIdctInt) {
+    for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctInt>(sh, t, c);
+    for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctInt>(sh, t, c);
+  } else {
+    for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctFloat>(sh, t, c);
+    for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctFloat>(sh, t, c);
+  }
+  for (int t = 0; t < kPixThreads; ++t) pix_planes(sh, t, c, E.out.data());
+  for (int t = 0; t < kPixThreads; ++t) pix_colour(sh, t, c, E.out.data());
+}
+
 void run_pixel(Emu& E, int mode, size_t first, size_t count) {
   Plan& p = E.plan;
   std::unique_ptr<PixShared> sh(new PixShared);
+  std::unique_ptr<PixSharedMfma> shm(new PixSharedMfma);
   for (size_t i = first; i < first + count; ++i) {
     PixCtx c;
     c.init(p.images.data(), p.pix[i]);
-    memset(sh.get(), 0, sizeof(PixShared));
-    for (int t = 0; t < kPixThreads; ++t) pix_stage(*sh, t, c, E.coefs.data(), p.qtabs.data());
-    if (mode == kIdctInt) {
-      for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctInt>(*sh, t, c);
-      for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctInt>(*sh, t, c);
-    } else if (mode == kIdctMfma) {
-      pix_idct_hilo_host(*sh, c);
-    } else {
-      for (int t = 0; t < kPixThreads; ++t) pix_rows<kIdctFloat>(*sh, t, c);
-      for (int t = 0; t < kPixThreads; ++t) pix_cols<kIdctFloat>(*sh, t, c);
-    }
-    for (int t = 0; t < kPixThreads; ++t) pix_planes(*sh, t, c, E.out.data());
-    for (int t = 0; t < kPixThreads; ++t) pix_colour(*sh, t, c, E.out.data());
+    if (mode == kIdctMfma) pixel_item(E, *shm, mode, c);
+    else pixel_item(E, *sh, mode, c);
   }
 }
 

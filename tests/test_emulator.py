@@ -19,6 +19,8 @@ SPECS = [
     synth.ImageSpec(512, 512, "420", 75, 6.0, 1000),
     synth.ImageSpec(640, 480, "444", 95, 8.0, 15),
     synth.ImageSpec(17, 9, "420", 90, 4.0, 12),
+    synth.ImageSpec(200, 136, "420", 80, 6.0, 31, qtables=3),
+    synth.ImageSpec(72, 40, "444", 60, 6.0, 32, qtables=3),
 ]
 
 

@@ -22,6 +22,8 @@ SMALL = [
     synth.ImageSpec(100, 60, "420", 70, 6.0, 9, restart_interval=3),
     synth.ImageSpec(300, 200, "420", 95, 12.0, 11, random_tile_frac=0.3),
     synth.ImageSpec(17, 9, "420", 90, 4.0, 12),
+    synth.ImageSpec(200, 136, "420", 80, 6.0, 31, qtables=3),
+    synth.ImageSpec(72, 40, "444", 60, 6.0, 32, qtables=3),
     synth.ImageSpec(8, 8, "gray", 100, 2.0, 13),
 ]
 
@@ -137,7 +139,8 @@ def test_mfma_idct_error_vs_fp64(gpu):
     # FP64 iDCT (T.81 A.3.3) and >= 99.9% equal to it; within 1 of the
     # reference's integer iDCT
     specs = [synth.config_plain_512(), synth.config_4k(), synth.ImageSpec(640, 480, "444", 95, 8.0, 5),
-             synth.ImageSpec(512, 384, "gray", 60, 8.0, 6)]
+             synth.ImageSpec(512, 384, "gray", 60, 8.0, 6),
+             synth.ImageSpec(800, 600, "420", 85, 8.0, 8, qtables=3)]
     for spec in specs:
         enc = synth.encode(spec, truth=True)
         b = BatchDecoder(IDCT_MFMA, planes=True)
