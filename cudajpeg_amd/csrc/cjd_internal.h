@@ -8,7 +8,6 @@ namespace cjd {
 constexpr int kLutBits = 9;                 // fast Huffman lookup width
 constexpr int kLutSize = 1 << kLutBits;
 constexpr int kMaxSlots = 10;               // blocks per MCU (T.81 B.2.3 limit)
-constexpr int kListLen = 8;                 // block starts remembered per hypothesis
 constexpr int kWG = 256;                    // entropy workgroup size (4 wave64)
 constexpr int kSegPadWords = 96;            // 0xFF words after every segment's data
 
@@ -66,7 +65,7 @@ struct SegmentDesc {
   uint32_t nblocks;     // blocks the segment must contain
   int64_t first_block;  // global coefficient-buffer index of its first block
   uint32_t first_item;  // index of its first work item
-  uint32_t pad_;
+  uint32_t sync_bits;   // sync point D: bits into a tile where hypotheses are compared
 };
 
 // One entropy workgroup: tiles [t0, t0 + ntiles) of a segment.  When t0 > 0 the
@@ -78,27 +77,27 @@ struct WorkItem {
   uint32_t prev_item;   // predecessor in the same segment, ~0 for the first
 };
 
-// Transfer of the decoder state across one workgroup: for each hypothesis h
-// of the lead-in tile, the hypothesis followed in the last tile (-1 when a
-// link on the way did not resolve) and the blocks / DC sums added on the way.
-// `inclusive` is the true state at the exit of the workgroup's last tile: the
-// hypothesis it lies on (-1 if none), its bit and slot, and the cumulative
-// blocks / DC sums of the segment.
+// Lookback record of one work item.  The aggregate holds under one
+// assumption: if the item's entry (the exit of its lead-in tile) is the main
+// exit (lead_p, lead_s) of that tile, the exit of its last tile is
+// (agg_p, agg_s) and the item adds agg_c (blocks, DC sums of components 0-2).
+// The inclusive state is the true exit of the last tile with the segment's
+// cumulative counts there.
 struct LookbackRec {
   int32_t status;         // 0 none, 1 aggregate, 2 inclusive  (written last)
-  int32_t incl_h;
+  uint32_t lead_p, lead_s;
+  uint32_t agg_p, agg_s;
+  int32_t agg_c[4];
   uint32_t incl_p, incl_s;
-  int32_t incl_c[4];      // blocks, dc0, dc1, dc2
-  int8_t agg_h[kMaxSlots];
-  int8_t pad_[6];
-  int32_t agg_c[kMaxSlots][4];
+  int32_t incl_c[4];
+  int32_t pad_[3];
 };
 
 // Per-image status bits (written by kernels with atomicOr).
 enum : uint32_t {
   kStatusSyntax = 1u,        // invalid Huffman code / coefficient index (reference SYNTAX_ERROR)
   kStatusTruncated = 2u,     // scan ended before all blocks were decoded
-  kStatusUnresolved = 4u,    // a tile link could not be resolved; rerun with the serial decoder
+  kStatusUnresolved = 4u,    // the decoder state could not be resolved; rerun with the serial decoder
   kStatusInconsistent = 8u,  // final decode disagreed with the resolved path; rerun serially
   kStatusFallback = 0x100u,  // host: the image was decoded again by the serial decoder
 };

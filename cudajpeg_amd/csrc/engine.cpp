@@ -109,11 +109,20 @@ extern "C" void cjd_batch_destroy(cjd_batch* b) { delete b; }
 extern "C" int cjd_batch_set_options(cjd_batch* b, int idct_mode, int entropy_mode, int tile_bits, int want_planes) {
   if (!b) return CJD_PROGRAMMER_ERROR;
   if (idct_mode < 0 || idct_mode > 2 || entropy_mode < 0 || entropy_mode > 1) return CJD_PROGRAMMER_ERROR;
-  if (tile_bits && (tile_bits < 2048 || (tile_bits & 31))) return CJD_PROGRAMMER_ERROR;
+  if (tile_bits && (tile_bits < 512 || (tile_bits & 31))) return CJD_PROGRAMMER_ERROR;
   b->idct_mode = idct_mode;
   b->entropy_mode = entropy_mode;
   if (tile_bits) b->plan.tile_bits = (uint32_t)tile_bits;
   b->plan.want_planes = want_planes ? 1 : 0;
+  return 0;
+}
+
+extern "C" int cjd_batch_set_tuning(cjd_batch* b, int sync_bits, int item_tiles) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  if (sync_bits < 0 || (sync_bits & 31) || item_tiles < 0 || item_tiles == 1 || item_tiles > kWG)
+    return CJD_PROGRAMMER_ERROR;
+  b->plan.sync_bits = (uint32_t)sync_bits;
+  b->plan.item_tiles = item_tiles ? (uint32_t)item_tiles : (uint32_t)kWG;
   return 0;
 }
 

@@ -3,7 +3,7 @@
 // k_entropy runs the phases of entropy_core.h for one work item (a run of
 // tiles of one restart segment) per 256-lane workgroup, with a decoupled
 // lookback across workgroups; k_entropy_serial is the one-lane-per-segment
-// decoder used when a tile link does not resolve.  Together they replace
+// decoder used when the final decode disagrees with the chain.  Together they replace
 // decodeScanGPU (src/decodeScanGPU.cu:123-491) and the speculative kernels of
 // src/entropyRLEdecodeGPU.cu:7-238.
 #include <hip/hip_runtime.h>
@@ -36,7 +36,10 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   const int tid = threadIdx.x;
   // Work items are taken in ticket order, so every predecessor an item waits
   // for in the lookback has already started.
-  if (tid == 0) sh.item = atomicAdd(ticket, 1u);
+  if (tid == 0) {
+    sh.item = atomicAdd(ticket, 1u);
+    sh.fix = 0;
+  }
   __syncthreads();
   const uint32_t item_id = sh.item;
   if (item_id >= nitems) return;
@@ -53,61 +56,57 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   __syncthreads();
   ent_phase2(sh, tid, c);
   __syncthreads();
-  ent_phase3_init(sh, tid, c);
-  __syncthreads();
-  int buf = 1;
-  for (int d = 1; d < c.J; d <<= 1) {
-    ent_phase3_step(sh, tid, c, d, buf);
+  if (sh.fix) {
+    if (tid == 0) ent_fix_chain(sh, c);
     __syncthreads();
-    buf ^= 1;
+  }
+  for (int d = 1; d < c.J; d <<= 1) {
+    int32_t v[4];
+    ent_scan_read(sh, tid, d, v);
+    __syncthreads();
+    if (tid < c.J) ent_scan_write(sh, tid, v);
+    __syncthreads();
   }
   LookbackRec* me = lb + item_id;
-  if (tid == 0) {
-    sh.buf = buf;
-    sh.need_walk = 0;
-    if (!ent_publish_aggregate(sh, c, me)) {
-      st_status(&me->status, 1);
-      // decoupled lookback: compose predecessors' aggregates until one has
-      // its inclusive state
-      LookAcc acc;
-      acc.identity(c.nh);
-      uint32_t q_item = c.it.prev_item;
-      int eh = -1;
-      uint32_t ep = 0, es = 0;
-      int32_t ec[4] = {0, 0, 0, 0};
-      for (;;) {
-        const LookbackRec* r = lb + q_item;
-        int s;
-        while ((s = ld_status(&r->status)) == 0) __builtin_amdgcn_s_sleep(2);
-        if (s == 2) {
-          if (!acc.apply(*r, sh, eh, ep, es, ec)) {
-            // not derivable through the aggregates: take the immediate
-            // predecessor's inclusive state
-            const LookbackRec* pr = lb + c.it.prev_item;
-            while (ld_status(&pr->status) != 2) __builtin_amdgcn_s_sleep(2);
-            acc.identity(c.nh);
-            acc.apply(*pr, sh, eh, ep, es, ec);
-          }
-          break;
+  if (tid == 0 && c.lead) {
+    ent_aggregate(sh, c, me);
+    st_status(&me->status, 1);
+    // decoupled lookback: compose predecessors' aggregates until one has its
+    // inclusive state
+    LookAcc acc;
+    acc.reset();
+    uint32_t q_item = c.it.prev_item, p = 0, s = 0;
+    int32_t cc[4] = {0, 0, 0, 0};
+    for (;;) {
+      const LookbackRec* r = lb + q_item;
+      int st;
+      while ((st = ld_status(&r->status)) == 0) __builtin_amdgcn_s_sleep(2);
+      if (st == 2) {
+        if (!acc.apply(*r, p, s, cc)) {
+          // an assumption on the way does not hold: take the immediate
+          // predecessor's inclusive state
+          const LookbackRec* pr = lb + c.it.prev_item;
+          while (ld_status(&pr->status) != 2) __builtin_amdgcn_s_sleep(2);
+          acc.reset();
+          acc.apply(*pr, p, s, cc);
         }
-        acc.absorb(*r, c.nh);
-        q_item = items[q_item].prev_item;
+        break;
       }
-      sh.entry_h = eh;
-      sh.entry_p = ep;
-      sh.entry_s = es;
-      for (int q = 0; q < 4; ++q) sh.entry_c[q] = ec[q];
+      acc.absorb(*r);
+      q_item = items[q_item].prev_item;
     }
+    sh.entry_p = p;
+    sh.entry_s = s;
+    for (int q = 0; q < 4; ++q) sh.entry_c[q] = cc[q];
+    ent_apply_entry(sh, c);
   }
   __syncthreads();
-  ent_states(sh, tid, c);
+  ent_absolute(sh, tid, c);
   __syncthreads();
   if (tid == 0) {
-    if (sh.need_walk) ent_walk(sh, c);
     ent_inclusive(sh, c, me);
     st_status(&me->status, 2);
   }
-  __syncthreads();
   ent_phase5(sh, tid, c, coefs, status);
 }
 

@@ -4,22 +4,33 @@
 // the CPU emulator used by the tests.
 //
 // Algorithm (replaces src/entropyRLEdecodeGPU.cu:7-238 and the host serial
-// decode of src/decodeScanGPU.cu:435-451):
-//  1. Every tile of S bits is decoded speculatively by one lane per MCU slot h
-//     ("hypothesis" h: a block of slot h starts at the tile's first bit).  Each
-//     lane remembers its first kListLen block starts (bit, slot, DC prefix) and
-//     its exit: the first block start at or after the tile end, with the block
-//     count and DC sums on the way.
-//  2. Link: from every hypothesis exit of tile t a lane decodes into tile t+1
-//     until it reaches a remembered block start of a hypothesis h' of t+1
-//     (the trajectories have synchronised): sigma_t(h) = h' plus the blocks /
-//     DC sums added in tile t+1.
-//  3. These maps compose associatively: pointer jumping in LDS gives, for
-//     every tile of the workgroup, the state reached from each hypothesis of
-//     the lead-in tile; a decoupled lookback across workgroups supplies the
-//     true state of the lead-in tile.
+// decode of src/decodeScanGPU.cu:435-451).  A restart segment is cut into
+// tiles of S bits; a work item is a run of up to kWG tiles, one lane each.
+// The decoder state at a block boundary is (bit, slot of the next block); a
+// trajectory is the sequence of block boundaries a decoder started from some
+// state passes through.  Huffman streams self-synchronise: trajectories
+// started at different states soon share a boundary and are identical from
+// there on.
+//  1. Main trajectory of each tile: from the tile's first bit one trajectory
+//     per MCU slot h ("hypothesis" h: a block of slot h starts there) is
+//     decoded until its first boundary at or after the sync point (D bits into
+//     the tile).  The boundary most of them reach is the tile's D-state; the
+//     main trajectory continues from it to the tile's exit (its first
+//     boundary at or after the tile end), counting blocks and DC sums.
+//  2. Link: from the main exit of tile j-1 a lane decodes into tile j up to
+//     its first boundary at or after the sync point.  If that is the D-state,
+//     the link has joined the main trajectory: tile j's exit is its main exit.
+//     Otherwise the lane decodes on to the tile end (a concrete exit).
+//  3. Chain: assuming the item's entry is the main exit of its lead-in tile
+//     (the previous item's last tile), every tile's exit follows from the
+//     links; a tile whose link did not start from its true entry is decoded
+//     again from that entry (rare).  Counts are prefix-summed.  Across items a
+//     decoupled lookback composes these single-state aggregates ("if the
+//     entry is E then the exit is X, plus counts") back to a predecessor that
+//     published its true exit.
 //  4. Every tile is decoded once more from its true entry (bit, slot, block
-//     index, DC predictors), writing quantised coefficients.
+//     index, DC predictors), writing quantised coefficients, and its exit is
+//     checked against the chain.
 //
 // Symbol semantics follow src/decodeScanGPU.cu:28-100 bit for bit on valid
 // streams.  During speculation an invalid code ends the block (an invalid DC
@@ -48,19 +59,28 @@ CJD_HD inline uint32_t bswap32(uint32_t v) {
 }
 
 // Sequential reader of an unstuffed segment, MSB first.  The next bits sit in
-// a 64-bit accumulator fed one 32-bit word at a time from a 4-word chunk; the
-// following chunk is already in flight (a 16-byte load issued one chunk ahead),
-// so a lane decoding symbols does not wait on memory.  Words past the
-// segment read as 0xFFFFFFFF (the reference pads with 0xFF beyond the end,
+// a 64-bit accumulator fed one 32-bit word at a time from two resident 4-word
+// chunks (current q, next n); a third chunk f is in flight.  Decode loops run
+// in periods of kPeriod symbol steps: at the top of a period rotate() moves
+// on to the next chunk when the current one is used up (q <- n, n <- f) and
+// issues the load of the following one.  A step appends at most one word, and
+// after rotate() at least 5 words are resident, so within a period no lane
+// touches a register whose load may still be in flight: the only wait on
+// global memory is for the chunk requested one period earlier.  Words are
+// kept as loaded and byte-swapped when appended.  Words past the segment read
+// as 0xFFFFFFFF (the reference pads with 0xFF beyond the end,
 // src/decodeScanGPU.cu:32).
+constexpr int kPeriod = 4;
+
 struct BitReader {
   const uint32_t* w;
   uint32_t nwords;
   uint64_t acc;   // next bits, MSB-aligned; nacc of them valid (> 32 after refill)
   int nacc;
-  uint32_t q0, q1, q2, q3;  // current chunk (byte-swapped)
+  uint32_t q0, q1, q2, q3;  // current chunk (as loaded)
   uint32_t n0, n1, n2, n3;  // next chunk
-  uint32_t qi;    // next word of the current chunk to append
+  uint32_t f0, f1, f2, f3;  // chunk after next, in flight
+  uint32_t qi;    // next word to append: 0..3 in q, 4..7 in n
   uint32_t chunk; // word index of q0
   uint32_t pos;   // stream bit of the top of acc
 
@@ -72,32 +92,49 @@ struct BitReader {
     wi = wi < nwords - 4 ? wi : nwords - 4;
 #if defined(__HIP_DEVICE_COMPILE__)
     const uint4 v = *reinterpret_cast<const uint4*>(w + wi);
-    a = bswap32(v.x);
-    b = bswap32(v.y);
-    c = bswap32(v.z);
-    d = bswap32(v.w);
+    a = v.x;
+    b = v.y;
+    c = v.z;
+    d = v.w;
 #else
-    a = bswap32(w[wi]);
-    b = bswap32(w[wi + 1]);
-    c = bswap32(w[wi + 2]);
-    d = bswap32(w[wi + 3]);
+    a = w[wi];
+    b = w[wi + 1];
+    c = w[wi + 2];
+    d = w[wi + 3];
 #endif
+  }
+  CJD_HD inline uint32_t word(uint32_t i) const {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (i > 7) __builtin_trap();  // CPU builds: the period invariant was broken
+#endif
+    const uint32_t a = (i & 4) ? n0 : q0, b = (i & 4) ? n1 : q1, c = (i & 4) ? n2 : q2, d = (i & 4) ? n3 : q3;
+    const uint32_t lo = (i & 1) ? b : a, hi = (i & 1) ? d : c;
+    return bswap32((i & 2) ? hi : lo);
   }
   CJD_HD inline void refill() {
     while (nacc <= 32) {
-      const uint32_t word = qi == 0 ? q0 : qi == 1 ? q1 : qi == 2 ? q2 : q3;
-      acc |= (uint64_t)word << (32 - nacc);
+      acc |= (uint64_t)word(qi) << (32 - nacc);
       nacc += 32;
-      if (++qi == 4) {
-        q0 = n0;
-        q1 = n1;
-        q2 = n2;
-        q3 = n3;
-        chunk += 4;
-        qi = 0;
-        load(chunk + 4, n0, n1, n2, n3);
-      }
+      ++qi;
     }
+  }
+  // Branch-free: the chunk shift is a select and the prefetch load is issued
+  // every period (re-reading the same chunk when nothing moved), so the load
+  // writes the loop-carried registers directly and nothing waits for it
+  // before the next period.
+  CJD_HD inline void rotate() {
+    const bool r = qi >= 4;
+    q0 = r ? n0 : q0;
+    q1 = r ? n1 : q1;
+    q2 = r ? n2 : q2;
+    q3 = r ? n3 : q3;
+    n0 = r ? f0 : n0;
+    n1 = r ? f1 : n1;
+    n2 = r ? f2 : n2;
+    n3 = r ? f3 : n3;
+    chunk += r ? 4u : 0u;
+    qi -= r ? 4u : 0u;
+    load(chunk + 8, f0, f1, f2, f3);
   }
   CJD_HD inline void seek(const uint32_t* words, uint32_t n, uint32_t p) {
     w = words;
@@ -106,6 +143,7 @@ struct BitReader {
     chunk = wi & ~3u;
     load(chunk, q0, q1, q2, q3);
     load(chunk + 4, n0, n1, n2, n3);
+    load(chunk + 8, f0, f1, f2, f3);
     qi = wi - chunk;
     acc = 0;
     nacc = 0;
@@ -115,6 +153,7 @@ struct BitReader {
     acc <<= o;
     nacc -= o;
     refill();
+    rotate();
   }
   CJD_HD inline uint32_t peek() const { return (uint32_t)(acc >> 32); }
   CJD_HD inline void skip(int n) {
@@ -159,6 +198,7 @@ CJD_HD inline int extend_value(uint32_t bits, int len, int size) {
 template <bool kWrite>
 CJD_HD inline void decode_block(BitReader& br, const HuffTable& dct, const HuffTable& act, int& dcdiff, int& err,
                                 int16_t* blk) {
+  br.rotate();
   uint32_t bits = br.peek();
   int len;
   int sym = huff_sym(dct, bits, len);
@@ -173,6 +213,7 @@ CJD_HD inline void decode_block(BitReader& br, const HuffTable& dct, const HuffT
   br.skip(len + size);
   int coef = 0;
   while (coef < 63) {
+    br.rotate();  // at most one word is appended per symbol
     bits = br.peek();
     sym = huff_sym(act, bits, len);
     if (!len) {
@@ -205,35 +246,29 @@ CJD_HD inline void status_or(uint32_t* p, uint32_t v) {
 }
 
 // ------------------------------------------------------------ LDS state --
+// Decoder state at a block boundary packed as bit << 4 | slot.
+CJD_HD inline uint64_t pack_state(uint32_t p, uint32_t s) { return ((uint64_t)p << 4) | s; }
+
 struct EntShared {
   HuffTable tab[4];
-  uint8_t dz[64];                     // zigzag index -> natural index
-  uint32_t exit_p[kWG];
-  uint32_t exit_s[kWG];
-  int32_t tot[kWG][4];   // blocks, dc0, dc1, dc2 of each hypothesis
-  uint32_t nlist[kWG];
+  uint8_t dz[64];                 // zigzag index -> natural index
+  uint32_t ep[kWG], es[kWG];      // main exit of each local tile (bit, slot)
+  uint32_t tp[kWG], ts[kWG];      // chain: exit of tile j (assumed, then true)
+  uint32_t up[kWG], us[kWG];      // chain: entry tile j's exit was computed from
+  int32_t tc[kWG][4];             // chain: blocks / DC sums gained in tile j, then
+                                  //   cumulative (relative to the item entry, then absolute)
   union {
+    int16_t coef[kWG][64];        // final decode: one block per lane
     struct {
-      uint32_t p[kWG][kListLen];
-      uint32_t s_dc0[kWG][kListLen];  // slot | dc0 << 16
-      uint32_t dc12[kWG][kListLen];   // dc1 | dc2 << 16
-    } l;
-    int16_t coef[kWG][64];            // final pass: one block per lane
+      uint64_t hyp[kMaxSlots][kWG];  // D-state reached by each hypothesis
+      uint64_t dstate[kWG];          // the tile's D-state (main), ~0 for none
+      int32_t mc[kWG][4];            // main counts from the D-state to the exit
+    } a;
   } u;
-  int32_t lk_h[kWG];                  // link of (tile j, hypothesis h) into tile j+1: hypothesis or -1
-  int32_t lk_c[kWG][4];               //   ... blocks, dc0, dc1, dc2 gained in tile j+1
-  uint32_t lk_p[kWG], lk_s[kWG];      //   ... exit of tile j+1 reached when the link is -1
-  int32_t mh[2][kWG];                 // prefix compositions (pointer jumping)
-  int32_t mc[2][kWG][4];
-  int32_t ex_h[kWG];                  // true state at the exit of each local tile:
-  uint32_t ex_p[kWG], ex_s[kWG];      //   hypothesis (-1: none, -2: not yet known), bit, slot,
-  int32_t ex_c[kWG][4];               //   cumulative blocks / DC sums of the segment
-  int32_t entry_h;                    // true state at the exit of local tile 0
-  uint32_t entry_p, entry_s;
-  int32_t entry_c[4];
   uint32_t item;
-  int32_t buf;                        // which mh/mc buffer holds the prefixes
-  int32_t need_walk;
+  int32_t fix;                    // the chain needs the sequential fix-up
+  uint32_t entry_p, entry_s;      // true state at the exit of the lead-in tile
+  int32_t entry_c[4];             //   and the segment's cumulative counts there
 };
 
 // Constants of one work item.
@@ -245,6 +280,7 @@ struct EntCtx {
   WorkItem it;
   int nh;       // hypotheses = blocks per MCU
   uint32_t S;   // tile bits
+  uint32_t D;   // sync point, bits into a tile
   int lead;     // 1 when local tile 0 is the lead-in (last tile of the previous item)
   int J;        // local tiles including the lead-in
   uint32_t tfirst;
@@ -256,6 +292,7 @@ struct EntCtx {
     im = images + sg.image;
     nh = im->nslots;
     S = sg.tile_bits;
+    D = sg.sync_bits;
     w = words + sg.word_base;
     nwords = sg.nwords;
     lead = item.t0 > 0 ? 1 : 0;
@@ -268,11 +305,9 @@ struct EntCtx {
   }
   CJD_HD int comp(int slot) const { return (int)((slotinfo >> (4 * slot + 2)) & 3); }
   CJD_HD uint32_t tile_end(uint32_t t) const { return (t + 1 >= sg.ntiles) ? sg.nbits : (t + 1) * S; }
-  CJD_HD const HuffTable& dct(const EntShared& sh, int slot) const {
-    return sh.tab[(slotinfo >> (4 * slot)) & 1];
-  }
-  CJD_HD const HuffTable& act(const EntShared& sh, int slot) const {
-    return sh.tab[2 + ((slotinfo >> (4 * slot + 1)) & 1)];
+  CJD_HD uint32_t sync_point(uint32_t t) const {
+    const uint32_t e = tile_end(t), q = t * S + D;
+    return q < e ? q : e;
   }
 };
 
@@ -316,296 +351,262 @@ CJD_HD inline bool sym_step(BitReader& br, const EntShared& sh, uint64_t slotinf
   return end;
 }
 
-// Phase 1: speculative decode of (local tile j, hypothesis h) = lane tid.
-CJD_HD inline void ent_phase1(EntShared& sh, int tid, const EntCtx& c) {
-  const int j = tid / c.nh, h = tid - j * c.nh;
-  if (j >= c.J) return;
-  const uint32_t t = c.tfirst + (uint32_t)j;
-  BitReader br;
-  br.seek(c.w, c.nwords, t * c.S);
-  int slot = (t == 0) ? 0 : h, coef = -1, err = 0;
-  const uint32_t end = c.tile_end(t);
-  int32_t nb = 0, dc[3] = {0, 0, 0};
-  for (;;) {
-    if (coef < 0) {
-      const uint32_t p = br.pos;
-      if (p >= end) break;
-      if (nb < kListLen) {
-        sh.u.l.p[tid][nb] = p;
-        sh.u.l.s_dc0[tid][nb] = (uint32_t)slot | ((uint32_t)(uint16_t)(int16_t)dc[0] << 16);
-        sh.u.l.dc12[tid][nb] = (uint32_t)(uint16_t)(int16_t)dc[1] | ((uint32_t)(uint16_t)(int16_t)dc[2] << 16);
-      }
-      ++nb;
+// The D-state every hypothesis of the lane's tile agrees with most (ties: the
+// lowest hypothesis).
+CJD_HD inline uint64_t ent_consensus(const EntShared& sh, int tid, int nh) {
+  uint64_t best = sh.u.a.hyp[0][tid];
+  int nbest = 0;
+  for (int h = 0; h < nh; ++h) {
+    const uint64_t v = sh.u.a.hyp[h][tid];
+    int n = 0;
+    for (int g = 0; g < nh; ++g) n += sh.u.a.hyp[g][tid] == v ? 1 : 0;
+    if (n > nbest) {
+      nbest = n;
+      best = v;
     }
-    const int s0 = slot;
-    int d, wp, wv;
-    sym_step(br, sh, c.slotinfo, c.nh, slot, coef, d, wp, wv, err);
-    add_dc(dc, c.comp(s0), d);
   }
-  sh.exit_p[tid] = br.pos;
-  sh.exit_s[tid] = (uint32_t)slot;
-  sh.tot[tid][0] = nb;
-  sh.tot[tid][1] = dc[0];
-  sh.tot[tid][2] = dc[1];
-  sh.tot[tid][3] = dc[2];
-  sh.nlist[tid] = nb < kListLen ? (uint32_t)nb : (uint32_t)kListLen;
+  return best;
 }
 
-// Decode from state (p, slot) into local tile j until the trajectory lands on
-// a remembered block start of one of tile j's hypotheses (hit, list index) or
-// reaches the tile end.  On a hit, cnt gets the blocks / DC sums of the whole
-// tile j along the path and (p, slot) its exit; otherwise the tile was decoded
-// directly and (p, slot) is the concrete exit.  Returns the hypothesis hit
-// (-1 for none).
-CJD_HD inline int ent_resync(const EntShared& sh, const EntCtx& c, int j, uint32_t& p, int& slot, int32_t* cnt) {
-  const int nh = c.nh;
-  const uint32_t end = c.tile_end(c.tfirst + (uint32_t)j);
-  int32_t rb = 0, r[3] = {0, 0, 0};
-  uint64_t cur = 0;  // 4-bit list cursor per hypothesis (kept in a register)
-  int hit = -1, hit_i = 0;
-  const int base = j * nh;
+// Phase 1: main trajectory of local tile tid.  One loop, one symbol per
+// iteration: hypotheses 0..nh-1 run to the sync point in turn, then the main
+// trajectory runs from the consensus D-state to the tile end.  Tile 0 of a
+// segment starts from the known state (bit 0, slot 0): its main trajectory is
+// the true one from the start.
+CJD_HD inline void ent_phase1(EntShared& sh, int tid, const EntCtx& c) {
+  if (tid >= c.J) return;
+  const uint32_t t = c.tfirst + (uint32_t)tid;
+  const uint32_t b0 = t * c.S, end = c.tile_end(t), thr = c.sync_point(t);
+  BitReader br;
+  br.seek(c.w, c.nwords, b0);
+  int mode = t == 0 ? c.nh : 0;  // hypothesis being decoded; nh = main
+  int slot = 0, coef = -1, err = 0;
+  int32_t nb = 0, dc[3] = {0, 0, 0};
+  sh.u.a.dstate[tid] = ~0ull;
+  for (bool done = false; !done;) {
+    br.rotate();
+    for (int u = 0; u < kPeriod; ++u) {
+      if (coef < 0) {
+        const uint32_t p = br.pos;
+        if (mode < c.nh && p >= thr) {
+          sh.u.a.hyp[mode][tid] = pack_state(p, (uint32_t)slot);
+          if (++mode < c.nh) {
+            br.seek(c.w, c.nwords, b0);
+            slot = mode;
+          } else {
+            const uint64_t m = ent_consensus(sh, tid, c.nh);
+            sh.u.a.dstate[tid] = m;
+            br.seek(c.w, c.nwords, (uint32_t)(m >> 4));
+            slot = (int)(m & 15);
+          }
+          continue;
+        }
+        if (mode == c.nh && p >= end) {
+          done = true;
+          break;
+        }
+        if (mode == c.nh) ++nb;
+      }
+      const int s0 = slot;
+      int d, wp, wv;
+      sym_step(br, sh, c.slotinfo, c.nh, slot, coef, d, wp, wv, err);
+      if (mode == c.nh) add_dc(dc, c.comp(s0), d);
+    }
+  }
+  sh.ep[tid] = br.pos;
+  sh.es[tid] = (uint32_t)slot;
+  sh.u.a.mc[tid][0] = nb;
+  sh.u.a.mc[tid][1] = dc[0];
+  sh.u.a.mc[tid][2] = dc[1];
+  sh.u.a.mc[tid][3] = dc[2];
+}
+
+// Decode local tile j from the state (p, slot) at its entry: up to the first
+// boundary at or after the sync point; if that is the tile's D-state, the
+// rest is the main trajectory (its exit and counts are taken from phase 1),
+// otherwise the decode goes on to the tile end.  (p, slot) become the exit,
+// cnt the blocks / DC sums gained in tile j.
+CJD_HD inline void ent_relink(const EntShared& sh, const EntCtx& c, int j, uint32_t& p, int& slot, int32_t* cnt) {
+  const uint32_t t = c.tfirst + (uint32_t)j;
+  const uint32_t end = c.tile_end(t), thr = c.sync_point(t);
+  const uint64_t ds = sh.u.a.dstate[j];
+  int32_t nb = 0, dc[3] = {0, 0, 0};
+  bool checked = false;
   BitReader br;
   br.seek(c.w, c.nwords, p);
   int coef = -1, err = 0;
-  for (;;) {
-    if (coef >= 0) {  // inside a block: one more symbol
+  for (bool done = false; !done;) {
+    br.rotate();
+    for (int u = 0; u < kPeriod; ++u) {
+      if (coef < 0) {
+        const uint32_t q = br.pos;
+        if (!checked && q >= thr) {
+          checked = true;
+          if (pack_state(q, (uint32_t)slot) == ds) {
+            cnt[0] = nb + sh.u.a.mc[j][0];
+            cnt[1] = dc[0] + sh.u.a.mc[j][1];
+            cnt[2] = dc[1] + sh.u.a.mc[j][2];
+            cnt[3] = dc[2] + sh.u.a.mc[j][3];
+            p = sh.ep[j];
+            slot = (int)sh.es[j];
+            return;
+          }
+        }
+        if (q >= end) {
+          done = true;
+          break;
+        }
+        ++nb;
+      }
       const int s0 = slot;
       int d, wp, wv;
-      sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
-      add_dc(r, c.comp(s0), d);
-      continue;
+      sym_step(br, sh, c.slotinfo, c.nh, slot, coef, d, wp, wv, err);
+      add_dc(dc, c.comp(s0), d);
     }
-    p = br.pos;
-    for (int k = 0; k < nh && hit < 0; ++k) {
-      const int L = base + k;
-      const int n = (int)sh.nlist[L];
-      int q = (int)((cur >> (4 * k)) & 15);
-      while (q < n && sh.u.l.p[L][q] < p) ++q;
-      cur = (cur & ~(15ull << (4 * k))) | ((uint64_t)q << (4 * k));
-      if (q < n && sh.u.l.p[L][q] == p && (int)(sh.u.l.s_dc0[L][q] & 0xFF) == slot) {
-        hit = k;
-        hit_i = q;
-      }
-    }
-    if (hit >= 0 || p >= end) break;
-    ++rb;
-    const int s0 = slot;
-    int d, wp, wv;
-    sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
-    add_dc(r, c.comp(s0), d);
   }
-  cnt[0] = rb;
-  cnt[1] = r[0];
-  cnt[2] = r[1];
-  cnt[3] = r[2];
-  if (hit >= 0) {
-    const int L = base + hit;
-    const uint32_t sd = sh.u.l.s_dc0[L][hit_i], d12 = sh.u.l.dc12[L][hit_i];
-    cnt[0] += sh.tot[L][0] - hit_i;
-    cnt[1] += sh.tot[L][1] - (int32_t)(int16_t)(sd >> 16);
-    cnt[2] += sh.tot[L][2] - (int32_t)(int16_t)(d12 & 0xFFFF);
-    cnt[3] += sh.tot[L][3] - (int32_t)(int16_t)(d12 >> 16);
-    p = sh.exit_p[L];
-    slot = (int)sh.exit_s[L];
-    return hit;
-  }
-  // landed at or past the end of tile j: it may coincide with an exit
-  for (int k = 0; k < nh; ++k) {
-    const int L = base + k;
-    if (sh.exit_p[L] == p && (int)sh.exit_s[L] == slot) return k;
-  }
-  return -1;
+  p = br.pos;
+  cnt[0] = nb;
+  cnt[1] = dc[0];
+  cnt[2] = dc[1];
+  cnt[3] = dc[2];
 }
 
-// Phase 2: link the exit of (j, h) = lane tid into tile j+1.
+// Phase 2: link local tile tid (>= 1) from the main exit of tile tid-1, and
+// seed the chain with the assumed states.  fix is set when some link did not
+// join the main trajectory (its successor's assumed entry is then wrong).
 CJD_HD inline void ent_phase2(EntShared& sh, int tid, const EntCtx& c) {
-  const int j = tid / c.nh;
-  if (j + 1 >= c.J) return;
-  uint32_t p = sh.exit_p[tid];
-  int slot = (int)sh.exit_s[tid];
+  if (tid >= c.J) return;
+  if (tid == 0) {
+    sh.tp[0] = sh.ep[0];
+    sh.ts[0] = sh.es[0];
+    sh.up[0] = 0;
+    sh.us[0] = 0;
+    for (int q = 0; q < 4; ++q) sh.tc[0][q] = c.lead ? 0 : sh.u.a.mc[0][q];
+    return;
+  }
+  uint32_t p = sh.ep[tid - 1];
+  int slot = (int)sh.es[tid - 1];
   int32_t cnt[4];
-  const int hit = ent_resync(sh, c, j + 1, p, slot, cnt);
-  sh.lk_h[tid] = hit;
-  sh.lk_p[tid] = p;
-  sh.lk_s[tid] = (uint32_t)slot;
-  for (int q = 0; q < 4; ++q) sh.lk_c[tid][q] = cnt[q];
+  sh.up[tid] = p;
+  sh.us[tid] = (uint32_t)slot;
+  ent_relink(sh, c, tid, p, slot, cnt);
+  sh.tp[tid] = p;
+  sh.ts[tid] = (uint32_t)slot;
+  for (int q = 0; q < 4; ++q) sh.tc[tid][q] = cnt[q];
+  if (p != sh.ep[tid] || (uint32_t)slot != sh.es[tid]) sh.fix = 1;
 }
 
-// Phase 3 init: F_j = link from tile j-1 (identity for j = 0) into buffer 1.
-CJD_HD inline void ent_phase3_init(EntShared& sh, int tid, const EntCtx& c) {
-  const int j = tid / c.nh, h = tid - j * c.nh;
-  if (j >= c.J) return;
-  if (j == 0) {
-    sh.mh[1][tid] = h;
-    for (int q = 0; q < 4; ++q) sh.mc[1][tid][q] = 0;
-  } else {
-    const int L = (j - 1) * c.nh + h;
-    sh.mh[1][tid] = sh.lk_h[L];
-    for (int q = 0; q < 4; ++q) sh.mc[1][tid][q] = sh.lk_c[L][q];
+// Sequential fix-up (one lane): every tile whose exit was computed from an
+// entry other than its predecessor's exit in the chain is decoded again from
+// that exit.  Runs over the whole item; most tiles only compare.
+CJD_HD inline void ent_fix_chain(EntShared& sh, const EntCtx& c) {
+  for (int j = 1; j < c.J; ++j) {
+    if (sh.up[j] == sh.tp[j - 1] && sh.us[j] == sh.ts[j - 1]) continue;
+    uint32_t p = sh.tp[j - 1];
+    int slot = (int)sh.ts[j - 1];
+    int32_t cnt[4];
+    sh.up[j] = p;
+    sh.us[j] = (uint32_t)slot;
+    ent_relink(sh, c, j, p, slot, cnt);
+    sh.tp[j] = p;
+    sh.ts[j] = (uint32_t)slot;
+    for (int q = 0; q < 4; ++q) sh.tc[j][q] = cnt[q];
   }
 }
 
-// Phase 3 step (pointer jumping): G_j <- G_j o G_{j-d}, from buffer `buf`
-// into buffer buf^1.  G_j(h) = state in local tile j reached from hypothesis h
-// of local tile 0.
-CJD_HD inline void ent_phase3_step(EntShared& sh, int tid, const EntCtx& c, int d, int buf) {
-  const int j = tid / c.nh, h = tid - j * c.nh;
-  if (j >= c.J) return;
-  const int nb = buf ^ 1;
-  if (j >= d) {
-    const int L0 = (j - d) * c.nh + h;
-    const int h1 = sh.mh[buf][L0];
-    if (h1 < 0) {
-      sh.mh[nb][tid] = -1;
-      for (int q = 0; q < 4; ++q) sh.mc[nb][tid][q] = 0;
-    } else {
-      const int L1 = j * c.nh + h1;
-      sh.mh[nb][tid] = sh.mh[buf][L1];
-      for (int q = 0; q < 4; ++q) sh.mc[nb][tid][q] = sh.mc[buf][L0][q] + sh.mc[buf][L1][q];
-    }
-  } else {
-    sh.mh[nb][tid] = sh.mh[buf][tid];
-    for (int q = 0; q < 4; ++q) sh.mc[nb][tid][q] = sh.mc[buf][tid][q];
-  }
+// Phase 3 (all lanes, kWG-lane Hillis-Steele scan in LDS, with barriers
+// supplied by the caller between steps): tc[j] <- tc[0] + ... + tc[j].
+// Step with distance d reads the values of the previous step: lanes first
+// read, then (after a barrier) write.
+CJD_HD inline void ent_scan_read(const EntShared& sh, int tid, int d, int32_t* v) {
+  for (int q = 0; q < 4; ++q) v[q] = tid >= d ? sh.tc[tid - d][q] : 0;
+}
+CJD_HD inline void ent_scan_write(EntShared& sh, int tid, const int32_t* v) {
+  for (int q = 0; q < 4; ++q) sh.tc[tid][q] += v[q];
 }
 
-// Phase 4a: publish this item's aggregate (the map from local tile 0's
-// hypotheses to the last tile).  For the first item of a segment the state at
-// tile 0 is known (hypothesis 0 decodes from the segment start): it is set and
-// true is returned.
-CJD_HD inline bool ent_publish_aggregate(EntShared& sh, const EntCtx& c, LookbackRec* me) {
-  const int Lq = (c.J - 1) * c.nh;
-  const int buf = sh.buf;
-  if (!c.lead) {
-    sh.entry_h = 0;
-    sh.entry_p = sh.exit_p[0];
-    sh.entry_s = sh.exit_s[0];
-    for (int q = 0; q < 4; ++q) sh.entry_c[q] = sh.tot[0][q];
-    return true;
-  }
-  for (int k = 0; k < c.nh; ++k) {
-    me->agg_h[k] = (int8_t)sh.mh[buf][Lq + k];
-    for (int q = 0; q < 4; ++q) me->agg_c[k][q] = sh.mc[buf][Lq + k][q];
-  }
-  return false;
+// Serial form of the scan (one lane), used after a late fix-up.
+CJD_HD inline void ent_scan_serial(EntShared& sh, const EntCtx& c) {
+  for (int j = 1; j < c.J; ++j)
+    for (int q = 0; q < 4; ++q) sh.tc[j][q] += sh.tc[j - 1][q];
 }
 
-// Composition accumulator of the lookback: acc = agg_{w-1} o ... o agg_q.
+// Composition of predecessors' aggregates in the lookback.  An aggregate says
+// "if the item's entry is (lead_p, lead_s), its exit is (agg_p, agg_s) and it
+// adds agg_c"; absorbing an earlier item checks that its aggregate exit is
+// the later item's assumed entry.
 struct LookAcc {
-  int h[kMaxSlots];
-  int32_t c[kMaxSlots][4];
-  bool ident;
-  CJD_HD void identity(int nh) {
-    ident = true;
-    for (int k = 0; k < nh; ++k) {
-      h[k] = k;
-      for (int q = 0; q < 4; ++q) c[k][q] = 0;
-    }
+  uint32_t lp, ls;     // assumed entry of the earliest absorbed item
+  uint32_t xp, xs;     // exit of the latest
+  int32_t c[4];
+  bool empty, valid;
+  CJD_HD void reset() {
+    empty = true;
+    valid = true;
+    for (int q = 0; q < 4; ++q) c[q] = 0;
   }
-  // acc <- acc o agg(r)
-  CJD_HD void absorb(const LookbackRec& r, int nh) {
-    ident = false;
-    int nh_[kMaxSlots];
-    int32_t nc[kMaxSlots][4];
-    for (int k = 0; k < nh; ++k) {
-      const int h1 = r.agg_h[k];
-      if (h1 < 0 || h[h1] < 0) {
-        nh_[k] = -1;
-        for (int q = 0; q < 4; ++q) nc[k][q] = 0;
-      } else {
-        nh_[k] = h[h1];
-        for (int q = 0; q < 4; ++q) nc[k][q] = r.agg_c[k][q] + c[h1][q];
-      }
+  CJD_HD void absorb(const LookbackRec& r) {
+    if (empty) {
+      xp = r.agg_p;
+      xs = r.agg_s;
+      empty = false;
+    } else {
+      valid = valid && r.agg_p == lp && r.agg_s == ls;
     }
-    for (int k = 0; k < nh; ++k) {
-      h[k] = nh_[k];
-      for (int q = 0; q < 4; ++q) c[k][q] = nc[k][q];
-    }
+    lp = r.lead_p;
+    ls = r.lead_s;
+    for (int q = 0; q < 4; ++q) c[q] += r.agg_c[q];
   }
-  // Entry state (at the exit of local tile 0) from a predecessor's inclusive
-  // state; false when it cannot be derived through the composed aggregates.
-  CJD_HD bool apply(const LookbackRec& r, const EntShared& sh, int& eh, uint32_t& ep, uint32_t& es,
-                    int32_t* ec) const {
-    if (ident) {
-      eh = r.incl_h;
-      ep = r.incl_p;
-      es = r.incl_s;
-      for (int q = 0; q < 4; ++q) ec[q] = r.incl_c[q];
+  // The exit of the latest absorbed item (or of r itself when none) from r's
+  // inclusive state; false when an assumption on the way does not hold.
+  CJD_HD bool apply(const LookbackRec& r, uint32_t& p, uint32_t& s, int32_t* cc) const {
+    if (empty) {
+      p = r.incl_p;
+      s = r.incl_s;
+      for (int q = 0; q < 4; ++q) cc[q] = r.incl_c[q];
       return true;
     }
-    const int hb = r.incl_h;
-    if (hb < 0 || h[hb] < 0) return false;
-    eh = h[hb];
-    ep = sh.exit_p[eh];  // local tile 0, hypothesis eh
-    es = sh.exit_s[eh];
-    for (int q = 0; q < 4; ++q) ec[q] = r.incl_c[q] + c[hb][q];
+    if (!valid || r.incl_p != lp || r.incl_s != ls) return false;
+    p = xp;
+    s = xs;
+    for (int q = 0; q < 4; ++q) cc[q] = r.incl_c[q] + c[q];
     return true;
   }
 };
 
-// Phase 4b (all lanes): true exit state of local tile j from the prefix
-// compositions; -2 where a link on the path did not resolve.
-CJD_HD inline void ent_states(EntShared& sh, int tid, const EntCtx& c) {
-  const int j = tid;
-  if (j >= c.J) return;
-  const int eh = sh.entry_h;
-  if (j == 0) {
-    sh.ex_h[0] = eh;
-    sh.ex_p[0] = sh.entry_p;
-    sh.ex_s[0] = sh.entry_s;
-    for (int q = 0; q < 4; ++q) sh.ex_c[0][q] = sh.entry_c[q];
-    return;
-  }
-  const int buf = sh.buf;
-  if (eh >= 0) {
-    const int L = j * c.nh + eh;
-    const int hj = sh.mh[buf][L];
-    if (hj >= 0) {
-      sh.ex_h[j] = hj;
-      sh.ex_p[j] = sh.exit_p[j * c.nh + hj];
-      sh.ex_s[j] = sh.exit_s[j * c.nh + hj];
-      for (int q = 0; q < 4; ++q) sh.ex_c[j][q] = sh.entry_c[q] + sh.mc[buf][L][q];
-      return;
-    }
-  }
-  sh.ex_h[j] = -2;
-  sh.need_walk = 1;
+// Aggregate of this item (tc already scanned: relative cumulative counts).
+CJD_HD inline void ent_aggregate(const EntShared& sh, const EntCtx& c, LookbackRec* me) {
+  me->lead_p = sh.ep[0];
+  me->lead_s = sh.es[0];
+  me->agg_p = sh.tp[c.J - 1];
+  me->agg_s = sh.ts[c.J - 1];
+  for (int q = 0; q < 4; ++q) me->agg_c[q] = sh.tc[c.J - 1][q];
 }
 
-// Phase 4c (one lane): fill the states the compositions could not give by
-// walking the tiles in order: a link where one exists, else the concrete exit
-// recorded by the link, else a resync decode from a concrete state.  Once a
-// composition fails every later one fails too (-1 propagates), so the walk
-// runs from the first unknown tile to the last.
-CJD_HD inline void ent_walk(EntShared& sh, const EntCtx& c) {
-  for (int j = 1; j < c.J; ++j) {
-    if (sh.ex_h[j] != -2) continue;
-    const int hp = sh.ex_h[j - 1];
-    int32_t cnt[4];
-    if (hp >= 0) {
-      const int L = (j - 1) * c.nh + hp;
-      sh.ex_h[j] = sh.lk_h[L];
-      sh.ex_p[j] = sh.lk_p[L];
-      sh.ex_s[j] = sh.lk_s[L];
-      for (int q = 0; q < 4; ++q) cnt[q] = sh.lk_c[L][q];
-    } else {
-      uint32_t p = sh.ex_p[j - 1];
-      int slot = (int)sh.ex_s[j - 1];
-      sh.ex_h[j] = ent_resync(sh, c, j, p, slot, cnt);
-      sh.ex_p[j] = p;
-      sh.ex_s[j] = (uint32_t)slot;
-    }
-    for (int q = 0; q < 4; ++q) sh.ex_c[j][q] = sh.ex_c[j - 1][q] + cnt[q];
-  }
-}
-
-// Phase 4d: the inclusive state of this item (exit of its last tile).
+// Inclusive state of this item: the true exit of its last tile and the
+// segment's cumulative counts there (tc absolute).
 CJD_HD inline void ent_inclusive(const EntShared& sh, const EntCtx& c, LookbackRec* me) {
-  const int j = c.J - 1;
-  me->incl_h = sh.ex_h[j];
-  me->incl_p = sh.ex_p[j];
-  me->incl_s = sh.ex_s[j];
-  for (int q = 0; q < 4; ++q) me->incl_c[q] = sh.ex_c[j][q];
+  me->incl_p = sh.tp[c.J - 1];
+  me->incl_s = sh.ts[c.J - 1];
+  for (int q = 0; q < 4; ++q) me->incl_c[q] = sh.tc[c.J - 1][q];
+}
+
+// After the lookback (one lane): the true entry differs from the assumed one
+// -> fix the chain from it and redo the scan serially.  Then every lane adds
+// the entry's cumulative counts (ent_absolute).
+CJD_HD inline void ent_apply_entry(EntShared& sh, const EntCtx& c) {
+  if (sh.entry_p == sh.ep[0] && sh.entry_s == sh.es[0]) return;
+  // undo the scan: per-tile counts again
+  for (int j = c.J - 1; j >= 1; --j)
+    for (int q = 0; q < 4; ++q) sh.tc[j][q] -= sh.tc[j - 1][q];
+  sh.tp[0] = sh.entry_p;
+  sh.ts[0] = sh.entry_s;
+  ent_fix_chain(sh, c);
+  ent_scan_serial(sh, c);
+}
+CJD_HD inline void ent_absolute(EntShared& sh, int tid, const EntCtx& c) {
+  if (tid >= c.J || !c.lead) return;
+  for (int q = 0; q < 4; ++q) sh.tc[tid][q] += sh.entry_c[q];
 }
 
 // Phase 5: final decode of local tile tid + lead from its true entry.
@@ -618,16 +619,18 @@ CJD_HD inline void ent_phase5(EntShared& sh, int tid, const EntCtx& c, int16_t* 
   uint32_t p;
   int slot;
   int32_t cp[4];
-  if (!c.lead && fj == 0) {
+  if (fj == 0) {
     p = 0;
     slot = 0;
     for (int q = 0; q < 4; ++q) cp[q] = 0;
   } else {
-    p = sh.ex_p[fj - 1];
-    slot = (int)sh.ex_s[fj - 1];
-    for (int q = 0; q < 4; ++q) cp[q] = sh.ex_c[fj - 1][q];
+    p = sh.tp[fj - 1];
+    slot = (int)sh.ts[fj - 1];
+    for (int q = 0; q < 4; ++q) cp[q] = sh.tc[fj - 1][q];
   }
-  const int32_t* cx = sh.ex_c[fj];
+  const uint32_t xp = sh.tp[fj], xs = sh.ts[fj];
+  int32_t cx[4];
+  for (int q = 0; q < 4; ++q) cx[q] = sh.tc[fj][q];
   const uint32_t end = c.tile_end(t);
   int64_t bidx = cp[0];
   int dc0 = cp[1], dc1 = cp[2], dc2 = cp[3];
@@ -637,35 +640,44 @@ CJD_HD inline void ent_phase5(EntShared& sh, int tid, const EntCtx& c, int16_t* 
   BitReader br;
   br.seek(c.w, c.nwords, p);
   int coef = -1;
-  for (;;) {
-    if (coef < 0) {
-      if (br.pos >= end) break;
-      if (bidx >= (int64_t)c.sg.nblocks) {
-        stopped = true;
+  for (bool done = false; !done;) {
+    br.rotate();
+    for (int u = 0; u < kPeriod; ++u) {
+      if (coef < 0) {
+        if (br.pos >= end) {
+          done = true;
+          break;
+        }
+        if (bidx >= (int64_t)c.sg.nblocks) {
+          stopped = true;
+          done = true;
+          break;
+        }
+        __builtin_memset(blk, 0, 128);
+      }
+      const int s0 = slot, comp = c.comp(s0);
+      int d, wp, wv;
+      const bool ended = sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
+      if (err) {
+        done = true;
         break;
       }
-      __builtin_memset(blk, 0, 128);
-    }
-    const int s0 = slot, comp = c.comp(s0);
-    int d, wp, wv;
-    const bool ended = sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
-    if (err) break;
-    dc0 += comp == 0 ? d : 0;
-    dc1 += comp == 1 ? d : 0;
-    dc2 += comp == 2 ? d : 0;
-    if (wp >= 0) blk[wp] = (int16_t)wv;
-    if (ended) {
-      blk[0] = (int16_t)(comp == 0 ? dc0 : comp == 1 ? dc1 : dc2);
-      __builtin_memcpy(coefs + (c.sg.first_block + bidx) * 64, blk, 128);
-      ++bidx;
+      dc0 += comp == 0 ? d : 0;
+      dc1 += comp == 1 ? d : 0;
+      dc2 += comp == 2 ? d : 0;
+      if (wp >= 0) blk[wp] = (int16_t)wv;
+      if (ended) {
+        blk[0] = (int16_t)(comp == 0 ? dc0 : comp == 1 ? dc1 : dc2);
+        __builtin_memcpy(coefs + (c.sg.first_block + bidx) * 64, blk, 128);
+        ++bidx;
+      }
     }
   }
   p = br.pos;
   if (err) {
     status_or(st, kStatusSyntax);
   } else if (!stopped) {
-    if (p != sh.ex_p[fj] || (uint32_t)slot != sh.ex_s[fj] || bidx != (int64_t)cx[0] || dc0 != cx[1] ||
-        dc1 != cx[2] || dc2 != cx[3])
+    if (p != xp || (uint32_t)slot != xs || bidx != (int64_t)cx[0] || dc0 != cx[1] || dc1 != cx[2] || dc2 != cx[3])
       status_or(st, kStatusInconsistent);
     if (t + 1 == c.sg.ntiles && bidx < (int64_t)c.sg.nblocks) status_or(st, kStatusTruncated);
   }

@@ -165,7 +165,8 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
 
   // segments, tiles and entropy work items
   const int nh = f.blocks_per_mcu;
-  const uint32_t per_first = (uint32_t)(kWG / nh), per_next = (uint32_t)(kWG / nh) - 1;
+  // one lane per tile; an item after the first also decodes its lead-in tile
+  const uint32_t per_first = b->item_tiles, per_next = b->item_tiles - 1;
   r.seg_first = (uint32_t)b->segs.size();
   for (size_t s = 0; s < parts.size(); ++s) {
     const std::vector<uint8_t>& data = parts[s];
@@ -184,6 +185,7 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
     b->scan_bytes += (int64_t)data.size();
     sd.image = r.desc;
     sd.tile_bits = b->tile_bits;
+    sd.sync_bits = b->sync_bits ? b->sync_bits : b->tile_bits / 4;
     sd.ntiles = std::max<uint32_t>(1, (sd.nbits + b->tile_bits - 1) / b->tile_bits);
     const int64_t mcu0 = (int64_t)s * (f.restart_interval > 0 ? f.restart_interval : nmcu);
     const int64_t mcus = std::min<int64_t>(nmcu - mcu0, f.restart_interval > 0 ? f.restart_interval : nmcu);

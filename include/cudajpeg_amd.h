@@ -85,9 +85,17 @@ void cjd_context_destroy(cjd_context *ctx);
 int cjd_batch_create(cjd_context *ctx, cjd_batch **out);
 void cjd_batch_destroy(cjd_batch *b);
 
-/* Options: iDCT variant, entropy decoder, tile length in bits (>= 2048,
- * 0 = default), whether planar component planes are also written. */
+/* Options: iDCT variant, entropy decoder, tile length in bits (>= 512, a
+ * multiple of 32, 0 = default 4096), whether planar component planes are
+ * also written. */
 int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int tile_bits, int want_planes);
+
+/* Tiled entropy decoder tuning (no counterpart in the reference, whose scan
+ * decoder has no tiles): sync point in bits into a tile (a multiple of 32,
+ * 0 = tile_bits / 4) and tiles per work item (2 .. 256, 0 = 256).  Smaller
+ * items spread one image over more workgroups (latency); larger ones amortise
+ * the lookback (throughput).  Takes effect at the next cjd_batch_stage. */
+int cjd_batch_set_tuning(cjd_batch *b, int sync_bits, int item_tiles);
 
 /* Parse, unstuff (src/decodeScanGPU.cu:11-24) and upload n files; plans the
  * entropy and pixel work lists.  errors[i] gets each file's parse result; a

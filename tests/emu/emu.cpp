@@ -26,7 +26,7 @@ struct Emu {
   std::vector<int16_t> coefs;
   std::vector<uint8_t> out;
   std::vector<uint32_t> status;
-  int unresolved_items = 0;
+  int fixes = 0;
   int fallbacks = 0;
 };
 
@@ -36,10 +36,10 @@ struct EntRun {
   std::vector<std::unique_ptr<EntShared>> shs;
   std::vector<EntCtx> ctx;
   std::vector<char> done;
-  int walks = 0;
+  int fixes = 0;  // items that needed the sequential fix-up (a link or the entry off the main chain)
 };
 
-// Lookback, state resolution and inclusive publication of item i, resolving
+// Lookback, entry application and inclusive publication of item i, resolving
 // predecessors first only when the kernel would have waited for them.
 void resolve(EntRun& R, size_t i) {
   if (R.done[i]) return;
@@ -47,36 +47,30 @@ void resolve(EntRun& R, size_t i) {
   const EntCtx& c = R.ctx[i];
   if (c.lead) {
     LookAcc acc;
-    acc.identity(c.nh);
-    uint32_t q = c.it.prev_item;
-    int eh = -1;
-    uint32_t ep = 0, es = 0;
-    int32_t ec[4] = {0, 0, 0, 0};
+    acc.reset();
+    uint32_t q = c.it.prev_item, p = 0, s = 0;
+    int32_t cc[4] = {0, 0, 0, 0};
     for (;;) {
       if (R.lb[q].status == 0) resolve(R, q);  // the kernel spins until it is published
       const LookbackRec& r = R.lb[q];
       if (r.status == 2) {
-        if (!acc.apply(r, sh, eh, ep, es, ec)) {
+        if (!acc.apply(r, p, s, cc)) {
           resolve(R, c.it.prev_item);
-          acc.identity(c.nh);
-          acc.apply(R.lb[c.it.prev_item], sh, eh, ep, es, ec);
+          acc.reset();
+          acc.apply(R.lb[c.it.prev_item], p, s, cc);
         }
         break;
       }
-      acc.absorb(r, c.nh);
+      acc.absorb(r);
       q = R.p->items[q].prev_item;
     }
-    sh.entry_h = eh;
-    sh.entry_p = ep;
-    sh.entry_s = es;
-    for (int k = 0; k < 4; ++k) sh.entry_c[k] = ec[k];
+    sh.entry_p = p;
+    sh.entry_s = s;
+    for (int k = 0; k < 4; ++k) sh.entry_c[k] = cc[k];
+    if (p != sh.ep[0] || s != sh.es[0]) ++R.fixes;
+    ent_apply_entry(sh, c);
   }
-  sh.need_walk = 0;
-  for (int t = 0; t < kWG; ++t) ent_states(sh, t, c);
-  if (sh.need_walk) {
-    ent_walk(sh, c);
-    ++R.walks;
-  }
+  for (int t = 0; t < kWG; ++t) ent_absolute(sh, t, c);
   ent_inclusive(sh, c, &R.lb[i]);
   R.lb[i].status = 2;
   R.done[i] = 1;
@@ -103,24 +97,31 @@ void run_entropy(Emu& E, int order) {
     memcpy(sh.dz, kDeZigZag, 64);
     for (int t = 0; t < kWG; ++t) ent_phase1(sh, t, c);
     for (int t = 0; t < kWG; ++t) ent_phase2(sh, t, c);
-    for (int t = 0; t < kWG; ++t) ent_phase3_init(sh, t, c);
-    int buf = 1;
-    for (int d = 1; d < c.J; d <<= 1) {
-      for (int t = 0; t < kWG; ++t) ent_phase3_step(sh, t, c, d, buf);
-      buf ^= 1;
+    if (sh.fix) {
+      ent_fix_chain(sh, c);
+      ++R.fixes;
     }
-    sh.buf = buf;
-    if (!ent_publish_aggregate(sh, c, &R.lb[i])) R.lb[i].status = 1;
+    for (int d = 1; d < c.J; d <<= 1) {  // each step: all lanes read, then all write
+      std::vector<int32_t> v((size_t)kWG * 4);
+      for (int t = 0; t < kWG; ++t) ent_scan_read(sh, t, d, &v[(size_t)t * 4]);
+      for (int t = 0; t < c.J; ++t) ent_scan_write(sh, t, &v[(size_t)t * 4]);
+    }
+    if (c.lead) {
+      ent_aggregate(sh, c, &R.lb[i]);
+      R.lb[i].status = 1;
+    }
   }
   // segment-first items resolve from the known start; then the rest in
   // order (predecessors inclusive) or backwards (predecessors aggregate-only)
   for (size_t i = 0; i < ni; ++i)
     if (!R.ctx[i].lead) resolve(R, i);
   for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
-  E.unresolved_items = R.walks;
+  E.fixes = R.fixes;
   for (size_t i = 0; i < ni; ++i)
     for (int t = 0; t < kWG; ++t) ent_phase5(*R.shs[i], t, R.ctx[i], E.coefs.data(), E.status.data());
 }
+
+void run_pixel(Emu& E, int mode, size_t first, size_t count);
 
 template <class SH>
 void pixel_item(Emu& E, SH& sh, int mode, const PixCtx& c) {
@@ -158,9 +159,11 @@ void run_pixel(Emu& E, int mode, size_t first, size_t count) {
 extern "C" {
 
 void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_mode, int tile_bits,
-              int want_planes, int order, int serial, int* errors) {
+              int want_planes, int order, int serial, int item_tiles, int sync_bits, int* errors) {
   Emu* E = new Emu;
   E->plan.tile_bits = tile_bits ? (uint32_t)tile_bits : 4096u;
+  E->plan.sync_bits = (uint32_t)sync_bits;
+  if (item_tiles) E->plan.item_tiles = (uint32_t)item_tiles;
   E->plan.want_planes = want_planes;
   plan_batch(E->plan, n, files, sizes, errors);
   E->coefs.assign((size_t)E->plan.total_blocks * 64, (int16_t)0x5555);
@@ -214,11 +217,11 @@ long emu_get(void* h, int i, int what, void* dst, size_t cap) {
   return (long)n;
 }
 
-// [items, items that needed a serial walk, fallbacks, segments]
+// [items, items that needed the sequential fix-up, fallbacks, segments]
 void emu_stats(void* h, long* out4) {
   Emu* E = static_cast<Emu*>(h);
   out4[0] = (long)E->plan.items.size();
-  out4[1] = E->unresolved_items;
+  out4[1] = E->fixes;
   out4[2] = E->fallbacks;
   out4[3] = (long)E->plan.segs.size();
 }

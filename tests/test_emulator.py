@@ -73,6 +73,28 @@ def test_emulated_large_high_entropy():
     assert np.array_equal(r.get(0, 1, (p.num_blocks, 64), np.int16), e.coefs)
 
 
+HARD = [synth.ImageSpec(1200, 900, "420", 63, 8.9, 5),
+        synth.ImageSpec(640, 480, "444", 95, 12.0, 15, random_tile_frac=0.5),
+        synth.ImageSpec(800, 600, "gray", 90, 12.0, 3),
+        synth.ImageSpec(1000, 700, "422", 80, 6.0, 9, restart_interval=7)]
+
+
+@pytest.mark.parametrize("item_tiles,tile_bits,sync_bits", [(2, 2048, 0), (3, 4096, 0), (8, 4096, 512),
+                                                            (8, 8192, 0)])
+@pytest.mark.parametrize("order", [0, 1])
+def test_emulated_small_items_chain_fixups(item_tiles, tile_bits, sync_bits, order):
+    # Items of 2-8 tiles: many items per segment, so the lookback composes
+    # chains of aggregates (order 1: every predecessor has only its
+    # aggregate), and links / entries that miss the main trajectory take the
+    # sequential fix-up.  Both must occur here and the result stay bit-exact.
+    encs = [synth.encode(s, truth=True) for s in HARD]
+    r = EmuResult([e.data for e in encs], 0, tile_bits, False, order, False, item_tiles, sync_bits)
+    assert not r.errors.any() and r.fallbacks == 0
+    assert r.fixes > 0 and r.items > 4 * len(HARD)
+    for i, (s, e) in enumerate(zip(HARD, encs)):
+        assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
+
+
 def test_emulated_corrupt_scan_reports_syntax():
     enc = synth.encode(synth.ImageSpec(64, 64, "420", 75, 6.0, 21))
     data = bytearray(enc.data)
