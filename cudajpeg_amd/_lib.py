@@ -68,7 +68,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_create": ([vp, ctypes.POINTER(vp)], i32),
         "cjd_batch_destroy": ([vp], None),
         "cjd_batch_set_options": ([vp, i32, i32, i32, i32], i32),
-        "cjd_batch_set_tuning": ([vp, i32, i32], i32),
+        "cjd_batch_set_tuning": ([vp, i32, i32, i32], i32),
         "cjd_batch_stage": ([vp, i32, vp, vp, vp], i32),
         "cjd_batch_decode": ([vp, vp], i32),
         "cjd_batch_decode_scan": ([vp, vp], i32),

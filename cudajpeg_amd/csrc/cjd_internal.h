@@ -65,7 +65,9 @@ struct SegmentDesc {
   uint32_t nblocks;     // blocks the segment must contain
   int64_t first_block;  // global coefficient-buffer index of its first block
   uint32_t first_item;  // index of its first work item
-  uint32_t sync_bits;   // sync point D: bits into a tile where hypotheses are compared
+  uint32_t sync1_bits;  // first sync point D1: bits into a tile where hypotheses are compared
+  uint32_t sync_bits;   // second sync point D (used when they disagree at D1)
+  uint32_t pad_;
 };
 
 // One entropy workgroup: tiles [t0, t0 + ntiles) of a segment.  When t0 > 0 the

@@ -117,10 +117,11 @@ extern "C" int cjd_batch_set_options(cjd_batch* b, int idct_mode, int entropy_mo
   return 0;
 }
 
-extern "C" int cjd_batch_set_tuning(cjd_batch* b, int sync_bits, int item_tiles) {
+extern "C" int cjd_batch_set_tuning(cjd_batch* b, int sync1_bits, int sync_bits, int item_tiles) {
   if (!b) return CJD_PROGRAMMER_ERROR;
-  if (sync_bits < 0 || (sync_bits & 31) || item_tiles < 0 || item_tiles == 1 || item_tiles > kWG)
+  if (sync1_bits < 0 || sync_bits < 0 || item_tiles < 0 || item_tiles == 1 || item_tiles > kWG)
     return CJD_PROGRAMMER_ERROR;
+  b->plan.sync1_bits = (uint32_t)sync1_bits;
   b->plan.sync_bits = (uint32_t)sync_bits;
   b->plan.item_tiles = item_tiles ? (uint32_t)item_tiles : (uint32_t)kWG;
   return 0;

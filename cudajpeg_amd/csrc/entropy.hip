@@ -25,6 +25,38 @@ __global__ void k_clear_u32(uint32_t* __restrict__ p, uint64_t n) {
     p[i] = 0;
 }
 
+// Fix-up rounds (all lanes): relink every tile whose entry changed, until no
+// tile changes.
+__device__ __forceinline__ void fix_rounds(EntShared& sh, int tid, const EntCtx& c) {
+  for (;;) {
+    uint32_t p = 0, s = 0;
+    const bool need = ent_fix_need(sh, tid, c, p, s);
+    if (tid == 0) sh.nchg = 0;
+    __syncthreads();  // reads of the previous state done, counter reset
+    if (need) {
+      ent_fix_apply(sh, tid, c, p, s);
+      atomicAdd(&sh.nchg, 1);
+    }
+    __syncthreads();
+    const int n = sh.nchg;
+    __syncthreads();  // every lane has read the counter before the next reset
+    if (!n) break;
+  }
+}
+
+// Inclusive prefix sum of the per-tile counts into tc (all lanes).
+__device__ __forceinline__ void scan_counts(EntShared& sh, int tid, const EntCtx& c) {
+  ent_scan_init(sh, tid, c);
+  __syncthreads();
+  for (int d = 1; d < c.J; d <<= 1) {
+    int32_t v[4];
+    ent_scan_read(sh, tid, d, v);
+    __syncthreads();
+    if (tid < c.J) ent_scan_write(sh, tid, v);
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ words,
                                                  const ImageDesc* __restrict__ images,
                                                  const SegmentDesc* __restrict__ segs,
@@ -36,10 +68,7 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   const int tid = threadIdx.x;
   // Work items are taken in ticket order, so every predecessor an item waits
   // for in the lookback has already started.
-  if (tid == 0) {
-    sh.item = atomicAdd(ticket, 1u);
-    sh.fix = 0;
-  }
+  if (tid == 0) sh.item = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t item_id = sh.item;
   if (item_id >= nitems) return;
@@ -56,17 +85,8 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   __syncthreads();
   ent_phase2(sh, tid, c);
   __syncthreads();
-  if (sh.fix) {
-    if (tid == 0) ent_fix_chain(sh, c);
-    __syncthreads();
-  }
-  for (int d = 1; d < c.J; d <<= 1) {
-    int32_t v[4];
-    ent_scan_read(sh, tid, d, v);
-    __syncthreads();
-    if (tid < c.J) ent_scan_write(sh, tid, v);
-    __syncthreads();
-  }
+  fix_rounds(sh, tid, c);
+  scan_counts(sh, tid, c);
   LookbackRec* me = lb + item_id;
   if (tid == 0 && c.lead) {
     ent_aggregate(sh, c, me);
@@ -98,9 +118,17 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
     sh.entry_p = p;
     sh.entry_s = s;
     for (int q = 0; q < 4; ++q) sh.entry_c[q] = cc[q];
-    ent_apply_entry(sh, c);
   }
   __syncthreads();
+  if (ent_entry_differs(sh, c)) {  // uniform: the chain is redone from the true entry
+    if (tid == 0) {
+      sh.tp[0] = sh.entry_p;
+      sh.ts[0] = sh.entry_s;
+    }
+    __syncthreads();
+    fix_rounds(sh, tid, c);
+    scan_counts(sh, tid, c);
+  }
   ent_absolute(sh, tid, c);
   __syncthreads();
   if (tid == 0) {

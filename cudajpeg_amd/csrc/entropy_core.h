@@ -254,19 +254,21 @@ struct EntShared {
   uint8_t dz[64];                 // zigzag index -> natural index
   uint32_t ep[kWG], es[kWG];      // main exit of each local tile (bit, slot)
   uint32_t tp[kWG], ts[kWG];      // chain: exit of tile j (assumed, then true)
-  uint32_t up[kWG], us[kWG];      // chain: entry tile j's exit was computed from
-  int32_t tc[kWG][4];             // chain: blocks / DC sums gained in tile j, then
-                                  //   cumulative (relative to the item entry, then absolute)
+  int32_t tc[kWG][4];             // chain: cumulative blocks / DC sums at the exit of tile j
+                                  //   (relative to the item entry, then absolute)
   union {
     int16_t coef[kWG][64];        // final decode: one block per lane
     struct {
-      uint64_t hyp[kMaxSlots][kWG];  // D-state reached by each hypothesis
-      uint64_t dstate[kWG];          // the tile's D-state (main), ~0 for none
+      uint64_t hyp[kMaxSlots][kWG];  // hypothesis states at a sync point
+      uint64_t dstate[kWG];          // the tile's D-state (main trajectory)
+      uint32_t dthr[kWG];            // the sync point it refers to
+      uint32_t up[kWG], us[kWG];     // chain: entry tile j's exit was computed from
       int32_t mc[kWG][4];            // main counts from the D-state to the exit
+      int32_t cnt[kWG][4];           // chain: blocks / DC sums gained in tile j
     } a;
   } u;
   uint32_t item;
-  int32_t fix;                    // the chain needs the sequential fix-up
+  int32_t nchg;                   // tiles relinked in the current fix-up round
   uint32_t entry_p, entry_s;      // true state at the exit of the lead-in tile
   int32_t entry_c[4];             //   and the segment's cumulative counts there
 };
@@ -280,7 +282,8 @@ struct EntCtx {
   WorkItem it;
   int nh;       // hypotheses = blocks per MCU
   uint32_t S;   // tile bits
-  uint32_t D;   // sync point, bits into a tile
+  uint32_t D1;  // first sync point, bits into a tile
+  uint32_t D;   // second sync point
   int lead;     // 1 when local tile 0 is the lead-in (last tile of the previous item)
   int J;        // local tiles including the lead-in
   uint32_t tfirst;
@@ -292,6 +295,7 @@ struct EntCtx {
     im = images + sg.image;
     nh = im->nslots;
     S = sg.tile_bits;
+    D1 = sg.sync1_bits;
     D = sg.sync_bits;
     w = words + sg.word_base;
     nwords = sg.nwords;
@@ -305,8 +309,8 @@ struct EntCtx {
   }
   CJD_HD int comp(int slot) const { return (int)((slotinfo >> (4 * slot + 2)) & 3); }
   CJD_HD uint32_t tile_end(uint32_t t) const { return (t + 1 >= sg.ntiles) ? sg.nbits : (t + 1) * S; }
-  CJD_HD uint32_t sync_point(uint32_t t) const {
-    const uint32_t e = tile_end(t), q = t * S + D;
+  CJD_HD uint32_t sync_point(uint32_t t, uint32_t d) const {
+    const uint32_t e = tile_end(t), q = t * S + d;
     return q < e ? q : e;
   }
 };
@@ -351,66 +355,90 @@ CJD_HD inline bool sym_step(BitReader& br, const EntShared& sh, uint64_t slotinf
   return end;
 }
 
-// The D-state every hypothesis of the lane's tile agrees with most (ties: the
-// lowest hypothesis).
-CJD_HD inline uint64_t ent_consensus(const EntShared& sh, int tid, int nh) {
+// The hypothesis state most hypotheses of the lane's tile share (ties: the
+// lowest hypothesis); n = how many share it.
+CJD_HD inline uint64_t ent_consensus(const EntShared& sh, int tid, int nh, int& n) {
   uint64_t best = sh.u.a.hyp[0][tid];
   int nbest = 0;
   for (int h = 0; h < nh; ++h) {
     const uint64_t v = sh.u.a.hyp[h][tid];
-    int n = 0;
-    for (int g = 0; g < nh; ++g) n += sh.u.a.hyp[g][tid] == v ? 1 : 0;
-    if (n > nbest) {
-      nbest = n;
+    int k = 0;
+    for (int g = 0; g < nh; ++g) k += sh.u.a.hyp[g][tid] == v ? 1 : 0;
+    if (k > nbest) {
+      nbest = k;
       best = v;
     }
   }
+  n = nbest;
   return best;
 }
 
-// Phase 1: main trajectory of local tile tid.  One loop, one symbol per
-// iteration: hypotheses 0..nh-1 run to the sync point in turn, then the main
-// trajectory runs from the consensus D-state to the tile end.  Tile 0 of a
-// segment starts from the known state (bit 0, slot 0): its main trajectory is
-// the true one from the start.
+// Phase 1: main trajectory of local tile tid.  One loop, one symbol per step:
+//   stage 1: hypotheses 0..nh-1 in turn from the tile start to the first sync
+//            point; if they all agree there, that state is the D-state;
+//   stage 2: otherwise each hypothesis continues from its stage-1 state to the
+//            second sync point and the consensus there is the D-state;
+//   main:    from the D-state to the tile end, counting blocks and DC sums.
+// Tile 0 of a segment starts from the known state (bit 0, slot 0): its main
+// trajectory is the true one from the start.
 CJD_HD inline void ent_phase1(EntShared& sh, int tid, const EntCtx& c) {
   if (tid >= c.J) return;
+  const int nh = c.nh, kMain = 2 * nh;
   const uint32_t t = c.tfirst + (uint32_t)tid;
-  const uint32_t b0 = t * c.S, end = c.tile_end(t), thr = c.sync_point(t);
+  const uint32_t b0 = t * c.S, end = c.tile_end(t), thr1 = c.sync_point(t, c.D1), thr2 = c.sync_point(t, c.D);
   BitReader br;
   br.seek(c.w, c.nwords, b0);
-  int mode = t == 0 ? c.nh : 0;  // hypothesis being decoded; nh = main
+  int mode = t == 0 ? kMain : 0;
   int slot = 0, coef = -1, err = 0;
   int32_t nb = 0, dc[3] = {0, 0, 0};
   sh.u.a.dstate[tid] = ~0ull;
+  sh.u.a.dthr[tid] = end;
   for (bool done = false; !done;) {
     br.rotate();
     for (int u = 0; u < kPeriod; ++u) {
       if (coef < 0) {
         const uint32_t p = br.pos;
-        if (mode < c.nh && p >= thr) {
-          sh.u.a.hyp[mode][tid] = pack_state(p, (uint32_t)slot);
-          if (++mode < c.nh) {
-            br.seek(c.w, c.nwords, b0);
-            slot = mode;
+        if (mode < kMain && p >= (mode < nh ? thr1 : thr2)) {
+          const int h = mode < nh ? mode : mode - nh;
+          sh.u.a.hyp[h][tid] = pack_state(p, (uint32_t)slot);
+          ++mode;
+          uint64_t next = 0;  // state to continue from
+          if (mode < nh) {
+            next = pack_state(b0, (uint32_t)mode);
+          } else if (mode == nh) {
+            int n;
+            const uint64_t m = ent_consensus(sh, tid, nh, n);
+            if (n == nh || thr1 >= thr2) {  // unanimous, or no second sync point
+              mode = kMain;
+              sh.u.a.dstate[tid] = m;
+              sh.u.a.dthr[tid] = thr1;
+              next = m;
+            } else {
+              next = sh.u.a.hyp[0][tid];
+            }
+          } else if (mode < kMain) {
+            next = sh.u.a.hyp[mode - nh][tid];
           } else {
-            const uint64_t m = ent_consensus(sh, tid, c.nh);
+            int n;
+            const uint64_t m = ent_consensus(sh, tid, nh, n);
             sh.u.a.dstate[tid] = m;
-            br.seek(c.w, c.nwords, (uint32_t)(m >> 4));
-            slot = (int)(m & 15);
+            sh.u.a.dthr[tid] = thr2;
+            next = m;
           }
+          br.seek(c.w, c.nwords, (uint32_t)(next >> 4));
+          slot = (int)(next & 15);
           continue;
         }
-        if (mode == c.nh && p >= end) {
+        if (mode == kMain && p >= end) {
           done = true;
           break;
         }
-        if (mode == c.nh) ++nb;
+        if (mode == kMain) ++nb;
       }
       const int s0 = slot;
       int d, wp, wv;
-      sym_step(br, sh, c.slotinfo, c.nh, slot, coef, d, wp, wv, err);
-      if (mode == c.nh) add_dc(dc, c.comp(s0), d);
+      sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
+      if (mode == kMain) add_dc(dc, c.comp(s0), d);
     }
   }
   sh.ep[tid] = br.pos;
@@ -422,13 +450,13 @@ CJD_HD inline void ent_phase1(EntShared& sh, int tid, const EntCtx& c) {
 }
 
 // Decode local tile j from the state (p, slot) at its entry: up to the first
-// boundary at or after the sync point; if that is the tile's D-state, the
-// rest is the main trajectory (its exit and counts are taken from phase 1),
-// otherwise the decode goes on to the tile end.  (p, slot) become the exit,
-// cnt the blocks / DC sums gained in tile j.
+// boundary at or after the tile's D-state sync point; if that is the D-state,
+// the rest is the main trajectory (its exit and counts are taken from phase
+// 1), otherwise the decode goes on to the tile end.  (p, slot) become the
+// exit, cnt the blocks / DC sums gained in tile j.
 CJD_HD inline void ent_relink(const EntShared& sh, const EntCtx& c, int j, uint32_t& p, int& slot, int32_t* cnt) {
   const uint32_t t = c.tfirst + (uint32_t)j;
-  const uint32_t end = c.tile_end(t), thr = c.sync_point(t);
+  const uint32_t end = c.tile_end(t), thr = sh.u.a.dthr[j];
   const uint64_t ds = sh.u.a.dstate[j];
   int32_t nb = 0, dc[3] = {0, 0, 0};
   bool checked = false;
@@ -472,63 +500,61 @@ CJD_HD inline void ent_relink(const EntShared& sh, const EntCtx& c, int j, uint3
 }
 
 // Phase 2: link local tile tid (>= 1) from the main exit of tile tid-1, and
-// seed the chain with the assumed states.  fix is set when some link did not
-// join the main trajectory (its successor's assumed entry is then wrong).
+// seed the chain with the assumed states (tile 0: its main exit; its counts
+// belong to this item only when it is the segment's first tile).
 CJD_HD inline void ent_phase2(EntShared& sh, int tid, const EntCtx& c) {
   if (tid >= c.J) return;
   if (tid == 0) {
     sh.tp[0] = sh.ep[0];
     sh.ts[0] = sh.es[0];
-    sh.up[0] = 0;
-    sh.us[0] = 0;
-    for (int q = 0; q < 4; ++q) sh.tc[0][q] = c.lead ? 0 : sh.u.a.mc[0][q];
+    for (int q = 0; q < 4; ++q) sh.u.a.cnt[0][q] = c.lead ? 0 : sh.u.a.mc[0][q];
     return;
   }
   uint32_t p = sh.ep[tid - 1];
   int slot = (int)sh.es[tid - 1];
   int32_t cnt[4];
-  sh.up[tid] = p;
-  sh.us[tid] = (uint32_t)slot;
+  sh.u.a.up[tid] = p;
+  sh.u.a.us[tid] = (uint32_t)slot;
   ent_relink(sh, c, tid, p, slot, cnt);
   sh.tp[tid] = p;
   sh.ts[tid] = (uint32_t)slot;
-  for (int q = 0; q < 4; ++q) sh.tc[tid][q] = cnt[q];
-  if (p != sh.ep[tid] || (uint32_t)slot != sh.es[tid]) sh.fix = 1;
+  for (int q = 0; q < 4; ++q) sh.u.a.cnt[tid][q] = cnt[q];
 }
 
-// Sequential fix-up (one lane): every tile whose exit was computed from an
-// entry other than its predecessor's exit in the chain is decoded again from
-// that exit.  Runs over the whole item; most tiles only compare.
-CJD_HD inline void ent_fix_chain(EntShared& sh, const EntCtx& c) {
-  for (int j = 1; j < c.J; ++j) {
-    if (sh.up[j] == sh.tp[j - 1] && sh.us[j] == sh.ts[j - 1]) continue;
-    uint32_t p = sh.tp[j - 1];
-    int slot = (int)sh.ts[j - 1];
-    int32_t cnt[4];
-    sh.up[j] = p;
-    sh.us[j] = (uint32_t)slot;
-    ent_relink(sh, c, j, p, slot, cnt);
-    sh.tp[j] = p;
-    sh.ts[j] = (uint32_t)slot;
-    for (int q = 0; q < 4; ++q) sh.tc[j][q] = cnt[q];
-  }
+// Fix-up round, read half (all lanes): does tile tid's exit come from an
+// entry other than tile tid-1's current exit?  (p, s) = that exit.
+CJD_HD inline bool ent_fix_need(const EntShared& sh, int tid, const EntCtx& c, uint32_t& p, uint32_t& s) {
+  if (tid < 1 || tid >= c.J) return false;
+  p = sh.tp[tid - 1];
+  s = sh.ts[tid - 1];
+  return p != sh.u.a.up[tid] || s != sh.u.a.us[tid];
+}
+// Fix-up round, write half: decode tile tid again from (p, s).  Rounds repeat
+// until no tile changes; a run of k consecutive tiles off the main trajectory
+// takes k rounds.
+CJD_HD inline void ent_fix_apply(EntShared& sh, int tid, const EntCtx& c, uint32_t p, uint32_t s) {
+  sh.u.a.up[tid] = p;
+  sh.u.a.us[tid] = s;
+  int slot = (int)s;
+  int32_t cnt[4];
+  ent_relink(sh, c, tid, p, slot, cnt);
+  sh.tp[tid] = p;
+  sh.ts[tid] = (uint32_t)slot;
+  for (int q = 0; q < 4; ++q) sh.u.a.cnt[tid][q] = cnt[q];
 }
 
-// Phase 3 (all lanes, kWG-lane Hillis-Steele scan in LDS, with barriers
-// supplied by the caller between steps): tc[j] <- tc[0] + ... + tc[j].
-// Step with distance d reads the values of the previous step: lanes first
-// read, then (after a barrier) write.
+// Scan (all lanes; Hillis-Steele over kWG lanes in LDS with barriers between
+// the steps supplied by the caller): tc[j] <- cnt[0] + ... + cnt[j].  A step
+// with distance d first reads (ent_scan_read), then, after a barrier, adds.
+CJD_HD inline void ent_scan_init(EntShared& sh, int tid, const EntCtx& c) {
+  if (tid >= c.J) return;
+  for (int q = 0; q < 4; ++q) sh.tc[tid][q] = sh.u.a.cnt[tid][q];
+}
 CJD_HD inline void ent_scan_read(const EntShared& sh, int tid, int d, int32_t* v) {
   for (int q = 0; q < 4; ++q) v[q] = tid >= d ? sh.tc[tid - d][q] : 0;
 }
 CJD_HD inline void ent_scan_write(EntShared& sh, int tid, const int32_t* v) {
   for (int q = 0; q < 4; ++q) sh.tc[tid][q] += v[q];
-}
-
-// Serial form of the scan (one lane), used after a late fix-up.
-CJD_HD inline void ent_scan_serial(EntShared& sh, const EntCtx& c) {
-  for (int j = 1; j < c.J; ++j)
-    for (int q = 0; q < 4; ++q) sh.tc[j][q] += sh.tc[j - 1][q];
 }
 
 // Composition of predecessors' aggregates in the lookback.  An aggregate says
@@ -574,7 +600,7 @@ struct LookAcc {
   }
 };
 
-// Aggregate of this item (tc already scanned: relative cumulative counts).
+// Aggregate of this item (tc scanned: relative cumulative counts).
 CJD_HD inline void ent_aggregate(const EntShared& sh, const EntCtx& c, LookbackRec* me) {
   me->lead_p = sh.ep[0];
   me->lead_s = sh.es[0];
@@ -591,18 +617,11 @@ CJD_HD inline void ent_inclusive(const EntShared& sh, const EntCtx& c, LookbackR
   for (int q = 0; q < 4; ++q) me->incl_c[q] = sh.tc[c.J - 1][q];
 }
 
-// After the lookback (one lane): the true entry differs from the assumed one
-// -> fix the chain from it and redo the scan serially.  Then every lane adds
-// the entry's cumulative counts (ent_absolute).
-CJD_HD inline void ent_apply_entry(EntShared& sh, const EntCtx& c) {
-  if (sh.entry_p == sh.ep[0] && sh.entry_s == sh.es[0]) return;
-  // undo the scan: per-tile counts again
-  for (int j = c.J - 1; j >= 1; --j)
-    for (int q = 0; q < 4; ++q) sh.tc[j][q] -= sh.tc[j - 1][q];
-  sh.tp[0] = sh.entry_p;
-  sh.ts[0] = sh.entry_s;
-  ent_fix_chain(sh, c);
-  ent_scan_serial(sh, c);
+// After the lookback: whether the true entry differs from the assumed one
+// (then tile 0's exit is set to it and the fix-up rounds and the scan run
+// again), and the absolute counts (every lane adds the entry's counts).
+CJD_HD inline bool ent_entry_differs(const EntShared& sh, const EntCtx& c) {
+  return c.lead && (sh.entry_p != sh.ep[0] || sh.entry_s != sh.es[0]);
 }
 CJD_HD inline void ent_absolute(EntShared& sh, int tid, const EntCtx& c) {
   if (tid >= c.J || !c.lead) return;

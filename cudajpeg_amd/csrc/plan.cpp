@@ -185,7 +185,12 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
     b->scan_bytes += (int64_t)data.size();
     sd.image = r.desc;
     sd.tile_bits = b->tile_bits;
-    sd.sync_bits = b->sync_bits ? b->sync_bits : b->tile_bits / 4;
+    // sync points: trajectories from the slot hypotheses mostly agree within
+    // a few blocks, and a main trajectory that starts off the true one still
+    // joins it before the tile end, so a short first sync point suffices
+    sd.sync_bits = b->sync_bits ? b->sync_bits : std::max<uint32_t>(64, b->tile_bits / 32);
+    sd.sync1_bits = b->sync1_bits ? b->sync1_bits : sd.sync_bits;
+    if (sd.sync1_bits > sd.sync_bits) sd.sync1_bits = sd.sync_bits;
     sd.ntiles = std::max<uint32_t>(1, (sd.nbits + b->tile_bits - 1) / b->tile_bits);
     const int64_t mcu0 = (int64_t)s * (f.restart_interval > 0 ? f.restart_interval : nmcu);
     const int64_t mcus = std::min<int64_t>(nmcu - mcu0, f.restart_interval > 0 ? f.restart_interval : nmcu);

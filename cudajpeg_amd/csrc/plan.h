@@ -27,7 +27,8 @@ struct ImageRec {
 
 struct Plan {
   uint32_t tile_bits = 4096;
-  uint32_t sync_bits = 0;         // sync point D in bits, 0 = tile_bits / 4
+  uint32_t sync_bits = 0;         // second sync point D in bits, 0 = tile_bits / 32 (>= 64)
+  uint32_t sync1_bits = 0;        // first sync point D1 in bits, 0 = D (one stage)
   uint32_t item_tiles = kWG;      // tiles per entropy work item (2 .. kWG)
   int want_planes = 0;
   std::vector<ImageRec> recs;

@@ -92,10 +92,11 @@ class BatchDecoder:
     def set_options(self, idct: int, entropy: int = L.ENTROPY_TILED, tile_bits: int = 0, planes: bool = False):
         _check(L.lib().cjd_batch_set_options(self.h, idct, entropy, tile_bits, int(planes)), "options: ")
 
-    def set_tuning(self, sync_bits: int = 0, item_tiles: int = 0):
-        """Tiled entropy decoder: sync point (bits into a tile) and tiles per
-        work item; applies from the next stage()."""
-        _check(L.lib().cjd_batch_set_tuning(self.h, sync_bits, item_tiles), "tuning: ")
+    def set_tuning(self, sync1_bits: int = 0, sync_bits: int = 0, item_tiles: int = 0):
+        """Tiled entropy decoder: first and second sync points (bits into a
+        tile) and tiles per work item; 0 = defaults; applies from the next
+        stage()."""
+        _check(L.lib().cjd_batch_set_tuning(self.h, sync1_bits, sync_bits, item_tiles), "tuning: ")
 
     def stage(self, files: list[bytes]) -> np.ndarray:
         n = len(files)

@@ -91,11 +91,12 @@ void cjd_batch_destroy(cjd_batch *b);
 int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int tile_bits, int want_planes);
 
 /* Tiled entropy decoder tuning (no counterpart in the reference, whose scan
- * decoder has no tiles): sync point in bits into a tile (a multiple of 32,
- * 0 = tile_bits / 4) and tiles per work item (2 .. 256, 0 = 256).  Smaller
- * items spread one image over more workgroups (latency); larger ones amortise
- * the lookback (throughput).  Takes effect at the next cjd_batch_stage. */
-int cjd_batch_set_tuning(cjd_batch *b, int sync_bits, int item_tiles);
+ * decoder has no tiles): the two sync points in bits into a tile where the
+ * per-slot trajectories are compared (0 = defaults; the first is used alone
+ * when the trajectories all agree there or when it is not below the second)
+ * and tiles per work item (2 .. 256, 0 = 256).  Takes effect at the next
+ * cjd_batch_stage. */
+int cjd_batch_set_tuning(cjd_batch *b, int sync1_bits, int sync_bits, int item_tiles);
 
 /* Parse, unstuff (src/decodeScanGPU.cu:11-24) and upload n files; plans the
  * entropy and pixel work lists.  errors[i] gets each file's parse result; a

@@ -36,8 +36,36 @@ struct EntRun {
   std::vector<std::unique_ptr<EntShared>> shs;
   std::vector<EntCtx> ctx;
   std::vector<char> done;
-  int fixes = 0;  // items that needed the sequential fix-up (a link or the entry off the main chain)
+  int fixes = 0;  // tiles decoded again in fix-up rounds (a link or the entry off the main chain)
 };
+
+// Fix-up rounds lane by lane (each round: all lanes read, then the lanes
+// that need it relink); returns the number of relinks.
+int fix_rounds(EntShared& sh, const EntCtx& c) {
+  int total = 0;
+  for (;;) {
+    std::vector<uint32_t> P(kWG), S(kWG);
+    std::vector<char> need(kWG);
+    for (int t = 0; t < kWG; ++t) need[t] = ent_fix_need(sh, t, c, P[t], S[t]);
+    int n = 0;
+    for (int t = 0; t < kWG; ++t)
+      if (need[t]) {
+        ent_fix_apply(sh, t, c, P[t], S[t]);
+        ++n;
+      }
+    total += n;
+    if (!n) return total;
+  }
+}
+
+void scan_counts(EntShared& sh, const EntCtx& c) {
+  for (int t = 0; t < kWG; ++t) ent_scan_init(sh, t, c);
+  for (int d = 1; d < c.J; d <<= 1) {  // each step: all lanes read, then all write
+    std::vector<int32_t> v((size_t)kWG * 4);
+    for (int t = 0; t < kWG; ++t) ent_scan_read(sh, t, d, &v[(size_t)t * 4]);
+    for (int t = 0; t < c.J; ++t) ent_scan_write(sh, t, &v[(size_t)t * 4]);
+  }
+}
 
 // Lookback, entry application and inclusive publication of item i, resolving
 // predecessors first only when the kernel would have waited for them.
@@ -67,8 +95,12 @@ void resolve(EntRun& R, size_t i) {
     sh.entry_p = p;
     sh.entry_s = s;
     for (int k = 0; k < 4; ++k) sh.entry_c[k] = cc[k];
-    if (p != sh.ep[0] || s != sh.es[0]) ++R.fixes;
-    ent_apply_entry(sh, c);
+    if (ent_entry_differs(sh, c)) {
+      sh.tp[0] = p;
+      sh.ts[0] = s;
+      R.fixes += fix_rounds(sh, c);
+      scan_counts(sh, c);
+    }
   }
   for (int t = 0; t < kWG; ++t) ent_absolute(sh, t, c);
   ent_inclusive(sh, c, &R.lb[i]);
@@ -86,7 +118,7 @@ void run_entropy(Emu& E, int order) {
   R.shs.resize(ni);
   R.ctx.resize(ni);
   R.done.assign(ni, 0);
-  // phases 1-3 and aggregate publication for every item
+  // phases 1-2, fix-up, scan and aggregate publication for every item
   for (size_t i = 0; i < ni; ++i) {
     R.shs[i].reset(new EntShared);
     EntShared& sh = *R.shs[i];
@@ -97,15 +129,8 @@ void run_entropy(Emu& E, int order) {
     memcpy(sh.dz, kDeZigZag, 64);
     for (int t = 0; t < kWG; ++t) ent_phase1(sh, t, c);
     for (int t = 0; t < kWG; ++t) ent_phase2(sh, t, c);
-    if (sh.fix) {
-      ent_fix_chain(sh, c);
-      ++R.fixes;
-    }
-    for (int d = 1; d < c.J; d <<= 1) {  // each step: all lanes read, then all write
-      std::vector<int32_t> v((size_t)kWG * 4);
-      for (int t = 0; t < kWG; ++t) ent_scan_read(sh, t, d, &v[(size_t)t * 4]);
-      for (int t = 0; t < c.J; ++t) ent_scan_write(sh, t, &v[(size_t)t * 4]);
-    }
+    R.fixes += fix_rounds(sh, c);
+    scan_counts(sh, c);
     if (c.lead) {
       ent_aggregate(sh, c, &R.lb[i]);
       R.lb[i].status = 1;
@@ -159,10 +184,12 @@ void run_pixel(Emu& E, int mode, size_t first, size_t count) {
 extern "C" {
 
 void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_mode, int tile_bits,
-              int want_planes, int order, int serial, int item_tiles, int sync_bits, int* errors) {
+              int want_planes, int order, int serial, int item_tiles, int sync1_bits, int sync_bits,
+              int* errors) {
   Emu* E = new Emu;
   E->plan.tile_bits = tile_bits ? (uint32_t)tile_bits : 4096u;
   E->plan.sync_bits = (uint32_t)sync_bits;
+  E->plan.sync1_bits = (uint32_t)sync1_bits;
   if (item_tiles) E->plan.item_tiles = (uint32_t)item_tiles;
   E->plan.want_planes = want_planes;
   plan_batch(E->plan, n, files, sizes, errors);
@@ -217,7 +244,7 @@ long emu_get(void* h, int i, int what, void* dst, size_t cap) {
   return (long)n;
 }
 
-// [items, items that needed the sequential fix-up, fallbacks, segments]
+// [items, tiles decoded again in fix-up rounds, fallbacks, segments]
 void emu_stats(void* h, long* out4) {
   Emu* E = static_cast<Emu*>(h);
   out4[0] = (long)E->plan.items.size();
