@@ -75,6 +75,45 @@ def cpu_baseline(encs, budget_s: float):
                       f"file in page cache, {done_t:.1f} s total, {done_px / 1e6:.0f} Mpix"}
 
 
+def serve_bench(args):
+    """Serving loop on the request mix: the mixed-batch distribution plus 5%
+    8192x8192 4:2:0 q85 images.  Peak = a burst of requests decoded flat out;
+    then Poisson arrivals at 70% of that rate for --serve-seconds, with
+    latency from submit to the frame ready in device memory."""
+    from cudajpeg_amd import Server, synth
+    rng = np.random.default_rng(args.seed + 1)
+    specs = synth.config_mixed(240, args.seed + 1000)
+    specs += [synth.ImageSpec(8192, 8192, "420", 85, float(rng.uniform(2, 12)), args.seed + 2000 + k)
+              for k in range(12)]
+    order = rng.permutation(len(specs))
+    specs = [specs[i] for i in order]
+    files = [e.data for e in synth.encode_many(specs)]
+    px = np.array([s.pixels for s in specs], np.float64)
+    srv = Server(max_batch=64, max_wait_us=2000)
+    # warm-up and peak: a burst of 2 x the pool
+    srv.replay(files[:64], np.zeros(64, np.int64))
+    n_peak = 2 * len(files)
+    burst = [files[k % len(files)] for k in range(n_peak)]
+    lat, err, wall = srv.replay(burst, np.zeros(n_peak, np.int64))
+    assert not err.any(), err
+    peak_req_s = n_peak / (wall / 1e9)
+    peak_mpix_s = float(sum(px[k % len(files)] for k in range(n_peak)) / (wall / 1e9) / 1e6)
+    # 70% of peak, Poisson arrivals
+    rate = 0.7 * peak_req_s
+    n = max(64, int(rate * args.serve_seconds))
+    arrival = np.cumsum(rng.exponential(1e9 / rate, n)).astype(np.int64)
+    reqs = [files[int(k)] for k in rng.integers(0, len(files), n)]
+    lat, err, wall = srv.replay(reqs, arrival)
+    srv.close()
+    assert not err.any(), err
+    ms = lat / 1e6
+    return {"request_mix": "240 mixed-batch images + 12 8192x8192 4:2:0 q85 (5%)", "peak_req_s": round(peak_req_s, 1),
+            "peak_mpix_s": round(peak_mpix_s, 1), "load_frac": 0.7, "offered_req_s": round(rate, 1),
+            "requests": n, "seconds": round(wall / 1e9, 2), "p50_ms": round(float(np.percentile(ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(ms, 99)), 3), "p999_ms": round(float(np.percentile(ms, 99.9)), 3),
+            "max_batch": 64, "max_wait_us": 2000}
+
+
 def measured_traffic(kernel: str, args) -> int | None:
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this same workload (profiles/hbm_traffic_mixed256.md, counters
@@ -99,6 +138,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (reads MIB, writes MIB) "
                          "three times: the reference point for scaling FETCH_SIZE/WRITE_SIZE counters")
@@ -210,6 +251,10 @@ def main():
         for d in per:
             d.close()
 
+    serving = None if args.no_serve else serve_bench(args)
+    if serving:
+        log(f"serving: {serving}")
+
     cpu = None if args.no_cpu else cpu_baseline(encs, args.cpu_seconds)
     if cpu:
         log(f"cpu baseline: {cpu}")
@@ -239,6 +284,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
                      "algorithmic_bytes": int(dom_bytes)},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

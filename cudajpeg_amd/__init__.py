@@ -7,7 +7,9 @@ reader API of include/format.h.
 from ._lib import (ENTROPY_SERIAL, ENTROPY_TILED, IDCT_FLOAT, IDCT_INT, IDCT_MFMA, OUT_COEFFICIENTS,
                    OUT_PIXELS, OUT_PLANE0, error_name)
 from .decoder import BatchDecoder, Context, DecodeError, JPGReader, decode, parse
+from .serve import Server, device_frame
 
-__all__ = ["BatchDecoder", "Context", "DecodeError", "JPGReader", "decode", "parse", "error_name",
+__all__ = ["BatchDecoder", "Context", "DecodeError", "JPGReader", "Server", "decode", "device_frame", "parse",
+           "error_name",
            "IDCT_INT", "IDCT_FLOAT", "IDCT_MFMA", "ENTROPY_TILED", "ENTROPY_SERIAL",
            "OUT_PIXELS", "OUT_COEFFICIENTS", "OUT_PLANE0"]

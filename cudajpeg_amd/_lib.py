@@ -20,10 +20,12 @@ OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 # Every symbol include/cudajpeg_amd.h and include/format.h declare.
 EXPORTS = [
     "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
-    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_stage", "cjd_batch_decode", "cjd_batch_decode_scan",
+    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
     "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_finish", "cjd_batch_capture",
     "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
     "cjd_batch_download", "cjd_batch_totals", "cjd_error_name",
+    "cjd_server_create", "cjd_server_submit", "cjd_server_poll", "cjd_server_release", "cjd_server_now_ns",
+    "cjd_server_replay", "cjd_server_destroy",
     "newJPGReader", "delJPGReader", "openJPG", "openJPGMem", "writeJPG", "printError", "read16", "skipBlock",
     "decodeSOF", "decodeDHT", "decodeDQT", "decodeDRI", "decodeScanGPU", "iDCT_resample_colourTransform",
 ]
@@ -49,6 +51,20 @@ class FrameInfo(ctypes.Structure):
     ]
 
 
+class ServerOptions(ctypes.Structure):
+    """cjd_server_options"""
+    _fields_ = [("max_batch", ctypes.c_int), ("max_wait_us", ctypes.c_int), ("frame_budget", ctypes.c_uint64),
+                ("slots", ctypes.c_int), ("idct_mode", ctypes.c_int), ("tile_bits", ctypes.c_int)]
+
+
+class Completion(ctypes.Structure):
+    """cjd_completion"""
+    _fields_ = [("id", ctypes.c_uint64), ("error", ctypes.c_int), ("batch_size", ctypes.c_int),
+                ("frame", ctypes.c_void_p), ("frame_bytes", ctypes.c_uint64),
+                ("width", ctypes.c_int), ("height", ctypes.c_int), ("channels", ctypes.c_int),
+                ("t_arrival_ns", ctypes.c_int64), ("t_launch_ns", ctypes.c_int64), ("t_ready_ns", ctypes.c_int64)]
+
+
 _lib = None
 
 
@@ -70,6 +86,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_set_options": ([vp, i32, i32, i32, i32], i32),
         "cjd_batch_set_tuning": ([vp, i32, i32, i32], i32),
         "cjd_batch_stage": ([vp, i32, vp, vp, vp], i32),
+        "cjd_batch_stage_async": ([vp, i32, vp, vp, vp, vp], i32),
         "cjd_batch_decode": ([vp, vp], i32),
         "cjd_batch_decode_scan": ([vp, vp], i32),
         "cjd_batch_transform": ([vp, vp], i32),
@@ -83,6 +100,13 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_download": ([vp, i32, i32, vp, sz], i32),
         "cjd_batch_totals": ([vp, vp, vp, vp, vp, vp], i32),
         "cjd_error_name": ([i32], ctypes.c_char_p),
+        "cjd_server_create": ([vp, ctypes.POINTER(ServerOptions), ctypes.POINTER(vp)], i32),
+        "cjd_server_submit": ([vp, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_uint64)], i32),
+        "cjd_server_poll": ([vp, ctypes.POINTER(Completion), i32, i32], i32),
+        "cjd_server_release": ([vp, ctypes.c_uint64], i32),
+        "cjd_server_now_ns": ([], ctypes.c_int64),
+        "cjd_server_replay": ([vp, i32, vp, vp, vp, vp, vp, vp], i32),
+        "cjd_server_destroy": ([vp], None),
         "newJPGReader": ([], vp),
         "delJPGReader": ([vp], None),
         "openJPG": ([vp, ctypes.c_char_p], i32),

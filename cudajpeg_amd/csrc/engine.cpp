@@ -20,9 +20,7 @@
 
 using namespace cjd;
 
-struct cjd_context {
-  int device = 0;
-};
+#include "engine_internal.h"
 
 namespace {
 
@@ -127,16 +125,27 @@ extern "C" int cjd_batch_set_tuning(cjd_batch* b, int sync1_bits, int sync_bits,
   return 0;
 }
 
+// Upload on the stream.  The host vectors belong to the batch and stay
+// unchanged until its next stage, which comes after this decode finished.
 template <class T>
-static int upload(DevBuf& d, const std::vector<T>& v) {
+static int upload(DevBuf& d, const std::vector<T>& v, hipStream_t s) {
   const size_t bytes = v.size() * sizeof(T);
   if (int e = d.ensure(bytes)) return e;
-  if (bytes && hipMemcpy(d.p, v.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+  if (bytes && hipMemcpyAsync(d.p, v.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    return CJD_DEVICE_MEM_ERROR;
   return 0;
 }
 
 extern "C" int cjd_batch_stage(cjd_batch* b, int n, const uint8_t* const* files, const size_t* sizes, int* errors) {
+  const int e = cjd_batch_stage_async(b, n, files, sizes, errors, nullptr);
+  if (e) return e;
+  return hipStreamSynchronize(nullptr) == hipSuccess ? 0 : CJD_DEVICE_MEM_ERROR;
+}
+
+extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* files, const size_t* sizes,
+                                     int* errors, void* stream) {
   if (!b || n < 0 || (n && (!files || !sizes))) return CJD_PROGRAMMER_ERROR;
+  hipStream_t s = (hipStream_t)stream;
   if (hipSetDevice(b->ctx->device) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
@@ -148,13 +157,13 @@ extern "C" int cjd_batch_stage(cjd_batch* b, int n, const uint8_t* const* files,
   }
   plan_batch(b->plan, n, files, sizes, errors);
   int e = 0;
-  if (!e) e = upload(b->d_words, b->plan.words);
-  if (!e) e = upload(b->d_images, b->plan.images);
-  if (!e) e = upload(b->d_segs, b->plan.segs);
-  if (!e) e = upload(b->d_items, b->plan.items);
-  if (!e) e = upload(b->d_pix, b->plan.pix);
-  if (!e) e = upload(b->d_tables, b->plan.tables);
-  if (!e) e = upload(b->d_qtabs, b->plan.qtabs);
+  if (!e) e = upload(b->d_words, b->plan.words, s);
+  if (!e) e = upload(b->d_images, b->plan.images, s);
+  if (!e) e = upload(b->d_segs, b->plan.segs, s);
+  if (!e) e = upload(b->d_items, b->plan.items, s);
+  if (!e) e = upload(b->d_pix, b->plan.pix, s);
+  if (!e) e = upload(b->d_tables, b->plan.tables, s);
+  if (!e) e = upload(b->d_qtabs, b->plan.qtabs, s);
   if (!e) e = b->d_coefs.ensure((size_t)b->plan.total_blocks * 128);
   if (!e) e = b->d_out.ensure(b->plan.out_bytes);
   // scratch: per-image status, ticket, lookback records (cleared per decode)
@@ -204,10 +213,10 @@ static int enqueue(cjd_batch* b, hipStream_t s, bool timed) {
 // whose tile links did not resolve; map status bits to reference error codes.
 // `redo` receives the images that were decoded again.
 static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo) {
-  if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-  if (b->status_words &&
-      hipMemcpy(b->h_status.data(), b->d_scratch.p, b->status_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
+  if (b->status_words && hipMemcpyAsync(b->h_status.data(), b->d_scratch.p, b->status_words * 4,
+                                        hipMemcpyDeviceToHost, s) != hipSuccess)
     return CJD_DEVICE_MEM_ERROR;
+  if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   for (size_t i = 0; i < b->plan.recs.size(); ++i) {
     ImageRec& r = b->plan.recs[i];
     if (r.error) continue;
@@ -215,13 +224,14 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
     if (st & (kStatusUnresolved | kStatusInconsistent)) {
       uint32_t zero = 0;
       uint32_t* status = b->d_scratch.as<uint32_t>() + i;
-      if (hipMemcpy(status, &zero, 4, hipMemcpyHostToDevice) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+      if (hipMemcpyAsync(status, &zero, 4, hipMemcpyHostToDevice, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
       if (launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(),
                                 b->d_segs.as<SegmentDesc>() + r.seg_first, r.seg_count, b->d_tables.as<HuffTable>(),
                                 b->d_coefs.as<int16_t>(), b->d_scratch.as<uint32_t>(), s) != hipSuccess)
         return CJD_KERNEL_LAUNCH_ERROR;
+      if (hipMemcpyAsync(&b->h_status[i], status, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return CJD_DEVICE_MEM_ERROR;
       if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-      if (hipMemcpy(&b->h_status[i], status, 4, hipMemcpyDeviceToHost) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
       b->h_status[i] |= kStatusFallback;
       ++b->fallbacks;
       if (redo) redo->push_back(i);

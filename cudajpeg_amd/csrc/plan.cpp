@@ -8,6 +8,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 
 #include "parse_internal.h"
 
@@ -48,56 +53,112 @@ namespace {
 
 // src/decodeScanGPU.cu:11-24 (remove 0x00 after 0xFF, keep RSTn, drop other
 // marker bytes) with the scan split at RSTn when a restart interval is set.
-// Returns false on an out-of-sequence RSTn (src/decodeScanGPU.cu:480-482).
-bool unstuff_segments(const uint8_t* src, size_t len, bool split, std::vector<std::vector<uint8_t>>& segs) {
-  segs.clear();
-  segs.emplace_back();
-  segs.back().reserve(len);
+// Runs between 0xFF bytes are found with memchr and block-copied.  With
+// dst == nullptr only the segment bounds (in unstuffed bytes, segment k =
+// [bounds[k], bounds[k+1])) are computed; otherwise segment k is written to
+// dst + dst_off[k].  Returns false on an out-of-sequence RSTn
+// (src/decodeScanGPU.cu:480-482).
+bool unstuff_segments(const uint8_t* src, size_t len, bool split, std::vector<size_t>* bounds, uint8_t* dst,
+                      const size_t* dst_off) {
+  if (bounds) bounds->assign(1, 0);
+  size_t o = 0, s = 0, seg = 0;
+  uint8_t* out = dst ? dst + dst_off[0] : nullptr;
   int next_rst = 0;
-  size_t s = 0;
   while (s < len) {
-    const uint8_t b = src[s++];
-    if (b != 0xFF) {
-      segs.back().push_back(b);
-      continue;
-    }
+    const void* ff = memchr(src + s, 0xFF, len - s);
+    const size_t e = ff ? (size_t)(static_cast<const uint8_t*>(ff) - src) : len;
+    if (out) memcpy(out + o, src + s, e - s);
+    o += e - s;
+    s = e;
+    if (s >= len) break;
+    ++s;  // the 0xFF
     const uint8_t nx = s < len ? src[s] : 0xFF;
     ++s;
     if (nx == 0x00) {
-      segs.back().push_back(0xFF);
+      if (out) out[o] = 0xFF;
+      ++o;
       continue;
     }
     if ((nx & 0xF8) == 0xD0) {
       if (split) {
         if ((nx & 7) != next_rst) return false;
         next_rst = (next_rst + 1) & 7;
-        segs.emplace_back();
+        if (bounds) bounds->push_back(bounds->back() + o);
+        ++seg;
+        if (dst) out = dst + dst_off[seg];
+        o = 0;
         continue;
       }
-      segs.back().push_back(0xFF);
-      segs.back().push_back(nx);
+      if (out) {
+        out[o] = 0xFF;
+        out[o + 1] = nx;
+      }
+      o += 2;
       continue;
     }
-    segs.back().push_back(0xFF);
+    if (out) out[o] = 0xFF;
+    ++o;
   }
+  if (bounds) bounds->push_back(bounds->back() + o);
   return true;
 }
 
-}  // namespace
+// Sizes of one image's share of the plan (pass 1) and its offsets in the
+// batch (pass 2).
+struct ImageWork {
+  std::vector<size_t> bounds;    // unstuffed segment bounds
+  int64_t nseg = 0, nitems = 0, npix = 0, nwords = 0;
+  int64_t seg0 = 0, item0 = 0, pix0 = 0, word0 = 0, desc = 0, coef0 = 0;
+  uint64_t out0 = 0, plane0[3] = {~0ull, ~0ull, ~0ull};
+};
 
-static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
+// Words of one segment: 16-byte aligned start (the decoder reads 4-word
+// chunks), the data, and kSegPadWords words of 0xFF padding.
+inline int64_t seg_words(size_t dlen) { return (int64_t)((((dlen + 3) / 4 + kSegPadWords) + 3) & ~(size_t)3); }
+
+inline uint32_t ntiles_of(size_t dlen, uint32_t S) {
+  return std::max<uint32_t>(1, (uint32_t)((dlen * 8 + S - 1) / S));
+}
+
+// Pass 1 (per image, in parallel): parse, find the segments, count.
+int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, size_t size) {
+  ParseState st;
+  int e = parse_until_scan(st, file, size, &r.f);
+  if (!e && (!st.at_scan || !st.have_sof)) e = CJD_SYNTAX_ERROR;
+  if (e) return e;
   const cjd_frame_info& f = r.f;
   if (f.blocks_per_mcu > kMaxSlots) return CJD_UNSUPPORTED_ERROR;
-  std::vector<std::vector<uint8_t>> parts;
-  if (!unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, parts))
+  if (!unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, &w.bounds, nullptr,
+                        nullptr))
     return CJD_SYNTAX_ERROR;
   const int64_t nmcu = (int64_t)f.mcus_x * f.mcus_y;
   int64_t nseg_expected = 1;
   if (f.restart_interval > 0) nseg_expected = (nmcu + f.restart_interval - 1) / f.restart_interval;
+  std::vector<size_t>& bd = w.bounds;
   // a trailing RSTn after the last interval leaves an empty extra segment
-  while ((int64_t)parts.size() > nseg_expected && parts.back().empty()) parts.pop_back();
-  if ((int64_t)parts.size() != nseg_expected) return CJD_SYNTAX_ERROR;
+  while ((int64_t)bd.size() - 1 > nseg_expected && bd[bd.size() - 1] == bd[bd.size() - 2]) bd.pop_back();
+  if ((int64_t)bd.size() - 1 != nseg_expected) return CJD_SYNTAX_ERROR;
+  w.nseg = nseg_expected;
+  w.nwords = 0;
+  w.nitems = 0;
+  const uint32_t per_first = b.item_tiles, per_next = b.item_tiles - 1;
+  for (int64_t s = 0; s < w.nseg; ++s) {
+    const size_t dlen = bd[s + 1] - bd[s];
+    w.nwords += seg_words(dlen);
+    const uint32_t nt = ntiles_of(dlen, b.tile_bits);
+    w.nitems += nt <= per_first ? 1 : 1 + (nt - per_first + per_next - 1) / per_next;
+  }
+  const int G = std::max(1, kPixBlocks / f.blocks_per_mcu);
+  w.npix = (int64_t)f.mcus_y * ((f.mcus_x + G - 1) / G);
+  return 0;
+}
 
+// Pass 3 (per image, in parallel): write the image's words, descriptors and
+// work items at the offsets of pass 2.
+void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint8_t* file) {
+  const cjd_frame_info& f = r.f;
+  const int64_t nmcu = (int64_t)f.mcus_x * f.mcus_y;
+  const int nh = f.blocks_per_mcu;
   ImageDesc d{};
   d.width = f.width;
   d.height = f.height;
@@ -107,17 +168,17 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
   d.nslots = f.blocks_per_mcu;
   d.mcu_w = f.mcu_width;
   d.mcu_h = f.mcu_height;
-  d.table_base = (int32_t)b->tables.size();
-  d.qtab_base = (int32_t)(b->qtabs.size() / 64);
+  d.table_base = (int32_t)(w.desc * 4);
+  d.qtab_base = (int32_t)(w.desc * 4);
   int slot = 0;
   for (int c = 0; c < f.num_channels; ++c) {
     d.comp_first_slot[c] = slot;
     d.comp_dq[c] = (uint8_t)f.dq_id[c];
     d.comp_sx[c] = (uint8_t)f.samples_x[c];
     d.comp_sy[c] = (uint8_t)f.samples_y[c];
-    int w = f.comp_width[c], h = f.comp_height[c], xs = 0, ys = 0;
-    while (w < f.width) { w <<= 1; ++xs; }   // CPUonly/src/pixelTransformCPU.c:113-114
-    while (h < f.height) { h <<= 1; ++ys; }
+    int ww = f.comp_width[c], hh = f.comp_height[c], xs = 0, ys = 0;
+    while (ww < f.width) { ww <<= 1; ++xs; }   // CPUonly/src/pixelTransformCPU.c:113-114
+    while (hh < f.height) { hh <<= 1; ++ys; }
     d.comp_xshift[c] = (uint8_t)xs;
     d.comp_yshift[c] = (uint8_t)ys;
     d.comp_stride[c] = f.comp_stride[c];
@@ -133,57 +194,42 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
     d.comp_tix[c] = (uint8_t)t;
   }
   for (int t = 0; t < 4; ++t) {
-    HuffTable ht;
-    build_table(f.huff_bits[t], f.huff_vals[t], ht);
-    b->tables.push_back(ht);
-    for (int k = 0; k < 64; ++k) b->qtabs.push_back(0);
-    uint16_t* q = &b->qtabs[b->qtabs.size() - 64];
+    build_table(f.huff_bits[t], f.huff_vals[t], b->tables[(size_t)(w.desc * 4 + t)]);
+    uint16_t* q = &b->qtabs[(size_t)(w.desc * 4 + t) * 64];
     for (int k = 0; k < 64; ++k) q[kZZ[k]] = f.dq_tables[t][k];
   }
-  d.coef_base = b->total_blocks;
+  d.coef_base = w.coef0;
   d.nblocks = f.num_blocks;
-  r.coef_base = d.coef_base;
-  b->total_blocks += f.num_blocks;
-  const uint64_t ob = (uint64_t)f.width * f.height * (f.num_channels == 3 ? 3 : 1);
-  r.out_offset = b->out_bytes;
-  r.out_bytes = ob;
-  d.out_offset = r.out_offset;
-  b->out_bytes += (ob + 255) & ~255ull;
-  for (int c = 0; c < 3; ++c) d.plane_offset[c] = ~0ull;
-  if (b->want_planes) {
-    for (int c = 0; c < f.num_channels; ++c) {
-      const uint64_t pb = (uint64_t)f.comp_stride[c] * f.comp_plane_height[c];
-      d.plane_offset[c] = r.plane_offset[c] = b->out_bytes;
-      b->out_bytes += (pb + 255) & ~255ull;
-    }
-  }
-  d.error_index = (int32_t)(&r - &b->recs[0]);
-  r.desc = (uint32_t)b->images.size();
-  b->images.push_back(d);
-  r.pixels = (int64_t)f.width * f.height;
-  b->total_pixels += r.pixels;
+  d.out_offset = w.out0;
+  for (int c = 0; c < 3; ++c) d.plane_offset[c] = w.plane0[c];
+  d.error_index = index;
+  b->images[(size_t)w.desc] = d;
 
-  // segments, tiles and entropy work items
-  const int nh = f.blocks_per_mcu;
-  // one lane per tile; an item after the first also decodes its lead-in tile
+  // words: 0xFF padding, then the unstuffed data written straight in
+  std::vector<size_t> dst_off((size_t)w.nseg);
+  int64_t wo = w.word0;
+  uint32_t* words = b->words.data();
+  for (int64_t s = 0; s < w.nseg; ++s) {
+    const size_t dlen = w.bounds[s + 1] - w.bounds[s];
+    const int64_t nw = seg_words(dlen);
+    std::fill(words + wo, words + wo + nw, 0xFFFFFFFFu);
+    dst_off[(size_t)s] = (size_t)wo * 4;
+    wo += nw;
+  }
+  unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, nullptr,
+                   reinterpret_cast<uint8_t*>(words), dst_off.data());
+
+  // segments, tiles and entropy work items: one lane per tile; an item after
+  // the first also decodes its lead-in tile
   const uint32_t per_first = b->item_tiles, per_next = b->item_tiles - 1;
-  r.seg_first = (uint32_t)b->segs.size();
-  for (size_t s = 0; s < parts.size(); ++s) {
-    const std::vector<uint8_t>& data = parts[s];
+  int64_t item = w.item0;
+  for (int64_t s = 0; s < w.nseg; ++s) {
+    const size_t dlen = w.bounds[s + 1] - w.bounds[s];
     SegmentDesc sd{};
-    sd.word_base = b->words.size();
-    sd.nbits = (uint32_t)(data.size() * 8);
-    // 16-byte aligned start (the decoder reads 4-word chunks) and
-    // kSegPadWords words of 0xFF padding past the data
-    b->words.resize((b->words.size() + 3) & ~(size_t)3, 0xFFFFFFFFu);
-    sd.word_base = b->words.size();
-    const size_t nw = (((data.size() + 3) / 4 + kSegPadWords) + 3) & ~(size_t)3;
-    sd.nwords = (uint32_t)nw;
-    const size_t w0 = b->words.size();
-    b->words.resize(w0 + nw, 0xFFFFFFFFu);
-    memcpy(&b->words[w0], data.data(), data.size());
-    b->scan_bytes += (int64_t)data.size();
-    sd.image = r.desc;
+    sd.word_base = dst_off[(size_t)s] / 4;
+    sd.nbits = (uint32_t)(dlen * 8);
+    sd.nwords = (uint32_t)seg_words(dlen);
+    sd.image = (uint32_t)w.desc;
     sd.tile_bits = b->tile_bits;
     // sync points: trajectories from the slot hypotheses mostly agree within
     // a few blocks, and a main trajectory that starts off the true one still
@@ -191,61 +237,172 @@ static int stage_image(Plan* b, ImageRec& r, const uint8_t* file) {
     sd.sync_bits = b->sync_bits ? b->sync_bits : std::max<uint32_t>(64, b->tile_bits / 32);
     sd.sync1_bits = b->sync1_bits ? b->sync1_bits : sd.sync_bits;
     if (sd.sync1_bits > sd.sync_bits) sd.sync1_bits = sd.sync_bits;
-    sd.ntiles = std::max<uint32_t>(1, (sd.nbits + b->tile_bits - 1) / b->tile_bits);
-    const int64_t mcu0 = (int64_t)s * (f.restart_interval > 0 ? f.restart_interval : nmcu);
+    sd.ntiles = ntiles_of(dlen, b->tile_bits);
+    const int64_t mcu0 = s * (f.restart_interval > 0 ? f.restart_interval : nmcu);
     const int64_t mcus = std::min<int64_t>(nmcu - mcu0, f.restart_interval > 0 ? f.restart_interval : nmcu);
     sd.nblocks = (uint32_t)(mcus * nh);
     sd.first_block = d.coef_base + mcu0 * nh;
-    sd.first_item = (uint32_t)b->items.size();
+    sd.first_item = (uint32_t)item;
     uint32_t t0 = 0, prev = ~0u;
     while (t0 < sd.ntiles) {
       WorkItem wi{};
-      wi.segment = (uint32_t)b->segs.size();
+      wi.segment = (uint32_t)(w.seg0 + s);
       wi.t0 = t0;
       wi.ntiles = std::min(sd.ntiles - t0, t0 == 0 ? per_first : per_next);
       wi.prev_item = prev;
-      prev = (uint32_t)b->items.size();
-      b->items.push_back(wi);
+      prev = (uint32_t)item;
+      b->items[(size_t)item++] = wi;
       t0 += wi.ntiles;
     }
-    b->segs.push_back(sd);
+    b->segs[(size_t)(w.seg0 + s)] = sd;
   }
-  r.seg_count = (uint32_t)b->segs.size() - r.seg_first;
 
   // pixel work items: runs of MCUs within one MCU row
   const int G = std::max(1, kPixBlocks / nh);
-  r.pix_first = (uint32_t)b->pix.size();
+  int64_t pi_i = w.pix0;
   for (int my = 0; my < f.mcus_y; ++my)
     for (int mx = 0; mx < f.mcus_x; mx += G) {
       PixItem pi{};
-      pi.image = r.desc;
+      pi.image = (uint32_t)w.desc;
       pi.mcu_first = (uint32_t)(my * f.mcus_x + mx);
       pi.count = (uint32_t)std::min(G, f.mcus_x - mx);
-      b->pix.push_back(pi);
+      b->pix[(size_t)pi_i++] = pi;
     }
-  r.pix_count = (uint32_t)b->pix.size() - r.pix_first;
-  return 0;
 }
+
+// A small persistent pool of host threads for the per-image passes.  run()
+// calls are serialised; the caller takes part and returns once every index
+// is done and no worker is still inside the job.
+class Pool {
+ public:
+  static Pool& get() {
+    static Pool p;
+    return p;
+  }
+  void run(int n, const std::function<void(int)>& fn) {
+    if (workers_.empty() || n < 2) {
+      for (int i = 0; i < n; ++i) fn(i);
+      return;
+    }
+    std::lock_guard<std::mutex> one(run_mu_);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &fn;
+      n_ = n;
+      next_ = 0;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (int i; (i = next_.fetch_add(1)) < n;) fn(i);
+    std::unique_lock<std::mutex> lk(mu_);
+    job_ = nullptr;  // no worker enters from here on
+    done_.wait(lk, [&] { return running_ == 0; });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+
+ private:
+  Pool() {
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int nw = (int)std::min<unsigned>(hc > 1 ? hc - 1 : 0, 15u);
+    for (int k = 0; k < nw; ++k) workers_.emplace_back([this] { loop(); });
+  }
+  void loop() {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_.wait(lk, [&] { return stop_ || (job_ && gen_ != seen); });
+      if (stop_) return;
+      seen = gen_;
+      const std::function<void(int)>* fn = job_;
+      const int n = n_;
+      ++running_;
+      lk.unlock();
+      for (int i; (i = next_.fetch_add(1)) < n;) (*fn)(i);
+      lk.lock();
+      if (--running_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex run_mu_, mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* job_ = nullptr;
+  std::atomic<int> next_{0};
+  int n_ = 0, running_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+}  // namespace
 
 void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes, int* errors) {
   b.recs.assign((size_t)n, ImageRec{});
-  b.images.clear();
-  b.segs.clear();
-  b.items.clear();
-  b.pix.clear();
-  b.tables.clear();
-  b.qtabs.clear();
-  b.words.clear();
-  b.total_blocks = b.total_pixels = b.scan_bytes = 0;
+  std::vector<ImageWork> work((size_t)n);
+  // pass 1: parse and count, per image in parallel
+  Pool::get().run(n, [&](int i) {
+    ImageRec& r = b.recs[(size_t)i];
+    r.error = image_pass1(b, r, work[(size_t)i], files[i], sizes[i]);
+  });
+  // pass 2: offsets, in image order
+  int64_t nimg = 0, nseg = 0, nitems = 0, npix = 0, nwords = 0, nblocks = 0;
+  b.total_pixels = b.scan_bytes = 0;
   b.out_bytes = 0;
   for (int i = 0; i < n; ++i) {
     ImageRec& r = b.recs[(size_t)i];
-    ParseState st;
-    r.error = parse_until_scan(st, files[i], sizes[i], &r.f);
-    if (!r.error && (!st.at_scan || !st.have_sof)) r.error = CJD_SYNTAX_ERROR;
-    if (!r.error) r.error = stage_image(&b, r, files[i]);
     if (errors) errors[i] = r.error;
+    if (r.error) continue;
+    ImageWork& w = work[(size_t)i];
+    const cjd_frame_info& f = r.f;
+    w.desc = nimg++;
+    w.seg0 = nseg;
+    w.item0 = nitems;
+    w.pix0 = npix;
+    w.word0 = nwords;
+    w.coef0 = nblocks;
+    nseg += w.nseg;
+    nitems += w.nitems;
+    npix += w.npix;
+    nwords += w.nwords;
+    nblocks += f.num_blocks;
+    const uint64_t ob = (uint64_t)f.width * f.height * (f.num_channels == 3 ? 3 : 1);
+    w.out0 = r.out_offset = b.out_bytes;
+    r.out_bytes = ob;
+    b.out_bytes += (ob + 255) & ~255ull;
+    if (b.want_planes)
+      for (int c = 0; c < f.num_channels; ++c) {
+        const uint64_t pb = (uint64_t)f.comp_stride[c] * f.comp_plane_height[c];
+        w.plane0[c] = r.plane_offset[c] = b.out_bytes;
+        b.out_bytes += (pb + 255) & ~255ull;
+      }
+    r.coef_base = w.coef0;
+    r.desc = (uint32_t)w.desc;
+    r.seg_first = (uint32_t)w.seg0;
+    r.seg_count = (uint32_t)w.nseg;
+    r.pix_first = (uint32_t)w.pix0;
+    r.pix_count = (uint32_t)w.npix;
+    r.pixels = (int64_t)f.width * f.height;
+    b.total_pixels += r.pixels;
+    b.scan_bytes += (int64_t)w.bounds.back();
   }
+  b.total_blocks = nblocks;
+  b.images.resize((size_t)nimg);
+  b.segs.resize((size_t)nseg);
+  b.items.resize((size_t)nitems);
+  b.pix.resize((size_t)npix);
+  b.tables.resize((size_t)nimg * 4);
+  b.qtabs.resize((size_t)nimg * 4 * 64);
+  b.words.resize((size_t)nwords);
+  // pass 3: fill, per image in parallel
+  Pool::get().run(n, [&](int i) {
+    ImageRec& r = b.recs[(size_t)i];
+    if (!r.error) image_pass3(&b, i, r, work[(size_t)i], files[i]);
+  });
 }
 
 }  // namespace cjd

@@ -103,6 +103,13 @@ int cjd_batch_set_tuning(cjd_batch *b, int sync1_bits, int sync_bits, int item_t
  * failed file is skipped by the decode.  Returns the first device error. */
 int cjd_batch_stage(cjd_batch *b, int n, const uint8_t *const *files, const size_t *sizes, int *errors);
 
+/* cjd_batch_stage with the uploads ordered on `stream` (a hipStream_t) and
+ * not waited for: staging one batch overlaps another batch's decode on
+ * another stream.  No counterpart in the reference (its uploads are
+ * synchronous copies, src/decodeScanGPU.cu:146-151). */
+int cjd_batch_stage_async(cjd_batch *b, int n, const uint8_t *const *files, const size_t *sizes, int *errors,
+                          void *stream);
+
 /* Enqueue the decode of the staged batch on `stream` (asynchronous):
  * entropy decode (replaces decodeScanGPU, src/decodeScanGPU.cu:123) and the
  * fused pixel stage (replaces iDCT_resample_colourTransform,
@@ -144,6 +151,56 @@ int cjd_batch_totals(cjd_batch *b, int64_t *images, int64_t *pixels, int64_t *sc
 
 /* Human-readable name of a reference error code (src/format.cu:62-78). */
 const char *cjd_error_name(int code);
+
+/* ---------------------------------------------------------------- serving --
+ * A request loop (no counterpart in the reference, which decodes one file per
+ * openJPG call, src/main.cu:18-50): JPEG byte streams are submitted as they
+ * arrive; a batcher thread flushes when max_batch requests wait or the oldest
+ * has waited max_wait_us, admits requests while the decoded frames held by
+ * the loop fit frame_budget bytes, stages each batch while earlier ones
+ * decode (`slots` batches, one stream each), and a completion thread hands
+ * finished frames to cjd_server_poll.  Frames stay in device memory until
+ * cjd_server_release; every completed request (also one with an error) must
+ * be released. */
+typedef struct cjd_server cjd_server;
+
+typedef struct {
+  int max_batch;          /* flush at this many waiting requests (0 = 64) */
+  int max_wait_us;        /* ... or when the oldest has waited this long (0 = 2000) */
+  uint64_t frame_budget;  /* bytes of frames admitted and not released (0 = 64 GiB) */
+  int slots;              /* batches in flight or holding frames (0 = 4) */
+  int idct_mode;          /* CJD_IDCT_* */
+  int tile_bits;          /* entropy tile length (0 = default) */
+} cjd_server_options;
+
+typedef struct {
+  uint64_t id;              /* from cjd_server_submit */
+  int error;                /* reference error code (include/format.h numbering) */
+  int batch_size;           /* requests decoded in the same launch */
+  void *frame;              /* device pointer to the RGB / gray frame (NULL on error) */
+  uint64_t frame_bytes;
+  int width, height, channels;
+  int64_t t_arrival_ns;     /* host monotonic clock: submit */
+  int64_t t_launch_ns;      /*   batch launched */
+  int64_t t_ready_ns;       /*   frame ready in device memory */
+} cjd_completion;
+
+int cjd_server_create(cjd_context *ctx, const cjd_server_options *opt, cjd_server **out);
+/* Copies the bytes; *id identifies the request.  Returns a parse error for a
+ * file without a decodable frame header (nothing is queued then). */
+int cjd_server_submit(cjd_server *s, const uint8_t *data, size_t size, uint64_t *id);
+/* Up to max completions, waiting up to wait_us for the first; returns the count. */
+int cjd_server_poll(cjd_server *s, cjd_completion *out, int max, int wait_us);
+int cjd_server_release(cjd_server *s, uint64_t id);
+/* Host monotonic clock of the loop's timestamps, in ns. */
+int64_t cjd_server_now_ns(void);
+/* Replays an arrival trace: file k is submitted at start + arrival_ns[k]
+ * (non-decreasing), every frame is released as it completes;
+ * latency_ns[k] = ready - submit, errors[k] its code; *wall_ns = last ready -
+ * start.  For load tests of the loop. */
+int cjd_server_replay(cjd_server *s, int n, const uint8_t *const *files, const size_t *sizes,
+                      const int64_t *arrival_ns, int64_t *latency_ns, int *errors, int64_t *wall_ns);
+void cjd_server_destroy(cjd_server *s);
 
 #ifdef __cplusplus
 }
