@@ -223,10 +223,18 @@ def main():
     # ------------------------------------------------------------------ plain
     plain_mpix = None
     lat = []
+    tune_table = None
     if not args.no_plain:
+        # per-image entropy parameters for one-image launches, autotuned per
+        # scan-size bucket on a few images of the batch
+        from cudajpeg_amd.autotune import autotune
+        t_at = time.time()
+        tune_table = autotune(files, "latency", IDCT_FLOAT, reps=2, per_bucket=2)
+        log(f"autotune (latency, {time.time() - t_at:.1f}s): {tune_table}")
         per = []
         for f in files:
             d = BatchDecoder(IDCT_FLOAT, tile_bits=args.tile_bits)
+            d.set_tune_table(tune_table)
             assert d.stage([f])[0] == 0
             per.append(d)
         with torch.cuda.stream(stream):
@@ -277,6 +285,7 @@ def main():
                    "scan_mb": round(tot["scan_bytes"] / 1e6, 1), "blocks": tot["blocks"],
                    "batched_idct": args.batched_idct, "plain_idct": "float", "parallelism": "1 GPU"},
         "plain_mpix_s": None if plain_mpix is None else round(plain_mpix, 1),
+        "plain_tune_table": tune_table,
         "ratio_batched_vs_plain": None if plain_mpix is None else round(batched_mpix / plain_mpix, 2),
         "p50_ms": None if not lat else round(float(np.percentile(lat, 50)), 3),
         "p99_ms": None if not lat else round(float(np.percentile(lat, 99)), 3),

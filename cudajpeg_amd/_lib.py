@@ -20,7 +20,7 @@ OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 # Every symbol include/cudajpeg_amd.h and include/format.h declare.
 EXPORTS = [
     "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
-    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
+    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
     "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_finish", "cjd_batch_capture",
     "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
     "cjd_batch_download", "cjd_batch_totals", "cjd_error_name",
@@ -49,6 +49,12 @@ class FrameInfo(ctypes.Structure):
         ("scan_offset", ctypes.c_uint64), ("scan_length", ctypes.c_uint64),
         ("error_line", ctypes.c_int32), ("pad_", ctypes.c_int32),
     ]
+
+
+class TuneEntry(ctypes.Structure):
+    """cjd_tune_entry"""
+    _fields_ = [("max_scan_bytes", ctypes.c_uint64), ("tile_bits", ctypes.c_int), ("sync1_bits", ctypes.c_int),
+                ("sync_bits", ctypes.c_int), ("item_tiles", ctypes.c_int)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -85,6 +91,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_destroy": ([vp], None),
         "cjd_batch_set_options": ([vp, i32, i32, i32, i32], i32),
         "cjd_batch_set_tuning": ([vp, i32, i32, i32], i32),
+        "cjd_batch_set_tune_table": ([vp, i32, ctypes.POINTER(TuneEntry)], i32),
         "cjd_batch_stage": ([vp, i32, vp, vp, vp], i32),
         "cjd_batch_stage_async": ([vp, i32, vp, vp, vp, vp], i32),
         "cjd_batch_decode": ([vp, vp], i32),

@@ -127,6 +127,22 @@ extern "C" int cjd_batch_set_tuning(cjd_batch* b, int sync1_bits, int sync_bits,
 
 // Upload on the stream.  The host vectors belong to the batch and stay
 // unchanged until its next stage, which comes after this decode finished.
+extern "C" int cjd_batch_set_tune_table(cjd_batch* b, int n, const cjd_tune_entry* entries) {
+  if (!b || n < 0 || (n && !entries)) return CJD_PROGRAMMER_ERROR;
+  std::vector<TuneEntry> t;
+  for (int i = 0; i < n; ++i) {
+    const cjd_tune_entry& x = entries[i];
+    if (x.tile_bits < 512 || (x.tile_bits & 31) || x.item_tiles < 2 || x.item_tiles > kWG || x.sync1_bits < 0 ||
+        x.sync_bits < 0)
+      return CJD_PROGRAMMER_ERROR;
+    t.push_back(TuneEntry{x.max_scan_bytes, (uint32_t)x.tile_bits, (uint32_t)x.sync1_bits, (uint32_t)x.sync_bits,
+                          (uint32_t)x.item_tiles});
+  }
+  std::sort(t.begin(), t.end(), [](const TuneEntry& a, const TuneEntry& c) { return a.max_scan_bytes < c.max_scan_bytes; });
+  b->plan.tune = t;
+  return 0;
+}
+
 template <class T>
 static int upload(DevBuf& d, const std::vector<T>& v, hipStream_t s) {
   const size_t bytes = v.size() * sizeof(T);

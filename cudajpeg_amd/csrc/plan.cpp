@@ -107,6 +107,7 @@ bool unstuff_segments(const uint8_t* src, size_t len, bool split, std::vector<si
 // batch (pass 2).
 struct ImageWork {
   std::vector<size_t> bounds;    // unstuffed segment bounds
+  uint32_t S = 4096, D1 = 0, D = 0, items = kWG;  // entropy parameters for this image
   int64_t nseg = 0, nitems = 0, npix = 0, nwords = 0;
   int64_t seg0 = 0, item0 = 0, pix0 = 0, word0 = 0, desc = 0, coef0 = 0;
   uint64_t out0 = 0, plane0[3] = {~0ull, ~0ull, ~0ull};
@@ -141,11 +142,28 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
   w.nseg = nseg_expected;
   w.nwords = 0;
   w.nitems = 0;
-  const uint32_t per_first = b.item_tiles, per_next = b.item_tiles - 1;
+  w.S = b.tile_bits;
+  w.D = b.sync_bits;
+  w.D1 = b.sync1_bits;
+  w.items = b.item_tiles;
+  for (const TuneEntry& t : b.tune)
+    if ((uint64_t)f.scan_length <= t.max_scan_bytes) {
+      w.S = t.tile_bits;
+      w.D1 = t.sync1_bits;
+      w.D = t.sync_bits;
+      w.items = t.item_tiles;
+      break;
+    }
+  // sync points: trajectories from the slot hypotheses mostly agree within a
+  // few blocks, and a main trajectory that starts off the true one still
+  // joins it before the tile end, so a short first sync point suffices
+  if (!w.D) w.D = std::max<uint32_t>(64, w.S / 32);
+  if (!w.D1 || w.D1 > w.D) w.D1 = w.D;
+  const uint32_t per_first = w.items, per_next = w.items - 1;
   for (int64_t s = 0; s < w.nseg; ++s) {
     const size_t dlen = bd[s + 1] - bd[s];
     w.nwords += seg_words(dlen);
-    const uint32_t nt = ntiles_of(dlen, b.tile_bits);
+    const uint32_t nt = ntiles_of(dlen, w.S);
     w.nitems += nt <= per_first ? 1 : 1 + (nt - per_first + per_next - 1) / per_next;
   }
   const int G = std::max(1, kPixBlocks / f.blocks_per_mcu);
@@ -221,7 +239,7 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
 
   // segments, tiles and entropy work items: one lane per tile; an item after
   // the first also decodes its lead-in tile
-  const uint32_t per_first = b->item_tiles, per_next = b->item_tiles - 1;
+  const uint32_t per_first = w.items, per_next = w.items - 1;
   int64_t item = w.item0;
   for (int64_t s = 0; s < w.nseg; ++s) {
     const size_t dlen = w.bounds[s + 1] - w.bounds[s];
@@ -230,14 +248,10 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
     sd.nbits = (uint32_t)(dlen * 8);
     sd.nwords = (uint32_t)seg_words(dlen);
     sd.image = (uint32_t)w.desc;
-    sd.tile_bits = b->tile_bits;
-    // sync points: trajectories from the slot hypotheses mostly agree within
-    // a few blocks, and a main trajectory that starts off the true one still
-    // joins it before the tile end, so a short first sync point suffices
-    sd.sync_bits = b->sync_bits ? b->sync_bits : std::max<uint32_t>(64, b->tile_bits / 32);
-    sd.sync1_bits = b->sync1_bits ? b->sync1_bits : sd.sync_bits;
-    if (sd.sync1_bits > sd.sync_bits) sd.sync1_bits = sd.sync_bits;
-    sd.ntiles = ntiles_of(dlen, b->tile_bits);
+    sd.tile_bits = w.S;
+    sd.sync_bits = w.D;
+    sd.sync1_bits = w.D1;
+    sd.ntiles = ntiles_of(dlen, w.S);
     const int64_t mcu0 = s * (f.restart_interval > 0 ? f.restart_interval : nmcu);
     const int64_t mcus = std::min<int64_t>(nmcu - mcu0, f.restart_interval > 0 ? f.restart_interval : nmcu);
     sd.nblocks = (uint32_t)(mcus * nh);

@@ -25,11 +25,19 @@ struct ImageRec {
   uint32_t pix_first = 0, pix_count = 0;
 };
 
+// Entropy-decoder parameters for images whose scan is at most max_scan_bytes
+// (the first matching entry of Plan::tune applies).
+struct TuneEntry {
+  uint64_t max_scan_bytes;
+  uint32_t tile_bits, sync1_bits, sync_bits, item_tiles;
+};
+
 struct Plan {
   uint32_t tile_bits = 4096;
   uint32_t sync_bits = 0;         // second sync point D in bits, 0 = tile_bits / 32 (>= 64)
   uint32_t sync1_bits = 0;        // first sync point D1 in bits, 0 = D (one stage)
   uint32_t item_tiles = kWG;      // tiles per entropy work item (2 .. kWG)
+  std::vector<TuneEntry> tune;    // per scan-size bucket overrides of the four above
   int want_planes = 0;
   std::vector<ImageRec> recs;
   std::vector<ImageDesc> images;

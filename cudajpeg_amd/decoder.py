@@ -98,6 +98,12 @@ class BatchDecoder:
         stage()."""
         _check(L.lib().cjd_batch_set_tuning(self.h, sync1_bits, sync_bits, item_tiles), "tuning: ")
 
+    def set_tune_table(self, table) -> None:
+        """Per-image entropy parameters by scan size: a list of
+        (max_scan_bytes, tile_bits, sync1_bits, sync_bits, item_tiles)."""
+        arr = (L.TuneEntry * max(1, len(table)))(*[L.TuneEntry(*t) for t in table])
+        _check(L.lib().cjd_batch_set_tune_table(self.h, len(table), arr), "tune table: ")
+
     def stage(self, files: list[bytes]) -> np.ndarray:
         n = len(files)
         self._bufs = [ctypes.create_string_buffer(f, len(f)) for f in files]

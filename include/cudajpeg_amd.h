@@ -98,6 +98,18 @@ int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int til
  * cjd_batch_stage. */
 int cjd_batch_set_tuning(cjd_batch *b, int sync1_bits, int sync_bits, int item_tiles);
 
+/* Per-image tuning by scan size (no counterpart in the reference): an image
+ * whose entropy-coded scan has at most max_scan_bytes bytes uses the first
+ * such entry's tile length, sync points (0 = defaults) and tiles per work
+ * item; images past every entry use the batch options.  n = 0 clears the
+ * table.  Entries come from an autotuner (cudajpeg_amd.autotune).  Takes
+ * effect at the next cjd_batch_stage. */
+typedef struct {
+  uint64_t max_scan_bytes;
+  int tile_bits, sync1_bits, sync_bits, item_tiles;
+} cjd_tune_entry;
+int cjd_batch_set_tune_table(cjd_batch *b, int n, const cjd_tune_entry *entries);
+
 /* Parse, unstuff (src/decodeScanGPU.cu:11-24) and upload n files; plans the
  * entropy and pixel work lists.  errors[i] gets each file's parse result; a
  * failed file is skipped by the decode.  Returns the first device error. */

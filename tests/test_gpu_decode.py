@@ -157,3 +157,22 @@ def test_mfma_idct_error_vs_fp64(gpu):
             assert d.max() <= 1 and (d == 0).mean() > 0.999
             assert np.abs(got - i32[c]).max() <= 1
         b.close()
+
+
+def test_tune_table_per_image_parameters(gpu):
+    # a per-scan-size table puts the small and the large images of one batch
+    # on different tile lengths / item sizes; coefficients stay bit-exact
+    specs = [synth.ImageSpec(96, 64, "420", 75, 6.0, 61), synth.ImageSpec(640, 480, "444", 90, 8.0, 62),
+             synth.ImageSpec(1024, 768, "420", 85, 10.0, 63, random_tile_frac=0.2)]
+    encs = [synth.encode(s, truth=True) for s in specs]
+    table = [(8 << 10, 1024, 0, 0, 16), (256 << 10, 2048, 0, 0, 64), (1 << 62, 8192, 0, 0, 256)]
+    b = BatchDecoder(IDCT_INT)
+    b.set_tune_table(table)
+    assert not b.stage([e.data for e in encs]).any()
+    b.decode()
+    assert not b.finish().any()
+    for i, (s, e) in enumerate(zip(specs, encs)):
+        assert np.array_equal(b.coefficients(i), e.coefs), s
+        ref, _ = _ref_pixels(e.data, e.coefs)
+        assert np.array_equal(b.pixels(i), ref), s
+    b.close()
