@@ -53,8 +53,8 @@ class FrameInfo(ctypes.Structure):
 
 class TuneEntry(ctypes.Structure):
     """cjd_tune_entry"""
-    _fields_ = [("max_scan_bytes", ctypes.c_uint64), ("tile_bits", ctypes.c_int), ("sync1_bits", ctypes.c_int),
-                ("sync_bits", ctypes.c_int), ("item_tiles", ctypes.c_int)]
+    _fields_ = [("max_scan_bytes", ctypes.c_uint64), ("tile_bits", ctypes.c_int), ("ck_bits", ctypes.c_int),
+                ("item_tiles", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -90,7 +90,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_create": ([vp, ctypes.POINTER(vp)], i32),
         "cjd_batch_destroy": ([vp], None),
         "cjd_batch_set_options": ([vp, i32, i32, i32, i32], i32),
-        "cjd_batch_set_tuning": ([vp, i32, i32, i32], i32),
+        "cjd_batch_set_tuning": ([vp, i32, i32], i32),
         "cjd_batch_set_tune_table": ([vp, i32, ctypes.POINTER(TuneEntry)], i32),
         "cjd_batch_stage": ([vp, i32, vp, vp, vp], i32),
         "cjd_batch_stage_async": ([vp, i32, vp, vp, vp, vp], i32),

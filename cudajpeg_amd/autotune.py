@@ -8,8 +8,8 @@ fastest:
   mode "throughput": the bucket's images decoded as one batch; the objective
                      is time per pixel.
 The result is a table for BatchDecoder.set_tune_table / cjd_batch_set_tune_table:
-rows (max_scan_bytes, tile_bits, sync1_bits, sync_bits, item_tiles), sync
-points left at their defaults (tile_bits / 32).
+rows (max_scan_bytes, tile_bits, ck_bits, item_tiles), the checkpoint spacing
+left at its default (tile_bits / 8).
 """
 from __future__ import annotations
 
@@ -26,7 +26,7 @@ CANDIDATES = tuple((s, it) for s in (1024, 2048, 4096, 8192) for it in (16, 64, 
 
 def _time(files, idct, S, items, reps):
     b = BatchDecoder(idct, tile_bits=S)
-    b.set_tuning(0, 0, items)
+    b.set_tuning(0, items)
     if b.stage(files).any():
         b.close()
         return float("inf")
@@ -66,7 +66,7 @@ def autotune(files: list[bytes], mode: str = "latency", idct: int = L.IDCT_FLOAT
                 log(f"autotune {mode} bucket <= {hi} ({len(pick)} images): tile {S} items {items}: {t:.3f} ms")
             if t < best_t:
                 best, best_t = (S, items), t
-        table.append((int(min(hi, (1 << 63) - 1)), best[0], 0, 0, best[1]))
+        table.append((int(min(hi, (1 << 63) - 1)), best[0], 0, best[1]))
     return table
 
 

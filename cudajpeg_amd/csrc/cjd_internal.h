@@ -11,6 +11,23 @@ constexpr int kMaxSlots = 10;               // blocks per MCU (T.81 B.2.3 limit)
 constexpr int kWG = 256;                    // entropy workgroup size (4 wave64)
 constexpr int kSegPadWords = 96;            // 0xFF words after every segment's data
 
+// Tiled decoder lookup tables: per image kLutEntries uint16 -- four first-
+// level tables of 2^kL1Bits entries (DC 0, DC 1, AC 0, AC 1), indexed by the
+// next kL1Bits stream bits, then a pool of second-level tables of 2^kL2Bits
+// entries for codes longer than kL1Bits, shared by the four.  Entry: bits 0-4
+// adv (code length + extra bits), 5-8 size (extra bits), 9-13 run + 1 of an
+// AC coefficient (16 for ZRL; 0 for EOB and DC symbols); 0 = no valid symbol.
+// Bit 15 set: bits 0-14 index the second-level table of this prefix, or
+// kLutSlow when the pool was full (canonical decode instead).
+constexpr int kL1Bits = 10;
+constexpr int kL1Size = 1 << kL1Bits;
+constexpr int kL2Bits = 6;
+constexpr int kL2Size = 1 << kL2Bits;
+constexpr int kL2Pool = 2048;
+constexpr int kLutEntries = 4 * kL1Size + kL2Pool;
+constexpr uint16_t kLutSlow = 0xFFFF;
+constexpr int kSinkWords = 4 * 64 + 64 * 8;  // per-item scratch of the tiled decoder (full streams write here)
+
 // Device Huffman table: a 2^kLutBits direct table for codes up to kLutBits
 // long plus canonical limits for longer codes (T.81 F.2.2.3).  LUT entry:
 // bits 0-4 code length (0 = longer code), bits 8-15 the symbol.  lim[l] is the
@@ -65,8 +82,10 @@ struct SegmentDesc {
   uint32_t nblocks;     // blocks the segment must contain
   int64_t first_block;  // global coefficient-buffer index of its first block
   uint32_t first_item;  // index of its first work item
-  uint32_t sync1_bits;  // first sync point D1: bits into a tile where hypotheses are compared
-  uint32_t sync_bits;   // second sync point D (used when they disagree at D1)
+  uint32_t ck_bits;     // checkpoint spacing C in bits
+  uint32_t nck;         // checkpoints per tile (at C, 2C, ... < S)
+  uint32_t rs;          // steps (entries) of a tile's speculative or prefix stream
+  uint32_t rd;          // entries of a tile's DC list (per stream)
   uint32_t pad_;
 };
 
@@ -77,11 +96,32 @@ struct WorkItem {
   uint32_t t0;
   uint32_t ntiles;
   uint32_t prev_item;   // predecessor in the same segment, ~0 for the first
+  uint64_t ebase;       // entry-buffer word of the item's streams, DC lists and checkpoints
+};
+
+// Stream row of a work item of J local tiles: the lanes of one wave write
+// one row of W consecutive words per step (W = 64, or J rounded up to 16 for
+// J < 64); nl = rows' words over all waves.
+inline uint32_t item_row(uint32_t J) { return J >= 64 ? 64u : (J + 15) & ~15u; }
+inline uint64_t item_lanes(uint32_t J) { return (uint64_t)((J + 63) / 64) * item_row(J); }
+// Entry-buffer words of a work item (layout in entropy_core.h).
+inline uint64_t item_words(uint32_t J, uint32_t rs, uint32_t rd, uint32_t nck) {
+  return item_lanes(J) * (2ull * rs + 2ull * rd + 2ull * nck) + kSinkWords;
+}
+
+// Where the quantised coefficients of one block are, after the entropy
+// decode: AC entries ebuf[eidx + W i] for i < n (bits 16-21 natural index,
+// 0-15 value; 0xFFFFFFFF = none) with n in bits 0-7 of nw and the stream row
+// W = 16 (1 + bits 8-9 of nw), and the absolute DC value.
+struct BlockRec {
+  uint32_t eidx;
+  uint16_t nw;
+  int16_t dc;
 };
 
 // Lookback record of one work item.  The aggregate holds under one
-// assumption: if the item's entry (the exit of its lead-in tile) is the main
-// exit (lead_p, lead_s) of that tile, the exit of its last tile is
+// assumption: if the item's entry (the exit of its lead-in tile) is the
+// speculative exit (lead_p, lead_s) of that tile, the exit of its last tile is
 // (agg_p, agg_s) and the item adds agg_c (blocks, DC sums of components 0-2).
 // The inclusive state is the true exit of the last tile with the segment's
 // cumulative counts there.
@@ -100,7 +140,7 @@ enum : uint32_t {
   kStatusSyntax = 1u,        // invalid Huffman code / coefficient index (reference SYNTAX_ERROR)
   kStatusTruncated = 2u,     // scan ended before all blocks were decoded
   kStatusUnresolved = 4u,    // the decoder state could not be resolved; rerun with the serial decoder
-  kStatusInconsistent = 8u,  // final decode disagreed with the resolved path; rerun serially
+  kStatusInconsistent = 8u,  // the tiled decoder could not hold the image's entries (invalid stream); rerun serially
   kStatusFallback = 0x100u,  // host: the image was decoded again by the serial decoder
 };
 

@@ -58,7 +58,8 @@ struct cjd_batch {
   int entropy_mode = CJD_ENTROPY_TILED;
   Plan plan;
 
-  DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_qtabs, d_coefs, d_out, d_scratch;
+  DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_luts, d_qtabs, d_coefs, d_out, d_scratch;
+  DevBuf d_ebuf, d_recs;  // tiled decoder: entry regions and block records
   size_t scratch_words = 0, status_words = 0, ticket_word = 0;
   std::vector<uint32_t> h_status;
   int64_t fallbacks = 0;
@@ -107,7 +108,7 @@ extern "C" void cjd_batch_destroy(cjd_batch* b) { delete b; }
 extern "C" int cjd_batch_set_options(cjd_batch* b, int idct_mode, int entropy_mode, int tile_bits, int want_planes) {
   if (!b) return CJD_PROGRAMMER_ERROR;
   if (idct_mode < 0 || idct_mode > 2 || entropy_mode < 0 || entropy_mode > 1) return CJD_PROGRAMMER_ERROR;
-  if (tile_bits && (tile_bits < 512 || (tile_bits & 31))) return CJD_PROGRAMMER_ERROR;
+  if (tile_bits && (tile_bits < 512 || tile_bits > 8192 || (tile_bits & 31))) return CJD_PROGRAMMER_ERROR;
   b->idct_mode = idct_mode;
   b->entropy_mode = entropy_mode;
   if (tile_bits) b->plan.tile_bits = (uint32_t)tile_bits;
@@ -115,12 +116,11 @@ extern "C" int cjd_batch_set_options(cjd_batch* b, int idct_mode, int entropy_mo
   return 0;
 }
 
-extern "C" int cjd_batch_set_tuning(cjd_batch* b, int sync1_bits, int sync_bits, int item_tiles) {
+extern "C" int cjd_batch_set_tuning(cjd_batch* b, int ck_bits, int item_tiles) {
   if (!b) return CJD_PROGRAMMER_ERROR;
-  if (sync1_bits < 0 || sync_bits < 0 || item_tiles < 0 || item_tiles == 1 || item_tiles > kWG)
+  if (ck_bits < 0 || (ck_bits && ck_bits < 64) || item_tiles < 0 || item_tiles == 1 || item_tiles > kWG)
     return CJD_PROGRAMMER_ERROR;
-  b->plan.sync1_bits = (uint32_t)sync1_bits;
-  b->plan.sync_bits = (uint32_t)sync_bits;
+  b->plan.ck_bits = (uint32_t)ck_bits;
   b->plan.item_tiles = item_tiles ? (uint32_t)item_tiles : (uint32_t)kWG;
   return 0;
 }
@@ -132,11 +132,10 @@ extern "C" int cjd_batch_set_tune_table(cjd_batch* b, int n, const cjd_tune_entr
   std::vector<TuneEntry> t;
   for (int i = 0; i < n; ++i) {
     const cjd_tune_entry& x = entries[i];
-    if (x.tile_bits < 512 || (x.tile_bits & 31) || x.item_tiles < 2 || x.item_tiles > kWG || x.sync1_bits < 0 ||
-        x.sync_bits < 0)
+    if (x.tile_bits < 512 || x.tile_bits > 8192 || (x.tile_bits & 31) || x.item_tiles < 2 || x.item_tiles > kWG ||
+        x.ck_bits < 0 || (x.ck_bits && x.ck_bits < 64))
       return CJD_PROGRAMMER_ERROR;
-    t.push_back(TuneEntry{x.max_scan_bytes, (uint32_t)x.tile_bits, (uint32_t)x.sync1_bits, (uint32_t)x.sync_bits,
-                          (uint32_t)x.item_tiles});
+    t.push_back(TuneEntry{x.max_scan_bytes, (uint32_t)x.tile_bits, (uint32_t)x.ck_bits, (uint32_t)x.item_tiles});
   }
   std::sort(t.begin(), t.end(), [](const TuneEntry& a, const TuneEntry& c) { return a.max_scan_bytes < c.max_scan_bytes; });
   b->plan.tune = t;
@@ -179,6 +178,10 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   if (!e) e = upload(b->d_items, b->plan.items, s);
   if (!e) e = upload(b->d_pix, b->plan.pix, s);
   if (!e) e = upload(b->d_tables, b->plan.tables, s);
+  if (!e) e = upload(b->d_luts, b->plan.luts, s);
+  if (!e && b->plan.entry_words >= ((int64_t)1 << 32)) e = CJD_OOM_ERROR;  // records hold 32-bit entry indexes
+  if (!e) e = b->d_ebuf.ensure((size_t)b->plan.entry_words * 4);
+  if (!e) e = b->d_recs.ensure((size_t)b->plan.total_blocks * sizeof(BlockRec));
   if (!e) e = upload(b->d_qtabs, b->plan.qtabs, s);
   if (!e) e = b->d_coefs.ensure((size_t)b->plan.total_blocks * 128);
   if (!e) e = b->d_out.ensure(b->plan.out_bytes);
@@ -203,10 +206,15 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
     he = launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                                (uint32_t)b->plan.segs.size(), b->d_tables.as<HuffTable>(), b->d_coefs.as<int16_t>(),
                                status, s);
-  else
+  else {
     he = launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                         b->d_items.as<WorkItem>(), (uint32_t)b->plan.items.size(), b->d_tables.as<HuffTable>(),
-                        b->d_coefs.as<int16_t>(), status, lb, ticket, s);
+                        b->d_luts.as<uint16_t>(), b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
+                        ticket, s);
+    if (he == hipSuccess)
+      he = launch_expand(b->d_recs.as<BlockRec>(), b->d_ebuf.as<uint32_t>(), (uint64_t)b->plan.entry_words,
+                         b->d_coefs.as<int16_t>(), (uint64_t)b->plan.total_blocks, s);
+  }
   if (he != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   if (timed && hipEventRecord(b->ev[1], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   return 0;

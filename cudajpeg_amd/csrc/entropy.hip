@@ -2,8 +2,10 @@
 //
 // k_entropy runs the phases of entropy_core.h for one work item (a run of
 // tiles of one restart segment) per 256-lane workgroup, with a decoupled
-// lookback across workgroups; k_entropy_serial is the one-lane-per-segment
-// decoder used when the final decode disagrees with the chain.  Together they replace
+// lookback across workgroups, and leaves a record per block; k_expand turns
+// the records into coefficient blocks; k_entropy_serial is the one-lane-per-
+// segment decoder (CJD_ENTROPY_SERIAL, and images the tiled decoder hands
+// back).  Together they replace
 // decodeScanGPU (src/decodeScanGPU.cu:123-491) and the speculative kernels of
 // src/entropyRLEdecodeGPU.cu:7-238.
 #include <hip/hip_runtime.h>
@@ -25,22 +27,51 @@ __global__ void k_clear_u32(uint32_t* __restrict__ p, uint64_t n) {
     p[i] = 0;
 }
 
+// Run the sh.ntask queued link tasks in rounds; after each round the
+// unfinished ones are packed into the lowest lanes (all lanes call this).
+__device__ __forceinline__ void run_links(EntShared& sh, int tid, const EntCtx& c) {
+  for (;;) {
+    __syncthreads();  // tasks and their count written
+    const int n = sh.ntask;
+    if (!n) break;
+    Task T{};
+    if (tid < n) T = task_get(sh, tid);
+    if (tid == 0) sh.nnext = 0;
+    __syncthreads();  // every task read before any is put back
+    if (tid < n) {
+      Lane L;
+      int j;
+      if (!task_round(sh, tid, c, T, L, j)) task_put(sh, atomicAdd(&sh.nnext, 1), j, L);
+    }
+    __syncthreads();
+    if (tid == 0) sh.ntask = sh.nnext;
+  }
+}
+
+// Queue a link task for tile tid when `need` (all lanes call this).
+__device__ __forceinline__ void queue_links(EntShared& sh, int tid, bool need) {
+  if (tid == 0) sh.ntask = 0;
+  __syncthreads();
+  if (need) task_new(sh, atomicAdd(&sh.ntask, 1), tid);
+}
+
 // Fix-up rounds (all lanes): relink every tile whose entry changed, until no
 // tile changes.
 __device__ __forceinline__ void fix_rounds(EntShared& sh, int tid, const EntCtx& c) {
   for (;;) {
     uint32_t p = 0, s = 0;
     const bool need = ent_fix_need(sh, tid, c, p, s);
-    if (tid == 0) sh.nchg = 0;
-    __syncthreads();  // reads of the previous state done, counter reset
-    if (need) {
-      ent_fix_apply(sh, tid, c, p, s);
-      atomicAdd(&sh.nchg, 1);
-    }
+    __syncthreads();  // reads of the previous state done
+    const bool task = need && link_begin(sh, c, tid, p, s);
+    queue_links(sh, tid, task);
     __syncthreads();
-    const int n = sh.nchg;
-    __syncthreads();  // every lane has read the counter before the next reset
-    if (!n) break;
+    const int n = sh.ntask;
+    if (!n) {
+      // a relink may have joined at once without a task: the exits moved, check again
+      if (!__syncthreads_or(need)) break;
+      continue;
+    }
+    run_links(sh, tid, c);
   }
 }
 
@@ -62,30 +93,41 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
                                                  const SegmentDesc* __restrict__ segs,
                                                  const WorkItem* __restrict__ items, uint32_t nitems,
                                                  const HuffTable* __restrict__ tables,
-                                                 int16_t* __restrict__ coefs, uint32_t* __restrict__ status,
+                                                 const uint16_t* __restrict__ luts, uint32_t* __restrict__ ebuf,
+                                                 BlockRec* __restrict__ recs, uint32_t* __restrict__ status,
                                                  LookbackRec* __restrict__ lb, uint32_t* __restrict__ ticket) {
   __shared__ EntShared sh;
   const int tid = threadIdx.x;
   // Work items are taken in ticket order, so every predecessor an item waits
   // for in the lookback has already started.
-  if (tid == 0) sh.item = atomicAdd(ticket, 1u);
+  if (tid == 0) {
+    sh.item = atomicAdd(ticket, 1u);
+    sh.lovf = 0;
+  }
   __syncthreads();
   const uint32_t item_id = sh.item;
   if (item_id >= nitems) return;
   EntCtx c;
-  c.init(words, images, segs, items[item_id]);
+  c.init(words, images, segs, tables, ebuf, items[item_id]);
   {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(tables + c.im->table_base);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(sh.tab);
-    for (int i = tid; i < (int)(sizeof(HuffTable) * 4 / 4); i += kWG) dst[i] = src[i];
+    const uint4* src = reinterpret_cast<const uint4*>(luts + (size_t)c.sg.image * kLutEntries);
+    uint4* dst = reinterpret_cast<uint4*>(sh.lut);
+    for (int i = tid; i < kLutEntries / 8; i += kWG) dst[i] = src[i];
     if (tid < 64) sh.dz[tid] = kDeZigZag[tid];
+    if (tid < 4) canon_fill(sh.canon[tid], c.tabs[tid]);
+    if (tid < c.nh) {
+      sh.tsel[2 * tid] = (uint16_t)(c.im->slot_ac[tid] * kL1Size);
+      sh.tsel[2 * tid + 1] = (uint16_t)(c.im->slot_dc[tid] * kL1Size);
+    }
   }
   __syncthreads();
-  ent_phase1(sh, tid, c);
+  ent_spec(sh, tid, c);
   __syncthreads();
-  ent_phase2(sh, tid, c);
-  __syncthreads();
+  queue_links(sh, tid, ent_phase2(sh, tid, c));
+  run_links(sh, tid, c);
   fix_rounds(sh, tid, c);
+  ent_count(sh, tid, c);
+  __syncthreads();
   scan_counts(sh, tid, c);
   LookbackRec* me = lb + item_id;
   if (tid == 0 && c.lead) {
@@ -122,11 +164,13 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   __syncthreads();
   if (ent_entry_differs(sh, c)) {  // uniform: the chain is redone from the true entry
     if (tid == 0) {
-      sh.tp[0] = sh.entry_p;
-      sh.ts[0] = sh.entry_s;
+      sh.xp[0] = sh.entry_p;
+      sh.xs[0] = (uint8_t)sh.entry_s;
     }
     __syncthreads();
     fix_rounds(sh, tid, c);
+    ent_count(sh, tid, c);
+    __syncthreads();
     scan_counts(sh, tid, c);
   }
   ent_absolute(sh, tid, c);
@@ -135,7 +179,41 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
     ent_inclusive(sh, c, me);
     st_status(&me->status, 2);
   }
-  ent_phase5(sh, tid, c, coefs, status);
+  ent_records(sh, tid, c, recs, status);
+}
+
+// Block records -> quantised coefficients (natural order, absolute DC), 256
+// consecutive blocks per workgroup, written as one contiguous 32 KB run.  A
+// block's AC entries are one stream row apart (one per step of its tile's
+// stream).
+__global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ recs, const uint32_t* __restrict__ ebuf,
+                                                uint64_t ewords, int16_t* __restrict__ coefs, uint64_t nblocks) {
+  __shared__ uint4 blk[256 * 8];
+  const int tid = threadIdx.x;
+  const uint64_t b0 = (uint64_t)blockIdx.x * 256;
+  const uint4 z = make_uint4(0, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) blk[tid * 8 + i] = z;
+  const uint64_t b = b0 + tid;
+  if (b < nblocks) {
+    const BlockRec r = recs[b];
+    const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
+    const uint64_t e0 = r.eidx + (uint64_t)n * W <= ewords ? r.eidx : 0;
+    int16_t* B = reinterpret_cast<int16_t*>(blk + tid * 8);
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t e = ebuf[e0 + (uint64_t)i * W];
+      if (e != 0xFFFFFFFFu) B[(e >> 16) & 63] = (int16_t)(e & 0xFFFFu);
+    }
+    B[0] = r.dc;
+  }
+  __syncthreads();
+  uint4* out = reinterpret_cast<uint4*>(coefs + b0 * 64);
+  const uint64_t nq = (nblocks - b0 < 256 ? nblocks - b0 : 256) * 8;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t q = (uint32_t)(tid + 256 * i);
+    if (q < nq) out[q] = blk[q];
+  }
 }
 
 __global__ void k_entropy_serial(const uint32_t* __restrict__ words, const ImageDesc* __restrict__ images,
@@ -156,11 +234,20 @@ hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s) {
 }
 
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
-                          const WorkItem* items, uint32_t nitems, const HuffTable* tables, int16_t* coefs,
-                          uint32_t* status, LookbackRec* lb, uint32_t* ticket, hipStream_t s) {
+                          const WorkItem* items, uint32_t nitems, const HuffTable* tables, const uint16_t* luts,
+                          uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb, uint32_t* ticket,
+                          hipStream_t s) {
   if (!nitems) return hipSuccess;
-  hipLaunchKernelGGL(k_entropy, dim3(nitems), dim3(kWG), 0, s, words, images, segs, items, nitems, tables, coefs,
-                     status, lb, ticket);
+  hipLaunchKernelGGL(k_entropy, dim3(nitems), dim3(kWG), 0, s, words, images, segs, items, nitems, tables, luts, ebuf,
+                     recs, status, lb, ticket);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand(const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, int16_t* coefs, uint64_t nblocks,
+                         hipStream_t s) {
+  if (!nblocks) return hipSuccess;
+  hipLaunchKernelGGL(k_expand, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, s, recs, ebuf, ewords, coefs,
+                     nblocks);
   return hipGetLastError();
 }
 

@@ -11,30 +11,25 @@
 // state passes through.  Huffman streams self-synchronise: trajectories
 // started at different states soon share a boundary and are identical from
 // there on.
-//  1. Main trajectory of each tile: from the tile's first bit one trajectory
-//     per MCU slot h ("hypothesis" h: a block of slot h starts there) is
-//     decoded until its first boundary at or after the sync point (D bits into
-//     the tile).  The boundary most of them reach is the tile's D-state; the
-//     main trajectory continues from it to the tile's exit (its first
-//     boundary at or after the tile end), counting blocks and DC sums.
-//  2. Link: from the main exit of tile j-1 a lane decodes into tile j up to
-//     its first boundary at or after the sync point.  If that is the D-state,
-//     the link has joined the main trajectory: tile j's exit is its main exit.
-//     Otherwise the lane decodes on to the tile end (a concrete exit).
-//  3. Chain: assuming the item's entry is the main exit of its lead-in tile
-//     (the previous item's last tile), every tile's exit follows from the
-//     links; a tile whose link did not start from its true entry is decoded
-//     again from that entry (rare).  Counts are prefix-summed.  Across items a
-//     decoupled lookback composes these single-state aggregates ("if the
-//     entry is E then the exit is X, plus counts") back to a predecessor that
-//     published its true exit.
-//  4. Every tile is decoded once more from its true entry (bit, slot, block
-//     index, DC predictors), writing quantised coefficients, and its exit is
-//     checked against the chain.
+//  1. Speculative pass: every tile is decoded from (its first bit, slot 0) to
+//     its exit (the first block boundary at or after the tile end), writing
+//     one entry per symbol and a checkpoint (state, entries so far) at its
+//     first block boundary past every C bits.
+//  2. Link: from its entry -- the exit of tile j-1 -- tile j is decoded into
+//     a prefix region until the trajectory passes through one of the
+//     speculative checkpoints; from there the tile is its speculative pass
+//     (same exit, same entries).  A link that never joins decodes to the tile
+//     end.  Tiles whose entry changes are linked again in fix-up rounds.
+//  3. Counts (blocks, DC sums) of every tile come from its DC entries and are
+//     prefix-summed.  Across items a decoupled lookback composes single-state
+//     aggregates ("if the entry is E then the exit is X, plus counts") back
+//     to a predecessor that published its true exit.
+//  4. Every block on the true trajectory gets a record: where its AC entries
+//     are and its absolute DC value (k_expand turns records into blocks).
 //
 // Symbol semantics follow src/decodeScanGPU.cu:28-100 bit for bit on valid
-// streams.  During speculation an invalid code ends the block (an invalid DC
-// code consumes one bit), so every trajectory advances.
+// streams.  An invalid code ends the block (an invalid DC code consumes one
+// bit), so every trajectory advances.
 #pragma once
 #include <stdint.h>
 #include <string.h>
@@ -42,6 +37,7 @@
 #include "cjd_internal.h"
 
 #if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define CJD_HD __host__ __device__
 #else
 #define CJD_HD
@@ -245,32 +241,74 @@ CJD_HD inline void status_or(uint32_t* p, uint32_t v) {
 #endif
 }
 
-// ------------------------------------------------------------ LDS state --
-// Decoder state at a block boundary packed as bit << 4 | slot.
-CJD_HD inline uint64_t pack_state(uint32_t p, uint32_t s) { return ((uint64_t)p << 4) | s; }
+// ------------------------------------------------------ tiled decoder --
+//
+// Lookup entry of a code of `len` bits for symbol `sym` (layout in
+// cjd_internal.h).  AC symbols of size 0 other than EOB (0x00) and ZRL (0xF0)
+// get 0: the reference raises SYNTAX_ERROR for them (src/decodeScanGPU.cu:91).
+CJD_HD inline uint16_t lut_entry(int len, int sym, bool ac) {
+  const int size = sym & 15;
+  if (!ac) return (uint16_t)((len + size) | (size << 5));
+  if (sym == 0x00) return (uint16_t)len;
+  if (sym == 0xF0) return (uint16_t)(len | (16 << 9));
+  if (!size) return 0;
+  return (uint16_t)((len + size) | (size << 5) | (((sym >> 4) + 1) << 9));
+}
+
+constexpr int kRingRows = 9;  // 8 word rows + a copy of row 0
+
+// Canonical decode of codes longer than kL1Bits bits of one table (T.81
+// F.2.2.3): lim[l - 11] is the first left-justified 16-bit value past all
+// codes of length <= l, valoff[l - 11] the index of the first symbol of
+// length l minus the first code of length l.
+struct CanonLong {
+  uint32_t lim[6];
+  int32_t valoff[6];
+  uint8_t vals[256];
+};
+
+CJD_HD inline void canon_fill(CanonLong& d, const HuffTable& t) {
+  for (int l = 11; l <= 16; ++l) {
+    d.lim[l - 11] = t.lim[l];
+    d.valoff[l - 11] = t.valoff[l];
+  }
+  for (int k = 0; k < 256; ++k) d.vals[k] = t.vals[k];
+}
 
 struct EntShared {
-  HuffTable tab[4];
-  uint8_t dz[64];                 // zigzag index -> natural index
-  uint32_t ep[kWG], es[kWG];      // main exit of each local tile (bit, slot)
-  uint32_t tp[kWG], ts[kWG];      // chain: exit of tile j (assumed, then true)
-  int32_t tc[kWG][4];             // chain: cumulative blocks / DC sums at the exit of tile j
-                                  //   (relative to the item entry, then absolute)
+  uint16_t lut[kLutEntries];
+  CanonLong canon[4];              // codes longer than kL1Bits when the second-level pool was full
+  uint16_t tsel[2 * kMaxSlots];    // first-level base of the table for (slot, next symbol is DC)
+  uint8_t dz[64];                  // zigzag index -> natural index
+  uint32_t ring[kRingRows][kWG];   // each lane's window of its stream: word w in row w & 7
+  // per local tile
+  uint32_t sxp[kWG];               // speculative exit (bit)
+  uint32_t xp[kWG];                // chain exit: assumed, then true
+  uint32_t lp[kWG];                // entry the tile's link was decoded from
+  uint8_t sxs[kWG], xs[kWG], ls[kWG];  // their slots
+  uint8_t nck[kWG];                // usable checkpoints of the speculative pass
+  uint8_t jslot[kWG];              // slot of the first suffix block
+  uint16_t ss[kWG], sdn[kWG];      // speculative pass: steps, blocks (DC entries)
+  uint16_t ps[kWG], pdn[kWG];      // link prefix: steps, blocks
+  uint16_t js[kWG], jdn[kWG];      // speculative steps and blocks before the join
   union {
-    int16_t coef[kWG][64];        // final decode: one block per lane
     struct {
-      uint64_t hyp[kMaxSlots][kWG];  // hypothesis states at a sync point
-      uint64_t dstate[kWG];          // the tile's D-state (main trajectory)
-      uint32_t dthr[kWG];            // the sync point it refers to
-      uint32_t up[kWG], us[kWG];     // chain: entry tile j's exit was computed from
-      int32_t mc[kWG][4];            // main counts from the D-state to the exit
-      int32_t cnt[kWG][4];           // chain: blocks / DC sums gained in tile j
-    } a;
+      int32_t cnt[kWG][4];         // blocks / DC sums of components 0-2 of tile j
+      int32_t tc[kWG][4];          // their inclusive prefix sums
+    } c;
+    struct {                       // link tasks (during the link and fix-up phases)
+      uint32_t pos[kWG];
+      uint32_t st[kWG];            // slot | (k + 1) << 4 | checkpoint << 11 | tile << 19
+      uint32_t sd[kWG];            // step | DC entries << 16
+      uint32_t last[kWG];          // DC-list entry of the current block
+    } t;
   } u;
+  int32_t ntask, nnext;
   uint32_t item;
-  int32_t nchg;                   // tiles relinked in the current fix-up round
-  uint32_t entry_p, entry_s;      // true state at the exit of the lead-in tile
-  int32_t entry_c[4];             //   and the segment's cumulative counts there
+  int32_t nchg;                    // tiles relinked in the current fix-up round
+  uint32_t lovf;                   // a link needed more entries than its tile has bits
+  uint32_t entry_p, entry_s;       // true state at the exit of the lead-in tile
+  int32_t entry_c[4];              //   and the segment's cumulative counts there
 };
 
 // Constants of one work item.
@@ -278,42 +316,455 @@ struct EntCtx {
   const uint32_t* w;
   uint32_t nwords;
   const ImageDesc* im;
+  const HuffTable* tabs;  // the image's canonical tables (codes the lookup tables could not hold)
+  uint32_t* ebuf;
   SegmentDesc sg;
   WorkItem it;
-  int nh;       // hypotheses = blocks per MCU
-  uint32_t S;   // tile bits
-  uint32_t D1;  // first sync point, bits into a tile
-  uint32_t D;   // second sync point
+  int nh;
+  uint32_t S;
   int lead;     // 1 when local tile 0 is the lead-in (last tile of the previous item)
   int J;        // local tiles including the lead-in
   uint32_t tfirst;
-  uint64_t slotinfo;  // 4 bits per MCU slot: DC table (bit 0), AC table - 2 (bit 1), component (bits 2-3)
+  uint64_t slotinfo;  // 4 bits per MCU slot: component in bits 2-3
 
-  CJD_HD void init(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs, const WorkItem& item) {
+  CJD_HD void init(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
+                   const HuffTable* tables, uint32_t* entries, const WorkItem& item) {
     it = item;
     sg = segs[item.segment];
     im = images + sg.image;
+    tabs = tables + im->table_base;
+    ebuf = entries;
     nh = im->nslots;
     S = sg.tile_bits;
-    D1 = sg.sync1_bits;
-    D = sg.sync_bits;
     w = words + sg.word_base;
     nwords = sg.nwords;
     lead = item.t0 > 0 ? 1 : 0;
     J = (int)item.ntiles + lead;
     tfirst = item.t0 - (uint32_t)lead;
     slotinfo = 0;
-    for (int k = 0; k < nh; ++k)
-      slotinfo |= (uint64_t)((im->slot_dc[k] & 1) | ((im->slot_ac[k] & 1) << 1) | (im->slot_comp[k] << 2))
-                  << (4 * k);
+    for (int k = 0; k < nh; ++k) slotinfo |= (uint64_t)(im->slot_comp[k] << 2) << (4 * k);
   }
   CJD_HD int comp(int slot) const { return (int)((slotinfo >> (4 * slot + 2)) & 3); }
   CJD_HD uint32_t tile_end(uint32_t t) const { return (t + 1 >= sg.ntiles) ? sg.nbits : (t + 1) * S; }
-  CJD_HD uint32_t sync_point(uint32_t t, uint32_t d) const {
-    const uint32_t e = tile_end(t), q = t * S + d;
-    return q < e ? q : e;
+  // Item layout (words from it.ebase), W = stream row (64, or J rounded up
+  // to 16 below 64 tiles), nl = W * waves:
+  //   speculative streams  nl * rs: entry of step s of local tile j at
+  //                        ((j / 64) * rs + s) * W + j % 64 (a wave's lanes
+  //                        write one contiguous row per step)
+  //   prefix streams       nl * rs, same layout
+  //   speculative DC lists nl * rd (tile j at j * rd), prefix DC lists nl * rd
+  //   checkpoints          nl * 2 * nck (tile j at j * 2 * nck)
+  //   sink                 kSinkWords
+  CJD_HD uint32_t W() const { return J >= 64 ? 64u : ((uint32_t)J + 15) & ~15u; }
+  CJD_HD uint64_t nl() const { return (uint64_t)((J + 63) / 64) * W(); }
+  CJD_HD uint32_t* sstream(int j) const {
+    return ebuf + it.ebase + (uint64_t)(j >> 6) * sg.rs * W() + (uint32_t)(j & 63);
   }
+  CJD_HD uint32_t* pstream(int j) const { return sstream(j) + nl() * sg.rs; }
+  CJD_HD uint32_t* sdcl(int j) const { return ebuf + it.ebase + 2 * nl() * sg.rs + (uint64_t)j * sg.rd; }
+  CJD_HD uint32_t* pdcl(int j) const { return sdcl(j) + nl() * sg.rd; }
+  CJD_HD uint32_t* ckpts(int j) const {
+    return ebuf + it.ebase + 2 * nl() * (sg.rs + sg.rd) + (uint64_t)j * 2 * sg.nck;
+  }
+  CJD_HD uint32_t* sink() const { return ebuf + it.ebase + 2 * nl() * (sg.rs + sg.rd + sg.nck); }
+  // is local tile j the segment's first tile (decoded from its known start)?
+  CJD_HD bool seg_first(int j) const { return tfirst + (uint32_t)j == 0; }
 };
+
+// Per-lane decoder of one trajectory.  The lane's next stream words sit in
+// its column of sh.ring: word w in row w & 7 (byte-swapped to MSB-first), row
+// 8 repeating row 0, so that any 32 bits at or after `pos` are two adjacent
+// rows.  Words c .. c+7 are resident; words c+8 .. c+11 (f0-f3) are in
+// flight.  Decoding runs in periods of kPeriod symbols: at the top of a
+// period, once the lane is past word c+3, the in-flight chunk replaces words
+// c .. c+3 and the next one is requested.  A symbol consumes at most 31 bits,
+// so from pos < 32(c+4) the kPeriod symbols of a period read at most up to
+// bit 32(c+4) - 1 + 31 (kPeriod - 1) + 31 < 32(c+8): inside the window.
+//
+// Every step writes one entry into the lane's stream at step index s, the
+// same for all lanes of a wave, so that the wave writes one contiguous row:
+// (natural index << 16 | value) of an AC coefficient; EOB and ZRL write a zero
+// value at a position that no later coefficient of the block uses; the entry
+// of a DC step is not used.  A DC symbol also appends (its step << 17 | value)
+// to the lane's DC list, with bit 16 set when the block has an error (an
+// error inside a block sets it in the block's list entry).
+struct Lane {
+  uint32_t pos, c;
+  uint32_t f0, f1, f2, f3;
+  int k;         // -1 when the next symbol is a DC symbol, else the last AC index
+  int slot;
+  uint32_t s;    // step index
+  uint32_t dn;   // DC-list entries
+  uint32_t lastdc;
+  uint64_t si, di;     // entry-buffer words of stream step 0 and DC-list entry 0
+  uint32_t R, Rd;      // stream steps and DC-list entries available
+  uint32_t W;          // stream row: words between steps
+  bool ovf;      // a stream or list was full: writing into the item's sink
+  bool hit;      // a capture happened in this period
+  int m;         // next checkpoint
+  uint32_t cap_p, cap_i;  // the capture: bit, slot | step << 4 | DC entries << 18
+};
+
+
+CJD_HD inline void ld4(const EntCtx& c, uint32_t wi, uint32_t& a, uint32_t& b, uint32_t& d, uint32_t& e) {
+  wi = wi < c.nwords - 4 ? wi : c.nwords - 4;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint4 v = *reinterpret_cast<const uint4*>(c.w + wi);
+  a = v.x;
+  b = v.y;
+  d = v.z;
+  e = v.w;
+#else
+  a = c.w[wi];
+  b = c.w[wi + 1];
+  d = c.w[wi + 2];
+  e = c.w[wi + 3];
+#endif
+}
+
+CJD_HD inline void ring_put(EntShared& sh, int tid, uint32_t row, uint32_t a, uint32_t b, uint32_t d, uint32_t e) {
+  sh.ring[row][tid] = bswap32(a);
+  sh.ring[row + 1][tid] = bswap32(b);
+  sh.ring[row + 2][tid] = bswap32(d);
+  sh.ring[row + 3][tid] = bswap32(e);
+  sh.ring[row ? row : 8][tid] = bswap32(a);  // row 8 repeats row 0 (else row is written twice)
+}
+
+CJD_HD inline void lane_seek(EntShared& sh, int tid, const EntCtx& c, Lane& L, uint32_t p) {
+  L.pos = p;
+  L.c = (p >> 5) & ~3u;
+  uint32_t a, b, d, e;
+  ld4(c, L.c, a, b, d, e);
+  ring_put(sh, tid, L.c & 7, a, b, d, e);
+  ld4(c, L.c + 4, a, b, d, e);
+  ring_put(sh, tid, (L.c + 4) & 7, a, b, d, e);
+  ld4(c, L.c + 8, L.f0, L.f1, L.f2, L.f3);
+#if defined(__HIP_DEVICE_COMPILE__)
+  // a seek is rare: wait here for the chunk in flight, so that the decode
+  // loop's own prefetch only ever waits for the load of the previous period
+  __builtin_amdgcn_s_waitcnt(0);
+#endif
+}
+
+CJD_HD inline void lane_rotate(EntShared& sh, int tid, const EntCtx& c, Lane& L) {
+  if (L.pos >= (L.c + 4) * 32) {
+    ring_put(sh, tid, L.c & 7, L.f0, L.f1, L.f2, L.f3);
+    L.c += 4;
+  }
+  ld4(c, L.c + 8, L.f0, L.f1, L.f2, L.f3);  // re-requests the same chunk when nothing moved
+}
+
+// The 32 bits at `pos`: rows r, r+1 with r = word of bit pos-1, shifted so
+// that bit pos comes first (at pos = 32w the shift is 0 and row r is unused).
+CJD_HD inline uint32_t lane_peek(const EntShared& sh, int tid, uint32_t pos) {
+  const uint32_t r = ((pos - 1) >> 5) & 7;
+  const uint32_t hi = sh.ring[r][tid], lo = sh.ring[r + 1][tid];
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit(hi, lo, (0u - pos) & 31);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> ((0u - pos) & 31));
+#endif
+}
+
+// Entry of a code longer than kL1Bits: second-level table, or the canonical
+// decode when the image's pool was full.
+CJD_HD inline uint32_t lut_long(const EntShared& sh, uint32_t e, uint32_t bits, uint32_t tb) {
+  if (e != kLutSlow) return sh.lut[(e & 0x7FFFu) + ((bits >> (32 - kL1Bits - kL2Bits)) & (kL2Size - 1))];
+  const int t = (int)(tb >> kL1Bits);
+  const CanonLong& T = sh.canon[t];
+  const uint32_t v = bits >> 16;
+  int l = kL1Bits + 1;
+  for (int k = 0; k < 6; ++k) l += v >= T.lim[k] ? 1 : 0;
+  if (l > 16) return 0u;
+  const int sym = T.vals[(T.valoff[l - 11] + (int)(v >> (16 - l))) & 0xFF];
+  return lut_entry(l, sym, t >= 2);
+}
+
+CJD_HD inline uint32_t ubfe(uint32_t v, uint32_t off, uint32_t width) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ubfe(v, off, width);
+#else
+  return width ? (v >> (off & 31)) & ((1u << width) - 1u) : 0u;
+#endif
+}
+
+// One symbol (src/decodeScanGPU.cu:78-100 for one symbol: DC difference, or
+// an AC run/size with EXTEND, EOB, ZRL, the coefficient index checks).
+// Straight-line code with one unconditional store; the second-level lookup
+// and the DC-list store are the only branches.  Before the symbol, the first
+// block boundary at or after thr in this period is captured.
+CJD_HD inline void lane_step(EntShared& sh, int tid, Lane& L, uint32_t thr, int nh, uint32_t* eb) {
+  const uint32_t bits = lane_peek(sh, tid, L.pos);
+  const bool isdc = L.k < 0;
+  const bool hc = isdc && !L.hit && L.pos >= thr;
+  const uint32_t info = (uint32_t)L.slot | (L.s << 4) | (L.dn << 18);
+  L.cap_p = hc ? L.pos : L.cap_p;
+  L.cap_i = hc ? info : L.cap_i;
+  L.hit = L.hit || hc;
+  const uint32_t tb = sh.tsel[(L.slot << 1) | (isdc ? 1 : 0)];
+  uint32_t e = sh.lut[tb + (bits >> (32 - kL1Bits))];
+  if (e & 0x8000u) e = lut_long(sh, e, bits, tb);
+  const uint32_t adv = e & 31u, size = (e >> 5) & 15u, runp1 = (e >> 9) & 31u;
+  const uint32_t raw = ubfe(bits, 32 - adv, size);
+  const uint32_t t = (1u << size) - 1u;
+  const int v = (int)raw - (raw <= (t >> 1) ? (int)t : 0);  // EXTEND (T.81 F.2.2.1)
+  const bool invalid = adv == 0;
+  L.pos += adv + ((invalid && isdc) ? 1u : 0u);  // an invalid DC code consumes one bit
+  const int nk = isdc ? 0 : L.k + (int)runp1;
+  const bool bad = invalid || nk > 63;
+  const bool end = isdc ? invalid : (runp1 == 0 || nk >= 63);
+  const uint32_t z = sh.dz[nk > 63 ? 63 : nk];
+  const uint32_t nat = runp1 == 0 ? 63u : z;
+  const uint32_t vv = (uint32_t)v & 0xFFFFu;
+  eb[L.si + (uint64_t)L.s * L.W] = (nat << 16) | vv;
+  const uint32_t dcent = (L.s << 17) | (invalid ? 0x10000u : 0u) | vv;
+  if (isdc || bad) eb[L.di + (isdc ? L.dn : L.dn - 1)] = isdc ? dcent : (L.lastdc | 0x10000u);
+  L.lastdc = isdc ? dcent : L.lastdc;
+  L.dn += isdc ? 1u : 0u;
+  L.s += 1;
+  L.k = end ? -1 : nk;
+  const int ns = L.slot + 1 == nh ? 0 : L.slot + 1;
+  L.slot = end ? ns : L.slot;
+}
+
+// Checkpoint m of local tile j (2 words at ckpts(j) + 2m): the speculative
+// trajectory's first block boundary at or after bit t*S + (m+1)*C, as (bit,
+// slot | step << 4 | DC entries << 18 before that block).
+
+// Return L to its capture (a block boundary).  The step index goes on: the
+// stream entries written since the capture are marked unused.
+CJD_HD inline void lane_restore(Lane& L, bool mark, uint32_t* eb) {
+  const uint32_t sc = (L.cap_i >> 4) & 0x3FFFu;
+  if (mark && !L.ovf)
+    for (uint32_t q = sc; q < L.s; ++q) eb[L.si + (uint64_t)q * L.W] = 0xFFFFFFFFu;
+  L.pos = L.cap_p;
+  L.slot = (int)(L.cap_i & 15u);
+  L.dn = L.cap_i >> 18;
+  L.k = -1;
+}
+
+// Decode a trajectory from L's state (a block boundary) up to its first block
+// boundary at or after `end`, where L is left (step index = the capture's).
+// The thresholds are the checkpoint bits t*S + (m+1)*C below `end`, then
+// `end`; the steps of a period capture the first boundary past the current
+// threshold and the top of the next period acts on it.  kLink = false
+// (speculative pass): a capture below `end` is checkpoint m (not recorded
+// after the streams filled; ncp = checkpoints recorded).  kLink = true: a
+// capture below `end` is compared with the speculative pass's checkpoint m (m
+// < nck) and the decode stops at the first one it equals (joined = m + 1).
+// When a boundary past the next threshold may have passed in the same period,
+// the lane decodes again from the capture (long blocks only).
+template <bool kLink>
+CJD_HD inline bool lane_run(EntShared& sh, int tid, const EntCtx& c, Lane& L, uint32_t b0, uint32_t end,
+                            uint32_t* ck, int nck, int& joined, int budget) {
+  const uint32_t C = c.sg.ck_bits;
+  const int M = (int)c.sg.nck;
+  const uint32_t t1 = b0 + (uint32_t)(L.m + 1) * C;
+  uint32_t thr = L.m < M && t1 < end ? t1 : end;
+  uint32_t want_p = ~0u, want_i = ~0u;
+  if (kLink && L.m < nck) {
+    want_p = ck[2 * L.m];
+    want_i = ck[2 * L.m + 1];
+  }
+  joined = 0;
+  L.hit = false;
+  L.cap_p = 0;
+  L.cap_i = 0;
+  for (int per = 0;; ++per) {
+    if (L.hit) {
+      L.hit = false;
+      bool stop = L.cap_p >= end;
+      if (!stop) {
+        if (kLink) {
+          if (L.m < nck && want_p == L.cap_p && ((want_i ^ L.cap_i) & 15u) == 0) {
+            joined = L.m + 1;
+            stop = true;
+          }
+        } else if (!L.ovf) {
+          ck[2 * L.m] = L.cap_p;
+          ck[2 * L.m + 1] = L.cap_i;
+        }
+      }
+      if (stop) {
+        lane_restore(L, false, c.ebuf);
+        L.s = (L.cap_i >> 4) & 0x3FFFu;
+        return true;
+      }
+      ++L.m;
+      const uint32_t tn = b0 + (uint32_t)(L.m + 1) * C;
+      thr = L.m < M && tn < end ? tn : end;
+      if (kLink && L.m < nck) {
+        want_p = ck[2 * L.m];
+        want_i = ck[2 * L.m + 1];
+      }
+      if (L.pos >= thr) {
+        lane_restore(L, true, c.ebuf);
+        lane_seek(sh, tid, c, L, L.pos);
+      }
+    }
+    if (budget >= 0 && per >= budget) return false;
+    lane_rotate(sh, tid, c, L);
+    if (L.ovf || L.s + kPeriod > L.R || L.dn + kPeriod + 1 > L.Rd) {
+      // keep decoding for the exit, writing into the item's sink: rows 0-3 of
+      // its stream part and entries 1-5 of the lane's 8-entry list part (the
+      // word indexes wrap modulo 2^64 to land there)
+      L.ovf = true;
+      const uint64_t sk = (uint64_t)(c.sink() - c.ebuf);
+      L.si = sk + (uint64_t)(tid & 63) - (uint64_t)L.s * L.W;
+      L.di = sk + 4 * 64 + (uint64_t)(tid & 63) * 8 + 2 - (uint64_t)L.dn;
+    }
+    for (int u = 0; u < kPeriod; ++u) lane_step(sh, tid, L, thr, c.nh, c.ebuf);
+  }
+}
+
+CJD_HD inline void lane_start(const EntCtx& c, Lane& L, const uint32_t* stream, const uint32_t* dcl, uint32_t R,
+                              uint32_t Rd, int slot) {
+  L.k = -1;
+  L.slot = slot;
+  L.s = 0;
+  L.dn = 0;
+  L.lastdc = 0;
+  L.si = (uint64_t)(stream - c.ebuf);
+  L.di = (uint64_t)(dcl - c.ebuf);
+  L.W = c.W();
+  L.m = 0;
+  L.R = R;
+  L.Rd = Rd;
+  L.ovf = false;
+}
+
+// Phase 1: speculative pass of local tile tid from (tile start, slot 0) --
+// the segment's known start for its first tile -- writing the tile's stream,
+// DC list and checkpoints.
+CJD_HD inline void ent_spec(EntShared& sh, int tid, const EntCtx& c) {
+  if (tid >= c.J) return;
+  const uint32_t t = c.tfirst + (uint32_t)tid;
+  const uint32_t b0 = t * c.S;
+  Lane L;
+  lane_start(c, L, c.sstream(tid), c.sdcl(tid), c.sg.rs, c.sg.rd, 0);
+  lane_seek(sh, tid, c, L, b0);
+  int joined;
+  lane_run<false>(sh, tid, c, L, b0, c.tile_end(t), c.ckpts(tid), 0, joined, -1);
+  const int ncp = L.m < (int)c.sg.nck ? L.m : (int)c.sg.nck;
+  sh.sxp[tid] = L.pos;
+  sh.sxs[tid] = (uint8_t)L.slot;
+  sh.xp[tid] = L.pos;
+  sh.xs[tid] = (uint8_t)L.slot;
+  sh.nck[tid] = (uint8_t)(L.ovf ? 0 : ncp);
+  sh.ss[tid] = (uint16_t)(L.ovf ? 0 : L.s);
+  sh.sdn[tid] = (uint16_t)(L.ovf ? 0 : L.dn);
+  // until linked: the tile's blocks are all of its speculative pass
+  sh.lp[tid] = b0;
+  sh.ls[tid] = 0;
+  sh.ps[tid] = 0;
+  sh.pdn[tid] = 0;
+  sh.js[tid] = 0;
+  sh.jdn[tid] = 0;
+  sh.jslot[tid] = 0;
+}
+
+// Links.  Local tile j linked from entry (p, s) decodes into its prefix
+// stream until the trajectory passes through a checkpoint of the speculative
+// pass (the rest of the tile is the speculative pass from there: same exit,
+// its entries after the checkpoint) or reaches the tile end (a concrete
+// exit).  An entry equal to the speculative pass's start joins at once.
+// Links run as tasks in rounds of kLinkBudget periods; after each round the
+// unfinished ones are packed into the lowest lanes, so that a few long links
+// occupy one wave instead of every wave of the workgroup.
+constexpr int kLinkBudget = 16;
+
+CJD_HD inline void link_joined(EntShared& sh, int j, uint32_t ps, uint32_t pdn, uint32_t info) {
+  sh.ps[j] = (uint16_t)ps;
+  sh.pdn[j] = (uint16_t)pdn;
+  sh.js[j] = (uint16_t)((info >> 4) & 0x3FFF);
+  sh.jdn[j] = (uint16_t)(info >> 18);
+  sh.jslot[j] = (uint8_t)(info & 15);
+  sh.xp[j] = sh.sxp[j];
+  sh.xs[j] = sh.sxs[j];
+}
+
+// Set tile j's link entry; false when it joins at once (no task needed).
+CJD_HD inline bool link_begin(EntShared& sh, const EntCtx& c, int j, uint32_t p, uint32_t s) {
+  const uint32_t b0 = (c.tfirst + (uint32_t)j) * c.S;
+  sh.lp[j] = p;
+  sh.ls[j] = (uint8_t)s;
+  if (sh.sdn[j] != 0 && p == b0 && s == 0) {  // speculative entries kept and the same start
+    link_joined(sh, j, 0, 0, 0);
+    return false;
+  }
+  return true;
+}
+
+CJD_HD inline void task_put(EntShared& sh, int idx, int j, const Lane& L) {
+  sh.u.t.pos[idx] = L.pos;
+  sh.u.t.st[idx] = (uint32_t)L.slot | ((uint32_t)(L.k + 1) << 4) | ((uint32_t)L.m << 11) | ((uint32_t)j << 19);
+  sh.u.t.sd[idx] = L.s | (L.dn << 16);
+  sh.u.t.last[idx] = L.lastdc;
+}
+
+// A new task: tile j from its link entry.
+CJD_HD inline void task_new(EntShared& sh, int idx, int j) {
+  sh.u.t.pos[idx] = sh.lp[j];
+  sh.u.t.st[idx] = (uint32_t)sh.ls[j] | ((uint32_t)j << 19);
+  sh.u.t.sd[idx] = 0;
+  sh.u.t.last[idx] = 0;
+}
+
+struct Task {
+  uint32_t pos, st, sd, last;
+};
+CJD_HD inline Task task_get(const EntShared& sh, int idx) {
+  return Task{sh.u.t.pos[idx], sh.u.t.st[idx], sh.u.t.sd[idx], sh.u.t.last[idx]};
+}
+
+// One round of task T on lane tid.  Returns true when the link finished
+// (results written); else L holds the state to put back.
+CJD_HD inline bool task_round(EntShared& sh, int tid, const EntCtx& c, const Task& T, Lane& L, int& j) {
+  const uint32_t st = T.st, sd = T.sd;
+  j = (int)(st >> 19);
+  const uint32_t t = c.tfirst + (uint32_t)j;
+  lane_start(c, L, c.pstream(j), c.pdcl(j), c.sg.rs, c.sg.rd, (int)(st & 15));
+  L.k = (int)((st >> 4) & 127) - 1;
+  L.m = (int)((st >> 11) & 255);
+  L.s = sd & 0xFFFF;
+  L.dn = sd >> 16;
+  L.lastdc = T.last;
+  lane_seek(sh, tid, c, L, T.pos);
+  uint32_t* ck = c.ckpts(j);
+  int joined;
+  if (!lane_run<true>(sh, tid, c, L, t * c.S, c.tile_end(t), ck, sh.nck[j], joined, kLinkBudget)) return false;
+  if (L.ovf) sh.lovf = 1;  // under two bits per symbol over a whole tile: left to the serial decoder
+  if (joined) {
+    link_joined(sh, j, L.ovf ? 0 : L.s, L.ovf ? 0 : L.dn, ck[2 * (joined - 1) + 1]);
+  } else {
+    sh.ps[j] = (uint16_t)(L.ovf ? 0 : L.s);
+    sh.pdn[j] = (uint16_t)(L.ovf ? 0 : L.dn);
+    sh.js[j] = sh.ss[j];
+    sh.jdn[j] = sh.sdn[j];
+    sh.jslot[j] = 0;
+    sh.xp[j] = L.pos;
+    sh.xs[j] = (uint8_t)L.slot;
+  }
+  return true;
+}
+
+// Phase 2: link entry of local tile tid -- the segment start for the
+// segment's first tile, else the speculative exit of tile tid-1 (the lead-in
+// has none); true when a link task is needed.
+CJD_HD inline bool ent_phase2(EntShared& sh, int tid, const EntCtx& c) {
+  if (tid >= c.J || (c.lead && tid == 0)) return false;
+  if (c.seg_first(tid)) return link_begin(sh, c, tid, 0, 0);
+  return link_begin(sh, c, tid, sh.sxp[tid - 1], sh.sxs[tid - 1]);
+}
+
+// Fix-up round, read half (all lanes): was tile tid linked from an entry
+// other than tile tid-1's current exit?  (p, s) = that exit.
+CJD_HD inline bool ent_fix_need(const EntShared& sh, int tid, const EntCtx& c, uint32_t& p, uint32_t& s) {
+  if (tid < 1 || tid >= c.J) return false;
+  p = sh.xp[tid - 1];
+  s = sh.xs[tid - 1];
+  return p != sh.lp[tid] || s != sh.ls[tid];
+}
 
 CJD_HD inline void add_dc(int32_t* c3, int comp, int d) {
   c3[0] += comp == 0 ? d : 0;
@@ -321,226 +772,58 @@ CJD_HD inline void add_dc(int32_t* c3, int comp, int d) {
   c3[2] += comp == 2 ? d : 0;
 }
 
-// One Huffman symbol of the block-structured stream (the body of
-// decode_block, one symbol per call, so that the lanes of a wave stay in step
-// symbol by symbol instead of waiting for each other's longest block).
-// State: slot of the current block; coef = -1 when the next symbol is a DC
-// (block start), else the index of the last coefficient.  Out: DC difference
-// (0 for AC symbols), natural position and value of a coefficient to store
-// (wpos = -1 for none).  Returns true when the block ended with this symbol.
-CJD_HD inline bool sym_step(BitReader& br, const EntShared& sh, uint64_t slotinfo, int nh, int& slot, int& coef,
-                            int& dcdiff, int& wpos, int& wval, int& err) {
-  const uint32_t bits = br.peek();
-  const bool isdc = coef < 0;
-  const int sinfo = (int)((slotinfo >> (4 * slot)) & 15);
-  const int tidx = isdc ? (sinfo & 1) : 2 + ((sinfo >> 1) & 1);
-  int len;
-  const int sym = huff_sym(sh.tab[tidx], bits, len);
-  const int size = sym & 15;
-  const int v = extend_value(bits, len, size);
-  // an invalid DC code consumes one bit, an invalid AC code none
-  br.skip(len ? len + size : (isdc ? 1 : 0));
-  const int nc = coef + (sym >> 4) + 1;
-  const bool bad = !len || (!isdc && sym && !size && sym != 0xF0) || (!isdc && sym && nc > 63);
-  const bool end = isdc ? !len : (bad || !sym || nc >= 63);
-  err |= bad ? 1 : 0;
-  dcdiff = isdc ? v : 0;
-  wpos = (!isdc && !bad && sym) ? (int)sh.dz[nc] : -1;
-  wval = v;
-  coef = isdc ? 0 : nc;
-  if (end) {
-    coef = -1;
-    if (++slot == nh) slot = 0;
-  }
-  return end;
+// Blocks of local tile j on the true trajectory: its prefix blocks, then the
+// speculative blocks from the join.  f(stream, dcl, d0, d1, send, slot) is
+// called for the two runs: DC-list entries [d0, d1) of list dcl, whose
+// blocks' AC entries are in the stream column `stream` (stride 64 words) and
+// end at step send for the last block; slot is the first block's slot.
+template <class F>
+CJD_HD inline void tile_runs(const EntShared& sh, const EntCtx& c, int j, F&& f) {
+  f(c.pstream(j), c.pdcl(j), 0u, (uint32_t)sh.pdn[j], (uint32_t)sh.ps[j], (int)sh.ls[j]);
+  f(c.sstream(j), c.sdcl(j), (uint32_t)sh.jdn[j], (uint32_t)sh.sdn[j], (uint32_t)sh.ss[j], (int)sh.jslot[j]);
 }
 
-// The hypothesis state most hypotheses of the lane's tile share (ties: the
-// lowest hypothesis); n = how many share it.
-CJD_HD inline uint64_t ent_consensus(const EntShared& sh, int tid, int nh, int& n) {
-  uint64_t best = sh.u.a.hyp[0][tid];
-  int nbest = 0;
-  for (int h = 0; h < nh; ++h) {
-    const uint64_t v = sh.u.a.hyp[h][tid];
-    int k = 0;
-    for (int g = 0; g < nh; ++g) k += sh.u.a.hyp[g][tid] == v ? 1 : 0;
-    if (k > nbest) {
-      nbest = k;
-      best = v;
+// f(e) for the DC-list entries dcl[d0 .. d1) in order, 16 entries per round
+// trip to memory (lists are 16-byte aligned and followed by at least 16
+// words of the item's region, so the reads past d1 stay inside it).
+template <class F>
+CJD_HD inline void dcl_for(const uint32_t* dcl, uint32_t d0, uint32_t d1, F&& f) {
+  for (uint32_t g = d0 & ~15u; g < d1; g += 16) {
+    uint32_t v[16];
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4* p = reinterpret_cast<const uint4*>(dcl + g);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint4 x = p[q];
+      v[4 * q] = x.x;
+      v[4 * q + 1] = x.y;
+      v[4 * q + 2] = x.z;
+      v[4 * q + 3] = x.w;
     }
+#else
+    for (int q = 0; q < 16; ++q) v[q] = dcl[g + q];
+#endif
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (g + q >= d0 && g + q < d1) f(v[q]);
   }
-  n = nbest;
-  return best;
 }
 
-// Phase 1: main trajectory of local tile tid.  One loop, one symbol per step:
-//   stage 1: hypotheses 0..nh-1 in turn from the tile start to the first sync
-//            point; if they all agree there, that state is the D-state;
-//   stage 2: otherwise each hypothesis continues from its stage-1 state to the
-//            second sync point and the consensus there is the D-state;
-//   main:    from the D-state to the tile end, counting blocks and DC sums.
-// Tile 0 of a segment starts from the known state (bit 0, slot 0): its main
-// trajectory is the true one from the start.
-CJD_HD inline void ent_phase1(EntShared& sh, int tid, const EntCtx& c) {
+// Counts of local tile tid (blocks, DC sums per component) from its DC
+// entries; the lead-in counts nothing (its blocks belong to the previous item).
+CJD_HD inline void ent_count(EntShared& sh, int tid, const EntCtx& c) {
   if (tid >= c.J) return;
-  const int nh = c.nh, kMain = 2 * nh;
-  const uint32_t t = c.tfirst + (uint32_t)tid;
-  const uint32_t b0 = t * c.S, end = c.tile_end(t), thr1 = c.sync_point(t, c.D1), thr2 = c.sync_point(t, c.D);
-  BitReader br;
-  br.seek(c.w, c.nwords, b0);
-  int mode = t == 0 ? kMain : 0;
-  int slot = 0, coef = -1, err = 0;
-  int32_t nb = 0, dc[3] = {0, 0, 0};
-  sh.u.a.dstate[tid] = ~0ull;
-  sh.u.a.dthr[tid] = end;
-  for (bool done = false; !done;) {
-    br.rotate();
-    for (int u = 0; u < kPeriod; ++u) {
-      if (coef < 0) {
-        const uint32_t p = br.pos;
-        if (mode < kMain && p >= (mode < nh ? thr1 : thr2)) {
-          const int h = mode < nh ? mode : mode - nh;
-          sh.u.a.hyp[h][tid] = pack_state(p, (uint32_t)slot);
-          ++mode;
-          uint64_t next = 0;  // state to continue from
-          if (mode < nh) {
-            next = pack_state(b0, (uint32_t)mode);
-          } else if (mode == nh) {
-            int n;
-            const uint64_t m = ent_consensus(sh, tid, nh, n);
-            if (n == nh || thr1 >= thr2) {  // unanimous, or no second sync point
-              mode = kMain;
-              sh.u.a.dstate[tid] = m;
-              sh.u.a.dthr[tid] = thr1;
-              next = m;
-            } else {
-              next = sh.u.a.hyp[0][tid];
-            }
-          } else if (mode < kMain) {
-            next = sh.u.a.hyp[mode - nh][tid];
-          } else {
-            int n;
-            const uint64_t m = ent_consensus(sh, tid, nh, n);
-            sh.u.a.dstate[tid] = m;
-            sh.u.a.dthr[tid] = thr2;
-            next = m;
-          }
-          br.seek(c.w, c.nwords, (uint32_t)(next >> 4));
-          slot = (int)(next & 15);
-          continue;
-        }
-        if (mode == kMain && p >= end) {
-          done = true;
-          break;
-        }
-        if (mode == kMain) ++nb;
-      }
-      const int s0 = slot;
-      int d, wp, wv;
-      sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
-      if (mode == kMain) add_dc(dc, c.comp(s0), d);
-    }
+  int32_t q[4] = {0, 0, 0, 0};
+  if (!(c.lead && tid == 0)) {
+    tile_runs(sh, c, tid, [&](const uint32_t*, const uint32_t* dcl, uint32_t d0, uint32_t d1, uint32_t, int slot) {
+      dcl_for(dcl, d0, d1, [&](uint32_t e) {
+        add_dc(q + 1, c.comp(slot), (int16_t)(e & 0xFFFFu));
+        slot = slot + 1 == c.nh ? 0 : slot + 1;
+      });
+      q[0] += (int32_t)(d1 - d0);
+    });
   }
-  sh.ep[tid] = br.pos;
-  sh.es[tid] = (uint32_t)slot;
-  sh.u.a.mc[tid][0] = nb;
-  sh.u.a.mc[tid][1] = dc[0];
-  sh.u.a.mc[tid][2] = dc[1];
-  sh.u.a.mc[tid][3] = dc[2];
-}
-
-// Decode local tile j from the state (p, slot) at its entry: up to the first
-// boundary at or after the tile's D-state sync point; if that is the D-state,
-// the rest is the main trajectory (its exit and counts are taken from phase
-// 1), otherwise the decode goes on to the tile end.  (p, slot) become the
-// exit, cnt the blocks / DC sums gained in tile j.
-CJD_HD inline void ent_relink(const EntShared& sh, const EntCtx& c, int j, uint32_t& p, int& slot, int32_t* cnt) {
-  const uint32_t t = c.tfirst + (uint32_t)j;
-  const uint32_t end = c.tile_end(t), thr = sh.u.a.dthr[j];
-  const uint64_t ds = sh.u.a.dstate[j];
-  int32_t nb = 0, dc[3] = {0, 0, 0};
-  bool checked = false;
-  BitReader br;
-  br.seek(c.w, c.nwords, p);
-  int coef = -1, err = 0;
-  for (bool done = false; !done;) {
-    br.rotate();
-    for (int u = 0; u < kPeriod; ++u) {
-      if (coef < 0) {
-        const uint32_t q = br.pos;
-        if (!checked && q >= thr) {
-          checked = true;
-          if (pack_state(q, (uint32_t)slot) == ds) {
-            cnt[0] = nb + sh.u.a.mc[j][0];
-            cnt[1] = dc[0] + sh.u.a.mc[j][1];
-            cnt[2] = dc[1] + sh.u.a.mc[j][2];
-            cnt[3] = dc[2] + sh.u.a.mc[j][3];
-            p = sh.ep[j];
-            slot = (int)sh.es[j];
-            return;
-          }
-        }
-        if (q >= end) {
-          done = true;
-          break;
-        }
-        ++nb;
-      }
-      const int s0 = slot;
-      int d, wp, wv;
-      sym_step(br, sh, c.slotinfo, c.nh, slot, coef, d, wp, wv, err);
-      add_dc(dc, c.comp(s0), d);
-    }
-  }
-  p = br.pos;
-  cnt[0] = nb;
-  cnt[1] = dc[0];
-  cnt[2] = dc[1];
-  cnt[3] = dc[2];
-}
-
-// Phase 2: link local tile tid (>= 1) from the main exit of tile tid-1, and
-// seed the chain with the assumed states (tile 0: its main exit; its counts
-// belong to this item only when it is the segment's first tile).
-CJD_HD inline void ent_phase2(EntShared& sh, int tid, const EntCtx& c) {
-  if (tid >= c.J) return;
-  if (tid == 0) {
-    sh.tp[0] = sh.ep[0];
-    sh.ts[0] = sh.es[0];
-    for (int q = 0; q < 4; ++q) sh.u.a.cnt[0][q] = c.lead ? 0 : sh.u.a.mc[0][q];
-    return;
-  }
-  uint32_t p = sh.ep[tid - 1];
-  int slot = (int)sh.es[tid - 1];
-  int32_t cnt[4];
-  sh.u.a.up[tid] = p;
-  sh.u.a.us[tid] = (uint32_t)slot;
-  ent_relink(sh, c, tid, p, slot, cnt);
-  sh.tp[tid] = p;
-  sh.ts[tid] = (uint32_t)slot;
-  for (int q = 0; q < 4; ++q) sh.u.a.cnt[tid][q] = cnt[q];
-}
-
-// Fix-up round, read half (all lanes): does tile tid's exit come from an
-// entry other than tile tid-1's current exit?  (p, s) = that exit.
-CJD_HD inline bool ent_fix_need(const EntShared& sh, int tid, const EntCtx& c, uint32_t& p, uint32_t& s) {
-  if (tid < 1 || tid >= c.J) return false;
-  p = sh.tp[tid - 1];
-  s = sh.ts[tid - 1];
-  return p != sh.u.a.up[tid] || s != sh.u.a.us[tid];
-}
-// Fix-up round, write half: decode tile tid again from (p, s).  Rounds repeat
-// until no tile changes; a run of k consecutive tiles off the main trajectory
-// takes k rounds.
-CJD_HD inline void ent_fix_apply(EntShared& sh, int tid, const EntCtx& c, uint32_t p, uint32_t s) {
-  sh.u.a.up[tid] = p;
-  sh.u.a.us[tid] = s;
-  int slot = (int)s;
-  int32_t cnt[4];
-  ent_relink(sh, c, tid, p, slot, cnt);
-  sh.tp[tid] = p;
-  sh.ts[tid] = (uint32_t)slot;
-  for (int q = 0; q < 4; ++q) sh.u.a.cnt[tid][q] = cnt[q];
+  for (int k = 0; k < 4; ++k) sh.u.c.cnt[tid][k] = q[k];
 }
 
 // Scan (all lanes; Hillis-Steele over kWG lanes in LDS with barriers between
@@ -548,13 +831,13 @@ CJD_HD inline void ent_fix_apply(EntShared& sh, int tid, const EntCtx& c, uint32
 // with distance d first reads (ent_scan_read), then, after a barrier, adds.
 CJD_HD inline void ent_scan_init(EntShared& sh, int tid, const EntCtx& c) {
   if (tid >= c.J) return;
-  for (int q = 0; q < 4; ++q) sh.tc[tid][q] = sh.u.a.cnt[tid][q];
+  for (int q = 0; q < 4; ++q) sh.u.c.tc[tid][q] = sh.u.c.cnt[tid][q];
 }
 CJD_HD inline void ent_scan_read(const EntShared& sh, int tid, int d, int32_t* v) {
-  for (int q = 0; q < 4; ++q) v[q] = tid >= d ? sh.tc[tid - d][q] : 0;
+  for (int q = 0; q < 4; ++q) v[q] = tid >= d ? sh.u.c.tc[tid - d][q] : 0;
 }
 CJD_HD inline void ent_scan_write(EntShared& sh, int tid, const int32_t* v) {
-  for (int q = 0; q < 4; ++q) sh.tc[tid][q] += v[q];
+  for (int q = 0; q < 4; ++q) sh.u.c.tc[tid][q] += v[q];
 }
 
 // Composition of predecessors' aggregates in the lookback.  An aggregate says
@@ -600,106 +883,82 @@ struct LookAcc {
   }
 };
 
-// Aggregate of this item (tc scanned: relative cumulative counts).
+
+// Aggregate of this item (tc scanned: relative cumulative counts): assumes
+// the item's entry is the speculative exit of its lead-in tile.
 CJD_HD inline void ent_aggregate(const EntShared& sh, const EntCtx& c, LookbackRec* me) {
-  me->lead_p = sh.ep[0];
-  me->lead_s = sh.es[0];
-  me->agg_p = sh.tp[c.J - 1];
-  me->agg_s = sh.ts[c.J - 1];
-  for (int q = 0; q < 4; ++q) me->agg_c[q] = sh.tc[c.J - 1][q];
+  me->lead_p = sh.sxp[0];
+  me->lead_s = sh.sxs[0];
+  me->agg_p = sh.xp[c.J - 1];
+  me->agg_s = sh.xs[c.J - 1];
+  for (int q = 0; q < 4; ++q) me->agg_c[q] = sh.u.c.tc[c.J - 1][q];
 }
 
 // Inclusive state of this item: the true exit of its last tile and the
 // segment's cumulative counts there (tc absolute).
 CJD_HD inline void ent_inclusive(const EntShared& sh, const EntCtx& c, LookbackRec* me) {
-  me->incl_p = sh.tp[c.J - 1];
-  me->incl_s = sh.ts[c.J - 1];
-  for (int q = 0; q < 4; ++q) me->incl_c[q] = sh.tc[c.J - 1][q];
+  me->incl_p = sh.xp[c.J - 1];
+  me->incl_s = sh.xs[c.J - 1];
+  for (int q = 0; q < 4; ++q) me->incl_c[q] = sh.u.c.tc[c.J - 1][q];
 }
 
 // After the lookback: whether the true entry differs from the assumed one
-// (then tile 0's exit is set to it and the fix-up rounds and the scan run
-// again), and the absolute counts (every lane adds the entry's counts).
+// (then the lead-in's exit is set to it and the fix-up rounds, the counts and
+// the scan run again), and the absolute counts (every lane adds the entry's).
 CJD_HD inline bool ent_entry_differs(const EntShared& sh, const EntCtx& c) {
-  return c.lead && (sh.entry_p != sh.ep[0] || sh.entry_s != sh.es[0]);
+  return c.lead && (sh.entry_p != sh.sxp[0] || sh.entry_s != sh.sxs[0]);
 }
 CJD_HD inline void ent_absolute(EntShared& sh, int tid, const EntCtx& c) {
   if (tid >= c.J || !c.lead) return;
-  for (int q = 0; q < 4; ++q) sh.tc[tid][q] += sh.entry_c[q];
+  for (int q = 0; q < 4; ++q) sh.u.c.tc[tid][q] += sh.entry_c[q];
 }
 
-// Phase 5: final decode of local tile tid + lead from its true entry.
-CJD_HD inline void ent_phase5(EntShared& sh, int tid, const EntCtx& c, int16_t* coefs, uint32_t* status) {
-  const int fj = tid + c.lead;
-  if (fj >= c.J) return;
-  const int nh = c.nh;
-  const uint32_t t = c.tfirst + (uint32_t)fj;
-  uint32_t* st = status + c.im->error_index;
-  uint32_t p;
-  int slot;
-  int32_t cp[4];
-  if (fj == 0) {
-    p = 0;
-    slot = 0;
-    for (int q = 0; q < 4; ++q) cp[q] = 0;
-  } else {
-    p = sh.tp[fj - 1];
-    slot = (int)sh.ts[fj - 1];
-    for (int q = 0; q < 4; ++q) cp[q] = sh.tc[fj - 1][q];
-  }
-  const uint32_t xp = sh.tp[fj], xs = sh.ts[fj];
-  int32_t cx[4];
-  for (int q = 0; q < 4; ++q) cx[q] = sh.tc[fj][q];
-  const uint32_t end = c.tile_end(t);
-  int64_t bidx = cp[0];
-  int dc0 = cp[1], dc1 = cp[2], dc2 = cp[3];
-  int16_t* blk = sh.u.coef[tid];
-  int err = 0;
-  bool stopped = false;
-  BitReader br;
-  br.seek(c.w, c.nwords, p);
-  int coef = -1;
-  for (bool done = false; !done;) {
-    br.rotate();
-    for (int u = 0; u < kPeriod; ++u) {
-      if (coef < 0) {
-        if (br.pos >= end) {
-          done = true;
-          break;
-        }
-        if (bidx >= (int64_t)c.sg.nblocks) {
-          stopped = true;
-          done = true;
-          break;
-        }
-        __builtin_memset(blk, 0, 128);
+// Final phase: the block records of local tile tid (absolute block index,
+// absolute DC), the syntax errors of its blocks within the segment, and the
+// truncation check at the segment's last tile.
+CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, BlockRec* recs, uint32_t* status) {
+  if (tid >= c.J || (c.lead && tid == 0)) return;
+  const int64_t seg_end = c.sg.first_block + (int64_t)c.sg.nblocks;
+  int64_t A = c.sg.first_block + (tid > 0 ? sh.u.c.tc[tid - 1][0] : 0);
+  int32_t pred[3] = {0, 0, 0};
+  if (tid > 0)
+    for (int q = 0; q < 3; ++q) pred[q] = sh.u.c.tc[tid - 1][q + 1];
+  bool err = false;
+  tile_runs(sh, c, tid, [&](const uint32_t* stream, const uint32_t* dcl, uint32_t d0, uint32_t d1, uint32_t send,
+                            int slot) {
+    const uint32_t base = (uint32_t)(stream - c.ebuf);
+    const uint32_t W = c.W();
+    // block d's AC entries end where block d+1's DC step is: emit a block
+    // when the next list entry is read
+    auto emit = [&](uint32_t e, uint32_t sn) {
+      const uint32_t st = e >> 17;
+      const int comp = c.comp(slot);
+      add_dc(pred, comp, (int16_t)(e & 0xFFFFu));
+      if (A < seg_end) {
+        BlockRec r;
+        r.eidx = base + (st + 1) * W;
+        r.nw = (uint16_t)((sn - st - 1) | ((W / 16 - 1) << 8));
+        r.dc = (int16_t)(comp == 0 ? pred[0] : comp == 1 ? pred[1] : pred[2]);
+        recs[A] = r;
+        err = err || (e & 0x10000u);
       }
-      const int s0 = slot, comp = c.comp(s0);
-      int d, wp, wv;
-      const bool ended = sym_step(br, sh, c.slotinfo, nh, slot, coef, d, wp, wv, err);
-      if (err) {
-        done = true;
-        break;
-      }
-      dc0 += comp == 0 ? d : 0;
-      dc1 += comp == 1 ? d : 0;
-      dc2 += comp == 2 ? d : 0;
-      if (wp >= 0) blk[wp] = (int16_t)wv;
-      if (ended) {
-        blk[0] = (int16_t)(comp == 0 ? dc0 : comp == 1 ? dc1 : dc2);
-        __builtin_memcpy(coefs + (c.sg.first_block + bidx) * 64, blk, 128);
-        ++bidx;
-      }
-    }
-  }
-  p = br.pos;
-  if (err) {
-    status_or(st, kStatusSyntax);
-  } else if (!stopped) {
-    if (p != xp || (uint32_t)slot != xs || bidx != (int64_t)cx[0] || dc0 != cx[1] || dc1 != cx[2] || dc2 != cx[3])
-      status_or(st, kStatusInconsistent);
-    if (t + 1 == c.sg.ntiles && bidx < (int64_t)c.sg.nblocks) status_or(st, kStatusTruncated);
-  }
+      ++A;
+      slot = slot + 1 == c.nh ? 0 : slot + 1;
+    };
+    uint32_t prev = 0;
+    bool have = false;
+    dcl_for(dcl, d0, d1, [&](uint32_t e) {
+      if (have) emit(prev, e >> 17);
+      prev = e;
+      have = true;
+    });
+    if (have) emit(prev, send);
+  });
+  const uint32_t sidx = (uint32_t)c.im->error_index;
+  if (err) status_or(status + sidx, kStatusSyntax);
+  if (sh.lovf) status_or(status + sidx, kStatusInconsistent);
+  if (c.tfirst + (uint32_t)tid + 1 == c.sg.ntiles && sh.u.c.tc[tid][0] < (int32_t)c.sg.nblocks)
+    status_or(status + sidx, kStatusTruncated);
 }
 
 // Serial decode of one whole segment from its known start (reference order,
@@ -728,5 +987,6 @@ CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* im
     if (++slot == im.nslots) slot = 0;
   }
 }
+
 
 }  // namespace cjd

@@ -14,6 +14,7 @@
 #include <mutex>
 #include <thread>
 
+#include "entropy_core.h"
 #include "parse_internal.h"
 
 namespace cjd {
@@ -47,6 +48,39 @@ void build_table(const uint8_t bits[16], const uint8_t vals[256], HuffTable& t) 
     code <<= 1;
   }
   memcpy(t.vals, vals, 256);
+}
+
+void build_lut(const uint8_t bits[16], const uint8_t vals[256], bool ac, uint16_t* lut, uint16_t* l1, int& pool_used) {
+  for (int e = 0; e < kL1Size; ++e) l1[e] = 0;
+  int code = 0, k = 0;
+  for (int l = 1; l <= 16; ++l) {
+    const int n = bits[l - 1];
+    for (int i = 0; i < n; ++i, ++k) {
+      const int cd = code + i;
+      const uint16_t ent = lut_entry(l, vals[k], ac);
+      if (l <= kL1Bits) {
+        const int lo = cd << (kL1Bits - l), hi = (cd + 1) << (kL1Bits - l);
+        for (int e = lo; e < hi && e < kL1Size; ++e) l1[e] = ent;
+        continue;
+      }
+      const int pre = cd >> (l - kL1Bits);
+      if (pre >= kL1Size) continue;  // not a prefix code: no stream can reach it
+      if (!(l1[pre] & 0x8000u)) {
+        if (pool_used + kL2Size > kL2Pool) {
+          l1[pre] = kLutSlow;
+        } else {
+          l1[pre] = (uint16_t)(0x8000u | (uint32_t)(4 * kL1Size + pool_used));
+          for (int e = 0; e < kL2Size; ++e) lut[4 * kL1Size + pool_used + e] = 0;
+          pool_used += kL2Size;
+        }
+      }
+      if (l1[pre] == kLutSlow) continue;
+      uint16_t* l2 = lut + (l1[pre] & 0x7FFFu);
+      const int sub = (cd << (kL1Bits + kL2Bits - l)) & (kL2Size - 1);
+      for (int e = sub; e < sub + (1 << (kL1Bits + kL2Bits - l)); ++e) l2[e] = ent;
+    }
+    code = (code + n) << 1;
+  }
 }
 
 namespace {
@@ -107,9 +141,9 @@ bool unstuff_segments(const uint8_t* src, size_t len, bool split, std::vector<si
 // batch (pass 2).
 struct ImageWork {
   std::vector<size_t> bounds;    // unstuffed segment bounds
-  uint32_t S = 4096, D1 = 0, D = 0, items = kWG;  // entropy parameters for this image
-  int64_t nseg = 0, nitems = 0, npix = 0, nwords = 0;
-  int64_t seg0 = 0, item0 = 0, pix0 = 0, word0 = 0, desc = 0, coef0 = 0;
+  uint32_t S = 4096, C = 512, items = kWG;  // entropy parameters for this image
+  int64_t nseg = 0, nitems = 0, npix = 0, nwords = 0, nent = 0;
+  int64_t seg0 = 0, item0 = 0, pix0 = 0, word0 = 0, desc = 0, coef0 = 0, ent0 = 0;
   uint64_t out0 = 0, plane0[3] = {~0ull, ~0ull, ~0ull};
 };
 
@@ -119,6 +153,35 @@ inline int64_t seg_words(size_t dlen) { return (int64_t)((((dlen + 3) / 4 + kSeg
 
 inline uint32_t ntiles_of(size_t dlen, uint32_t S) {
   return std::max<uint32_t>(1, (uint32_t)((dlen * 8 + S - 1) / S));
+}
+
+// Stream and list sizes of the tiled decoder: a speculative or prefix stream
+// holds S/2 + 128 symbols (a stream under two bits per symbol on average over
+// a tile runs into the limit: the speculative pass then leaves the tile to
+// its link, a link hands the image to the serial decoder), a DC list one
+// block per two symbols (every valid block has a DC symbol and at least one
+// AC symbol) plus slack.
+struct TileGeom {
+  uint32_t rs, rd, nck;
+};
+inline TileGeom tile_geom(uint32_t S, uint32_t C) {
+  TileGeom g;
+  g.rs = S / 2 + 128;
+  g.rd = ((g.rs / 2 + 64) + 3) & ~3u;
+  g.nck = (S - 1) / C;
+  return g;
+}
+
+// Work items of a segment of nt tiles: the first takes per_first tiles, the
+// others per_next each plus the lead-in; f(t0, ntiles) for each.
+template <class F>
+inline void for_items(uint32_t nt, uint32_t per_first, uint32_t per_next, F&& f) {
+  uint32_t t0 = 0;
+  while (t0 < nt) {
+    const uint32_t n = std::min(nt - t0, t0 == 0 ? per_first : per_next);
+    f(t0, n);
+    t0 += n;
+  }
 }
 
 // Pass 1 (per image, in parallel): parse, find the segments, count.
@@ -143,28 +206,25 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
   w.nwords = 0;
   w.nitems = 0;
   w.S = b.tile_bits;
-  w.D = b.sync_bits;
-  w.D1 = b.sync1_bits;
+  w.C = b.ck_bits;
   w.items = b.item_tiles;
   for (const TuneEntry& t : b.tune)
     if ((uint64_t)f.scan_length <= t.max_scan_bytes) {
       w.S = t.tile_bits;
-      w.D1 = t.sync1_bits;
-      w.D = t.sync_bits;
+      w.C = t.ck_bits;
       w.items = t.item_tiles;
       break;
     }
-  // sync points: trajectories from the slot hypotheses mostly agree within a
-  // few blocks, and a main trajectory that starts off the true one still
-  // joins it before the tile end, so a short first sync point suffices
-  if (!w.D) w.D = std::max<uint32_t>(64, w.S / 32);
-  if (!w.D1 || w.D1 > w.D) w.D1 = w.D;
+  if (!w.C) w.C = std::max<uint32_t>(64, w.S / 16);
   const uint32_t per_first = w.items, per_next = w.items - 1;
+  const TileGeom g = tile_geom(w.S, w.C);
   for (int64_t s = 0; s < w.nseg; ++s) {
     const size_t dlen = bd[s + 1] - bd[s];
     w.nwords += seg_words(dlen);
-    const uint32_t nt = ntiles_of(dlen, w.S);
-    w.nitems += nt <= per_first ? 1 : 1 + (nt - per_first + per_next - 1) / per_next;
+    for_items(ntiles_of(dlen, w.S), per_first, per_next, [&](uint32_t t0, uint32_t n) {
+      ++w.nitems;
+      w.nent += (int64_t)item_words(n + (t0 > 0 ? 1u : 0u), g.rs, g.rd, g.nck);
+    });
   }
   const int G = std::max(1, kPixBlocks / f.blocks_per_mcu);
   w.npix = (int64_t)f.mcus_y * ((f.mcus_x + G - 1) / G);
@@ -211,6 +271,12 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
     if (t == d.ntab) d.tab_dq[d.ntab++] = d.comp_dq[c];
     d.comp_tix[c] = (uint8_t)t;
   }
+  {
+    uint16_t* lut = &b->luts[(size_t)w.desc * kLutEntries];
+    int pool = 0;
+    for (int t = 0; t < 4; ++t) build_lut(f.huff_bits[t], f.huff_vals[t], t >= 2, lut, lut + t * kL1Size, pool);
+    for (int e = 4 * kL1Size + pool; e < kLutEntries; ++e) lut[e] = 0;
+  }
   for (int t = 0; t < 4; ++t) {
     build_table(f.huff_bits[t], f.huff_vals[t], b->tables[(size_t)(w.desc * 4 + t)]);
     uint16_t* q = &b->qtabs[(size_t)(w.desc * 4 + t) * 64];
@@ -240,7 +306,9 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
   // segments, tiles and entropy work items: one lane per tile; an item after
   // the first also decodes its lead-in tile
   const uint32_t per_first = w.items, per_next = w.items - 1;
+  const TileGeom g = tile_geom(w.S, w.C);
   int64_t item = w.item0;
+  int64_t eo = w.ent0;
   for (int64_t s = 0; s < w.nseg; ++s) {
     const size_t dlen = w.bounds[s + 1] - w.bounds[s];
     SegmentDesc sd{};
@@ -249,25 +317,28 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
     sd.nwords = (uint32_t)seg_words(dlen);
     sd.image = (uint32_t)w.desc;
     sd.tile_bits = w.S;
-    sd.sync_bits = w.D;
-    sd.sync1_bits = w.D1;
+    sd.ck_bits = w.C;
+    sd.nck = g.nck;
+    sd.rs = g.rs;
+    sd.rd = g.rd;
     sd.ntiles = ntiles_of(dlen, w.S);
     const int64_t mcu0 = s * (f.restart_interval > 0 ? f.restart_interval : nmcu);
     const int64_t mcus = std::min<int64_t>(nmcu - mcu0, f.restart_interval > 0 ? f.restart_interval : nmcu);
     sd.nblocks = (uint32_t)(mcus * nh);
     sd.first_block = d.coef_base + mcu0 * nh;
     sd.first_item = (uint32_t)item;
-    uint32_t t0 = 0, prev = ~0u;
-    while (t0 < sd.ntiles) {
+    uint32_t prev = ~0u;
+    for_items(sd.ntiles, per_first, per_next, [&](uint32_t t0, uint32_t n) {
       WorkItem wi{};
       wi.segment = (uint32_t)(w.seg0 + s);
       wi.t0 = t0;
-      wi.ntiles = std::min(sd.ntiles - t0, t0 == 0 ? per_first : per_next);
+      wi.ntiles = n;
       wi.prev_item = prev;
+      wi.ebase = (uint64_t)eo;
+      eo += (int64_t)item_words(n + (t0 > 0 ? 1u : 0u), g.rs, g.rd, g.nck);
       prev = (uint32_t)item;
       b->items[(size_t)item++] = wi;
-      t0 += wi.ntiles;
-    }
+    });
     b->segs[(size_t)(w.seg0 + s)] = sd;
   }
 
@@ -364,7 +435,7 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     r.error = image_pass1(b, r, work[(size_t)i], files[i], sizes[i]);
   });
   // pass 2: offsets, in image order
-  int64_t nimg = 0, nseg = 0, nitems = 0, npix = 0, nwords = 0, nblocks = 0;
+  int64_t nimg = 0, nseg = 0, nitems = 0, npix = 0, nwords = 0, nblocks = 0, nent = 0;
   b.total_pixels = b.scan_bytes = 0;
   b.out_bytes = 0;
   for (int i = 0; i < n; ++i) {
@@ -379,6 +450,8 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     w.pix0 = npix;
     w.word0 = nwords;
     w.coef0 = nblocks;
+    w.ent0 = nent;
+    nent += w.nent;
     nseg += w.nseg;
     nitems += w.nitems;
     npix += w.npix;
@@ -405,11 +478,13 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     b.scan_bytes += (int64_t)w.bounds.back();
   }
   b.total_blocks = nblocks;
+  b.entry_words = nent;
   b.images.resize((size_t)nimg);
   b.segs.resize((size_t)nseg);
   b.items.resize((size_t)nitems);
   b.pix.resize((size_t)npix);
   b.tables.resize((size_t)nimg * 4);
+  b.luts.resize((size_t)nimg * kLutEntries);
   b.qtabs.resize((size_t)nimg * 4 * 64);
   b.words.resize((size_t)nwords);
   // pass 3: fill, per image in parallel

@@ -29,15 +29,14 @@ struct ImageRec {
 // (the first matching entry of Plan::tune applies).
 struct TuneEntry {
   uint64_t max_scan_bytes;
-  uint32_t tile_bits, sync1_bits, sync_bits, item_tiles;
+  uint32_t tile_bits, ck_bits, item_tiles;
 };
 
 struct Plan {
   uint32_t tile_bits = 4096;
-  uint32_t sync_bits = 0;         // second sync point D in bits, 0 = tile_bits / 32 (>= 64)
-  uint32_t sync1_bits = 0;        // first sync point D1 in bits, 0 = D (one stage)
+  uint32_t ck_bits = 0;           // checkpoint spacing C in bits, 0 = tile_bits / 16 (>= 64)
   uint32_t item_tiles = kWG;      // tiles per entropy work item (2 .. kWG)
-  std::vector<TuneEntry> tune;    // per scan-size bucket overrides of the four above
+  std::vector<TuneEntry> tune;    // per scan-size bucket overrides of the three above
   int want_planes = 0;
   std::vector<ImageRec> recs;
   std::vector<ImageDesc> images;
@@ -45,14 +44,19 @@ struct Plan {
   std::vector<WorkItem> items;
   std::vector<PixItem> pix;
   std::vector<HuffTable> tables;
+  std::vector<uint16_t> luts;    // kLutEntries per image
   std::vector<uint16_t> qtabs;   // natural order, 64 per table
   std::vector<uint32_t> words;   // unstuffed segments, word aligned, 0xFF padded
   int64_t total_blocks = 0, total_pixels = 0, scan_bytes = 0;
+  int64_t entry_words = 0;       // entry buffer of the tiled decoder (regions, checkpoints, sinks)
   uint64_t out_bytes = 0;
 };
 
 // Build the plan for n files; errors[i] gets each file's parse/stage result.
 void plan_batch(Plan& p, int n, const uint8_t* const* files, const size_t* sizes, int* errors);
 void build_table(const uint8_t bits[16], const uint8_t vals[256], HuffTable& t);
+// First-level table l1 (kL1Size entries inside lut) of one Huffman table;
+// second-level tables are taken from lut's pool (pool_used entries so far).
+void build_lut(const uint8_t bits[16], const uint8_t vals[256], bool ac, uint16_t* lut, uint16_t* l1, int& pool_used);
 
 }  // namespace cjd

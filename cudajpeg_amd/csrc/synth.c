@@ -215,6 +215,12 @@ void synth_quant_table(int quality, int chroma, uint8_t out_natural[64]) {
 }
 
 /* Encoder-side code tables (T.81 C.2 canonical construction). */
+/* "Long codes" AC code lengths (huff = 1): the Annex K symbol order, 10
+ * short codes, 60 codes of 11 bits and the rest of 16 bits -- a valid table
+ * whose long codes need more second-level lookup tables than common tables. */
+static const uint8_t k_ac_long_bits[16] = {0, 2, 2, 2, 2, 2, 0, 0, 0, 0, 60, 0, 0, 0, 0, 92};
+static const uint8_t k_ac_long_bits_chr[16] = {0, 2, 2, 2, 2, 2, 0, 0, 0, 0, 60, 0, 0, 0, 0, 92};
+
 typedef struct { uint16_t code[256]; uint8_t len[256]; } ehuff_t;
 static void build_ehuff(const uint8_t bits[16], const uint8_t *vals, ehuff_t *t) {
   memset(t, 0, sizeof(*t));
@@ -327,7 +333,8 @@ static void put_header(bw_t *w, const synth_params *p, uint8_t q[3][64]) {
     raw(w, (uint8_t)comp_qtab(p, c));
   }
   /* DHT */
-  const uint8_t *bits[4] = {k_dc_lum_bits, k_ac_lum_bits, k_dc_chr_bits, k_ac_chr_bits};
+  const uint8_t *bits[4] = {k_dc_lum_bits, p->huff == 1 ? k_ac_long_bits : k_ac_lum_bits, k_dc_chr_bits,
+                            p->huff == 1 ? k_ac_long_bits_chr : k_ac_chr_bits};
   const uint8_t *vals[4] = {k_dc_vals, k_ac_lum_vals, k_dc_vals, k_ac_chr_vals};
   const uint8_t cls[4] = {0x00, 0x10, 0x01, 0x11};
   for (int t = 0; t < (C == 1 ? 2 : 4); t++) {
@@ -414,9 +421,9 @@ long synth_encode(const synth_params *p, const uint8_t *pix, uint8_t *out, long 
 
   ehuff_t dc_t[2], ac_t[2];
   build_ehuff(k_dc_lum_bits, k_dc_vals, &dc_t[0]);
-  build_ehuff(k_ac_lum_bits, k_ac_lum_vals, &ac_t[0]);
+  build_ehuff(p->huff == 1 ? k_ac_long_bits : k_ac_lum_bits, k_ac_lum_vals, &ac_t[0]);
   build_ehuff(k_dc_chr_bits, k_dc_vals, &dc_t[1]);
-  build_ehuff(k_ac_chr_bits, k_ac_chr_vals, &ac_t[1]);
+  build_ehuff(p->huff == 1 ? k_ac_long_bits_chr : k_ac_chr_bits, k_ac_chr_vals, &ac_t[1]);
 
   bw_t w = {out, cap, 0, 0, 0, 0, 0};
   put_header(&w, p, q);

@@ -17,7 +17,7 @@ typedef struct {
   double amp_total;         /* sum of the 6 sinusoid amplitudes */
   double random_tile_frac;  /* fraction of 64x64 tiles replaced by uniform noise */
   int32_t qtables;          /* quant tables for YCbCr: 2 (Y | CbCr) or 3 (Y | Cb | Cr) */
-  int32_t pad_;
+  int32_t huff;             /* AC Huffman tables: 0 Annex K, 1 long codes (60 of 11 bits, 92 of 16) */
 } synth_params;
 
 /* W*H*ncomp interleaved pixels; returns 0 on success. */

@@ -92,16 +92,15 @@ class BatchDecoder:
     def set_options(self, idct: int, entropy: int = L.ENTROPY_TILED, tile_bits: int = 0, planes: bool = False):
         _check(L.lib().cjd_batch_set_options(self.h, idct, entropy, tile_bits, int(planes)), "options: ")
 
-    def set_tuning(self, sync1_bits: int = 0, sync_bits: int = 0, item_tiles: int = 0):
-        """Tiled entropy decoder: first and second sync points (bits into a
-        tile) and tiles per work item; 0 = defaults; applies from the next
-        stage()."""
-        _check(L.lib().cjd_batch_set_tuning(self.h, sync1_bits, sync_bits, item_tiles), "tuning: ")
+    def set_tuning(self, ck_bits: int = 0, item_tiles: int = 0):
+        """Tiled entropy decoder: checkpoint spacing in bits and tiles per
+        work item; 0 = defaults; applies from the next stage()."""
+        _check(L.lib().cjd_batch_set_tuning(self.h, ck_bits, item_tiles), "tuning: ")
 
     def set_tune_table(self, table) -> None:
         """Per-image entropy parameters by scan size: a list of
-        (max_scan_bytes, tile_bits, sync1_bits, sync_bits, item_tiles)."""
-        arr = (L.TuneEntry * max(1, len(table)))(*[L.TuneEntry(*t) for t in table])
+        (max_scan_bytes, tile_bits, ck_bits, item_tiles)."""
+        arr = (L.TuneEntry * max(1, len(table)))(*[L.TuneEntry(*t[:4], 0) for t in table])
         _check(L.lib().cjd_batch_set_tune_table(self.h, len(table), arr), "tune table: ")
 
     def stage(self, files: list[bytes]) -> np.ndarray:

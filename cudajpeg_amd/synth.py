@@ -33,7 +33,7 @@ class _Params(ctypes.Structure):
         ("amp_total", ctypes.c_double),
         ("random_tile_frac", ctypes.c_double),
         ("qtables", ctypes.c_int32),
-        ("pad_", ctypes.c_int32),
+        ("huff", ctypes.c_int32),
     ]
 
 
@@ -74,12 +74,13 @@ class ImageSpec:
     amp_total: float = 80.0
     restart_interval: int = 0
     qtables: int = 2               # YCbCr: 2 (Y | CbCr) or 3 (Y | Cb | Cr) quant tables
+    huff: int = 0                  # AC Huffman tables: 0 Annex K, 1 long codes (60 of 11 bits)
 
     def _params(self) -> _Params:
         ncomp, h, v = SAMPLING[self.sampling]
         return _Params(self.seed, self.width, self.height, ncomp, h, v, self.quality,
                        self.restart_interval, self.sigma, self.amp_total, self.random_tile_frac,
-                       self.qtables, 0)
+                       self.qtables, self.huff)
 
     @property
     def ncomp(self) -> int:

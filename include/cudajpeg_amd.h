@@ -85,28 +85,27 @@ void cjd_context_destroy(cjd_context *ctx);
 int cjd_batch_create(cjd_context *ctx, cjd_batch **out);
 void cjd_batch_destroy(cjd_batch *b);
 
-/* Options: iDCT variant, entropy decoder, tile length in bits (>= 512, a
- * multiple of 32, 0 = default 4096), whether planar component planes are
+/* Options: iDCT variant, entropy decoder, tile length in bits (512 .. 8192,
+ * a multiple of 32, 0 = default 4096), whether planar component planes are
  * also written. */
 int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int tile_bits, int want_planes);
 
 /* Tiled entropy decoder tuning (no counterpart in the reference, whose scan
- * decoder has no tiles): the two sync points in bits into a tile where the
- * per-slot trajectories are compared (0 = defaults; the first is used alone
- * when the trajectories all agree there or when it is not below the second)
- * and tiles per work item (2 .. 256, 0 = 256).  Takes effect at the next
+ * decoder has no tiles): the spacing in bits of the checkpoints where a
+ * tile's link can join its speculative pass (>= 64, 0 = tile length / 16) and
+ * tiles per work item (2 .. 256, 0 = 256).  Takes effect at the next
  * cjd_batch_stage. */
-int cjd_batch_set_tuning(cjd_batch *b, int sync1_bits, int sync_bits, int item_tiles);
+int cjd_batch_set_tuning(cjd_batch *b, int ck_bits, int item_tiles);
 
 /* Per-image tuning by scan size (no counterpart in the reference): an image
  * whose entropy-coded scan has at most max_scan_bytes bytes uses the first
- * such entry's tile length, sync points (0 = defaults) and tiles per work
- * item; images past every entry use the batch options.  n = 0 clears the
- * table.  Entries come from an autotuner (cudajpeg_amd.autotune).  Takes
+ * such entry's tile length, checkpoint spacing (0 = default) and tiles per
+ * work item; images past every entry use the batch options.  n = 0 clears
+ * the table.  Entries come from an autotuner (cudajpeg_amd.autotune).  Takes
  * effect at the next cjd_batch_stage. */
 typedef struct {
   uint64_t max_scan_bytes;
-  int tile_bits, sync1_bits, sync_bits, item_tiles;
+  int tile_bits, ck_bits, item_tiles, reserved;
 } cjd_tune_entry;
 int cjd_batch_set_tune_table(cjd_batch *b, int n, const cjd_tune_entry *entries);
 

@@ -15,7 +15,7 @@ def lib():
         subprocess.run(["make", "-s", "-C", HERE], check=True)
         L = ctypes.CDLL(os.path.join(HERE, "libcjd_emu.so"))
         L.emu_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_void_p]
         L.emu_run.restype = ctypes.c_void_p
         L.emu_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
@@ -28,7 +28,7 @@ def lib():
 
 class EmuResult:
     def __init__(self, files, idct_mode=0, tile_bits=4096, want_planes=False, order=0, serial=False,
-                 item_tiles=0, sync_bits=0, sync1_bits=0):
+                 item_tiles=0, ck_bits=0):
         L = lib()
         n = len(files)
         self._keep = [ctypes.create_string_buffer(f, len(f)) for f in files]
@@ -36,7 +36,7 @@ class EmuResult:
         sizes = (ctypes.c_size_t * n)(*[len(f) for f in files])
         self.errors = np.zeros(n, np.int32)
         self.h = L.emu_run(n, ptrs, sizes, idct_mode, tile_bits, int(want_planes), order, int(serial),
-                           item_tiles, sync1_bits, sync_bits, self.errors.ctypes.data)
+                           item_tiles, ck_bits, self.errors.ctypes.data)
         st = np.zeros(4, np.int64)
         L.emu_stats(self.h, st.ctypes.data)
         self.items, self.fixes, self.fallbacks, self.segments = (int(x) for x in st)

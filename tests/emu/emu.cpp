@@ -26,6 +26,8 @@ struct Emu {
   std::vector<int16_t> coefs;
   std::vector<uint8_t> out;
   std::vector<uint32_t> status;
+  std::vector<uint32_t> ebuf;
+  std::vector<BlockRec> recs;
   int fixes = 0;
   int fallbacks = 0;
 };
@@ -36,25 +38,47 @@ struct EntRun {
   std::vector<std::unique_ptr<EntShared>> shs;
   std::vector<EntCtx> ctx;
   std::vector<char> done;
-  int fixes = 0;  // tiles decoded again in fix-up rounds (a link or the entry off the main chain)
+  int fixes = 0;  // links decoded again in fix-up rounds (entries off the speculative chain)
 };
 
-// Fix-up rounds lane by lane (each round: all lanes read, then the lanes
-// that need it relink); returns the number of relinks.
+// Link tasks: every queued task runs a round (task t on lane t), then the
+// unfinished ones are packed, as in the kernel.
+void run_links(EntShared& sh, const EntCtx& c) {
+  for (;;) {
+    const int n = sh.ntask;
+    if (!n) return;
+    std::vector<Task> T((size_t)n);
+    for (int t = 0; t < n; ++t) T[(size_t)t] = task_get(sh, t);
+    sh.nnext = 0;
+    for (int t = 0; t < n; ++t) {
+      Lane L;
+      int j;
+      if (!task_round(sh, t, c, T[(size_t)t], L, j)) task_put(sh, sh.nnext++, j, L);
+    }
+    sh.ntask = sh.nnext;
+  }
+}
+
+void queue_links(EntShared& sh, const std::vector<char>& task) {
+  sh.ntask = 0;
+  for (int t = 0; t < kWG; ++t)
+    if (task[(size_t)t]) task_new(sh, sh.ntask++, t);
+}
+
+// Fix-up rounds (each round: all lanes read, then the lanes that need it
+// relink); returns the number of relinks.
 int fix_rounds(EntShared& sh, const EntCtx& c) {
   int total = 0;
   for (;;) {
     std::vector<uint32_t> P(kWG), S(kWG);
-    std::vector<char> need(kWG);
-    for (int t = 0; t < kWG; ++t) need[t] = ent_fix_need(sh, t, c, P[t], S[t]);
+    std::vector<char> need(kWG), task(kWG);
     int n = 0;
-    for (int t = 0; t < kWG; ++t)
-      if (need[t]) {
-        ent_fix_apply(sh, t, c, P[t], S[t]);
-        ++n;
-      }
-    total += n;
+    for (int t = 0; t < kWG; ++t) n += (need[(size_t)t] = ent_fix_need(sh, t, c, P[(size_t)t], S[(size_t)t])) ? 1 : 0;
     if (!n) return total;
+    total += n;
+    for (int t = 0; t < kWG; ++t) task[(size_t)t] = need[(size_t)t] && link_begin(sh, c, t, P[(size_t)t], S[(size_t)t]);
+    queue_links(sh, task);
+    run_links(sh, c);
   }
 }
 
@@ -65,6 +89,10 @@ void scan_counts(EntShared& sh, const EntCtx& c) {
     for (int t = 0; t < kWG; ++t) ent_scan_read(sh, t, d, &v[(size_t)t * 4]);
     for (int t = 0; t < c.J; ++t) ent_scan_write(sh, t, &v[(size_t)t * 4]);
   }
+}
+
+void count_all(EntShared& sh, const EntCtx& c) {
+  for (int t = 0; t < kWG; ++t) ent_count(sh, t, c);
 }
 
 // Lookback, entry application and inclusive publication of item i, resolving
@@ -96,9 +124,10 @@ void resolve(EntRun& R, size_t i) {
     sh.entry_s = s;
     for (int k = 0; k < 4; ++k) sh.entry_c[k] = cc[k];
     if (ent_entry_differs(sh, c)) {
-      sh.tp[0] = p;
-      sh.ts[0] = s;
+      sh.xp[0] = p;
+      sh.xs[0] = (uint8_t)s;
       R.fixes += fix_rounds(sh, c);
+      count_all(sh, c);
       scan_counts(sh, c);
     }
   }
@@ -108,9 +137,29 @@ void resolve(EntRun& R, size_t i) {
   R.done[i] = 1;
 }
 
+// k_expand, block by block.
+void expand(Emu& E) {
+  const uint64_t ew = E.ebuf.size();
+  for (int64_t b = 0; b < E.plan.total_blocks; ++b) {
+    const BlockRec r = E.recs[(size_t)b];
+    int16_t* B = E.coefs.data() + b * 64;
+    for (int k = 0; k < 64; ++k) B[k] = 0;
+    const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
+    const uint64_t e0 = r.eidx + (uint64_t)n * W <= ew ? r.eidx : 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t e = E.ebuf[e0 + (uint64_t)i * W];
+      if (e != 0xFFFFFFFFu) B[(e >> 16) & 63] = (int16_t)(e & 0xFFFFu);
+    }
+    B[0] = r.dc;
+  }
+}
+
 void run_entropy(Emu& E, int order) {
   Plan& p = E.plan;
   const size_t ni = p.items.size();
+  E.ebuf.assign((size_t)p.entry_words + 16, 0x7EADBEEFu);
+  BlockRec junk{0, 0, 0x5555};  // eidx, nw, dc
+  E.recs.assign((size_t)p.total_blocks, junk);
   EntRun R;
   R.p = &p;
   R.lb.resize(ni);
@@ -118,18 +167,28 @@ void run_entropy(Emu& E, int order) {
   R.shs.resize(ni);
   R.ctx.resize(ni);
   R.done.assign(ni, 0);
-  // phases 1-2, fix-up, scan and aggregate publication for every item
+  // speculative pass, links, fix-up, counts, scan and aggregate publication
+  // for every item
   for (size_t i = 0; i < ni; ++i) {
     R.shs[i].reset(new EntShared);
     EntShared& sh = *R.shs[i];
     memset(&sh, 0, sizeof(sh));
     EntCtx& c = R.ctx[i];
-    c.init(p.words.data(), p.images.data(), p.segs.data(), p.items[i]);
-    memcpy(sh.tab, &p.tables[c.im->table_base], sizeof(sh.tab));
+    c.init(p.words.data(), p.images.data(), p.segs.data(), p.tables.data(), E.ebuf.data(), p.items[i]);
+    memcpy(sh.lut, &p.luts[(size_t)c.sg.image * kLutEntries], sizeof(sh.lut));
     memcpy(sh.dz, kDeZigZag, 64);
-    for (int t = 0; t < kWG; ++t) ent_phase1(sh, t, c);
-    for (int t = 0; t < kWG; ++t) ent_phase2(sh, t, c);
+    for (int k = 0; k < 4; ++k) canon_fill(sh.canon[k], c.tabs[k]);
+    for (int k = 0; k < c.nh; ++k) {
+      sh.tsel[2 * k] = (uint16_t)(c.im->slot_ac[k] * kL1Size);
+      sh.tsel[2 * k + 1] = (uint16_t)(c.im->slot_dc[k] * kL1Size);
+    }
+    for (int t = 0; t < kWG; ++t) ent_spec(sh, t, c);
+    std::vector<char> task(kWG);
+    for (int t = 0; t < kWG; ++t) task[(size_t)t] = ent_phase2(sh, t, c);
+    queue_links(sh, task);
+    run_links(sh, c);
     R.fixes += fix_rounds(sh, c);
+    count_all(sh, c);
     scan_counts(sh, c);
     if (c.lead) {
       ent_aggregate(sh, c, &R.lb[i]);
@@ -143,7 +202,8 @@ void run_entropy(Emu& E, int order) {
   for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
   E.fixes = R.fixes;
   for (size_t i = 0; i < ni; ++i)
-    for (int t = 0; t < kWG; ++t) ent_phase5(*R.shs[i], t, R.ctx[i], E.coefs.data(), E.status.data());
+    for (int t = 0; t < kWG; ++t) ent_records(*R.shs[i], t, R.ctx[i], E.recs.data(), E.status.data());
+  expand(E);
 }
 
 void run_pixel(Emu& E, int mode, size_t first, size_t count);
@@ -184,12 +244,10 @@ void run_pixel(Emu& E, int mode, size_t first, size_t count) {
 extern "C" {
 
 void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_mode, int tile_bits,
-              int want_planes, int order, int serial, int item_tiles, int sync1_bits, int sync_bits,
-              int* errors) {
+              int want_planes, int order, int serial, int item_tiles, int ck_bits, int* errors) {
   Emu* E = new Emu;
   E->plan.tile_bits = tile_bits ? (uint32_t)tile_bits : 4096u;
-  E->plan.sync_bits = (uint32_t)sync_bits;
-  E->plan.sync1_bits = (uint32_t)sync1_bits;
+  E->plan.ck_bits = (uint32_t)ck_bits;
   if (item_tiles) E->plan.item_tiles = (uint32_t)item_tiles;
   E->plan.want_planes = want_planes;
   plan_batch(E->plan, n, files, sizes, errors);
@@ -244,13 +302,19 @@ long emu_get(void* h, int i, int what, void* dst, size_t cap) {
   return (long)n;
 }
 
-// [items, tiles decoded again in fix-up rounds, fallbacks, segments]
+// [items, links decoded again in fix-up rounds, fallbacks, segments]
 void emu_stats(void* h, long* out4) {
   Emu* E = static_cast<Emu*>(h);
   out4[0] = (long)E->plan.items.size();
   out4[1] = E->fixes;
   out4[2] = E->fallbacks;
   out4[3] = (long)E->plan.segs.size();
+}
+
+// per-image status bits of the last run (before the host maps them to codes)
+void emu_status(void* h, uint32_t* out, int n) {
+  Emu* E = static_cast<Emu*>(h);
+  for (int i = 0; i < n && i < (int)E->status.size(); ++i) out[i] = E->status[(size_t)i];
 }
 
 void emu_free(void* h) { delete static_cast<Emu*>(h); }

@@ -79,16 +79,16 @@ HARD = [synth.ImageSpec(1200, 900, "420", 63, 8.9, 5),
         synth.ImageSpec(1000, 700, "422", 80, 6.0, 9, restart_interval=7)]
 
 
-@pytest.mark.parametrize("item_tiles,tile_bits,sync_bits", [(2, 2048, 0), (3, 4096, 0), (8, 4096, 512),
-                                                            (8, 8192, 0)])
+@pytest.mark.parametrize("item_tiles,tile_bits,ck_bits", [(2, 2048, 0), (3, 4096, 0), (8, 4096, 128),
+                                                          (8, 8192, 0)])
 @pytest.mark.parametrize("order", [0, 1])
-def test_emulated_small_items_chain_fixups(item_tiles, tile_bits, sync_bits, order):
+def test_emulated_small_items_chain_fixups(item_tiles, tile_bits, ck_bits, order):
     # Items of 2-8 tiles: many items per segment, so the lookback composes
     # chains of aggregates (order 1: every predecessor has only its
     # aggregate), and links / entries that miss the main trajectory take the
     # sequential fix-up.  Both must occur here and the result stay bit-exact.
     encs = [synth.encode(s, truth=True) for s in HARD]
-    r = EmuResult([e.data for e in encs], 0, tile_bits, False, order, False, item_tiles, sync_bits)
+    r = EmuResult([e.data for e in encs], 0, tile_bits, False, order, False, item_tiles, ck_bits)
     assert not r.errors.any() and r.fallbacks == 0
     assert r.fixes > 0 and r.items > 4 * len(HARD)
     for i, (s, e) in enumerate(zip(HARD, encs)):
@@ -120,3 +120,18 @@ def test_emulated_mfma_contraction_matches_fp64(encoded):
             assert np.abs(got - f64[c]).max() <= 1, s
             assert (got == f64[c]).mean() > 0.999, s
             assert np.abs(got - i32[c]).max() <= 1, s
+
+
+@pytest.mark.parametrize("tile_bits", [1024, 4096])
+def test_emulated_long_code_tables(tile_bits):
+    # AC tables with 60 codes of 11 bits and 92 of 16: the image's long codes
+    # need more second-level lookup tables than the pool holds, so the chroma
+    # AC table decodes its long codes canonically; parity must not change.
+    specs = [synth.ImageSpec(320, 240, "444", 90, 8.0, 71, huff=1),
+             synth.ImageSpec(400, 300, "420", 75, 8.0, 72, huff=1, restart_interval=5)]
+    encs = [synth.encode(s, truth=True) for s in specs]
+    r = EmuResult([e.data for e in encs], 0, tile_bits, False, 0)
+    assert not r.errors.any()
+    for i, (s, e) in enumerate(zip(specs, encs)):
+        assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
+

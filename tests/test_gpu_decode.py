@@ -25,6 +25,9 @@ SMALL = [
     synth.ImageSpec(200, 136, "420", 80, 6.0, 31, qtables=3),
     synth.ImageSpec(72, 40, "444", 60, 6.0, 32, qtables=3),
     synth.ImageSpec(8, 8, "gray", 100, 2.0, 13),
+    # long-code AC tables: more second-level lookup tables than the pool holds
+    synth.ImageSpec(320, 240, "444", 90, 8.0, 71, huff=1),
+    synth.ImageSpec(400, 300, "420", 75, 8.0, 72, huff=1, restart_interval=5),
 ]
 
 
@@ -165,7 +168,7 @@ def test_tune_table_per_image_parameters(gpu):
     specs = [synth.ImageSpec(96, 64, "420", 75, 6.0, 61), synth.ImageSpec(640, 480, "444", 90, 8.0, 62),
              synth.ImageSpec(1024, 768, "420", 85, 10.0, 63, random_tile_frac=0.2)]
     encs = [synth.encode(s, truth=True) for s in specs]
-    table = [(8 << 10, 1024, 0, 0, 16), (256 << 10, 2048, 0, 0, 64), (1 << 62, 8192, 0, 0, 256)]
+    table = [(8 << 10, 1024, 0, 16), (256 << 10, 2048, 256, 64), (1 << 62, 8192, 0, 256)]
     b = BatchDecoder(IDCT_INT)
     b.set_tune_table(table)
     assert not b.stage([e.data for e in encs]).any()
