@@ -6,16 +6,22 @@ quality U{60..95}, noise U[2,12] -- with the scans already resident in HBM
 and the RGB/gray frames left in HBM.
 
 Reported:
-  value              batched form (all images in one entropy launch and one
-                     pixel launch, captured as a HIP graph), Mpixel/s (W*H)
-  plain_mpix_s       plain form: one image per launch, FP32 VALU iDCT
-  ratio              batched / plain
-  p50/p99_ms         per-image decode latency of the plain form
+  value              batched form (all images in one entropy launch, one
+                     record expansion and one pixel launch, captured as a
+                     HIP graph), Mpixel/s (W*H)
+  kernel_ms          device time per kernel (HIP events on the decode stream)
   roofline           dominant kernel, achieved algorithmic GB/s vs 8 TB/s;
-                     traffic = its HBM bytes per launch from the committed
-                     FETCH_SIZE/WRITE_SIZE summary of this workload
+                     traffic = its HBM bytes per launch from a committed
+                     FETCH_SIZE/WRITE_SIZE summary of this workload and these
+                     kernels, null when there is none
   cpu_baseline       the reference CPUonly decoder (built from its sources
                      into oracle/_ref) on a bounded sample, one host core
+With --plain (off by default):
+  plain_mpix_s       plain form: one image per launch, FP32 VALU iDCT, with
+                     per-size entropy parameters from a run-time autotune
+  ratio              batched / plain
+  p50/p99_ms         per-image decode latency of the plain form
+With --serve (off by default): the serving loop at 70% of its peak rate.
 """
 from __future__ import annotations
 
@@ -118,7 +124,7 @@ def measured_traffic(kernel: str, args) -> int | None:
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this same workload (profiles/hbm_traffic_mixed256.md, counters
     scaled by a copy kernel of known size); None for any other workload."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed256.md")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed256_entries.md")
     if args.images != 256 or args.seed != 4000 or args.tile_bits or not os.path.exists(path):
         return None
     text = open(path).read()
@@ -134,11 +140,16 @@ def main():
     ap.add_argument("--images", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4000)
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--item-tiles", type=int, default=0, help="tiles per entropy work item (0 = default)")
+    ap.add_argument("--ck-bits", type=int, default=0, help="checkpoint spacing of the tiled decoder (0 = default)")
     ap.add_argument("--batched-idct", default="mfma", choices=["int", "float", "mfma"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--plain", action="store_true",
+                    help="also run the plain form (autotune, then one image per launch) and its latencies")
+    ap.add_argument("--serve", action="store_true", help="also run the serving-loop load test")
+    ap.add_argument("--no-plain", action="store_true", help="the default; kept for older command lines")
+    ap.add_argument("--no-serve", action="store_true", help="the default; kept for older command lines")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (reads MIB, writes MIB) "
@@ -178,6 +189,8 @@ def main():
     idct = {"int": IDCT_INT, "float": IDCT_FLOAT, "mfma": IDCT_MFMA}[args.batched_idct]
     # ---------------------------------------------------------------- batched
     b = BatchDecoder(idct, tile_bits=args.tile_bits)
+    if args.item_tiles or args.ck_bits:
+        b.set_tuning(args.ck_bits, args.item_tiles)
     errs = b.stage(files)
     assert not errs.any(), errs
     # host side of the boundary: parse + unstuff + plan + upload of the scans
@@ -205,12 +218,15 @@ def main():
         torch.cuda.synchronize()
         t_batched = (time.perf_counter() - t_start) / args.steps
         # per-kernel device times (events on the decode stream), averaged
-        ent, pix = [], []
+        ent, pix, kent, kexp = [], [], [], []
         for _ in range(args.steps):
             b.decode(stream)
             a, c, _ = b.timings()
+            ka, kx = b.kernel_timings()
             ent.append(a)
             pix.append(c)
+            kent.append(ka)
+            kexp.append(kx)
         assert not b.finish(stream).any()
     batched_mpix = total_px / t_batched / 1e6
     # frames back to host memory (the reader API's pixels buffer)
@@ -219,14 +235,18 @@ def main():
         b.pixels(i)
     d2h_ms = (time.perf_counter() - t1) * 1e3
     ms_ent, ms_pix = float(np.mean(ent)), float(np.mean(pix))
-    log(f"batched: {t_batched * 1e3:.2f} ms/step, {batched_mpix:.0f} Mpix/s; entropy {ms_ent:.2f} ms, "
-        f"pixel {ms_pix:.2f} ms")
+    ms_kent, ms_kexp = float(np.mean(kent)), float(np.mean(kexp))
+    log(f"batched: {t_batched * 1e3:.2f} ms/step, {batched_mpix:.0f} Mpix/s; entropy stage {ms_ent:.2f} ms "
+        f"(k_entropy {ms_kent:.2f}, k_expand {ms_kexp:.2f}), pixel {ms_pix:.2f} ms")
 
-    # algorithmic bytes per launch
+    # algorithmic bytes per launch: the entropy decode reads the scan once and
+    # yields 128 B of coefficients per block (k_entropy encodes them as
+    # entries and block records, k_expand lays them out densely); the pixel
+    # kernel reads the coefficients and writes the frames
     ent_bytes = tot["scan_bytes"] + tot["blocks"] * 128
     pix_bytes = tot["blocks"] * 128 + tot["out_bytes"]
-    if ms_ent >= ms_pix:
-        dom, dom_bytes, dom_ms = "cjd::k_entropy", ent_bytes, ms_ent
+    if ms_kent >= ms_pix:
+        dom, dom_bytes, dom_ms = "cjd::k_entropy", ent_bytes, ms_kent
     else:
         dom, dom_bytes, dom_ms = f"cjd::k_pixel<{idct}>", pix_bytes, ms_pix
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -236,7 +256,7 @@ def main():
     plain_mpix = None
     lat = []
     tune_table = None
-    if not args.no_plain:
+    if args.plain and not args.no_plain:
         # per-image entropy parameters for one-image launches, autotuned per
         # scan-size bucket on a few images of the batch
         from cudajpeg_amd.autotune import autotune
@@ -271,7 +291,7 @@ def main():
         for d in per:
             d.close()
 
-    serving = None if args.no_serve else serve_bench(args)
+    serving = serve_bench(args) if args.serve and not args.no_serve else None
     if serving:
         log(f"serving: {serving}")
 
@@ -305,7 +325,8 @@ def main():
         "ratio_batched_vs_plain": None if plain_mpix is None else round(batched_mpix / plain_mpix, 2),
         "p50_ms": None if not lat else round(float(np.percentile(lat, 50)), 3),
         "p99_ms": None if not lat else round(float(np.percentile(lat, 99)), 3),
-        "kernel_ms": {"k_entropy": round(ms_ent, 3), "k_pixel": round(ms_pix, 3)},
+        "kernel_ms": {"k_entropy": round(ms_kent, 3), "k_expand": round(ms_kexp, 3), "k_pixel": round(ms_pix, 3),
+                      "entropy_stage": round(ms_ent, 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
                      "algorithmic_bytes": int(dom_bytes)},

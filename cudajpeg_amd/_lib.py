@@ -22,7 +22,7 @@ EXPORTS = [
     "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
     "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
     "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_finish", "cjd_batch_capture",
-    "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
+    "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_kernel_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
     "cjd_batch_download", "cjd_batch_totals", "cjd_error_name",
     "cjd_server_create", "cjd_server_submit", "cjd_server_poll", "cjd_server_release", "cjd_server_now_ns",
     "cjd_server_replay", "cjd_server_destroy",
@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_capture": ([vp, vp], i32),
         "cjd_batch_replay": ([vp, vp], i32),
         "cjd_batch_timings": ([vp, vp, vp, vp], i32),
+        "cjd_batch_kernel_timings": ([vp, vp, vp], i32),
         "cjd_batch_image_info": ([vp, i32, FP], i32),
         "cjd_batch_device_ptr": ([vp, i32, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)], i32),
         "cjd_batch_download": ([vp, i32, i32, vp, sz], i32),

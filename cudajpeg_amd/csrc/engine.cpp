@@ -64,7 +64,9 @@ struct cjd_batch {
   std::vector<uint32_t> h_status;
   int64_t fallbacks = 0;
 
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // timed decodes: 0 start, 3 after the scratch clear, 4 after the entropy
+  // kernel, 1 after the entropy stage (k_expand included), 2 after the pixels
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -201,6 +203,7 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
   LookbackRec* lb = reinterpret_cast<LookbackRec*>(scratch + b->ticket_word + 4);
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   if (launch_clear(scratch, b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   hipError_t he;
   if (b->entropy_mode == CJD_ENTROPY_SERIAL)
     he = launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
@@ -211,6 +214,7 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
                         b->d_items.as<WorkItem>(), (uint32_t)b->plan.items.size(), b->d_tables.as<HuffTable>(),
                         b->d_luts.as<uint16_t>(), b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
                         ticket, s);
+    if (he == hipSuccess && timed && hipEventRecord(b->ev[4], s) != hipSuccess) he = hipErrorUnknown;
     if (he == hipSuccess)
       he = launch_expand(b->d_recs.as<BlockRec>(), b->d_ebuf.as<uint32_t>(), (uint64_t)b->plan.entry_words,
                          b->d_coefs.as<int16_t>(), (uint64_t)b->plan.total_blocks, s);
@@ -339,6 +343,17 @@ extern "C" int cjd_batch_timings(cjd_batch* b, float* ms_entropy, float* ms_pixe
   if (ms_entropy) *ms_entropy = a;
   if (ms_pixel) *ms_pixel = c;
   if (ms_total) *ms_total = t;
+  return 0;
+}
+
+extern "C" int cjd_batch_kernel_timings(cjd_batch* b, float* ms_entropy_kernel, float* ms_expand) {
+  if (!b || !b->timed || b->entropy_mode != CJD_ENTROPY_TILED) return CJD_PROGRAMMER_ERROR;
+  if (hipEventSynchronize(b->ev[1]) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  float a = 0, x = 0;
+  (void)hipEventElapsedTime(&a, b->ev[3], b->ev[4]);
+  (void)hipEventElapsedTime(&x, b->ev[4], b->ev[1]);
+  if (ms_entropy_kernel) *ms_entropy_kernel = a;
+  if (ms_expand) *ms_expand = x;
   return 0;
 }
 

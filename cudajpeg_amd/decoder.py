@@ -143,6 +143,12 @@ class BatchDecoder:
         _check(L.lib().cjd_batch_timings(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "timings: ")
         return a.value, b.value, c.value
 
+    def kernel_timings(self) -> tuple[float, float]:
+        """(tiled entropy kernel alone, record expansion) of the last decode, ms."""
+        a, x = ctypes.c_float(), ctypes.c_float()
+        _check(L.lib().cjd_batch_kernel_timings(self.h, ctypes.byref(a), ctypes.byref(x)), "kernel timings: ")
+        return a.value, x.value
+
     def info(self, i: int) -> L.FrameInfo:
         f = L.FrameInfo()
         L.lib().cjd_batch_image_info(self.h, i, ctypes.byref(f))

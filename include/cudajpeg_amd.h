@@ -147,8 +147,14 @@ int cjd_batch_capture(cjd_batch *b, void *stream);
 int cjd_batch_replay(cjd_batch *b, void *stream);
 
 /* Device time of the last cjd_batch_decode (ms), measured with HIP events on
- * the decode stream: entropy kernel, pixel kernel, whole decode. */
+ * the decode stream: entropy stage (scratch clear, tiled decode, record
+ * expansion), pixel kernel, whole decode. */
 int cjd_batch_timings(cjd_batch *b, float *ms_entropy, float *ms_pixel, float *ms_total);
+/* The entropy stage of the last cjd_batch_decode split by kernel (ms): the
+ * tiled decode kernel alone and the expansion of its block records into
+ * coefficient blocks (tiled entropy mode only).  No counterpart in the
+ * reference; used by the benchmark's per-kernel roofline. */
+int cjd_batch_kernel_timings(cjd_batch *b, float *ms_entropy_kernel, float *ms_expand);
 
 /* Geometry and device pointers of image i's outputs. */
 int cjd_batch_image_info(cjd_batch *b, int i, cjd_frame_info *info);

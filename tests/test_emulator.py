@@ -122,16 +122,20 @@ def test_emulated_mfma_contraction_matches_fp64(encoded):
             assert np.abs(got - i32[c]).max() <= 1, s
 
 
-@pytest.mark.parametrize("tile_bits", [1024, 4096])
-def test_emulated_long_code_tables(tile_bits):
+@pytest.mark.parametrize("tile_bits,item_tiles", [(1024, 0), (4096, 0), (512, 3)])
+def test_emulated_long_code_tables(tile_bits, item_tiles):
     # AC tables with 60 codes of 11 bits and 92 of 16: the image's long codes
     # need more second-level lookup tables than the pool holds, so the chroma
     # AC table decodes its long codes canonically; parity must not change.
+    # Items of 3 tiles: entries differ from the assumed ones, so links run
+    # again after the counts (which overlay the canonical tables in LDS).
     specs = [synth.ImageSpec(320, 240, "444", 90, 8.0, 71, huff=1),
              synth.ImageSpec(400, 300, "420", 75, 8.0, 72, huff=1, restart_interval=5)]
     encs = [synth.encode(s, truth=True) for s in specs]
-    r = EmuResult([e.data for e in encs], 0, tile_bits, False, 0)
+    r = EmuResult([e.data for e in encs], 0, tile_bits, False, 0, False, item_tiles)
     assert not r.errors.any()
+    if item_tiles:
+        assert r.fixes > 0
     for i, (s, e) in enumerate(zip(specs, encs)):
         assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
 
