@@ -162,20 +162,6 @@ def test_mfma_idct_error_vs_fp64(gpu):
         b.close()
 
 
-def test_kernel_timings_split_the_entropy_stage(gpu, small_encoded):
-    # the per-kernel events lie inside the entropy stage: k_entropy alone plus
-    # the record expansion never exceed it (the scratch clear is the rest)
-    b = BatchDecoder(IDCT_INT)
-    assert not b.stage([e.data for e in small_encoded]).any()
-    b.decode()
-    assert not b.finish().any()
-    stage, _, _ = b.timings()
-    k, x = b.kernel_timings()
-    print("entropy stage", stage, "k_entropy", k, "k_expand", x, flush=True)
-    assert 0 < k <= stage and 0 <= x <= stage and k + x <= stage + 1e-3
-    b.close()
-
-
 def test_tune_table_per_image_parameters(gpu):
     # a per-scan-size table puts the small and the large images of one batch
     # on different tile lengths / item sizes; coefficients stay bit-exact
