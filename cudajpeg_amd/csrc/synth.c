@@ -364,7 +364,10 @@ long synth_encode(const synth_params *p, const uint8_t *pix, uint8_t *out, long 
   int hs[3] = {1, 1, 1}, vs[3] = {1, 1, 1};
   if (C == 3) { hs[0] = p->h_samp; vs[0] = p->v_samp; }
   if (W <= 0 || H <= 0 || W > 65535 || H > 65535) return -1;
-  if (hs[0] < 1 || hs[0] > 2 || vs[0] < 1 || vs[0] > 2) return -1;
+  /* luma factors 1, 2 or 4 (powers of two, as the reference accepts,
+     src/format.cu:274-276), at most 10 blocks per MCU with the chroma */
+  if ((hs[0] != 1 && hs[0] != 2 && hs[0] != 4) || (vs[0] != 1 && vs[0] != 2 && vs[0] != 4) || hs[0] * vs[0] > 8)
+    return -1;
   cos_init();
   const int hmax = hs[0], vmax = vs[0];
   const int nbx = (W + 8 * hmax - 1) / (8 * hmax), nby = (H + 8 * vmax - 1) / (8 * vmax);

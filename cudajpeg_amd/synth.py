@@ -58,7 +58,9 @@ def _lib():
     return _LIB
 
 
-SAMPLING = {"420": (3, 2, 2), "422": (3, 2, 1), "444": (3, 1, 1), "gray": (1, 1, 1)}
+# (components, luma h, luma v); chroma is 1x1
+SAMPLING = {"420": (3, 2, 2), "422": (3, 2, 1), "444": (3, 1, 1), "gray": (1, 1, 1), "411": (3, 4, 1),
+            "440": (3, 1, 2), "410": (3, 4, 2)}
 
 
 @dataclass
@@ -66,7 +68,7 @@ class ImageSpec:
     """One synthetic image: geometry, sampling, quality and content knobs."""
     width: int
     height: int
-    sampling: str = "420"          # "420" | "422" | "444" | "gray"
+    sampling: str = "420"          # a key of SAMPLING
     quality: int = 75
     sigma: float = 6.0
     seed: int = 0

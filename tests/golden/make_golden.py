@@ -31,6 +31,9 @@ CASES = [
     ("c420_96x64_q85_rst2", synth.ImageSpec(96, 64, "420", 85, 6.0, 106, restart_interval=2)),
     ("c420_80x80_q95_noise", synth.ImageSpec(80, 80, "420", 95, 12.0, 107, random_tile_frac=0.5)),
     ("gray_8x8_q100", synth.ImageSpec(8, 8, "gray", 100, 2.0, 108)),
+    ("c411_72x40_q85", synth.ImageSpec(72, 40, "411", 85, 6.0, 109)),
+    ("c440_41x35_q70", synth.ImageSpec(41, 35, "440", 70, 6.0, 110)),
+    ("c410_70x34_q90", synth.ImageSpec(70, 34, "410", 90, 8.0, 111)),
 ]
 
 
@@ -46,7 +49,7 @@ def main():
         out[f"{name}/file"] = np.frombuffer(enc.data, np.uint8)
         for c, w in enumerate(work):
             out[f"{name}/work{c}"] = w
-        with tempfile.NamedTemporaryFile(suffix=".jpg", delete=False) as f:
+        with tempfile.NamedTemporaryFile(suffix=".jpg", dir=HERE, delete=False) as f:
             f.write(enc.data)
             path = f.name
         e, pix = oracle.ref_cpuonly_decode(path, spec.width * spec.height * 3)

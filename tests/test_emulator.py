@@ -21,6 +21,10 @@ SPECS = [
     synth.ImageSpec(17, 9, "420", 90, 4.0, 12),
     synth.ImageSpec(200, 136, "420", 80, 6.0, 31, qtables=3),
     synth.ImageSpec(72, 40, "444", 60, 6.0, 32, qtables=3),
+    # luma sampled 4x1, 1x2 and 4x2 (6, 4 and 10 blocks per MCU)
+    synth.ImageSpec(131, 77, "411", 85, 6.0, 41),
+    synth.ImageSpec(90, 101, "440", 75, 6.0, 42),
+    synth.ImageSpec(150, 66, "410", 90, 8.0, 43, restart_interval=4),
 ]
 
 

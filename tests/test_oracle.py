@@ -56,6 +56,9 @@ def test_golden_pixels_match_reference_cpuonly():
     synth.ImageSpec(100, 60, "420", 70, 6.0, 9, restart_interval=3),
     synth.ImageSpec(160, 120, "420", 95, 12.0, 11, random_tile_frac=0.3),
     synth.ImageSpec(512, 512, "420", 75, 6.0, 1000),
+    synth.ImageSpec(131, 77, "411", 85, 6.0, 41),
+    synth.ImageSpec(90, 101, "440", 75, 6.0, 42),
+    synth.ImageSpec(150, 66, "410", 90, 8.0, 43, restart_interval=4),
 ])
 def test_oracle_matches_encoder_truth(spec):
     enc = synth.encode(spec, truth=True)
