@@ -7,8 +7,7 @@ and the RGB/gray frames left in HBM.
 
 Reported:
   value              batched form (all images in one entropy launch, one
-                     record expansion and one pixel launch, captured as a
-                     HIP graph), Mpixel/s (W*H)
+                     record expansion and one pixel launch), Mpixel/s (W*H)
   kernel_ms          device time per kernel (HIP events on the decode stream)
   roofline           dominant kernel, achieved algorithmic GB/s vs 8 TB/s;
                      traffic = its HBM bytes per launch from a committed
@@ -145,6 +144,7 @@ def main():
     ap.add_argument("--batched-idct", default="mfma", choices=["int", "float", "mfma"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="time replays of a captured HIP graph of the batch")
     ap.add_argument("--plain", action="store_true",
                     help="also run the plain form (autotune, then one image per launch) and its latencies")
     ap.add_argument("--serve", action="store_true", help="also run the serving-loop load test")
@@ -206,14 +206,18 @@ def main():
         assert not b.finish(stream).any()
         ms_ent, ms_pix, ms_all = b.timings()
         log(f"batched first decode: entropy {ms_ent:.2f} ms, pixel {ms_pix:.2f} ms, fallbacks {b.fallbacks()}")
-        b.capture(stream)
+        # one step = the batch's launches, enqueued directly (--graph: replayed
+        # from a captured HIP graph instead)
+        run = b.replay if args.graph else b.decode
+        if args.graph:
+            b.capture(stream)
         for _ in range(args.warmup):
-            b.replay(stream)
+            run(stream)
         stream.synchronize()
         torch.cuda.synchronize()
         t_start = time.perf_counter()
         for _ in range(args.steps):
-            b.replay(stream)
+            run(stream)
         stream.synchronize()
         torch.cuda.synchronize()
         t_batched = (time.perf_counter() - t_start) / args.steps
@@ -315,7 +319,8 @@ def main():
         "config": {"workload": f"mixed batch of {args.images} baseline JPEGs, 256x256..4096x3072",
                    "images": args.images, "mpix": round(total_px / 1e6, 1),
                    "scan_mb": round(tot["scan_bytes"] / 1e6, 1), "blocks": tot["blocks"],
-                   "batched_idct": args.batched_idct, "plain_idct": "float", "parallelism": "1 GPU"},
+                   "batched_idct": args.batched_idct, "plain_idct": "float", "parallelism": "1 GPU",
+                   "launch": "hipGraph replay" if args.graph else "direct"},
         "pcie_inclusive": {"stage_ms": round(stage_ms, 2), "download_ms": round(d2h_ms, 2),
                            "mpix_s_with_stage": round(total_px / ((stage_ms + t_batched * 1e3) * 1e-3) / 1e6, 1),
                            "mpix_s_with_stage_and_download":

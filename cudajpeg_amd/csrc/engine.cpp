@@ -196,13 +196,15 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   return e;
 }
 
-static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
+// clear = false leaves out the scratch clear (graph capture: the replay
+// launches it before the graph).
+static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed, bool clear = true) {
   uint32_t* scratch = b->d_scratch.as<uint32_t>();
   uint32_t* status = scratch;
   uint32_t* ticket = scratch + b->ticket_word;
   LookbackRec* lb = reinterpret_cast<LookbackRec*>(scratch + b->ticket_word + 4);
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-  if (launch_clear(scratch, b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  if (clear && launch_clear(scratch, b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   hipError_t he;
   if (b->entropy_mode == CJD_ENTROPY_SERIAL)
@@ -296,7 +298,8 @@ extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
   b->graph_exec = nullptr;
   b->graph = nullptr;
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-  int e = enqueue(b, s, false);
+  int e = enqueue_entropy(b, s, false, false);
+  if (!e) e = enqueue_pixel(b, s, false);
   hipGraph_t g = nullptr;
   if (hipStreamEndCapture(s, &g) != hipSuccess || e) {
     if (g) (void)hipGraphDestroy(g);
@@ -310,7 +313,14 @@ extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
 extern "C" int cjd_batch_replay(cjd_batch* b, void* stream) {
   if (!b || !b->graph_exec) return CJD_PROGRAMMER_ERROR;
   b->timed = false;
-  return hipGraphLaunch(b->graph_exec, (hipStream_t)stream) == hipSuccess ? 0 : CJD_KERNEL_LAUNCH_ERROR;
+  hipStream_t s = (hipStream_t)stream;
+  // The scratch (work-item ticket, lookback records, image status) is cleared
+  // by a launch of its own, ordered on the stream before the graph.  Replays
+  // of a graph that began with the clear intermittently reported errors for
+  // images that decode correctly, which a clear not yet complete when the
+  // entropy kernel takes its first tickets would cause.
+  if (launch_clear(b->d_scratch.as<uint32_t>(), b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  return hipGraphLaunch(b->graph_exec, s) == hipSuccess ? 0 : CJD_KERNEL_LAUNCH_ERROR;
 }
 
 extern "C" int cjd_batch_finish(cjd_batch* b, void* stream, int* errors) {

@@ -185,24 +185,33 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
 // Block records -> quantised coefficients (natural order, absolute DC), 256
 // consecutive blocks per workgroup, written as one contiguous 32 KB run.  A
 // block's AC entries are one stream row apart (one per step of its tile's
-// stream).
+// stream), so every entry is a separate memory transaction: a lane issues
+// them eight at a time before using any.  The blocks are assembled in LDS at
+// a stride of 33 words, so that lanes writing the same coefficient index of
+// their blocks hit different banks.
+constexpr int kExpandStride = 33;  // LDS words per block (32 + 1 of padding)
+
 __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ recs, const uint32_t* __restrict__ ebuf,
                                                 uint64_t ewords, int16_t* __restrict__ coefs, uint64_t nblocks) {
-  __shared__ uint4 blk[256 * 8];
+  __shared__ uint32_t blk[256 * kExpandStride];
   const int tid = threadIdx.x;
   const uint64_t b0 = (uint64_t)blockIdx.x * 256;
-  const uint4 z = make_uint4(0, 0, 0, 0);
+  uint32_t* mine = blk + tid * kExpandStride;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) blk[tid * 8 + i] = z;
+  for (int i = 0; i < 32; ++i) mine[i] = 0;
   const uint64_t b = b0 + tid;
   if (b < nblocks) {
     const BlockRec r = recs[b];
     const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
-    const uint64_t e0 = r.eidx + (uint64_t)n * W <= ewords ? r.eidx : 0;
-    int16_t* B = reinterpret_cast<int16_t*>(blk + tid * 8);
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t e = ebuf[e0 + (uint64_t)i * W];
-      if (e != 0xFFFFFFFFu) B[(e >> 16) & 63] = (int16_t)(e & 0xFFFFu);
+    const uint32_t* E = ebuf + (r.eidx + (uint64_t)n * W <= ewords ? r.eidx : 0);
+    int16_t* B = reinterpret_cast<int16_t*>(mine);
+    for (uint32_t i0 = 0; i0 < n; i0 += 8) {
+      uint32_t e[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) e[k] = i0 + k < n ? E[(i0 + k) * W] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (e[k] != 0xFFFFFFFFu) B[(e[k] >> 16) & 63] = (int16_t)(e[k] & 0xFFFFu);
     }
     B[0] = r.dc;
   }
@@ -211,8 +220,11 @@ __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ rec
   const uint64_t nq = (nblocks - b0 < 256 ? nblocks - b0 : 256) * 8;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const uint32_t q = (uint32_t)(tid + 256 * i);
-    if (q < nq) out[q] = blk[q];
+    const uint32_t q = (uint32_t)(tid + 256 * i);  // 16-byte piece q & 7 of block q >> 3
+    if (q < nq) {
+      const uint32_t* w = blk + (q >> 3) * kExpandStride + (q & 7) * 4;
+      out[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
   }
 }
 

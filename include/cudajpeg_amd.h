@@ -142,7 +142,9 @@ int cjd_batch_fallbacks(cjd_batch *b);
  * tile links did not resolve, and report per-image reference error codes. */
 int cjd_batch_finish(cjd_batch *b, void *stream, int *errors);
 
-/* Capture the decode of the staged batch into a HIP graph / replay it. */
+/* Capture the decode of the staged batch into a HIP graph / replay it.  The
+ * graph holds the entropy, expansion and pixel launches; each replay first
+ * launches the scratch clear on the stream, then the graph. */
 int cjd_batch_capture(cjd_batch *b, void *stream);
 int cjd_batch_replay(cjd_batch *b, void *stream);
 
