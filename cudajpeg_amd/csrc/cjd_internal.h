@@ -118,6 +118,9 @@ struct BlockRec {
   uint16_t nw;
   int16_t dc;
 };
+// nw flag: the block is in the dense coefficient buffer (written by the
+// serial decoder), not in entries.
+constexpr uint16_t kRecDense = 0x8000;
 
 // Lookback record of one work item.  The aggregate holds under one
 // assumption: if the item's entry (the exit of its lead-in tile) is the

@@ -60,6 +60,10 @@ struct cjd_batch {
 
   DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_luts, d_qtabs, d_coefs, d_out, d_scratch;
   DevBuf d_ebuf, d_recs;  // tiled decoder: entry regions and block records
+  // The pixel kernel reads the block records; the dense coefficient buffer
+  // is filled from them (k_expand) only when a caller asks for it.
+  bool coefs_stale = false;
+  hipStream_t last_stream = nullptr;
   size_t scratch_words = 0, status_words = 0, ticket_word = 0;
   std::vector<uint32_t> h_status;
   int64_t fallbacks = 0;
@@ -210,26 +214,29 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed, bool clear =
   if (b->entropy_mode == CJD_ENTROPY_SERIAL)
     he = launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                                (uint32_t)b->plan.segs.size(), b->d_tables.as<HuffTable>(), b->d_coefs.as<int16_t>(),
-                               status, s);
+                               b->d_recs.as<BlockRec>(), status, s);
   else {
     he = launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                         b->d_items.as<WorkItem>(), (uint32_t)b->plan.items.size(), b->d_tables.as<HuffTable>(),
                         b->d_luts.as<uint16_t>(), b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
                         ticket, s);
     if (he == hipSuccess && timed && hipEventRecord(b->ev[4], s) != hipSuccess) he = hipErrorUnknown;
-    if (he == hipSuccess)
-      he = launch_expand(b->d_recs.as<BlockRec>(), b->d_ebuf.as<uint32_t>(), (uint64_t)b->plan.entry_words,
-                         b->d_coefs.as<int16_t>(), (uint64_t)b->plan.total_blocks, s);
+    b->coefs_stale = true;
   }
+  b->last_stream = s;
   if (he != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   if (timed && hipEventRecord(b->ev[1], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   return 0;
 }
 
+static hipError_t pixel_range(cjd_batch* b, uint32_t first, uint32_t count, hipStream_t s) {
+  return launch_pixel(b->idct_mode, b->d_images.as<ImageDesc>(), b->d_pix.as<PixItem>() + first, count,
+                      b->d_recs.as<BlockRec>(), b->d_ebuf.as<uint32_t>(), (uint64_t)b->plan.entry_words,
+                      b->d_coefs.as<int16_t>(), b->d_qtabs.as<uint16_t>(), b->d_out.as<uint8_t>(), s);
+}
+
 static int enqueue_pixel(cjd_batch* b, hipStream_t s, bool timed) {
-  if (launch_pixel(b->idct_mode, b->d_images.as<ImageDesc>(), b->d_pix.as<PixItem>(), (uint32_t)b->plan.pix.size(),
-                   b->d_coefs.as<int16_t>(), b->d_qtabs.as<uint16_t>(), b->d_out.as<uint8_t>(), s) != hipSuccess)
-    return CJD_KERNEL_LAUNCH_ERROR;
+  if (pixel_range(b, 0, (uint32_t)b->plan.pix.size(), s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   if (timed && hipEventRecord(b->ev[2], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   return 0;
 }
@@ -257,7 +264,8 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
       if (hipMemcpyAsync(status, &zero, 4, hipMemcpyHostToDevice, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
       if (launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(),
                                 b->d_segs.as<SegmentDesc>() + r.seg_first, r.seg_count, b->d_tables.as<HuffTable>(),
-                                b->d_coefs.as<int16_t>(), b->d_scratch.as<uint32_t>(), s) != hipSuccess)
+                                b->d_coefs.as<int16_t>(), b->d_recs.as<BlockRec>(), b->d_scratch.as<uint32_t>(),
+                                s) != hipSuccess)
         return CJD_KERNEL_LAUNCH_ERROR;
       if (hipMemcpyAsync(&b->h_status[i], status, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
         return CJD_DEVICE_MEM_ERROR;
@@ -314,6 +322,8 @@ extern "C" int cjd_batch_replay(cjd_batch* b, void* stream) {
   if (!b || !b->graph_exec) return CJD_PROGRAMMER_ERROR;
   b->timed = false;
   hipStream_t s = (hipStream_t)stream;
+  b->coefs_stale = b->entropy_mode != CJD_ENTROPY_SERIAL;
+  b->last_stream = s;
   // The scratch (work-item ticket, lookback records, image status) is cleared
   // by a launch of its own, ordered on the stream before the graph.  Replays
   // of a graph that began with the clear intermittently reported errors for
@@ -331,9 +341,7 @@ extern "C" int cjd_batch_finish(cjd_batch* b, void* stream, int* errors) {
   if (e) return e;
   for (size_t i : redo) {
     const ImageRec& r = b->plan.recs[i];
-    if (launch_pixel(b->idct_mode, b->d_images.as<ImageDesc>(), b->d_pix.as<PixItem>() + r.pix_first, r.pix_count,
-                     b->d_coefs.as<int16_t>(), b->d_qtabs.as<uint16_t>(), b->d_out.as<uint8_t>(), s) != hipSuccess)
-      return CJD_KERNEL_LAUNCH_ERROR;
+    if (pixel_range(b, r.pix_first, r.pix_count, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   }
   if (!redo.empty() && hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   if (errors)
@@ -383,6 +391,15 @@ extern "C" int cjd_batch_device_ptr(cjd_batch* b, int i, int what, void** ptr, s
     p = b->d_out.as<uint8_t>() + r.out_offset;
     n = r.out_bytes;
   } else if (what == CJD_OUT_COEFFICIENTS) {
+    if (b->coefs_stale) {
+      // blocks the serial decoder wrote are in place already; k_expand fills
+      // the others from their records
+      if (launch_expand(b->d_recs.as<BlockRec>(), b->d_ebuf.as<uint32_t>(), (uint64_t)b->plan.entry_words,
+                        b->d_coefs.as<int16_t>(), (uint64_t)b->plan.total_blocks, b->last_stream) != hipSuccess ||
+          hipStreamSynchronize(b->last_stream) != hipSuccess)
+        return CJD_KERNEL_LAUNCH_ERROR;
+      b->coefs_stale = false;
+    }
     p = b->d_coefs.as<int16_t>() + r.coef_base * 64;
     n = (size_t)r.f.num_blocks * 128;
   } else if (what >= CJD_OUT_PLANE0 && what < CJD_OUT_PLANE0 + r.f.num_channels) {

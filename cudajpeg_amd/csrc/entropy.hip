@@ -183,8 +183,10 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
 }
 
 // Block records -> quantised coefficients (natural order, absolute DC), 256
-// consecutive blocks per workgroup, written as one contiguous 32 KB run.  A
-// block's AC entries are one stream row apart (one per step of its tile's
+// consecutive blocks per workgroup, written as one contiguous 32 KB run
+// (blocks the serial decoder wrote are left as they are; the pixel kernel
+// reads records, so this runs when coefficients are asked for).  A block's AC
+// entries are one stream row apart (one per step of its tile's
 // stream), so every entry is a separate memory transaction: a lane issues
 // them eight at a time before using any.  The blocks are assembled in LDS at
 // a stride of 33 words, so that lanes writing the same coefficient index of
@@ -200,7 +202,7 @@ __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ rec
 #pragma unroll
   for (int i = 0; i < 32; ++i) mine[i] = 0;
   const uint64_t b = b0 + tid;
-  if (b < nblocks) {
+  if (b < nblocks && !(recs[b].nw & kRecDense)) {
     const BlockRec r = recs[b];
     const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
     const uint32_t* E = ebuf + (r.eidx + (uint64_t)n * W <= ewords ? r.eidx : 0);
@@ -221,7 +223,7 @@ __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ rec
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const uint32_t q = (uint32_t)(tid + 256 * i);  // 16-byte piece q & 7 of block q >> 3
-    if (q < nq) {
+    if (q < nq && !(recs[b0 + (q >> 3)].nw & kRecDense)) {
       const uint32_t* w = blk + (q >> 3) * kExpandStride + (q & 7) * 4;
       out[q] = make_uint4(w[0], w[1], w[2], w[3]);
     }
@@ -231,10 +233,10 @@ __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ rec
 __global__ void k_entropy_serial(const uint32_t* __restrict__ words, const ImageDesc* __restrict__ images,
                                  const SegmentDesc* __restrict__ segs, uint32_t nsegs,
                                  const HuffTable* __restrict__ tables, int16_t* __restrict__ coefs,
-                                 uint32_t* __restrict__ status) {
+                                 BlockRec* __restrict__ recs, uint32_t* __restrict__ status) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsegs) return;
-  ent_serial_segment(words, images, segs[s], tables, coefs, status);
+  ent_serial_segment(words, images, segs[s], tables, coefs, recs, status);
 }
 
 hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s) {
@@ -264,11 +266,11 @@ hipError_t launch_expand(const BlockRec* recs, const uint32_t* ebuf, uint64_t ew
 }
 
 hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
-                                 uint32_t nsegs, const HuffTable* tables, int16_t* coefs, uint32_t* status,
-                                 hipStream_t s) {
+                                 uint32_t nsegs, const HuffTable* tables, int16_t* coefs, BlockRec* recs,
+                                 uint32_t* status, hipStream_t s) {
   if (!nsegs) return hipSuccess;
   hipLaunchKernelGGL(k_entropy_serial, dim3((nsegs + 63) / 64), dim3(64), 0, s, words, images, segs, nsegs, tables,
-                     coefs, status);
+                     coefs, recs, status);
   return hipGetLastError();
 }
 

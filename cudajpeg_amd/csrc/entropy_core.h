@@ -964,7 +964,7 @@ CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, Bl
 // Serial decode of one whole segment from its known start (reference order,
 // src/decodeScanGPU.cu:435-451).
 CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* images, const SegmentDesc& sg,
-                                      const HuffTable* tables, int16_t* coefs, uint32_t* status) {
+                                      const HuffTable* tables, int16_t* coefs, BlockRec* recs, uint32_t* status) {
   const ImageDesc& im = images[sg.image];
   const HuffTable* tab = tables + im.table_base;
   BitReader br;
@@ -984,6 +984,11 @@ CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* im
     blk[0] = (int16_t)dc[comp];
     int16_t* g = coefs + (sg.first_block + b) * 64;
     for (int k = 0; k < 64; ++k) g[k] = blk[k];
+    BlockRec r;
+    r.eidx = 0;
+    r.nw = kRecDense;
+    r.dc = blk[0];
+    recs[sg.first_block + b] = r;
     if (++slot == im.nslots) slot = 0;
   }
 }

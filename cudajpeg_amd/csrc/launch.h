@@ -14,9 +14,10 @@ hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const 
 hipError_t launch_expand(const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, int16_t* coefs, uint64_t nblocks,
                          hipStream_t s);
 hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
-                                 uint32_t nsegs, const HuffTable* tables, int16_t* coefs, uint32_t* status,
-                                 hipStream_t s);
+                                 uint32_t nsegs, const HuffTable* tables, int16_t* coefs, BlockRec* recs,
+                                 uint32_t* status, hipStream_t s);
 hipError_t launch_pixel(int mode, const ImageDesc* images, const PixItem* items, uint32_t nitems,
-                        const int16_t* coefs, const uint16_t* qtabs, uint8_t* out, hipStream_t s);
+                        const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, const int16_t* coefs,
+                        const uint16_t* qtabs, uint8_t* out, hipStream_t s);
 
 }  // namespace cjd

@@ -85,12 +85,14 @@ __device__ __forceinline__ void pix_idct_mfma(PixSharedMfma& sh, int tid, const 
   b_image = c.ntab <= 2 ? c.it.image : ~0u;
 }
 
-// One workgroup decodes kPixItemsPerWG consecutive items.  Barriers: staging
-// -> iDCT (coefficients, descriptors, quant tables); iDCT -> colour
-// (samples); colour -> next item's staging.
+// One workgroup decodes kPixItemsPerWG consecutive items.  Barriers: zeroed
+// blocks -> block fill; staging -> iDCT (coefficients, descriptors, quant
+// tables); iDCT -> colour (samples); colour -> next item's staging.
 template <int kMode>
 __global__ void __launch_bounds__(kPixThreads) k_pixel(const ImageDesc* __restrict__ images,
                                                        const PixItem* __restrict__ items, uint32_t nitems,
+                                                       const BlockRec* __restrict__ recs,
+                                                       const uint32_t* __restrict__ ebuf, uint64_t ewords,
                                                        const int16_t* __restrict__ coefs,
                                                        const uint16_t* __restrict__ qtabs,
                                                        uint8_t* __restrict__ out) {
@@ -106,7 +108,9 @@ __global__ void __launch_bounds__(kPixThreads) k_pixel(const ImageDesc* __restri
     PixCtx c;
     c.init(images, items[idx]);
     __syncthreads();  // the previous item is done with coef, samp, q and binfo
-    pix_stage(sh, tid, c, coefs, qtabs);
+    pix_stage_zero(sh, tid, c, qtabs);
+    __syncthreads();
+    pix_stage_blocks(sh, tid, c, recs, ebuf, ewords, coefs);
     __syncthreads();
     if constexpr (kMode == kIdctMfma) {
       pix_idct_mfma(sh, tid, c, B, b_image);
@@ -122,18 +126,22 @@ __global__ void __launch_bounds__(kPixThreads) k_pixel(const ImageDesc* __restri
 }
 
 hipError_t launch_pixel(int mode, const ImageDesc* images, const PixItem* items, uint32_t nitems,
-                        const int16_t* coefs, const uint16_t* qtabs, uint8_t* out, hipStream_t s) {
+                        const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, const int16_t* coefs,
+                        const uint16_t* qtabs, uint8_t* out, hipStream_t s) {
   if (!nitems) return hipSuccess;
   const dim3 grid((nitems + kPixItemsPerWG - 1) / kPixItemsPerWG);
   switch (mode) {
     case kIdctInt:
-      hipLaunchKernelGGL(k_pixel<kIdctInt>, grid, dim3(kPixThreads), 0, s, images, items, nitems, coefs, qtabs, out);
+      hipLaunchKernelGGL(k_pixel<kIdctInt>, grid, dim3(kPixThreads), 0, s, images, items, nitems, recs, ebuf, ewords,
+                         coefs, qtabs, out);
       break;
     case kIdctFloat:
-      hipLaunchKernelGGL(k_pixel<kIdctFloat>, grid, dim3(kPixThreads), 0, s, images, items, nitems, coefs, qtabs, out);
+      hipLaunchKernelGGL(k_pixel<kIdctFloat>, grid, dim3(kPixThreads), 0, s, images, items, nitems, recs, ebuf,
+                         ewords, coefs, qtabs, out);
       break;
     case kIdctMfma:
-      hipLaunchKernelGGL(k_pixel<kIdctMfma>, grid, dim3(kPixThreads), 0, s, images, items, nitems, coefs, qtabs, out);
+      hipLaunchKernelGGL(k_pixel<kIdctMfma>, grid, dim3(kPixThreads), 0, s, images, items, nitems, recs, ebuf, ewords,
+                         coefs, qtabs, out);
       break;
     default:
       return hipErrorInvalidValue;

@@ -204,6 +204,17 @@ CJD_HD inline void stage_piece(PixSharedMfma& sh, int i, const u32x4& v) {
   __builtin_memcpy(&sh.coef[0][0] + i * 8, h, 16);
 }
 
+// Coefficient v at natural index k of block b, and a stream entry (natural
+// index << 16 | value; 0xFFFFFFFF = none).
+CJD_HD inline void stage_coef(PixShared& sh, int b, uint32_t k, int16_t v) { sh.coef[b][k] = v; }
+CJD_HD inline void stage_coef(PixSharedMfma& sh, int b, uint32_t k, int16_t v) {
+  sh.coef[b][k] = half_bits(round_to_half((float)v));
+}
+template <class SH>
+CJD_HD inline void stage_entry(SH& sh, int b, uint32_t e) {
+  if (e != 0xFFFFFFFFu) stage_coef(sh, b, (e >> 16) & 63u, (int16_t)(e & 0xFFFFu));
+}
+
 struct PixCtx {
   const ImageDesc* im;
   PixItem it;
@@ -244,27 +255,16 @@ CJD_HD inline void pix_stage_cos(SH& sh, int tid) {
   if (tid < 64) sh.cosf[tid] = kCosF[tid >> 3][tid & 7];
 }
 
-// Stage one item: coefficients (contiguous: MCUs in scan order; all of a
-// lane's 16-byte loads are issued before its LDS writes), quant tables and
-// block descriptors.
+// Stage one item in two phases with a barrier between them.
+//
+// pix_stage_zero: every coefficient of the item's blocks to zero, the quant
+// tables and the block descriptors.
 template <class SH>
-CJD_HD inline void pix_stage(SH& sh, int tid, const PixCtx& c, const int16_t* coefs, const uint16_t* qtabs) {
-  // four named vectors, not an array: a private array of this size would be
-  // moved into LDS by the compiler
+CJD_HD inline void pix_stage_zero(SH& sh, int tid, const PixCtx& c, const uint16_t* qtabs) {
   static_assert(kPixBlocks * 8 == 4 * kPixThreads, "four 16-byte pieces per lane");
-  const int64_t first = c.im->coef_base + (int64_t)c.it.mcu_first * c.nslots;
-  const int16_t* src = coefs + first * 64;
   const int lim = c.nblk * 8;
-  const int i0 = tid, i1 = tid + kPixThreads, i2 = tid + 2 * kPixThreads, i3 = tid + 3 * kPixThreads;
-  u32x4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0}, v2 = {0, 0, 0, 0}, v3 = {0, 0, 0, 0};
-  if (i0 < lim) __builtin_memcpy(&v0, src + i0 * 8, 16);
-  if (i1 < lim) __builtin_memcpy(&v1, src + i1 * 8, 16);
-  if (i2 < lim) __builtin_memcpy(&v2, src + i2 * 8, 16);
-  if (i3 < lim) __builtin_memcpy(&v3, src + i3 * 8, 16);
-  if (i0 < lim) stage_piece(sh, i0, v0);
-  if (i1 < lim) stage_piece(sh, i1, v1);
-  if (i2 < lim) stage_piece(sh, i2, v2);
-  if (i3 < lim) stage_piece(sh, i3, v3);
+  const u32x4 z = {0, 0, 0, 0};
+  for (int i = tid; i < lim; i += kPixThreads) __builtin_memcpy(&sh.coef[0][0] + i * 8, &z, 16);
   for (int i = tid; i < c.ntab * 64; i += kPixThreads) {
     const int t = i >> 6;
     sh.q[t][i & 63] = qtabs[(c.im->qtab_base + c.im->tab_dq[t]) * 64 + (i & 63)];
@@ -278,6 +278,49 @@ CJD_HD inline void pix_stage(SH& sh, int tid, const PixCtx& c, const int16_t* co
     const int tw = c.tw(comp);
     sh.binfo[b] = binfo_pack(c.toff(comp) + sy * 8 * tw + (m * sxf + sx) * 8, tw, c.im->comp_tix[comp]);
   }
+}
+
+// pix_stage_blocks: lanes 2b and 2b+1 fill block b of the item (blocks are
+// the item's MCUs in scan order) from its record.  A block the serial decoder
+// wrote is copied from the dense buffer, four 16-byte pieces per lane; for
+// the others each lane writes every second AC entry of the record (entries
+// one stream row apart, loaded four at a time; an entry's natural index is
+// unique within its block, so the two lanes' writes never meet) and the even
+// lane writes the absolute DC value.  A record whose entries would run past
+// the entry buffer (a corrupt stream) reads from its start instead, as the
+// expansion kernel does.
+template <class SH>
+CJD_HD inline void pix_stage_blocks(SH& sh, int tid, const PixCtx& c, const BlockRec* recs, const uint32_t* ebuf,
+                                    uint64_t ewords, const int16_t* coefs) {
+  const int b = tid >> 1, h = tid & 1;
+  if (b >= c.nblk) return;
+  const int64_t g = c.im->coef_base + (int64_t)c.it.mcu_first * c.nslots + b;
+  const BlockRec r = recs[g];
+  if (r.nw & kRecDense) {
+    const int16_t* src = coefs + g * 64 + h * 32;
+    u32x4 v0, v1, v2, v3;
+    __builtin_memcpy(&v0, src, 16);
+    __builtin_memcpy(&v1, src + 8, 16);
+    __builtin_memcpy(&v2, src + 16, 16);
+    __builtin_memcpy(&v3, src + 24, 16);
+    stage_piece(sh, b * 8 + h * 4, v0);
+    stage_piece(sh, b * 8 + h * 4 + 1, v1);
+    stage_piece(sh, b * 8 + h * 4 + 2, v2);
+    stage_piece(sh, b * 8 + h * 4 + 3, v3);
+    return;
+  }
+  const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
+  const uint32_t* E = ebuf + (r.eidx + (uint64_t)n * W <= ewords ? r.eidx : 0);
+  uint32_t i = (uint32_t)h;
+  for (; i + 6 < n; i += 8) {
+    const uint32_t e0 = E[i * W], e1 = E[(i + 2) * W], e2 = E[(i + 4) * W], e3 = E[(i + 6) * W];
+    stage_entry(sh, b, e0);
+    stage_entry(sh, b, e1);
+    stage_entry(sh, b, e2);
+    stage_entry(sh, b, e3);
+  }
+  for (; i < n; i += 2) stage_entry(sh, b, E[i * W]);
+  if (h == 0) stage_coef(sh, b, 0, r.dc);
 }
 
 // Row pass: lane -> (block, row).

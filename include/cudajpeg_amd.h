@@ -153,12 +153,16 @@ int cjd_batch_replay(cjd_batch *b, void *stream);
  * expansion), pixel kernel, whole decode. */
 int cjd_batch_timings(cjd_batch *b, float *ms_entropy, float *ms_pixel, float *ms_total);
 /* The entropy stage of the last cjd_batch_decode split by kernel (ms): the
- * tiled decode kernel alone and the expansion of its block records into
- * coefficient blocks (tiled entropy mode only).  No counterpart in the
- * reference; used by the benchmark's per-kernel roofline. */
+ * tiled decode kernel alone, and the rest of the stage after it (0 unless a
+ * kernel runs there; the record expansion runs when coefficients are asked
+ * for).  Tiled entropy mode only.  No counterpart in the reference; used by
+ * the benchmark's per-kernel roofline. */
 int cjd_batch_kernel_timings(cjd_batch *b, float *ms_entropy_kernel, float *ms_expand);
 
-/* Geometry and device pointers of image i's outputs. */
+/* Geometry and device pointers of image i's outputs.  The pixel kernel reads
+ * the tiled decoder's block records, not dense coefficients: the first
+ * CJD_OUT_COEFFICIENTS request after a decode fills the coefficient buffer of
+ * the whole batch from them (a kernel on the decode's stream, waited for). */
 int cjd_batch_image_info(cjd_batch *b, int i, cjd_frame_info *info);
 int cjd_batch_device_ptr(cjd_batch *b, int i, int what, void **ptr, size_t *bytes);
 /* Copy image i's output `what` to host memory (synchronous). */

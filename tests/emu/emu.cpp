@@ -28,6 +28,7 @@ struct Emu {
   std::vector<uint32_t> status;
   std::vector<uint32_t> ebuf;
   std::vector<BlockRec> recs;
+  bool coefs_stale = false;  // records not yet expanded into coefs
   int fixes = 0;
   int fallbacks = 0;
 };
@@ -137,11 +138,12 @@ void resolve(EntRun& R, size_t i) {
   R.done[i] = 1;
 }
 
-// k_expand, block by block.
+// k_expand, block by block (blocks the serial decoder wrote stay).
 void expand(Emu& E) {
-  const uint64_t ew = E.ebuf.size();
+  const uint64_t ew = E.plan.entry_words;
   for (int64_t b = 0; b < E.plan.total_blocks; ++b) {
     const BlockRec r = E.recs[(size_t)b];
+    if (r.nw & kRecDense) continue;
     int16_t* B = E.coefs.data() + b * 64;
     for (int k = 0; k < 64; ++k) B[k] = 0;
     const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
@@ -157,9 +159,7 @@ void expand(Emu& E) {
 void run_entropy(Emu& E, int order) {
   Plan& p = E.plan;
   const size_t ni = p.items.size();
-  E.ebuf.assign((size_t)p.entry_words + 16, 0x7EADBEEFu);
-  BlockRec junk{0, 0, 0x5555};  // eidx, nw, dc
-  E.recs.assign((size_t)p.total_blocks, junk);
+  E.ebuf.assign((size_t)p.entry_words, 0x7EADBEEFu);
   EntRun R;
   R.p = &p;
   R.lb.resize(ni);
@@ -203,7 +203,7 @@ void run_entropy(Emu& E, int order) {
   E.fixes = R.fixes;
   for (size_t i = 0; i < ni; ++i)
     for (int t = 0; t < kWG; ++t) ent_records(*R.shs[i], t, R.ctx[i], E.recs.data(), E.status.data());
-  expand(E);
+  E.coefs_stale = true;
 }
 
 void run_pixel(Emu& E, int mode, size_t first, size_t count);
@@ -213,7 +213,9 @@ void pixel_item(Emu& E, SH& sh, int mode, const PixCtx& c) {
   Plan& p = E.plan;
   memset(&sh, 0, sizeof(SH));
   for (int t = 0; t < kPixThreads; ++t) pix_stage_cos(sh, t);
-  for (int t = 0; t < kPixThreads; ++t) pix_stage(sh, t, c, E.coefs.data(), p.qtabs.data());
+  for (int t = 0; t < kPixThreads; ++t) pix_stage_zero(sh, t, c, p.qtabs.data());
+  for (int t = 0; t < kPixThreads; ++t)
+    pix_stage_blocks(sh, t, c, E.recs.data(), E.ebuf.data(), (uint64_t)p.entry_words, E.coefs.data());
   if constexpr (std::is_same<SH, PixSharedMfma>::value) {
     pix_idct_mfma_emul(sh, c);
   } else if (mode == kIdctInt) {
@@ -252,12 +254,14 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
   E->plan.want_planes = want_planes;
   plan_batch(E->plan, n, files, sizes, errors);
   E->coefs.assign((size_t)E->plan.total_blocks * 64, (int16_t)0x5555);
-  E->out.assign(E->plan.out_bytes + 256, 0xAB);
-  E->status.assign((size_t)n + 4, 0);
+  BlockRec junk{0, 0, 0x5555};  // eidx, nw, dc
+  E->recs.assign((size_t)E->plan.total_blocks, junk);
+  E->out.assign(E->plan.out_bytes, 0xAB);
+  E->status.assign((size_t)n, 0);
   if (serial) {
     for (auto& s : E->plan.segs)
       ent_serial_segment(E->plan.words.data(), E->plan.images.data(), s, E->plan.tables.data(), E->coefs.data(),
-                         E->status.data());
+                         E->recs.data(), E->status.data());
   } else {
     run_entropy(*E, order);
   }
@@ -269,7 +273,7 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
       ++E->fallbacks;
       for (uint32_t s = 0; s < r.seg_count; ++s)
         ent_serial_segment(E->plan.words.data(), E->plan.images.data(), E->plan.segs[r.seg_first + s],
-                           E->plan.tables.data(), E->coefs.data(), E->status.data());
+                           E->plan.tables.data(), E->coefs.data(), E->recs.data(), E->status.data());
     }
     if (E->status[(size_t)i] & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
     if (errors) errors[i] = r.error;
@@ -289,6 +293,10 @@ long emu_get(void* h, int i, int what, void* dst, size_t cap) {
     src = E->out.data() + r.out_offset;
     n = r.out_bytes;
   } else if (what == 1) {
+    if (E->coefs_stale) {
+      expand(*E);
+      E->coefs_stale = false;
+    }
     src = reinterpret_cast<const uint8_t*>(E->coefs.data() + r.coef_base * 64);
     n = (size_t)r.f.num_blocks * 128;
   } else {

@@ -10,8 +10,8 @@ pytestmark = pytest.mark.gpu
 
 def test_kernel_timings_split_the_entropy_stage(gpu):
     # the per-kernel events lie inside the entropy stage: the tiled decode
-    # kernel plus the record expansion never exceed it (the scratch clear is
-    # the rest), and every time is positive
+    # kernel plus what follows it in the stage never exceed it (the scratch
+    # clear is the rest)
     specs = [synth.ImageSpec(320, 240, "420", 80, 6.0, 81), synth.ImageSpec(200, 136, "444", 90, 8.0, 82)]
     b = BatchDecoder(IDCT_INT)
     assert not b.stage([synth.encode(s).data for s in specs]).any()
@@ -20,6 +20,6 @@ def test_kernel_timings_split_the_entropy_stage(gpu):
     stage, pixel, total = b.timings()
     k, x = b.kernel_timings()
     print("entropy stage", stage, "k_entropy", k, "k_expand", x, "pixel", pixel, "total", total, flush=True)
-    assert 0 < k <= stage and 0 < x <= stage and k + x <= stage + 1e-3
+    assert 0 < k <= stage and 0 <= x <= stage and k + x <= stage + 1e-3
     assert 0 < pixel <= total and stage <= total
     b.close()
