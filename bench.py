@@ -148,7 +148,6 @@ def main():
     ap.add_argument("--batched-idct", default="mfma", choices=["int", "float", "mfma"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="time replays of a captured HIP graph of the batch")
     ap.add_argument("--plain", action="store_true",
                     help="also run the plain form (autotune, then one image per launch) and its latencies")
     ap.add_argument("--serve", action="store_true", help="also run the serving-loop load test")
@@ -210,11 +209,8 @@ def main():
         assert not b.finish(stream).any()
         ms_ent, ms_pix, ms_all = b.timings()
         log(f"batched first decode: entropy {ms_ent:.2f} ms, pixel {ms_pix:.2f} ms, fallbacks {b.fallbacks()}")
-        # one step = the batch's launches, enqueued directly (--graph: replayed
-        # from a captured HIP graph instead)
-        run = b.replay if args.graph else b.decode
-        if args.graph:
-            b.capture(stream)
+        # one step = the batch's launches, enqueued directly
+        run = b.decode
         for _ in range(args.warmup):
             run(stream)
         stream.synchronize()
@@ -324,7 +320,7 @@ def main():
                    "images": args.images, "mpix": round(total_px / 1e6, 1),
                    "scan_mb": round(tot["scan_bytes"] / 1e6, 1), "blocks": tot["blocks"],
                    "batched_idct": args.batched_idct, "plain_idct": "float", "parallelism": "1 GPU",
-                   "launch": "hipGraph replay" if args.graph else "direct",
+                   "launch": "direct",
                    "tile_bits": args.tile_bits or 4096, "item_tiles": args.item_tiles or 256},
         "pcie_inclusive": {"stage_ms": round(stage_ms, 2), "download_ms": round(d2h_ms, 2),
                            "mpix_s_with_stage": round(total_px / ((stage_ms + t_batched * 1e3) * 1e-3) / 1e6, 1),

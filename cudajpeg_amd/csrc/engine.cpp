@@ -24,6 +24,10 @@ using namespace cjd;
 
 namespace {
 
+// Graph capture of a batch's launches (cjd_batch_capture / replay): off, see
+// cjd_batch_capture.
+constexpr bool kCaptureEnabled = false;
+
 // Grow-only device buffer (src/format.cu:81-110 policy).
 struct DevBuf {
   void* p = nullptr;
@@ -300,6 +304,10 @@ extern "C" int cjd_batch_decode(cjd_batch* b, void* stream) {
 
 extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
   if (!b) return CJD_PROGRAMMER_ERROR;
+  // Replays of the captured decode have intermittently reported errors for
+  // images that decode correctly with direct launches; until that is
+  // understood a capture is refused.
+  if (!kCaptureEnabled) return CJD_UNSUPPORTED_ERROR;
   hipStream_t s = (hipStream_t)stream;
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);

@@ -143,8 +143,11 @@ int cjd_batch_fallbacks(cjd_batch *b);
 int cjd_batch_finish(cjd_batch *b, void *stream, int *errors);
 
 /* Capture the decode of the staged batch into a HIP graph / replay it.  The
- * graph holds the entropy, expansion and pixel launches; each replay first
- * launches the scratch clear on the stream, then the graph. */
+ * graph holds the entropy and pixel launches; each replay first launches the
+ * scratch clear on the stream, then the graph.  Currently refused
+ * (CJD_UNSUPPORTED_ERROR): replays have intermittently reported errors for
+ * images that decode correctly with direct launches, and the cause is not
+ * yet known.  Direct launches (cjd_batch_decode) are unaffected. */
 int cjd_batch_capture(cjd_batch *b, void *stream);
 int cjd_batch_replay(cjd_batch *b, void *stream);
 

@@ -23,3 +23,18 @@ def test_kernel_timings_split_the_entropy_stage(gpu):
     assert 0 < k <= stage and 0 <= x <= stage and k + x <= stage + 1e-3
     assert 0 < pixel <= total and stage <= total
     b.close()
+
+
+def test_graph_capture_is_refused(gpu):
+    # capture/replay is switched off (replays intermittently reported errors
+    # for images that decode correctly); the refusal is the reference's
+    # UNSUPPORTED_ERROR and leaves the batch usable for direct launches
+    from cudajpeg_amd import DecodeError
+    b = BatchDecoder(IDCT_INT)
+    assert not b.stage([synth.encode(synth.ImageSpec(64, 48, "420", 75, 6.0, 1)).data]).any()
+    with pytest.raises(DecodeError) as exc:
+        b.capture(None)
+    assert exc.value.code == 2
+    b.decode()
+    assert not b.finish().any()
+    b.close()
