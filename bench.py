@@ -7,9 +7,9 @@ and the RGB/gray frames left in HBM.
 
 Reported:
   value              batched form (all images in one entropy launch and one
-                     pixel launch), Mpixel/s (W*H), with --in-flight (2)
-                     decoders of the batch taking the steps in turn on their
-                     own streams; ms_per_step_one_in_flight is the same with one
+                     pixel launch), Mpixel/s (W*H); --in-flight N > 1 lets N
+                     decoders of the batch take the steps in turn on their own
+                     streams (off by default)
   kernel_ms          device time per kernel (HIP events on the decode stream)
   roofline           dominant kernel, achieved algorithmic GB/s vs 8 TB/s;
                      traffic = its HBM bytes per launch from a committed
@@ -148,7 +148,7 @@ def main():
     ap.add_argument("--item-tiles", type=int, default=128, help="tiles per entropy work item (0 = library default)")
     ap.add_argument("--ck-bits", type=int, default=0, help="checkpoint spacing of the tiled decoder (0 = default)")
     ap.add_argument("--batched-idct", default="mfma", choices=["int", "float", "mfma"])
-    ap.add_argument("--in-flight", type=int, default=2,
+    ap.add_argument("--in-flight", type=int, default=1,
                     help="decoders of the batch taking the timed steps in turn, one stream each")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
