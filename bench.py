@@ -45,6 +45,16 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def check(errs, what: str) -> None:
+    """Stop with the failing images and their error codes when a decode
+    reported errors."""
+    errs = np.asarray(errs)
+    bad = np.flatnonzero(errs)
+    if bad.size:
+        raise SystemExit(f"bench.py: {what}: {bad.size} image(s) with errors: "
+                         + ", ".join(f"#{i} code {int(errs[i])}" for i in bad[:16]))
+
+
 def cpu_baseline(encs, budget_s: float):
     """Reference CPUonly (oracle/_ref) or, if it was not built, the oracle's
     C restatement, on a bounded sample of the same workload, single thread."""
@@ -198,8 +208,7 @@ def main():
     b = BatchDecoder(idct, tile_bits=args.tile_bits)
     if args.item_tiles or args.ck_bits:
         b.set_tuning(args.ck_bits, args.item_tiles)
-    errs = b.stage(files)
-    assert not errs.any(), errs
+    check(b.stage(files), "stage")
     # host side of the boundary: parse + unstuff + plan + upload of the scans
     stage_ms = []
     for _ in range(3):
@@ -210,7 +219,7 @@ def main():
     tot = b.totals()
     with torch.cuda.stream(stream):
         b.decode(stream)
-        assert not b.finish(stream).any()
+        check(b.finish(stream), "first decode")
         ms_ent, ms_pix, ms_all = b.timings()
         log(f"batched first decode: entropy {ms_ent:.2f} ms, pixel {ms_pix:.2f} ms, fallbacks {b.fallbacks()}")
         # one step = one decode of the batch (its launches, enqueued directly);
@@ -232,7 +241,7 @@ def main():
             d = BatchDecoder(idct, tile_bits=args.tile_bits)
             if args.item_tiles or args.ck_bits:
                 d.set_tuning(args.ck_bits, args.item_tiles)
-            assert not d.stage(files).any()
+            check(d.stage(files), f"stage of decoder {len(decs)}")
             decs.append(d)
             streams.append(torch.cuda.Stream())
         n_fl = len(decs)
@@ -246,8 +255,8 @@ def main():
             st.synchronize()
         torch.cuda.synchronize()
         t_batched = (time.perf_counter() - t_start) / args.steps
-        for d, st in zip(decs[1:], streams[1:]):
-            assert not d.finish(st).any()
+        for k, (d, st) in enumerate(zip(decs[1:], streams[1:])):
+            check(d.finish(st), f"timed steps, decoder {k + 1} (fallbacks {d.fallbacks()})")
             d.close()
         # per-kernel device times (events on the decode stream), averaged
         ent, pix, kent, kexp = [], [], [], []
@@ -259,7 +268,7 @@ def main():
             pix.append(c)
             kent.append(ka)
             kexp.append(kx)
-        assert not b.finish(stream).any()
+        check(b.finish(stream), f"timed steps, decoder 0 (fallbacks {b.fallbacks()})")
     batched_mpix = total_px / t_batched / 1e6
     # frames back to host memory (the reader API's pixels buffer)
     t1 = time.perf_counter()

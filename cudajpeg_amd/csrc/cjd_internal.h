@@ -128,8 +128,16 @@ constexpr uint16_t kRecDense = 0x8000;
 // (agg_p, agg_s) and the item adds agg_c (blocks, DC sums of components 0-2).
 // The inclusive state is the true exit of the last tile with the segment's
 // cumulative counts there.
+//
+// Epochs.  Every entropy launch on a batch's scratch carries an epoch number
+// (1 .. kEpochMax, one more than the launch before), and the scratch words
+// are tagged with the epoch of the launch that wrote them, so nothing has to
+// clear them between decodes: a record or status word of an older launch
+// simply does not match.  The status word of a record is
+// lookback_word(epoch, 1) once its aggregate is published and
+// lookback_word(epoch, 2) with its inclusive state (written last).
 struct LookbackRec {
-  int32_t status;         // 0 none, 1 aggregate, 2 inclusive  (written last)
+  int32_t status;         // lookback_word(epoch, 1 aggregate | 2 inclusive)
   uint32_t lead_p, lead_s;
   uint32_t agg_p, agg_s;
   int32_t agg_c[4];
@@ -138,7 +146,14 @@ struct LookbackRec {
   int32_t pad_[3];
 };
 
-// Per-image status bits (written by kernels with atomicOr).
+constexpr uint32_t kEpochMax = 0xFFFFFFu;
+inline constexpr int32_t lookback_word(uint32_t epoch, int st) { return (int32_t)(epoch << 2 | (uint32_t)st); }
+
+// Per-image status bits.  A status word holds (epoch << 8) | bits; a word of
+// another epoch holds no bits for this one (see status_or in entropy_core.h).
+inline constexpr uint32_t status_bits(uint32_t word, uint32_t epoch) {
+  return (word >> 8) == epoch ? word & 0xFFu : 0u;
+}
 enum : uint32_t {
   kStatusSyntax = 1u,        // invalid Huffman code / coefficient index (reference SYNTAX_ERROR)
   kStatusTruncated = 2u,     // scan ended before all blocks were decoded

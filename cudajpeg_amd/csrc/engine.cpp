@@ -24,10 +24,6 @@ using namespace cjd;
 
 namespace {
 
-// Graph capture of a batch's launches (cjd_batch_capture / replay): off, see
-// cjd_batch_capture.
-constexpr bool kCaptureEnabled = false;
-
 // Grow-only device buffer (src/format.cu:81-110 policy).
 struct DevBuf {
   void* p = nullptr;
@@ -68,22 +64,26 @@ struct cjd_batch {
   // is filled from them (k_expand) only when a caller asks for it.
   bool coefs_stale = false;
   hipStream_t last_stream = nullptr;
-  size_t scratch_words = 0, status_words = 0, ticket_word = 0;
-  std::vector<uint32_t> h_status;
+  // scratch: the 64-bit work-item ticket counter (words 0-1 of a 4-word
+  // head), then status_words per-image status words, then the lookback
+  // records.  Nothing in it is cleared between decodes: the ticket counter
+  // only grows (a launch takes its nitems tickets from ticket_base on) and the
+  // status words and records are tagged with the launch's epoch.
+  size_t scratch_words = 0, status_words = 0;
+  uint64_t ticket_base = 0;
+  uint32_t epoch = 0;          // of the last entropy launch
+  bool ticket_unknown = false;  // a launch failed: the counter is cleared before the next
+  std::vector<uint32_t> h_status;  // status bits of the last decode (host)
   int64_t fallbacks = 0;
 
-  // timed decodes: 0 start, 3 after the scratch clear, 4 after the entropy
-  // kernel, 1 after the entropy stage (k_expand included), 2 after the pixels
+  // timed decodes: 0 start, 3 before and 4 after the entropy kernel, 1 after
+  // the entropy stage, 2 after the pixels
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
 
   ~cjd_batch() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
   }
 };
 
@@ -172,14 +172,6 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   if (!b || n < 0 || (n && (!files || !sizes))) return CJD_PROGRAMMER_ERROR;
   hipStream_t s = (hipStream_t)stream;
   if (hipSetDevice(b->ctx->device) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
-  if (b->graph_exec) {
-    (void)hipGraphExecDestroy(b->graph_exec);
-    b->graph_exec = nullptr;
-  }
-  if (b->graph) {
-    (void)hipGraphDestroy(b->graph);
-    b->graph = nullptr;
-  }
   plan_batch(b->plan, n, files, sizes, errors);
   int e = 0;
   if (!e) e = upload(b->d_words, b->plan.words, s);
@@ -195,35 +187,70 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   if (!e) e = upload(b->d_qtabs, b->plan.qtabs, s);
   if (!e) e = b->d_coefs.ensure((size_t)b->plan.total_blocks * 128);
   if (!e) e = b->d_out.ensure(b->plan.out_bytes);
-  // scratch: per-image status, ticket, lookback records (cleared per decode)
+  // scratch (layout at cjd_batch): a new allocation is cleared whole and
+  // starts the ticket counter again; otherwise the status words and records
+  // are cleared (this batch's layout differs from the last one's) and the
+  // counter and the epochs go on.  The clear is ordered on the staging stream
+  // like the uploads.
   b->status_words = ((size_t)n + 3) & ~(size_t)3;
-  b->ticket_word = b->status_words;
-  b->scratch_words = b->status_words + 4 + b->plan.items.size() * (sizeof(LookbackRec) / 4);
-  if (!e) e = b->d_scratch.ensure(b->scratch_words * 4);
+  b->scratch_words = 4 + b->status_words + b->plan.items.size() * (sizeof(LookbackRec) / 4);
+  if (!e) {
+    const bool fresh = !b->d_scratch.p || b->d_scratch.cap < b->scratch_words * 4;
+    e = b->d_scratch.ensure(b->scratch_words * 4);
+    if (!e && fresh) {
+      b->ticket_base = 0;
+      b->epoch = 0;
+    }
+    const size_t from = fresh ? 0 : 4;
+    if (!e && launch_clear(b->d_scratch.as<uint32_t>() + from, b->scratch_words - from, s) != hipSuccess)
+      e = CJD_KERNEL_LAUNCH_ERROR;
+  }
   b->h_status.assign(b->status_words, 0);
   return e;
 }
 
-// clear = false leaves out the scratch clear (graph capture: the replay
-// launches it before the graph).
-static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed, bool clear = true) {
+// Epoch of the next entropy launch on b's scratch.  After a failed launch the
+// ticket counter may not match ticket_base: the whole scratch is cleared
+// (ordered on s) and counting starts again.  After kEpochMax launches the
+// status words and records are cleared and the epochs start again at 1.
+static int next_epoch(cjd_batch* b, hipStream_t s, uint32_t* epoch) {
+  if (b->ticket_unknown) {
+    if (launch_clear(b->d_scratch.as<uint32_t>(), b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+    b->ticket_base = 0;
+    b->epoch = 0;
+    b->ticket_unknown = false;
+  } else if (b->epoch >= kEpochMax) {
+    if (launch_clear(b->d_scratch.as<uint32_t>() + 4, b->scratch_words - 4, s) != hipSuccess)
+      return CJD_KERNEL_LAUNCH_ERROR;
+    b->epoch = 0;
+  }
+  *epoch = ++b->epoch;
+  return 0;
+}
+
+static uint32_t* status_of(cjd_batch* b) { return b->d_scratch.as<uint32_t>() + 4; }
+
+static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
   uint32_t* scratch = b->d_scratch.as<uint32_t>();
-  uint32_t* status = scratch;
-  uint32_t* ticket = scratch + b->ticket_word;
-  LookbackRec* lb = reinterpret_cast<LookbackRec*>(scratch + b->ticket_word + 4);
+  uint32_t* status = status_of(b);
+  LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
+  const uint32_t nitems = (uint32_t)b->plan.items.size();
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-  if (clear && launch_clear(scratch, b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  uint32_t epoch = 0;
+  if (int e = next_epoch(b, s, &epoch)) return e;
   if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   hipError_t he;
   if (b->entropy_mode == CJD_ENTROPY_SERIAL)
     he = launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                                (uint32_t)b->plan.segs.size(), b->d_tables.as<HuffTable>(), b->d_coefs.as<int16_t>(),
-                               b->d_recs.as<BlockRec>(), status, s);
+                               b->d_recs.as<BlockRec>(), status, epoch, s);
   else {
     he = launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
-                        b->d_items.as<WorkItem>(), (uint32_t)b->plan.items.size(), b->d_tables.as<HuffTable>(),
-                        b->d_luts.as<uint16_t>(), b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
-                        ticket, s);
+                        b->d_items.as<WorkItem>(), nitems, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
+                        b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
+                        reinterpret_cast<unsigned long long*>(scratch), b->ticket_base, epoch, s);
+    b->ticket_base += nitems;
+    b->ticket_unknown = he != hipSuccess;
     if (he == hipSuccess && timed && hipEventRecord(b->ev[4], s) != hipSuccess) he = hipErrorUnknown;
     b->coefs_stale = true;
   }
@@ -254,27 +281,28 @@ static int enqueue(cjd_batch* b, hipStream_t s, bool timed) {
 // whose tile links did not resolve; map status bits to reference error codes.
 // `redo` receives the images that were decoded again.
 static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo) {
-  if (b->status_words && hipMemcpyAsync(b->h_status.data(), b->d_scratch.p, b->status_words * 4,
+  if (b->status_words && hipMemcpyAsync(b->h_status.data(), status_of(b), b->status_words * 4,
                                         hipMemcpyDeviceToHost, s) != hipSuccess)
     return CJD_DEVICE_MEM_ERROR;
   if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  const uint32_t epoch = b->epoch;
+  for (uint32_t& w : b->h_status) w = status_bits(w, epoch);
   for (size_t i = 0; i < b->plan.recs.size(); ++i) {
     ImageRec& r = b->plan.recs[i];
     if (r.error) continue;
     const uint32_t st = b->h_status[i];
     if (st & (kStatusUnresolved | kStatusInconsistent)) {
-      uint32_t zero = 0;
-      uint32_t* status = b->d_scratch.as<uint32_t>() + i;
-      if (hipMemcpyAsync(status, &zero, 4, hipMemcpyHostToDevice, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+      // a launch of its own, so its status word starts without bits
+      uint32_t e2 = 0, w = 0;
+      if (int e = next_epoch(b, s, &e2)) return e;
+      uint32_t* status = status_of(b);
       if (launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(),
                                 b->d_segs.as<SegmentDesc>() + r.seg_first, r.seg_count, b->d_tables.as<HuffTable>(),
-                                b->d_coefs.as<int16_t>(), b->d_recs.as<BlockRec>(), b->d_scratch.as<uint32_t>(),
-                                s) != hipSuccess)
+                                b->d_coefs.as<int16_t>(), b->d_recs.as<BlockRec>(), status, e2, s) != hipSuccess)
         return CJD_KERNEL_LAUNCH_ERROR;
-      if (hipMemcpyAsync(&b->h_status[i], status, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
-        return CJD_DEVICE_MEM_ERROR;
+      if (hipMemcpyAsync(&w, status + i, 4, hipMemcpyDeviceToHost, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
       if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-      b->h_status[i] |= kStatusFallback;
+      b->h_status[i] = status_bits(w, e2) | kStatusFallback;
       ++b->fallbacks;
       if (redo) redo->push_back(i);
     }
@@ -302,43 +330,17 @@ extern "C" int cjd_batch_decode(cjd_batch* b, void* stream) {
   return enqueue(b, (hipStream_t)stream, true);
 }
 
+// Every entropy launch takes its epoch and ticket base as kernel arguments, so
+// a captured graph would replay the first decode's values: capture is refused.
 extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
-  if (!b) return CJD_PROGRAMMER_ERROR;
-  // Replays of the captured decode have intermittently reported errors for
-  // images that decode correctly with direct launches; until that is
-  // understood a capture is refused.
-  if (!kCaptureEnabled) return CJD_UNSUPPORTED_ERROR;
-  hipStream_t s = (hipStream_t)stream;
-  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->graph_exec = nullptr;
-  b->graph = nullptr;
-  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-  int e = enqueue_entropy(b, s, false, false);
-  if (!e) e = enqueue_pixel(b, s, false);
-  hipGraph_t g = nullptr;
-  if (hipStreamEndCapture(s, &g) != hipSuccess || e) {
-    if (g) (void)hipGraphDestroy(g);
-    return e ? e : CJD_KERNEL_LAUNCH_ERROR;
-  }
-  b->graph = g;
-  if (hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-  return 0;
+  (void)stream;
+  return b ? CJD_UNSUPPORTED_ERROR : CJD_PROGRAMMER_ERROR;
 }
 
 extern "C" int cjd_batch_replay(cjd_batch* b, void* stream) {
-  if (!b || !b->graph_exec) return CJD_PROGRAMMER_ERROR;
-  b->timed = false;
-  hipStream_t s = (hipStream_t)stream;
-  b->coefs_stale = b->entropy_mode != CJD_ENTROPY_SERIAL;
-  b->last_stream = s;
-  // The scratch (work-item ticket, lookback records, image status) is cleared
-  // by a launch of its own, ordered on the stream before the graph.  Replays
-  // of a graph that began with the clear intermittently reported errors for
-  // images that decode correctly, which a clear not yet complete when the
-  // entropy kernel takes its first tickets would cause.
-  if (launch_clear(b->d_scratch.as<uint32_t>(), b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-  return hipGraphLaunch(b->graph_exec, s) == hipSuccess ? 0 : CJD_KERNEL_LAUNCH_ERROR;
+  (void)b;
+  (void)stream;
+  return CJD_PROGRAMMER_ERROR;  // nothing is ever captured
 }
 
 extern "C" int cjd_batch_finish(cjd_batch* b, void* stream, int* errors) {

@@ -95,13 +95,16 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
                                                  const HuffTable* __restrict__ tables,
                                                  const uint16_t* __restrict__ luts, uint32_t* __restrict__ ebuf,
                                                  BlockRec* __restrict__ recs, uint32_t* __restrict__ status,
-                                                 LookbackRec* __restrict__ lb, uint32_t* __restrict__ ticket) {
+                                                 LookbackRec* __restrict__ lb,
+                                                 unsigned long long* __restrict__ ticket, uint64_t ticket_base,
+                                                 uint32_t epoch) {
   __shared__ EntShared sh;
   const int tid = threadIdx.x;
   // Work items are taken in ticket order, so every predecessor an item waits
-  // for in the lookback has already started.
+  // for in the lookback has already started.  The ticket counter is never
+  // cleared: each launch takes nitems tickets from ticket_base on.
   if (tid == 0) {
-    sh.item = atomicAdd(ticket, 1u);
+    sh.item = (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
     sh.lovf = 0;
   }
   __syncthreads();
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   LookbackRec* me = lb + item_id;
   if (tid == 0 && c.lead) {
     ent_aggregate(sh, c, me);
-    st_status(&me->status, 1);
+    st_status(&me->status, lookback_word(epoch, 1));
     // decoupled lookback: compose predecessors' aggregates until one has its
     // inclusive state
     LookAcc acc;
@@ -141,14 +144,15 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
     int32_t cc[4] = {0, 0, 0, 0};
     for (;;) {
       const LookbackRec* r = lb + q_item;
+      // a record of an older launch does not match this epoch: wait for it
       int st;
-      while ((st = ld_status(&r->status)) == 0) __builtin_amdgcn_s_sleep(2);
-      if (st == 2) {
+      while (((uint32_t)(st = ld_status(&r->status)) >> 2) != epoch) __builtin_amdgcn_s_sleep(2);
+      if ((st & 3) == 2) {
         if (!acc.apply(*r, p, s, cc)) {
           // an assumption on the way does not hold: take the immediate
           // predecessor's inclusive state
           const LookbackRec* pr = lb + c.it.prev_item;
-          while (ld_status(&pr->status) != 2) __builtin_amdgcn_s_sleep(2);
+          while (ld_status(&pr->status) != lookback_word(epoch, 2)) __builtin_amdgcn_s_sleep(2);
           acc.reset();
           acc.apply(*pr, p, s, cc);
         }
@@ -177,9 +181,9 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   __syncthreads();
   if (tid == 0) {
     ent_inclusive(sh, c, me);
-    st_status(&me->status, 2);
+    st_status(&me->status, lookback_word(epoch, 2));
   }
-  ent_records(sh, tid, c, recs, status);
+  ent_records(sh, tid, c, recs, status, epoch);
 }
 
 // Block records -> quantised coefficients (natural order, absolute DC), 256
@@ -233,10 +237,10 @@ __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ rec
 __global__ void k_entropy_serial(const uint32_t* __restrict__ words, const ImageDesc* __restrict__ images,
                                  const SegmentDesc* __restrict__ segs, uint32_t nsegs,
                                  const HuffTable* __restrict__ tables, int16_t* __restrict__ coefs,
-                                 BlockRec* __restrict__ recs, uint32_t* __restrict__ status) {
+                                 BlockRec* __restrict__ recs, uint32_t* __restrict__ status, uint32_t epoch) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsegs) return;
-  ent_serial_segment(words, images, segs[s], tables, coefs, recs, status);
+  ent_serial_segment(words, images, segs[s], tables, coefs, recs, status, epoch);
 }
 
 hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s) {
@@ -249,11 +253,11 @@ hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s) {
 
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                           const WorkItem* items, uint32_t nitems, const HuffTable* tables, const uint16_t* luts,
-                          uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb, uint32_t* ticket,
-                          hipStream_t s) {
+                          uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb,
+                          unsigned long long* ticket, uint64_t ticket_base, uint32_t epoch, hipStream_t s) {
   if (!nitems) return hipSuccess;
   hipLaunchKernelGGL(k_entropy, dim3(nitems), dim3(kWG), 0, s, words, images, segs, items, nitems, tables, luts, ebuf,
-                     recs, status, lb, ticket);
+                     recs, status, lb, ticket, ticket_base, epoch);
   return hipGetLastError();
 }
 
@@ -267,10 +271,10 @@ hipError_t launch_expand(const BlockRec* recs, const uint32_t* ebuf, uint64_t ew
 
 hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                                  uint32_t nsegs, const HuffTable* tables, int16_t* coefs, BlockRec* recs,
-                                 uint32_t* status, hipStream_t s) {
+                                 uint32_t* status, uint32_t epoch, hipStream_t s) {
   if (!nsegs) return hipSuccess;
   hipLaunchKernelGGL(k_entropy_serial, dim3((nsegs + 63) / 64), dim3(64), 0, s, words, images, segs, nsegs, tables,
-                     coefs, recs, status);
+                     coefs, recs, status, epoch);
   return hipGetLastError();
 }
 

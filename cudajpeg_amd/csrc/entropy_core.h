@@ -233,11 +233,20 @@ CJD_HD inline void decode_block(BitReader& br, const HuffTable& dct, const HuffT
   }
 }
 
-CJD_HD inline void status_or(uint32_t* p, uint32_t v) {
+// Set status bits v in the word at p for launch `epoch`: a word of an older
+// launch is replaced, one of this launch gains the bits (compare-and-swap, so
+// that lanes of several workgroups can report the same image).
+CJD_HD inline void status_or(uint32_t* p, uint32_t v, uint32_t epoch) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  atomicOr(p, v);
+  uint32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (;;) {
+    const uint32_t nv = ((old >> 8) == epoch ? old : epoch << 8) | v;
+    if (nv == old ||
+        __hip_atomic_compare_exchange_strong(p, &old, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      break;
+  }
 #else
-  *p |= v;
+  *p = ((*p >> 8) == epoch ? *p : epoch << 8) | v;
 #endif
 }
 
@@ -916,7 +925,8 @@ CJD_HD inline void ent_absolute(EntShared& sh, int tid, const EntCtx& c) {
 // Final phase: the block records of local tile tid (absolute block index,
 // absolute DC), the syntax errors of its blocks within the segment, and the
 // truncation check at the segment's last tile.
-CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, BlockRec* recs, uint32_t* status) {
+CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, BlockRec* recs, uint32_t* status,
+                               uint32_t epoch) {
   if (tid >= c.J || (c.lead && tid == 0)) return;
   const int64_t seg_end = c.sg.first_block + (int64_t)c.sg.nblocks;
   int64_t A = c.sg.first_block + (tid > 0 ? sh.u.c.tc[tid - 1][0] : 0);
@@ -955,16 +965,17 @@ CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, Bl
     if (have) emit(prev, send);
   });
   const uint32_t sidx = (uint32_t)c.im->error_index;
-  if (err) status_or(status + sidx, kStatusSyntax);
-  if (sh.lovf) status_or(status + sidx, kStatusInconsistent);
+  if (err) status_or(status + sidx, kStatusSyntax, epoch);
+  if (sh.lovf) status_or(status + sidx, kStatusInconsistent, epoch);
   if (c.tfirst + (uint32_t)tid + 1 == c.sg.ntiles && sh.u.c.tc[tid][0] < (int32_t)c.sg.nblocks)
-    status_or(status + sidx, kStatusTruncated);
+    status_or(status + sidx, kStatusTruncated, epoch);
 }
 
 // Serial decode of one whole segment from its known start (reference order,
 // src/decodeScanGPU.cu:435-451).
 CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* images, const SegmentDesc& sg,
-                                      const HuffTable* tables, int16_t* coefs, BlockRec* recs, uint32_t* status) {
+                                      const HuffTable* tables, int16_t* coefs, BlockRec* recs, uint32_t* status,
+                                      uint32_t epoch) {
   const ImageDesc& im = images[sg.image];
   const HuffTable* tab = tables + im.table_base;
   BitReader br;
@@ -976,7 +987,7 @@ CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* im
     int d, err = 0;
     decode_block<true>(br, tab[im.slot_dc[slot]], tab[im.slot_ac[slot]], d, err, blk);
     if (err) {
-      status_or(status + im.error_index, kStatusSyntax);
+      status_or(status + im.error_index, kStatusSyntax, epoch);
       return;
     }
     const int comp = im.slot_comp[slot];

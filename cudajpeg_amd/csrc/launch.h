@@ -9,13 +9,13 @@ namespace cjd {
 hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s);
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                           const WorkItem* items, uint32_t nitems, const HuffTable* tables, const uint16_t* luts,
-                          uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb, uint32_t* ticket,
-                          hipStream_t s);
+                          uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb,
+                          unsigned long long* ticket, uint64_t ticket_base, uint32_t epoch, hipStream_t s);
 hipError_t launch_expand(const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, int16_t* coefs, uint64_t nblocks,
                          hipStream_t s);
 hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                                  uint32_t nsegs, const HuffTable* tables, int16_t* coefs, BlockRec* recs,
-                                 uint32_t* status, hipStream_t s);
+                                 uint32_t* status, uint32_t epoch, hipStream_t s);
 hipError_t launch_pixel(int mode, const ImageDesc* images, const PixItem* items, uint32_t nitems,
                         const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, const int16_t* coefs,
                         const uint16_t* qtabs, uint8_t* out, hipStream_t s);

@@ -114,17 +114,22 @@ int cjd_batch_set_tune_table(cjd_batch *b, int n, const cjd_tune_entry *entries)
  * failed file is skipped by the decode.  Returns the first device error. */
 int cjd_batch_stage(cjd_batch *b, int n, const uint8_t *const *files, const size_t *sizes, int *errors);
 
-/* cjd_batch_stage with the uploads ordered on `stream` (a hipStream_t) and
- * not waited for: staging one batch overlaps another batch's decode on
- * another stream.  No counterpart in the reference (its uploads are
- * synchronous copies, src/decodeScanGPU.cu:146-151). */
+/* cjd_batch_stage with the uploads (and the clear of the batch's scratch)
+ * ordered on `stream` (a hipStream_t) and not waited for: staging one batch
+ * overlaps another batch's decode on another stream.  The caller orders the
+ * batch's next decode after this stream.  No counterpart in the reference
+ * (its uploads are synchronous copies, src/decodeScanGPU.cu:146-151). */
 int cjd_batch_stage_async(cjd_batch *b, int n, const uint8_t *const *files, const size_t *sizes, int *errors,
                           void *stream);
 
 /* Enqueue the decode of the staged batch on `stream` (asynchronous):
  * entropy decode (replaces decodeScanGPU, src/decodeScanGPU.cu:123) and the
  * fused pixel stage (replaces iDCT_resample_colourTransform,
- * src/pixelTransformGPU.cu:621). */
+ * src/pixelTransformGPU.cu:621).  Two launches, nothing else: the batch's
+ * scratch (work-item ticket counter, per-image status, lookback records) is
+ * not cleared between decodes, since every entropy launch tags what it writes
+ * there with an epoch of its own.  Decodes of one batch are ordered by the
+ * caller (one stream, or events); different batches are independent. */
 int cjd_batch_decode(cjd_batch *b, void *stream);
 
 /* The two halves of cjd_batch_decode.  decode_scan replaces decodeScanGPU
@@ -142,18 +147,16 @@ int cjd_batch_fallbacks(cjd_batch *b);
  * tile links did not resolve, and report per-image reference error codes. */
 int cjd_batch_finish(cjd_batch *b, void *stream, int *errors);
 
-/* Capture the decode of the staged batch into a HIP graph / replay it.  The
- * graph holds the entropy and pixel launches; each replay first launches the
- * scratch clear on the stream, then the graph.  Currently refused
- * (CJD_UNSUPPORTED_ERROR): replays have intermittently reported errors for
- * images that decode correctly with direct launches, and the cause is not
- * yet known.  Direct launches (cjd_batch_decode) are unaffected. */
+/* Graph capture of a batch's decode: refused (CJD_UNSUPPORTED_ERROR).  Each
+ * entropy launch takes its epoch and ticket base as kernel arguments, so a
+ * captured graph would replay the first decode's values.  replay returns
+ * CJD_PROGRAMMER_ERROR (nothing is captured).  A decode is two launches
+ * (cjd_batch_decode), so a graph would save little. */
 int cjd_batch_capture(cjd_batch *b, void *stream);
 int cjd_batch_replay(cjd_batch *b, void *stream);
 
 /* Device time of the last cjd_batch_decode (ms), measured with HIP events on
- * the decode stream: entropy stage (scratch clear, tiled decode, record
- * expansion), pixel kernel, whole decode. */
+ * the decode stream: entropy stage, pixel kernel, whole decode. */
 int cjd_batch_timings(cjd_batch *b, float *ms_entropy, float *ms_pixel, float *ms_total);
 /* The entropy stage of the last cjd_batch_decode split by kernel (ms): the
  * tiled decode kernel alone, and the rest of the stage after it (0 unless a

@@ -25,13 +25,17 @@ struct Emu {
   Plan plan;
   std::vector<int16_t> coefs;
   std::vector<uint8_t> out;
-  std::vector<uint32_t> status;
+  std::vector<uint32_t> status;  // device words, tagged with the launch epoch
+  std::vector<uint32_t> bits;    // the host's status bits per image
   std::vector<uint32_t> ebuf;
   std::vector<BlockRec> recs;
   bool coefs_stale = false;  // records not yet expanded into coefs
   int fixes = 0;
   int fallbacks = 0;
 };
+
+// epoch of the emulated entropy launch
+constexpr uint32_t kEpoch = 6;
 
 struct EntRun {
   Plan* p;
@@ -108,9 +112,10 @@ void resolve(EntRun& R, size_t i) {
     uint32_t q = c.it.prev_item, p = 0, s = 0;
     int32_t cc[4] = {0, 0, 0, 0};
     for (;;) {
-      if (R.lb[q].status == 0) resolve(R, q);  // the kernel spins until it is published
+      // the kernel spins until a record of its own epoch is published
+      if (((uint32_t)R.lb[q].status >> 2) != kEpoch) resolve(R, q);
       const LookbackRec& r = R.lb[q];
-      if (r.status == 2) {
+      if (r.status == lookback_word(kEpoch, 2)) {
         if (!acc.apply(r, p, s, cc)) {
           resolve(R, c.it.prev_item);
           acc.reset();
@@ -134,7 +139,7 @@ void resolve(EntRun& R, size_t i) {
   }
   for (int t = 0; t < kWG; ++t) ent_absolute(sh, t, c);
   ent_inclusive(sh, c, &R.lb[i]);
-  R.lb[i].status = 2;
+  R.lb[i].status = lookback_word(kEpoch, 2);
   R.done[i] = 1;
 }
 
@@ -162,8 +167,11 @@ void run_entropy(Emu& E, int order) {
   E.ebuf.assign((size_t)p.entry_words, 0x7EADBEEFu);
   EntRun R;
   R.p = &p;
+  // records left by an older launch (epoch kEpoch - 1, inclusive), which the
+  // lookback must not take
   R.lb.resize(ni);
-  memset(R.lb.data(), 0, ni * sizeof(LookbackRec));
+  memset(R.lb.data(), 0x5A, ni * sizeof(LookbackRec));
+  for (auto& r : R.lb) r.status = lookback_word(kEpoch - 1, 2);
   R.shs.resize(ni);
   R.ctx.resize(ni);
   R.done.assign(ni, 0);
@@ -192,7 +200,7 @@ void run_entropy(Emu& E, int order) {
     scan_counts(sh, c);
     if (c.lead) {
       ent_aggregate(sh, c, &R.lb[i]);
-      R.lb[i].status = 1;
+      R.lb[i].status = lookback_word(kEpoch, 1);
     }
   }
   // segment-first items resolve from the known start; then the rest in
@@ -202,7 +210,7 @@ void run_entropy(Emu& E, int order) {
   for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
   E.fixes = R.fixes;
   for (size_t i = 0; i < ni; ++i)
-    for (int t = 0; t < kWG; ++t) ent_records(*R.shs[i], t, R.ctx[i], E.recs.data(), E.status.data());
+    for (int t = 0; t < kWG; ++t) ent_records(*R.shs[i], t, R.ctx[i], E.recs.data(), E.status.data(), kEpoch);
   E.coefs_stale = true;
 }
 
@@ -257,25 +265,31 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
   BlockRec junk{0, 0, 0x5555};  // eidx, nw, dc
   E->recs.assign((size_t)E->plan.total_blocks, junk);
   E->out.assign(E->plan.out_bytes, 0xAB);
-  E->status.assign((size_t)n, 0);
+  // status words of an older launch with every bit set: this launch must
+  // not read them as its own
+  E->status.assign((size_t)n, (kEpoch - 1) << 8 | 0xFFu);
   if (serial) {
     for (auto& s : E->plan.segs)
       ent_serial_segment(E->plan.words.data(), E->plan.images.data(), s, E->plan.tables.data(), E->coefs.data(),
-                         E->recs.data(), E->status.data());
+                         E->recs.data(), E->status.data(), kEpoch);
   } else {
     run_entropy(*E, order);
   }
+  E->bits.assign((size_t)n, 0);
   for (int i = 0; i < n; ++i) {
     ImageRec& r = E->plan.recs[(size_t)i];
     if (r.error) continue;
-    if (E->status[(size_t)i] & (kStatusUnresolved | kStatusInconsistent)) {
-      E->status[(size_t)i] = kStatusFallback;
+    uint32_t& st = E->bits[(size_t)i];
+    st = status_bits(E->status[(size_t)i], kEpoch);
+    if (st & (kStatusUnresolved | kStatusInconsistent)) {
+      // the serial redo is a launch of its own (next epoch)
       ++E->fallbacks;
       for (uint32_t s = 0; s < r.seg_count; ++s)
         ent_serial_segment(E->plan.words.data(), E->plan.images.data(), E->plan.segs[r.seg_first + s],
-                           E->plan.tables.data(), E->coefs.data(), E->recs.data(), E->status.data());
+                           E->plan.tables.data(), E->coefs.data(), E->recs.data(), E->status.data(), kEpoch + 1);
+      st = status_bits(E->status[(size_t)i], kEpoch + 1) | kStatusFallback;
     }
-    if (E->status[(size_t)i] & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
+    if (st & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
     if (errors) errors[i] = r.error;
   }
   run_pixel(*E, idct_mode, 0, E->plan.pix.size());
@@ -322,7 +336,7 @@ void emu_stats(void* h, long* out4) {
 // per-image status bits of the last run (before the host maps them to codes)
 void emu_status(void* h, uint32_t* out, int n) {
   Emu* E = static_cast<Emu*>(h);
-  for (int i = 0; i < n && i < (int)E->status.size(); ++i) out[i] = E->status[(size_t)i];
+  for (int i = 0; i < n && i < (int)E->bits.size(); ++i) out[i] = E->bits[(size_t)i];
 }
 
 void emu_free(void* h) { delete static_cast<Emu*>(h); }

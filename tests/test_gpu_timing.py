@@ -10,8 +10,7 @@ pytestmark = pytest.mark.gpu
 
 def test_kernel_timings_split_the_entropy_stage(gpu):
     # the per-kernel events lie inside the entropy stage: the tiled decode
-    # kernel plus what follows it in the stage never exceed it (the scratch
-    # clear is the rest)
+    # kernel plus what follows it in the stage never exceed it
     specs = [synth.ImageSpec(320, 240, "420", 80, 6.0, 81), synth.ImageSpec(200, 136, "444", 90, 8.0, 82)]
     b = BatchDecoder(IDCT_INT)
     assert not b.stage([synth.encode(s).data for s in specs]).any()
@@ -26,15 +25,17 @@ def test_kernel_timings_split_the_entropy_stage(gpu):
 
 
 def test_graph_capture_is_refused(gpu):
-    # capture/replay is switched off (replays intermittently reported errors
-    # for images that decode correctly); the refusal is the reference's
-    # UNSUPPORTED_ERROR and leaves the batch usable for direct launches
+    # a decode's launches take per-launch arguments (epoch, ticket base), so
+    # capture is refused with the reference's UNSUPPORTED_ERROR; replay has
+    # nothing to run; the batch stays usable for direct launches
     from cudajpeg_amd import DecodeError
     b = BatchDecoder(IDCT_INT)
     assert not b.stage([synth.encode(synth.ImageSpec(64, 48, "420", 75, 6.0, 1)).data]).any()
     with pytest.raises(DecodeError) as exc:
         b.capture(None)
     assert exc.value.code == 2
+    with pytest.raises(DecodeError):
+        b.replay(None)
     b.decode()
     assert not b.finish().any()
     b.close()
