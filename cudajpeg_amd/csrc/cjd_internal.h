@@ -99,9 +99,11 @@ struct WorkItem {
   uint64_t ebase;       // entry-buffer word of the item's streams, DC lists and checkpoints
 };
 
-// Stream row of a work item of J local tiles: the lanes of one wave write
-// one row of W consecutive words per step (W = 64, or J rounded up to 16 for
-// J < 64); nl = rows' words over all waves.
+// Stream row of a work item of J local tiles: W lanes (W = 64, or J rounded
+// up to 16 for J < 64); a stream row holds 4 consecutive steps of each of
+// its W lanes (4 W words, lane-major: a lane's 4 entries are one 16-byte
+// group), so a wave writes one contiguous 1 KB row per 4 steps; nl = lanes
+// over all waves.
 inline uint32_t item_row(uint32_t J) { return J >= 64 ? 64u : (J + 15) & ~15u; }
 inline uint64_t item_lanes(uint32_t J) { return (uint64_t)((J + 63) / 64) * item_row(J); }
 // Entry-buffer words of a work item (layout in entropy_core.h).
@@ -109,10 +111,16 @@ inline uint64_t item_words(uint32_t J, uint32_t rs, uint32_t rd, uint32_t nck) {
   return item_lanes(J) * (2ull * rs + 2ull * rd + 2ull * nck) + kSinkWords;
 }
 
+// Word of entry i of a stream whose entry 0 is at word e (a lane's stream
+// of row width W): entries go 4 to a 16-byte group, groups 4 W words apart.
+inline constexpr uint64_t entry_word(uint64_t e, uint32_t W, uint32_t i) {
+  return (e & ~3ull) + (uint64_t)(((uint32_t)(e & 3u) + i) >> 2) * 4u * W + (((uint32_t)e + i) & 3u);
+}
+
 // Where the quantised coefficients of one block are, after the entropy
-// decode: AC entries ebuf[eidx + W i] for i < n (bits 16-21 natural index,
-// 0-15 value; 0xFFFFFFFF = none) with n in bits 0-7 of nw and the stream row
-// W = 16 (1 + bits 8-9 of nw), and the absolute DC value.
+// decode: AC entries ebuf[entry_word(eidx, W, i)] for i < n (bits 16-21
+// natural index, 0-15 value; 0xFFFFFFFF = none) with n in bits 0-7 of nw and
+// the stream row W = 16 (1 + bits 8-9 of nw), and the absolute DC value.
 struct BlockRec {
   uint32_t eidx;
   uint16_t nw;

@@ -190,11 +190,11 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
 // consecutive blocks per workgroup, written as one contiguous 32 KB run
 // (blocks the serial decoder wrote are left as they are; the pixel kernel
 // reads records, so this runs when coefficients are asked for).  A block's AC
-// entries are one stream row apart (one per step of its tile's
-// stream), so every entry is a separate memory transaction: a lane issues
-// them eight at a time before using any.  The blocks are assembled in LDS at
-// a stride of 33 words, so that lanes writing the same coefficient index of
-// their blocks hit different banks.
+// entries are consecutive entries of its tile's stream, 16-byte groups of 4
+// one stream row apart: a lane issues its block's group loads four at a
+// time before using any.  The blocks are assembled in LDS at a stride of 33
+// words, so that lanes writing the same coefficient index of their blocks hit
+// different banks.
 constexpr int kExpandStride = 33;  // LDS words per block (32 + 1 of padding)
 
 __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ recs, const uint32_t* __restrict__ ebuf,
@@ -209,15 +209,26 @@ __global__ void __launch_bounds__(256) k_expand(const BlockRec* __restrict__ rec
   if (b < nblocks && !(recs[b].nw & kRecDense)) {
     const BlockRec r = recs[b];
     const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
-    const uint32_t* E = ebuf + (r.eidx + (uint64_t)n * W <= ewords ? r.eidx : 0);
+    const uint32_t ph = r.eidx & 3u, ng = n ? (ph + n + 3) >> 2 : 0u;
+    const uint64_t g0 = r.eidx & ~3ull, row = 4ull * W;
     int16_t* B = reinterpret_cast<int16_t*>(mine);
-    for (uint32_t i0 = 0; i0 < n; i0 += 8) {
-      uint32_t e[8];
+    if (ng && g0 + (ng - 1) * row + 4 <= ewords) {  // else a corrupt record: no AC entries
+      for (uint32_t g = 0; g < ng; g += 4) {
+        uint4 v[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) e[k] = i0 + k < n ? E[(i0 + k) * W] : 0xFFFFFFFFu;
+        for (int k = 0; k < 4; ++k)
+          v[k] = g + k < ng ? *reinterpret_cast<const uint4*>(ebuf + g0 + (g + k) * row)
+                            : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (e[k] != 0xFFFFFFFFu) B[(e[k] >> 16) & 63] = (int16_t)(e[k] & 0xFFFFu);
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t e[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+          for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t kk = 4 * (g + k) + q;
+            if (kk >= ph && kk < ph + n && e[q] != 0xFFFFFFFFu) B[(e[q] >> 16) & 63] = (int16_t)(e[q] & 0xFFFFu);
+          }
+        }
+      }
     }
     B[0] = r.dc;
   }

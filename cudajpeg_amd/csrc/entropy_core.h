@@ -358,8 +358,9 @@ struct EntCtx {
   // Item layout (words from it.ebase), W = stream row (64, or J rounded up
   // to 16 below 64 tiles), nl = W * waves:
   //   speculative streams  nl * rs: entry of step s of local tile j at
-  //                        ((j / 64) * rs + s) * W + j % 64 (a wave's lanes
-  //                        write one contiguous row per step)
+  //                        ((j / 64) * rs + s - s % 4) * W + 4 (j % 64) + s % 4
+  //                        (a wave's lanes write one contiguous row of 16-byte
+  //                        groups per 4 steps)
   //   prefix streams       nl * rs, same layout
   //   speculative DC lists nl * rd (tile j at j * rd), prefix DC lists nl * rd
   //   checkpoints          nl * 2 * nck (tile j at j * 2 * nck)
@@ -367,7 +368,7 @@ struct EntCtx {
   CJD_HD uint32_t W() const { return J >= 64 ? 64u : ((uint32_t)J + 15) & ~15u; }
   CJD_HD uint64_t nl() const { return (uint64_t)((J + 63) / 64) * W(); }
   CJD_HD uint32_t* sstream(int j) const {
-    return ebuf + it.ebase + (uint64_t)(j >> 6) * sg.rs * W() + (uint32_t)(j & 63);
+    return ebuf + it.ebase + (uint64_t)(j >> 6) * sg.rs * W() + 4u * (uint32_t)(j & 63);
   }
   CJD_HD uint32_t* pstream(int j) const { return sstream(j) + nl() * sg.rs; }
   CJD_HD uint32_t* sdcl(int j) const { return ebuf + it.ebase + 2 * nl() * sg.rs + (uint64_t)j * sg.rd; }
@@ -499,10 +500,11 @@ CJD_HD inline uint32_t ubfe(uint32_t v, uint32_t off, uint32_t width) {
 
 // One symbol (src/decodeScanGPU.cu:78-100 for one symbol: DC difference, or
 // an AC run/size with EXTEND, EOB, ZRL, the coefficient index checks).
-// Straight-line code with one unconditional store; the second-level lookup
-// and the DC-list store are the only branches.  Before the symbol, the first
-// block boundary at or after thr in this period is captured.
-CJD_HD inline void lane_step(EntShared& sh, int tid, Lane& L, uint32_t thr, int nh, uint32_t* eb) {
+// Straight-line code returning the step's stream entry (the period stores its
+// four entries as one group); the second-level lookup and the DC-list store
+// are the only branches.  Before the symbol, the first block boundary at or
+// after thr in this period is captured.
+CJD_HD inline uint32_t lane_step(EntShared& sh, int tid, Lane& L, uint32_t thr, int nh, uint32_t* eb) {
   const uint32_t bits = lane_peek(sh, tid, L.pos);
   const bool isdc = L.k < 0;
   const bool hc = isdc && !L.hit && L.pos >= thr;
@@ -525,7 +527,7 @@ CJD_HD inline void lane_step(EntShared& sh, int tid, Lane& L, uint32_t thr, int 
   const uint32_t z = sh.dz[nk > 63 ? 63 : nk];
   const uint32_t nat = runp1 == 0 ? 63u : z;
   const uint32_t vv = (uint32_t)v & 0xFFFFu;
-  eb[L.si + (uint64_t)L.s * L.W] = (nat << 16) | vv;
+  const uint32_t entry = (nat << 16) | vv;
   const uint32_t dcent = (L.s << 17) | (invalid ? 0x10000u : 0u) | vv;
   if (isdc || bad) eb[L.di + (isdc ? L.dn : L.dn - 1)] = isdc ? dcent : (L.lastdc | 0x10000u);
   L.lastdc = isdc ? dcent : L.lastdc;
@@ -534,6 +536,19 @@ CJD_HD inline void lane_step(EntShared& sh, int tid, Lane& L, uint32_t thr, int 
   L.k = end ? -1 : nk;
   const int ns = L.slot + 1 == nh ? 0 : L.slot + 1;
   L.slot = end ? ns : L.slot;
+  return entry;
+}
+
+// The four entries of a period, as one 16-byte store.
+CJD_HD inline void store_group(uint32_t* p, uint32_t a, uint32_t b, uint32_t d, uint32_t e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *reinterpret_cast<uint4*>(p) = make_uint4(a, b, d, e);
+#else
+  p[0] = a;
+  p[1] = b;
+  p[2] = d;
+  p[3] = e;
+#endif
 }
 
 // Checkpoint m of local tile j (2 words at ckpts(j) + 2m): the speculative
@@ -545,7 +560,7 @@ CJD_HD inline void lane_step(EntShared& sh, int tid, Lane& L, uint32_t thr, int 
 CJD_HD inline void lane_restore(Lane& L, bool mark, uint32_t* eb) {
   const uint32_t sc = (L.cap_i >> 4) & 0x3FFFu;
   if (mark && !L.ovf)
-    for (uint32_t q = sc; q < L.s; ++q) eb[L.si + (uint64_t)q * L.W] = 0xFFFFFFFFu;
+    for (uint32_t q = sc; q < L.s; ++q) eb[L.si + (uint64_t)(q & ~3u) * L.W + (q & 3u)] = 0xFFFFFFFFu;
   L.pos = L.cap_p;
   L.slot = (int)(L.cap_i & 15u);
   L.dn = L.cap_i >> 18;
@@ -563,6 +578,8 @@ CJD_HD inline void lane_restore(Lane& L, bool mark, uint32_t* eb) {
 // < nck) and the decode stops at the first one it equals (joined = m + 1).
 // When a boundary past the next threshold may have passed in the same period,
 // the lane decodes again from the capture (long blocks only).
+static_assert(kPeriod == 4, "a period's four entries are one 16-byte group of the stream");
+
 template <bool kLink>
 CJD_HD inline bool lane_run(EntShared& sh, int tid, const EntCtx& c, Lane& L, uint32_t b0, uint32_t end,
                             uint32_t* ck, int nck, int& joined, int budget) {
@@ -614,15 +631,25 @@ CJD_HD inline bool lane_run(EntShared& sh, int tid, const EntCtx& c, Lane& L, ui
     if (budget >= 0 && per >= budget) return false;
     lane_rotate(sh, tid, c, L);
     if (L.ovf || L.s + kPeriod > L.R || L.dn + kPeriod + 1 > L.Rd) {
-      // keep decoding for the exit, writing into the item's sink: rows 0-3 of
-      // its stream part and entries 1-5 of the lane's 8-entry list part (the
-      // word indexes wrap modulo 2^64 to land there)
+      // keep decoding for the exit, writing into the item's sink: the lane's
+      // group of its 256-word stream part and entries 1-5 of the lane's
+      // 8-entry list part (the word indexes wrap modulo 2^64 to land there)
       L.ovf = true;
       const uint64_t sk = (uint64_t)(c.sink() - c.ebuf);
-      L.si = sk + (uint64_t)(tid & 63) - (uint64_t)L.s * L.W;
+      L.si = sk + 4u * (uint64_t)(tid & 63) - (uint64_t)L.s * L.W;
       L.di = sk + 4 * 64 + (uint64_t)(tid & 63) * 8 + 2 - (uint64_t)L.dn;
     }
-    for (int u = 0; u < kPeriod; ++u) lane_step(sh, tid, L, thr, c.nh, c.ebuf);
+    // a period starts at a step that is a multiple of 4: its entries are one
+    // group of the stream
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (L.s & 3u) __builtin_trap();
+#endif
+    uint32_t* const grp = c.ebuf + (L.si + (uint64_t)L.s * L.W);
+    const uint32_t e0 = lane_step(sh, tid, L, thr, c.nh, c.ebuf);
+    const uint32_t e1 = lane_step(sh, tid, L, thr, c.nh, c.ebuf);
+    const uint32_t e2 = lane_step(sh, tid, L, thr, c.nh, c.ebuf);
+    const uint32_t e3 = lane_step(sh, tid, L, thr, c.nh, c.ebuf);
+    store_group(grp, e0, e1, e2, e3);
   }
 }
 
@@ -946,7 +973,7 @@ CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, Bl
       add_dc(pred, comp, (int16_t)(e & 0xFFFFu));
       if (A < seg_end) {
         BlockRec r;
-        r.eidx = base + (st + 1) * W;
+        r.eidx = (uint32_t)entry_word(base, W, st + 1);
         r.nw = (uint16_t)((sn - st - 1) | ((W / 16 - 1) << 8));
         r.dc = (int16_t)(comp == 0 ? pred[0] : comp == 1 ? pred[1] : pred[2]);
         recs[A] = r;

@@ -280,15 +280,27 @@ CJD_HD inline void pix_stage_zero(SH& sh, int tid, const PixCtx& c, const uint16
   }
 }
 
+// The entries of 16-byte group v that belong to a block: group entry q is
+// the block's entry k0 + q - ph when 0 <= k0 + q - ph < n.
+template <class SH>
+CJD_HD inline void stage_group(SH& sh, int b, const u32x4& v, uint32_t k0, uint32_t ph, uint32_t n) {
+  uint32_t e[4];
+  __builtin_memcpy(e, &v, 16);
+#pragma unroll
+  for (uint32_t q = 0; q < 4; ++q)
+    if (k0 + q >= ph && k0 + q < ph + n) stage_entry(sh, b, e[q]);
+}
+
 // pix_stage_blocks: lanes 2b and 2b+1 fill block b of the item (blocks are
 // the item's MCUs in scan order) from its record.  A block the serial decoder
-// wrote is copied from the dense buffer, four 16-byte pieces per lane; for
-// the others each lane writes every second AC entry of the record (entries
-// one stream row apart, loaded four at a time; an entry's natural index is
-// unique within its block, so the two lanes' writes never meet) and the even
-// lane writes the absolute DC value.  A record whose entries would run past
-// the entry buffer (a corrupt stream) reads from its start instead, as the
-// expansion kernel does.
+// wrote is copied from the dense buffer, four 16-byte pieces per lane.  For
+// the others the block's AC entries are consecutive entries of a tile stream:
+// 16-byte groups of 4, one stream row (4 W words) apart; lane h loads groups
+// h, h+2, ... two at a time and stages the entries in them that are the
+// block's (an entry's natural index is unique within its block, so the two
+// lanes' writes never meet), and the even lane writes the absolute DC value.
+// A record whose groups would run past the entry buffer (a corrupt stream)
+// stages no AC entries.
 template <class SH>
 CJD_HD inline void pix_stage_blocks(SH& sh, int tid, const PixCtx& c, const BlockRec* recs, const uint32_t* ebuf,
                                     uint64_t ewords, const int16_t* coefs) {
@@ -310,16 +322,17 @@ CJD_HD inline void pix_stage_blocks(SH& sh, int tid, const PixCtx& c, const Bloc
     return;
   }
   const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
-  const uint32_t* E = ebuf + (r.eidx + (uint64_t)n * W <= ewords ? r.eidx : 0);
-  uint32_t i = (uint32_t)h;
-  for (; i + 6 < n; i += 8) {
-    const uint32_t e0 = E[i * W], e1 = E[(i + 2) * W], e2 = E[(i + 4) * W], e3 = E[(i + 6) * W];
-    stage_entry(sh, b, e0);
-    stage_entry(sh, b, e1);
-    stage_entry(sh, b, e2);
-    stage_entry(sh, b, e3);
+  const uint32_t ph = r.eidx & 3u, ng = n ? (ph + n + 3) >> 2 : 0u;
+  const uint64_t g0 = r.eidx & ~3ull, row = 4ull * W;
+  if (ng && g0 + (ng - 1) * row + 4 <= ewords) {
+    for (uint32_t g = (uint32_t)h; g < ng; g += 4) {
+      u32x4 v0, v1 = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+      __builtin_memcpy(&v0, ebuf + g0 + g * row, 16);
+      if (g + 2 < ng) __builtin_memcpy(&v1, ebuf + g0 + (g + 2) * row, 16);
+      stage_group(sh, b, v0, 4 * g, ph, n);
+      stage_group(sh, b, v1, 4 * (g + 2), ph, n);
+    }
   }
-  for (; i < n; i += 2) stage_entry(sh, b, E[i * W]);
   if (h == 0) stage_coef(sh, b, 0, r.dc);
 }
 

@@ -152,9 +152,10 @@ void expand(Emu& E) {
     int16_t* B = E.coefs.data() + b * 64;
     for (int k = 0; k < 64; ++k) B[k] = 0;
     const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
-    const uint64_t e0 = r.eidx + (uint64_t)n * W <= ew ? r.eidx : 0;
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t e = E.ebuf[e0 + (uint64_t)i * W];
+    // a record whose groups would run past the entry buffer stages no AC entries
+    const bool ok = n == 0 || entry_word(r.eidx, W, n - 1) - ((r.eidx + n - 1) & 3u) + 4 <= ew;
+    for (uint32_t i = 0; ok && i < n; ++i) {
+      const uint32_t e = E.ebuf[entry_word(r.eidx, W, i)];
       if (e != 0xFFFFFFFFu) B[(e >> 16) & 63] = (int16_t)(e & 0xFFFFu);
     }
     B[0] = r.dc;
