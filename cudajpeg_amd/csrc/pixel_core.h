@@ -410,21 +410,31 @@ CJD_HD inline void pix_planes(const SH& sh, int tid, const PixCtx& c, uint8_t* o
 // back from the register's previous value).
 CJD_HD inline uint32_t clip_shr8(int v) { return (uint32_t)(v < 0 ? 0 : (v > 65535 ? 65535 : v)) >> 8; }
 
-// Aligned 4-byte store (device: one dword store; host: memcpy).
-CJD_HD inline void store_u32(uint8_t* p, uint32_t v) {
+// Aligned 16-byte store (device: one dwordx4 store; host: memcpy).
+CJD_HD inline void store_u32x4(uint8_t* p, uint32_t a, uint32_t b, uint32_t d, uint32_t e) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  *reinterpret_cast<uint32_t*>(p) = v;
+  *reinterpret_cast<uint4*>(p) = make_uint4(a, b, d, e);
 #else
-  memcpy(p, &v, 4);
+  const uint32_t v[4] = {a, b, d, e};
+  memcpy(p, v, 16);
+#endif
+}
+
+// Bytes s .. s+3 of the 8-byte value hi:lo (s = 0..3).
+CJD_HD inline uint32_t align_bytes(uint32_t hi, uint32_t lo, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbyte(hi, lo, s);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * s));
 #endif
 }
 
 // Colour conversion / gray copy (CPUonly/src/pixelTransformCPU.c:130-172).
-// Each output row segment of the run is written as aligned 4-byte words:
-// wave -> row, lane -> word.  A word of interleaved RGB covers bytes of at
-// most two pixels, so a lane converts those two pixels and shifts the word
-// out of their six bytes.  The partial words at the two ends of a segment
-// (shared with neighbouring runs or images) are written byte by byte.
+// Each output row segment of the run is written as aligned 16-byte chunks:
+// wave -> row, lane -> chunk.  A chunk of interleaved RGB covers bytes of at
+// most six pixels, so a lane converts those six and shifts the chunk out of
+// their 18 bytes.  The partial chunks at the two ends of a segment (shared
+// with neighbouring runs or images) are written byte by byte.
 template <class SH>
 CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* out) {
   const ImageDesc& im = *c.im;
@@ -442,26 +452,29 @@ CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* o
   const int xs2 = im.comp_xshift[2], ys2 = im.comp_yshift[2];
   for (int yy = wave; yy < rows; yy += kPixWaves) {
     const uint64_t a0 = im.out_offset + ((uint64_t)(y0 + yy) * (uint64_t)im.width + (uint64_t)x0) * (uint64_t)bpp;
-    const int mis = (int)(a0 & 3u);
-    const int nw = (mis + n + 3) >> 2;
+    const int mis = (int)(a0 & 15u);
+    const int nch = (mis + n + 15) >> 4;
     uint8_t* base = out + (a0 - (uint64_t)mis);
     const uint8_t* srow0 = &sh.samp[c.toff0 + (yy >> ys0) * c.tw0];
     const uint8_t* srow1 = &sh.samp[c.toff1 + (yy >> ys1) * c.tw1];
     const uint8_t* srow2 = &sh.samp[c.toff2 + (yy >> ys2) * c.tw2];
-    for (int w = lane; w < nw; w += 64) {
-      const int pos = 4 * w - mis;  // segment byte of the word's first byte (-3..)
-      uint32_t word;
+    for (int w = lane; w < nch; w += 64) {
+      const int pos = 16 * w - mis;  // segment byte of the chunk's first byte (-15..)
+      uint32_t o[4];
       if (bpp == 1) {
-        word = 0;
-        for (int j = 0; j < 4; ++j) {
-          int q = pos + j;
-          q = q < 0 ? 0 : (q >= rw ? rw - 1 : q);
-          word |= (uint32_t)srow0[q] << (8 * j);
+        for (int k = 0; k < 4; ++k) {
+          uint32_t word = 0;
+          for (int j = 0; j < 4; ++j) {
+            int q = pos + 4 * k + j;
+            q = q < 0 ? 0 : (q >= rw ? rw - 1 : q);
+            word |= (uint32_t)srow0[q] << (8 * j);
+          }
+          o[k] = word;
         }
       } else {
-        const int q0 = (pos + 3) / 3 - 1;  // pixel of byte pos (floor, pos >= -3)
-        uint64_t rgb = 0;
-        for (int e = 0; e < 2; ++e) {
+        const int q0 = (pos + 15) / 3 - 5;  // pixel of byte pos (floor, pos >= -15)
+        uint32_t px[6];
+        for (int e = 0; e < 6; ++e) {
           int xx = q0 + e;
           xx = xx < 0 ? 0 : (xx >= rw ? rw - 1 : xx);
           const int Y = (int)srow0[xx >> xs0] << 8;
@@ -470,16 +483,23 @@ CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* o
           const uint32_t r = clip_shr8(Y + 359 * cr + 128);
           const uint32_t g = clip_shr8(Y - 88 * cb - 183 * cr + 128);
           const uint32_t b = clip_shr8(Y + 454 * cb + 128);
-          rgb |= (uint64_t)(r | (g << 8) | (b << 16)) << (24 * e);
+          px[e] = r | (g << 8) | (b << 16);
         }
-        word = (uint32_t)(rgb >> (8 * (pos - 3 * q0)));
+        // the six pixels' 18 bytes as dwords, then the 16 from byte pos - 3 q0
+        const uint32_t d0 = px[0] | (px[1] << 24), d1 = (px[1] >> 8) | (px[2] << 16);
+        const uint32_t d2 = (px[2] >> 16) | (px[3] << 8), d3 = px[4] | (px[5] << 24), d4 = px[5] >> 8;
+        const uint32_t sft = (uint32_t)(pos - 3 * q0);
+        o[0] = align_bytes(d1, d0, sft);
+        o[1] = align_bytes(d2, d1, sft);
+        o[2] = align_bytes(d3, d2, sft);
+        o[3] = align_bytes(d4, d3, sft);
       }
-      uint8_t* dst = base + 4 * w;
-      if (pos >= 0 && pos + 4 <= n) {
-        store_u32(dst, word);
+      uint8_t* dst = base + 16 * w;
+      if (pos >= 0 && pos + 16 <= n) {
+        store_u32x4(dst, o[0], o[1], o[2], o[3]);
       } else {
-        for (int j = 0; j < 4; ++j)
-          if (pos + j >= 0 && pos + j < n) dst[j] = (uint8_t)(word >> (8 * j));
+        for (int j = 0; j < 16; ++j)
+          if (pos + j >= 0 && pos + j < n) dst[j] = (uint8_t)(o[j >> 2] >> (8 * (j & 3)));
       }
     }
   }
