@@ -20,30 +20,47 @@ struct MfmaB {
   half8_t hi[2][2], lo[2][2];
 };
 
+// The 8 products of a tile pair whose blocks all use table slot TL: per K
+// half, hi and lo parts, the two tiles interleaved.
+template <int TL>
+__device__ __forceinline__ void mfma_one_table(const MfmaB& B, const half8_t (&a)[2][2], float4_t (&acc)[2]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(B.hi[TL][h], a[jj][h], acc[jj], 0, 0, 0);
+      acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(B.lo[TL][h], a[jj][h], acc[jj], 0, 0, 0);
+    }
+}
+
 // Batched-form iDCT of one item on the matrix cores (see the MFMA section of
-// pixel_core.h).  Wave w computes N tile w (pixels 16w .. 16w+15) of every
-// 16-block tile, so its B operands depend only on the image's quant tables:
-// they are built in registers and kept while consecutive items belong to the
-// same image (`b_image`, ~0 = none).  Tiles go two at a time; all LDS reads
-// of a pair (A operands, table indices, output descriptors) are issued
-// together from clamped, always-valid rows, and both table slots always run
-// (rows of other tables are masked to zero), so the loop has no per-lane
-// branches around loads or MFMAs.
+// pixel_core.h).  The contraction runs as C[pixel][block] = M_t^T x coef^T:
+// wave w computes N tile w (pixels 16w .. 16w+15) of every 16-block tile, so
+// its matrix operands depend only on the image's quant tables and are built
+// in registers and kept while consecutive items belong to the same image
+// (`b_image`, ~0 = none); lane l receives 4 consecutive pixels (one 4-byte
+// sample word) of block l % 16.  The item's blocks sit in LDS in table order,
+// and tiles go two at a time: a pair whose blocks all use one table runs that
+// table's 8 products only; a pair across a table boundary runs every table of
+// the pass with the blocks of the other tables masked to zero.  All LDS reads
+// of a pair are issued together from clamped, always-valid rows.
 __device__ __forceinline__ void pix_idct_mfma(PixSharedMfma& sh, int tid, const PixCtx& c, MfmaB& B,
                                               uint32_t& b_image) {
   const int n = tid >> 6, l = tid & 63;
   const int ntiles = (c.nblk + 15) >> 4;
   const int last = c.nblk - 1;
   const half8_t zero = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int y = 2 * n + ((l & 15) >> 3), x = l & 7, kofs = 8 * (l >> 4);
+  const int kofs = 8 * (l >> 4);
+  const int yo = 2 * n + (l >> 5), xo = 4 * ((l >> 4) & 1);  // the lane's pixel row and first column
   for (int t0 = 0; t0 < c.ntab; t0 += 2) {
     const int nt = c.ntab - t0 < 2 ? c.ntab - t0 : 2;
     if (t0 > 0 || b_image != c.it.image) {
+      // constant loop bounds keep the operand arrays in registers
       for (int tl = 0; tl < 2; ++tl)
         for (int h = 0; h < 2; ++h) {
-          // slot 1 repeats slot 0 when there is one table: its rows are all masked
+          if (tl >= nt) continue;
           uint16_t hb[8], lb[8];
-          mfma_b_lane(sh.q[t0 + (tl < nt ? tl : 0)], sh.cosf, n, h, l, hb, lb);
+          mfma_b_lane(sh.q[t0 + tl], sh.cosf, n, h, l, hb, lb);
           __builtin_memcpy(&B.hi[tl][h], hb, 16);
           __builtin_memcpy(&B.lo[tl][h], lb, 16);
         }
@@ -51,34 +68,39 @@ __device__ __forceinline__ void pix_idct_mfma(PixSharedMfma& sh, int tid, const 
     for (int tile = 0; tile < ntiles; tile += 2) {
       half8_t a[2][2];
       int tix[2];
-      uint32_t ob[2][4];
+      uint32_t info[2];
       for (int jj = 0; jj < 2; ++jj) {
-        const int brow = (tile + jj) * 16 + (l & 15);
-        const int r = brow <= last ? brow : last;
+        const int pj = (tile + jj) * 16 + (l & 15);
+        const int r = pj <= last ? pj : last;
         __builtin_memcpy(&a[jj][0], &sh.coef[r][kofs], 16);
         __builtin_memcpy(&a[jj][1], &sh.coef[r][32 + kofs], 16);
-        const int t = binfo_tix(sh.binfo[r]);
-        tix[jj] = brow <= last ? t : -1;
-        for (int i = 0; i < 4; ++i) {
-          const int b = (tile + jj) * 16 + 4 * (l >> 4) + i;
-          ob[jj][i] = sh.binfo[b <= last ? b : last];
-        }
+        info[jj] = sh.binfo[r];
+        tix[jj] = pj <= last ? binfo_tix(info[jj]) : -1;
       }
+      const int f = tile * 16, e = f + 31 <= last ? f + 31 : last;
+      const int tlo = __builtin_amdgcn_readfirstlane(binfo_tix(sh.binfo[f])) - t0;
+      const int thi = __builtin_amdgcn_readfirstlane(binfo_tix(sh.binfo[e])) - t0;
       float4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-      for (int tl = 0; tl < 2; ++tl)
-        for (int h = 0; h < 2; ++h)
-          for (int jj = 0; jj < 2; ++jj) {
-            const half8_t am = tix[jj] == t0 + tl ? a[jj][h] : zero;
-            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, B.hi[tl][h], acc[jj], 0, 0, 0);
-            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(am, B.lo[tl][h], acc[jj], 0, 0, 0);
-          }
+      if (tlo == thi) {
+        if (tlo == 0)
+          mfma_one_table<0>(B, a, acc);
+        else if (tlo == 1 && nt > 1)
+          mfma_one_table<1>(B, a, acc);
+      } else {
+        for (int tl = 0; tl < 2; ++tl)
+          for (int h = 0; h < 2; ++h)
+            for (int jj = 0; jj < 2; ++jj) {
+              if (tl >= nt) continue;
+              const half8_t bm = tix[jj] == t0 + tl ? a[jj][h] : zero;
+              acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(B.hi[tl][h], bm, acc[jj], 0, 0, 0);
+              acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(B.lo[tl][h], bm, acc[jj], 0, 0, 0);
+            }
+      }
       for (int jj = 0; jj < 2; ++jj)
-        for (int i = 0; i < 4; ++i) {
-          const int b = (tile + jj) * 16 + 4 * (l >> 4) + i;
-          const uint32_t info = ob[jj][i];
-          const int tb = binfo_tix(info);
-          if (b <= last && tb >= t0 && tb < t0 + nt)
-            sh.samp[binfo_base(info) + y * binfo_tw(info) + x] = clip8((int)floorf(acc[jj][i] + 128.5f));
+        if (tix[jj] >= t0 && tix[jj] < t0 + nt) {
+          uint32_t word = 0;
+          for (int i = 0; i < 4; ++i) word |= (uint32_t)clip8((int)floorf(acc[jj][i] + 128.5f)) << (8 * i);
+          *reinterpret_cast<uint32_t*>(&sh.samp[binfo_base(info[jj]) + yo * binfo_tw(info[jj]) + xo]) = word;
         }
     }
   }

@@ -24,6 +24,8 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+
+#include <type_traits>
 #if !defined(__HIPCC__)
 #include <vector>
 #endif
@@ -186,7 +188,7 @@ struct PixShared {
 // write samples concurrently).
 struct PixSharedMfma {
   alignas(16) uint16_t coef[kPixBlocks][64];
-  uint8_t samp[kPixBlocks * 64];
+  alignas(16) uint8_t samp[kPixBlocks * 64];
   uint16_t q[3][64];
   uint32_t binfo[kPixBlocks];
   float cosf[64];
@@ -249,6 +251,28 @@ struct PixCtx {
   }
 };
 
+// LDS row of block b of the item (b in scan order).  The MFMA form keeps the
+// blocks in quant-table order (stable: the item's blocks of lower tables,
+// then this table's blocks of earlier MCUs, then its earlier slots in this
+// MCU), so that most 16-block tiles of the contraction use one table.
+template <class SH>
+CJD_HD inline int stage_row(const PixCtx& c, int b) {
+  if constexpr (std::is_same<SH, PixSharedMfma>::value) {
+    const int m = b / c.nslots, slot = b - m * c.nslots;
+    const int t = c.im->comp_tix[c.im->slot_comp[slot]];
+    int lower = 0, same = 0, before = 0;
+    for (int s2 = 0; s2 < c.nslots; ++s2) {
+      const int t2 = c.im->comp_tix[c.im->slot_comp[s2]];
+      lower += t2 < t ? 1 : 0;
+      same += t2 == t ? 1 : 0;
+      before += (t2 == t && s2 < slot) ? 1 : 0;
+    }
+    return lower * (int)c.it.count + m * same + before;
+  } else {
+    return b;
+  }
+}
+
 // The cosine table, once per workgroup.
 template <class SH>
 CJD_HD inline void pix_stage_cos(SH& sh, int tid) {
@@ -276,7 +300,7 @@ CJD_HD inline void pix_stage_zero(SH& sh, int tid, const PixCtx& c, const uint16
     const int k = slot - c.im->comp_first_slot[comp];
     const int sx = k % sxf, sy = k / sxf;
     const int tw = c.tw(comp);
-    sh.binfo[b] = binfo_pack(c.toff(comp) + sy * 8 * tw + (m * sxf + sx) * 8, tw, c.im->comp_tix[comp]);
+    sh.binfo[stage_row<SH>(c, b)] = binfo_pack(c.toff(comp) + sy * 8 * tw + (m * sxf + sx) * 8, tw, c.im->comp_tix[comp]);
   }
 }
 
@@ -292,7 +316,7 @@ CJD_HD inline void stage_group(SH& sh, int b, const u32x4& v, uint32_t k0, uint3
 }
 
 // pix_stage_blocks: lanes 2b and 2b+1 fill block b of the item (blocks are
-// the item's MCUs in scan order) from its record.  A block the serial decoder
+// the item's MCUs in scan order) from its record, into LDS row stage_row(b).  A block the serial decoder
 // wrote is copied from the dense buffer, four 16-byte pieces per lane.  For
 // the others the block's AC entries are consecutive entries of a tile stream:
 // 16-byte groups of 4, one stream row (4 W words) apart; lane h loads groups
@@ -308,6 +332,7 @@ CJD_HD inline void pix_stage_blocks(SH& sh, int tid, const PixCtx& c, const Bloc
   if (b >= c.nblk) return;
   const int64_t g = c.im->coef_base + (int64_t)c.it.mcu_first * c.nslots + b;
   const BlockRec r = recs[g];
+  const int row = stage_row<SH>(c, b);
   if (r.nw & kRecDense) {
     const int16_t* src = coefs + g * 64 + h * 32;
     u32x4 v0, v1, v2, v3;
@@ -315,25 +340,25 @@ CJD_HD inline void pix_stage_blocks(SH& sh, int tid, const PixCtx& c, const Bloc
     __builtin_memcpy(&v1, src + 8, 16);
     __builtin_memcpy(&v2, src + 16, 16);
     __builtin_memcpy(&v3, src + 24, 16);
-    stage_piece(sh, b * 8 + h * 4, v0);
-    stage_piece(sh, b * 8 + h * 4 + 1, v1);
-    stage_piece(sh, b * 8 + h * 4 + 2, v2);
-    stage_piece(sh, b * 8 + h * 4 + 3, v3);
+    stage_piece(sh, row * 8 + h * 4, v0);
+    stage_piece(sh, row * 8 + h * 4 + 1, v1);
+    stage_piece(sh, row * 8 + h * 4 + 2, v2);
+    stage_piece(sh, row * 8 + h * 4 + 3, v3);
     return;
   }
   const uint32_t n0 = r.nw & 255u, n = n0 < 64 ? n0 : 64, W = 16 * (1 + ((r.nw >> 8) & 3u));
   const uint32_t ph = r.eidx & 3u, ng = n ? (ph + n + 3) >> 2 : 0u;
-  const uint64_t g0 = r.eidx & ~3ull, row = 4ull * W;
-  if (ng && g0 + (ng - 1) * row + 4 <= ewords) {
+  const uint64_t g0 = r.eidx & ~3ull, gap = 4ull * W;  // groups are one stream row apart
+  if (ng && g0 + (ng - 1) * gap + 4 <= ewords) {
     for (uint32_t g = (uint32_t)h; g < ng; g += 4) {
       u32x4 v0, v1 = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-      __builtin_memcpy(&v0, ebuf + g0 + g * row, 16);
-      if (g + 2 < ng) __builtin_memcpy(&v1, ebuf + g0 + (g + 2) * row, 16);
-      stage_group(sh, b, v0, 4 * g, ph, n);
-      stage_group(sh, b, v1, 4 * (g + 2), ph, n);
+      __builtin_memcpy(&v0, ebuf + g0 + g * gap, 16);
+      if (g + 2 < ng) __builtin_memcpy(&v1, ebuf + g0 + (g + 2) * gap, 16);
+      stage_group(sh, row, v0, 4 * g, ph, n);
+      stage_group(sh, row, v1, 4 * (g + 2), ph, n);
     }
   }
-  if (h == 0) stage_coef(sh, b, 0, r.dc);
+  if (h == 0) stage_coef(sh, row, 0, r.dc);
 }
 
 // Row pass: lane -> (block, row).
@@ -518,13 +543,16 @@ CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* o
 //
 // Lane layout of v_mfma_f32_16x16x32_f16 (checked on the device): A[row][k]
 // lane l holds row l%16, k = 8*(l/16) + i; B[k][col] lane l holds col l%16,
-// the same k; C lane l holds col l%16, rows 4*(l/16) + i.  With K half h
-// (coefficients 32h..32h+31) and N tile n (pixels 16n..16n+15), lane l's B
-// element i is M_t[32h + 8(l/16) + i][16n + l%16].
+// the same k; C lane l holds col l%16, rows 4*(l/16) + i.  The contraction
+// runs as C[pixel][block] = M_t^T x coef^T: with K half h (coefficients
+// 32h..32h+31) and N tile n (pixels 16n..16n+15), lane l's A element i is
+// M_t[32h + 8(l/16) + i][16n + l%16] and its B element i is coefficient
+// 32h + 8(l/16) + i of block l%16; its 4 results are pixels 16n + 4(l/16) + i
+// of that block, 4 consecutive samples of one row.
 
-// B operand of lane l for quant table q (natural order), N tile n and K half
-// h: its 8 elements of M = (q[k] * Cu[u][x]) * Cv[v][y] in FP32, split into
-// FP16 hi and lo bit patterns.
+// Table operand (A) of lane l for quant table q (natural order), N tile n and
+// K half h: its 8 elements of M = (q[k] * Cu[u][x]) * Cv[v][y] in FP32, split
+// into FP16 hi and lo bit patterns.
 CJD_HD inline void mfma_b_lane(const uint16_t* q, const float* cosf, int n, int h, int l, uint16_t hb[8],
                                uint16_t lb[8]) {
   const int p = n * 16 + (l & 15), x = p & 7, y = p >> 3;
@@ -538,69 +566,77 @@ CJD_HD inline void mfma_b_lane(const uint16_t* q, const float* cosf, int n, int 
   }
 }
 
-// Write one contraction result (pixel p of block b) into its sample tile.
-CJD_HD inline void pix_put(PixSharedMfma& sh, int b, int p, float v) {
-  const uint32_t info = sh.binfo[b];
-  sh.samp[binfo_base(info) + (p >> 3) * binfo_tw(info) + (p & 7)] = clip8((int)floorf(v + 128.5f));
-}
-
 // Host emulation of the device MFMA phase, all table passes: the same
-// operands (A from the FP16 coefficients in LDS, B from mfma_b_lane) held by
-// the same lanes, with each 16x16x32 product summed in k order in FP32 (the
-// device's summation order inside one MFMA is the hardware's).  Checks the
-// operand layout, masking and output addressing on the CPU.
+// operands (the table operand from mfma_b_lane, the block operand from the
+// FP16 coefficients in LDS, blocks in table order) held by the same lanes,
+// the same tiles skipped and the same masking, with each 16x16x32 product
+// summed in k order in FP32 (the device's summation order inside one MFMA is
+// the hardware's), and each lane's 4 results written as one sample word.
+// Checks the operand layout, the table order and the output addressing on
+// the CPU.
 #if !defined(__HIPCC__)
 inline void pix_idct_mfma_emul(PixSharedMfma& sh, const PixCtx& c) {
   const int ntiles = (c.nblk + 15) >> 4;
+  const int last = c.nblk - 1;
   for (int t0 = 0; t0 < c.ntab; t0 += 2) {
     const int nt = c.ntab - t0 < 2 ? c.ntab - t0 : 2;
     for (int n = 0; n < 4; ++n) {  // the wave that owns N tile n
-      float B[2][2][2][64][8];     // [table][h][hi, lo][lane][element]
+      float A[2][2][2][64][8];     // [table][h][hi, lo][lane][element]
       for (int tl = 0; tl < nt; ++tl)
         for (int h = 0; h < 2; ++h)
           for (int l = 0; l < 64; ++l) {
             uint16_t hb[8], lb[8];
             mfma_b_lane(sh.q[t0 + tl], sh.cosf, n, h, l, hb, lb);
             for (int i = 0; i < 8; ++i) {
-              B[tl][h][0][l][i] = half_value(hb[i]);
-              B[tl][h][1][l][i] = half_value(lb[i]);
+              A[tl][h][0][l][i] = half_value(hb[i]);
+              A[tl][h][1][l][i] = half_value(lb[i]);
             }
           }
       for (int tile = 0; tile < ntiles; ++tile) {
-        float a[64][2][8];
-        int tix[64];
+        const int f = tile * 16;
+        // the pair's tables (the device takes tiles two at a time): of the
+        // first and last block of the pair, in table order
+        const int fp = f & ~31, ep = fp + 31 <= last ? fp + 31 : last;
+        const int tlo = binfo_tix(sh.binfo[fp]) - t0, thi = binfo_tix(sh.binfo[ep]) - t0;
+        float bv[64][2][8];
+        int tix[64], blk[64];
         for (int l = 0; l < 64; ++l) {
-          const int brow = tile * 16 + (l & 15);
-          tix[l] = brow < c.nblk ? binfo_tix(sh.binfo[brow]) : -1;
+          const int pj = f + (l & 15);
+          blk[l] = pj <= last ? pj : last;
+          tix[l] = pj <= last ? binfo_tix(sh.binfo[blk[l]]) : -1;
           for (int h = 0; h < 2; ++h)
-            for (int i = 0; i < 8; ++i)
-              a[l][h][i] = brow < c.nblk ? half_value(sh.coef[brow][h * 32 + 8 * (l >> 4) + i]) : 0.f;
+            for (int i = 0; i < 8; ++i) bv[l][h][i] = half_value(sh.coef[blk[l]][h * 32 + 8 * (l >> 4) + i]);
         }
         float acc[64][4] = {};
-        for (int tl = 0; tl < nt; ++tl)
+        for (int tl = 0; tl < nt; ++tl) {
+          // one table in the pair: only its pass, unmasked; else every pass, masked
+          const bool single = tlo == thi;
+          if (single && tl != tlo) continue;
           for (int h = 0; h < 2; ++h)
             for (int hl = 0; hl < 2; ++hl) {
-              float prod[16][16];
+              float prod[16][16];  // [pixel row][block col]
               for (int row = 0; row < 16; ++row)
                 for (int col = 0; col < 16; ++col) {
                   float s = 0.f;
                   for (int k = 0; k < 32; ++k) {
                     const int la = row + 16 * (k >> 3), lb = col + 16 * (k >> 3);
-                    const float av = tix[la] == t0 + tl ? a[la][h][k & 7] : 0.f;
-                    s += av * B[tl][h][hl][lb][k & 7];
+                    const float b = (single || tix[lb] == t0 + tl) ? bv[lb][h][k & 7] : 0.f;
+                    s += A[tl][h][hl][la][k & 7] * b;
                   }
                   prod[row][col] = s;
                 }
               for (int l = 0; l < 64; ++l)
                 for (int i = 0; i < 4; ++i) acc[l][i] += prod[4 * (l >> 4) + i][l & 15];
             }
-        for (int l = 0; l < 64; ++l)
-          for (int i = 0; i < 4; ++i) {
-            const int b = tile * 16 + 4 * (l >> 4) + i;
-            if (b >= c.nblk) continue;
-            const int tb = binfo_tix(sh.binfo[b]);
-            if (tb >= t0 && tb < t0 + nt) pix_put(sh, b, n * 16 + (l & 15), acc[l][i]);
-          }
+        }
+        for (int l = 0; l < 64; ++l) {
+          if (tix[l] < t0 || tix[l] >= t0 + nt) continue;
+          const uint32_t info = sh.binfo[blk[l]];
+          const int yo = 2 * n + (l >> 5), xo = 4 * ((l >> 4) & 1);
+          uint8_t* dst = &sh.samp[binfo_base(info) + yo * binfo_tw(info) + xo];
+          if ((binfo_base(info) + yo * binfo_tw(info) + xo) & 3) __builtin_trap();  // the device writes a word
+          for (int i = 0; i < 4; ++i) dst[i] = clip8((int)floorf(acc[l][i] + 128.5f));
+        }
       }
     }
   }
