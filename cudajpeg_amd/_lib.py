@@ -20,7 +20,7 @@ OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 # Every symbol include/cudajpeg_amd.h and include/format.h declare.
 EXPORTS = [
     "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
-    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
+    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_overlap", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
     "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_finish", "cjd_batch_capture",
     "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_kernel_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
     "cjd_batch_download", "cjd_batch_totals", "cjd_error_name",
@@ -91,6 +91,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_destroy": ([vp], None),
         "cjd_batch_set_options": ([vp, i32, i32, i32, i32], i32),
         "cjd_batch_set_tuning": ([vp, i32, i32], i32),
+        "cjd_batch_set_overlap": ([vp, i32], i32),
         "cjd_batch_set_tune_table": ([vp, i32, ctypes.POINTER(TuneEntry)], i32),
         "cjd_batch_stage": ([vp, i32, vp, vp, vp], i32),
         "cjd_batch_stage_async": ([vp, i32, vp, vp, vp, vp], i32),

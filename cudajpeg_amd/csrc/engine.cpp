@@ -76,14 +76,26 @@ struct cjd_batch {
   std::vector<uint32_t> h_status;  // status bits of the last decode (host)
   int64_t fallbacks = 0;
 
-  // timed decodes: 0 start, 3 before and 4 after the entropy kernel, 1 after
-  // the entropy stage, 2 after the pixels
+  // Two-part decode (cjd_batch_set_overlap): the entropy decode of work
+  // items [0, split_item) -- whole images -- then of the rest on the decode
+  // stream, while the pixel kernel of those images' pixel items [0,
+  // split_pix) runs on `aux`; 0 = no split for this batch.
+  bool overlap = true;
+  uint32_t split_item = 0, split_pix = 0;
+  hipStream_t aux = nullptr;
+  hipEvent_t ex[2] = {nullptr, nullptr};  // first entropy part done; first pixel part done
+
+  // timed decodes: 0 start, 3 before and 4 after the entropy kernel(s), 1
+  // after the entropy stage, 2 after the pixels
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timed = false;
 
   ~cjd_batch() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : ex)
+      if (e) (void)hipEventDestroy(e);
+    if (aux) (void)hipStreamDestroy(aux);
   }
 };
 
@@ -161,6 +173,36 @@ static int upload(DevBuf& d, const std::vector<T>& v, hipStream_t s) {
   return 0;
 }
 
+// Split point of the two-part decode: the first image boundary at or after
+// half of the work items, where both work lists are in image order (else,
+// or for small batches, no split).
+static void plan_split(cjd_batch* b) {
+  const Plan& p = b->plan;
+  b->split_item = b->split_pix = 0;
+  const size_t ni = p.items.size(), np = p.pix.size();
+  if (ni < 64 || np < 2) return;
+  auto img = [&](size_t i) { return p.segs[p.items[i].segment].image; };
+  for (size_t i = 1; i < ni; ++i)
+    if (img(i) < img(i - 1)) return;
+  for (size_t i = 1; i < np; ++i)
+    if (p.pix[i].image < p.pix[i - 1].image) return;
+  size_t m = ni / 2;
+  while (m < ni && img(m) == img(m - 1)) ++m;
+  if (m >= ni) return;
+  const uint32_t d = img(m);
+  size_t q = 0;
+  while (q < np && p.pix[q].image < d) ++q;
+  if (q == 0 || q >= np) return;
+  b->split_item = (uint32_t)m;
+  b->split_pix = (uint32_t)q;
+}
+
+extern "C" int cjd_batch_set_overlap(cjd_batch* b, int on) {
+  if (!b) return CJD_PROGRAMMER_ERROR;
+  b->overlap = on != 0;
+  return 0;
+}
+
 extern "C" int cjd_batch_stage(cjd_batch* b, int n, const uint8_t* const* files, const size_t* sizes, int* errors) {
   const int e = cjd_batch_stage_async(b, n, files, sizes, errors, nullptr);
   if (e) return e;
@@ -206,6 +248,7 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
       e = CJD_KERNEL_LAUNCH_ERROR;
   }
   b->h_status.assign(b->status_words, 0);
+  plan_split(b);
   return e;
 }
 
@@ -246,7 +289,7 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
                                b->d_recs.as<BlockRec>(), status, epoch, s);
   else {
     he = launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
-                        b->d_items.as<WorkItem>(), nitems, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
+                        b->d_items.as<WorkItem>(), 0, nitems, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
                         b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
                         reinterpret_cast<unsigned long long*>(scratch), b->ticket_base, epoch, s);
     b->ticket_base += nitems;
@@ -272,7 +315,58 @@ static int enqueue_pixel(cjd_batch* b, hipStream_t s, bool timed) {
   return 0;
 }
 
+// Two-part decode on s: entropy of the first images, [then] entropy of the
+// rest while `aux` runs the first images' pixel kernel, then the rest's pixel
+// kernel; s waits for aux at the end, so the decode is complete when s is.
+// Both entropy launches share the epoch (their images, work items and
+// records are disjoint) and take consecutive tickets.
+static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed) {
+  if (!b->aux) {
+    if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess) {
+      b->aux = nullptr;
+      return CJD_DEVICE_MEM_ERROR;
+    }
+    for (auto& e : b->ex)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+  }
+  uint32_t* scratch = b->d_scratch.as<uint32_t>();
+  uint32_t* status = status_of(b);
+  LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
+  const uint32_t nitems = (uint32_t)b->plan.items.size(), m = b->split_item;
+  const uint32_t npix = (uint32_t)b->plan.pix.size(), q = b->split_pix;
+  if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  uint32_t epoch = 0;
+  if (int e = next_epoch(b, s, &epoch)) return e;
+  if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  b->coefs_stale = true;
+  b->last_stream = s;
+  const uint32_t bounds[3] = {0, m, nitems};
+  for (int part = 0; part < 2; ++part) {
+    const hipError_t he =
+        launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
+                       b->d_items.as<WorkItem>(), bounds[part], bounds[part + 1], b->d_tables.as<HuffTable>(),
+                       b->d_luts.as<uint16_t>(), b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
+                       reinterpret_cast<unsigned long long*>(scratch), b->ticket_base, epoch, s);
+    b->ticket_base += bounds[part + 1] - bounds[part];
+    if (he != hipSuccess) {
+      b->ticket_unknown = true;
+      return CJD_KERNEL_LAUNCH_ERROR;
+    }
+    if (part == 0 && hipEventRecord(b->ex[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  }
+  if (timed && (hipEventRecord(b->ev[4], s) != hipSuccess || hipEventRecord(b->ev[1], s) != hipSuccess))
+    return CJD_KERNEL_LAUNCH_ERROR;
+  if (hipStreamWaitEvent(b->aux, b->ex[0], 0) != hipSuccess || pixel_range(b, 0, q, b->aux) != hipSuccess ||
+      hipEventRecord(b->ex[1], b->aux) != hipSuccess)
+    return CJD_KERNEL_LAUNCH_ERROR;
+  if (pixel_range(b, q, npix - q, s) != hipSuccess || hipStreamWaitEvent(s, b->ex[1], 0) != hipSuccess)
+    return CJD_KERNEL_LAUNCH_ERROR;
+  if (timed && hipEventRecord(b->ev[2], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  return 0;
+}
+
 static int enqueue(cjd_batch* b, hipStream_t s, bool timed) {
+  if (b->overlap && b->split_item && b->entropy_mode == CJD_ENTROPY_TILED) return enqueue_split(b, s, timed);
   int e = enqueue_entropy(b, s, timed);
   return e ? e : enqueue_pixel(b, s, timed);
 }

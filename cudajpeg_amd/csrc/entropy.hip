@@ -91,8 +91,8 @@ __device__ __forceinline__ void scan_counts(EntShared& sh, int tid, const EntCtx
 __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ words,
                                                  const ImageDesc* __restrict__ images,
                                                  const SegmentDesc* __restrict__ segs,
-                                                 const WorkItem* __restrict__ items, uint32_t nitems,
-                                                 const HuffTable* __restrict__ tables,
+                                                 const WorkItem* __restrict__ items, uint32_t item0,
+                                                 uint32_t item_end, const HuffTable* __restrict__ tables,
                                                  const uint16_t* __restrict__ luts, uint32_t* __restrict__ ebuf,
                                                  BlockRec* __restrict__ recs, uint32_t* __restrict__ status,
                                                  LookbackRec* __restrict__ lb,
@@ -102,14 +102,15 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   const int tid = threadIdx.x;
   // Work items are taken in ticket order, so every predecessor an item waits
   // for in the lookback has already started.  The ticket counter is never
-  // cleared: each launch takes nitems tickets from ticket_base on.
+  // cleared: a launch over items [item0, item_end) takes that many tickets
+  // from ticket_base on.
   if (tid == 0) {
-    sh.item = (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
+    sh.item = item0 + (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
     sh.lovf = 0;
   }
   __syncthreads();
   const uint32_t item_id = sh.item;
-  if (item_id >= nitems) return;
+  if (item_id >= item_end) return;
   EntCtx c;
   c.init(words, images, segs, tables, ebuf, items[item_id]);
   {
@@ -263,12 +264,12 @@ hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s) {
 }
 
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
-                          const WorkItem* items, uint32_t nitems, const HuffTable* tables, const uint16_t* luts,
-                          uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb,
+                          const WorkItem* items, uint32_t item0, uint32_t item_end, const HuffTable* tables,
+                          const uint16_t* luts, uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb,
                           unsigned long long* ticket, uint64_t ticket_base, uint32_t epoch, hipStream_t s) {
-  if (!nitems) return hipSuccess;
-  hipLaunchKernelGGL(k_entropy, dim3(nitems), dim3(kWG), 0, s, words, images, segs, items, nitems, tables, luts, ebuf,
-                     recs, status, lb, ticket, ticket_base, epoch);
+  if (item_end <= item0) return hipSuccess;
+  hipLaunchKernelGGL(k_entropy, dim3(item_end - item0), dim3(kWG), 0, s, words, images, segs, items, item0, item_end,
+                     tables, luts, ebuf, recs, status, lb, ticket, ticket_base, epoch);
   return hipGetLastError();
 }
 

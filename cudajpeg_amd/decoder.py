@@ -97,6 +97,11 @@ class BatchDecoder:
         work item; 0 = defaults; applies from the next stage()."""
         _check(L.lib().cjd_batch_set_tuning(self.h, ck_bits, item_tiles), "tuning: ")
 
+    def set_overlap(self, on: bool) -> None:
+        """Two-part decode (default on): the first images' pixel kernel runs
+        beside the entropy decode of the rest, on a second stream."""
+        _check(L.lib().cjd_batch_set_overlap(self.h, int(bool(on))), "overlap: ")
+
     def set_tune_table(self, table) -> None:
         """Per-image entropy parameters by scan size: a list of
         (max_scan_bytes, tile_bits, ck_bits, item_tiles)."""
