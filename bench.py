@@ -6,10 +6,10 @@ quality U{60..95}, noise U[2,12] -- with the scans already resident in HBM
 and the RGB/gray frames left in HBM.
 
 Reported:
-  value              batched form, Mpixel/s (W*H): each decode is the
-                     two-part decode (entropy of the first images, then of
-                     the rest beside the first images' pixel kernel on a
-                     second stream, then the rest's pixel kernel); --in-flight
+  value              batched form, Mpixel/s (W*H): each decode runs in
+                     --parts runs of images (2): each run's pixel kernel on a
+                     second stream beside the next runs' entropy decodes,
+                     the last run's after its own; --in-flight
                      N > 1 lets N decoders of the batch take the steps in turn
                      on their own streams (off by default)
   kernel_ms          device time per kernel (HIP events on the decode stream,
@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--item-tiles", type=int, default=128, help="tiles per entropy work item (0 = library default)")
     ap.add_argument("--ck-bits", type=int, default=0, help="checkpoint spacing of the tiled decoder (0 = default)")
     ap.add_argument("--batched-idct", default="mfma", choices=["int", "float", "mfma"])
+    ap.add_argument("--parts", type=int, default=2,
+                    help="decode in parts: each run's pixel kernel beside the next runs' entropy (1 = off)")
     ap.add_argument("--in-flight", type=int, default=1,
                     help="decoders of the batch taking the timed steps in turn, one stream each")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -209,6 +211,7 @@ def main():
     idct = {"int": IDCT_INT, "float": IDCT_FLOAT, "mfma": IDCT_MFMA}[args.batched_idct]
     # ---------------------------------------------------------------- batched
     b = BatchDecoder(idct, tile_bits=args.tile_bits)
+    b.set_overlap(args.parts)
     if args.item_tiles or args.ck_bits:
         b.set_tuning(args.ck_bits, args.item_tiles)
     check(b.stage(files), "stage")
@@ -242,6 +245,7 @@ def main():
         decs, streams = [b], [stream]
         for _ in range(args.in_flight - 1):
             d = BatchDecoder(idct, tile_bits=args.tile_bits)
+            d.set_overlap(args.parts)
             if args.item_tiles or args.ck_bits:
                 d.set_tuning(args.ck_bits, args.item_tiles)
             check(d.stage(files), f"stage of decoder {len(decs)}")
@@ -264,7 +268,7 @@ def main():
         # per-kernel device times (events on the decode stream), averaged,
         # with the two kernels one after the other so that each time is the
         # kernel's own
-        b.set_overlap(False)
+        b.set_overlap(1)
         ent, pix, kent, kexp = [], [], [], []
         for _ in range(args.steps):
             b.decode(stream)
@@ -275,7 +279,7 @@ def main():
             kent.append(ka)
             kexp.append(kx)
         check(b.finish(stream), f"timed steps, decoder 0 (fallbacks {b.fallbacks()})")
-        b.set_overlap(True)
+        b.set_overlap(args.parts)
     batched_mpix = total_px / t_batched / 1e6
     # frames back to host memory (the reader API's pixels buffer)
     t1 = time.perf_counter()
@@ -366,7 +370,7 @@ def main():
                    "images": args.images, "mpix": round(total_px / 1e6, 1),
                    "scan_mb": round(tot["scan_bytes"] / 1e6, 1), "blocks": tot["blocks"],
                    "batched_idct": args.batched_idct, "plain_idct": "float", "parallelism": "1 GPU",
-                   "launch": "direct, two-part", "in_flight": n_fl,
+                   "launch": "direct", "parts": args.parts, "in_flight": n_fl,
                    "tile_bits": args.tile_bits or 4096, "item_tiles": args.item_tiles or 256},
         "pcie_inclusive": {"stage_ms": round(stage_ms, 2), "download_ms": round(d2h_ms, 2),
                            "mpix_s_with_stage": round(total_px / ((stage_ms + t_batched * 1e3) * 1e-3) / 1e6, 1),

@@ -97,10 +97,10 @@ class BatchDecoder:
         work item; 0 = defaults; applies from the next stage()."""
         _check(L.lib().cjd_batch_set_tuning(self.h, ck_bits, item_tiles), "tuning: ")
 
-    def set_overlap(self, on: bool) -> None:
-        """Two-part decode (default on): the first images' pixel kernel runs
-        beside the entropy decode of the rest, on a second stream."""
-        _check(L.lib().cjd_batch_set_overlap(self.h, int(bool(on))), "overlap: ")
+    def set_overlap(self, parts: int = 2) -> None:
+        """Decode in parts (default 2; 1 = off): each run's pixel kernel runs
+        beside the next runs' entropy decodes, on a second stream."""
+        _check(L.lib().cjd_batch_set_overlap(self.h, int(parts)), "overlap: ")
 
     def set_tune_table(self, table) -> None:
         """Per-image entropy parameters by scan size: a list of

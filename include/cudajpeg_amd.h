@@ -97,16 +97,18 @@ int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int til
  * cjd_batch_stage. */
 int cjd_batch_set_tuning(cjd_batch *b, int ck_bits, int item_tiles);
 
-/* Two-part decode (no counterpart in the reference; on by default): for a
- * batch of at least 64 entropy work items, cjd_batch_decode decodes the
- * entropy of the first images (about half the work items), then that of the
- * rest while the first images' pixel kernel runs on a second stream of the
- * batch, then the rest's pixel kernel; the decode stream waits for the second
- * stream, so the decode is complete when its stream is.  Results are the
- * same either way.  on = 0 runs the two kernels one after the other (the
- * per-kernel timings of cjd_batch_kernel_timings are then each kernel's
- * own). */
-int cjd_batch_set_overlap(cjd_batch *b, int on);
+/* Decode in parts (no counterpart in the reference; 2 parts by default): for
+ * a batch of at least 64 entropy work items, cjd_batch_decode cuts the images
+ * into `parts` runs of about equal work (whole images) and launches the runs'
+ * entropy decodes one after another on the decode stream; the pixel kernel of
+ * every run but the last runs on a second stream of the batch as soon as its
+ * run's entropy is done, beside the next runs' entropy decodes.  The decode
+ * stream waits for the second stream, so the decode is complete when its
+ * stream is.  Results are the same for every value.  parts = 0 or 1 runs the
+ * two kernels one after the other (the per-kernel timings of
+ * cjd_batch_kernel_timings are then each kernel's own); at most 4.  Takes
+ * effect at once for the staged batch. */
+int cjd_batch_set_overlap(cjd_batch *b, int parts);
 
 /* Per-image tuning by scan size (no counterpart in the reference): an image
  * whose entropy-coded scan has at most max_scan_bytes bytes uses the first

@@ -1,7 +1,7 @@
-"""GPU test of the two-part decode (cjd_batch_set_overlap): the pixel kernel
-of the first images runs on a second stream beside the entropy decode of the
-rest.  Frames and coefficients must equal those of the two kernels run one
-after the other, decode after decode."""
+"""GPU test of the decode in parts (cjd_batch_set_overlap): the pixel kernel
+of each run of images but the last runs on a second stream beside the next
+runs' entropy decodes.  Frames and coefficients must equal those of the two
+kernels run one after the other, decode after decode."""
 import numpy as np
 import pytest
 
@@ -17,21 +17,30 @@ def files():
 
 
 @pytest.mark.parametrize("idct", [IDCT_INT, IDCT_MFMA])
-def test_two_part_decode_equals_one_after_the_other(gpu, files, idct):
+def test_decode_in_parts_equals_one_after_the_other(gpu, files, idct):
     got = []
-    for on in (False, True):
+    for parts in (1, 2, 3, 4):
         b = BatchDecoder(idct, tile_bits=8192)
         b.set_tuning(0, 128)
-        b.set_overlap(on)
         assert not b.stage(files).any()
+        b.set_overlap(parts)
         for _ in range(3):
             b.decode()
         errs = b.finish()
-        print("overlap", on, "errors", errs.tolist(), "fallbacks", b.fallbacks(), flush=True)
+        print("parts", parts, "errors", errs.tolist(), "fallbacks", b.fallbacks(), flush=True)
         assert not errs.any()
         got.append(([b.pixels(i) for i in range(len(files))], [b.coefficients(i) for i in range(len(files))]))
         b.close()
-    (p0, c0), (p1, c1) = got
-    for i in range(len(files)):
-        assert np.array_equal(p0[i], p1[i]), i
-        assert np.array_equal(c0[i], c1[i]), i
+    p0, c0 = got[0]
+    for parts, (p1, c1) in zip((2, 3, 4), got[1:]):
+        for i in range(len(files)):
+            assert np.array_equal(p0[i], p1[i]), (parts, i)
+            assert np.array_equal(c0[i], c1[i]), (parts, i)
+
+
+def test_overlap_parts_out_of_range_is_refused(gpu):
+    from cudajpeg_amd import DecodeError
+    b = BatchDecoder()
+    with pytest.raises(DecodeError):
+        b.set_overlap(5)
+    b.close()
