@@ -19,7 +19,9 @@ Reported:
                      FETCH_SIZE/WRITE_SIZE summary of this workload and these
                      kernels, null when there is none
   cpu_baseline       the reference CPUonly decoder (built from its sources
-                     into oracle/_ref) on a bounded sample, one host core
+                     into oracle/_ref) on a bounded sample, one host core;
+                     all_cores: the whole batch over --cpu-workers (16)
+                     processes started together
 With --plain (off by default):
   plain_mpix_s       plain form: one image per launch, FP32 VALU iDCT, with
                      per-size entropy parameters from a run-time autotune
@@ -95,6 +97,76 @@ def cpu_baseline(encs, budget_s: float):
                       f"file in page cache, {done_t:.1f} s total, {done_px / 1e6:.0f} Mpix"}
 
 
+_CPU_WORKER = r"""
+import os, sys, time
+sys.path.insert(0, sys.argv[1])
+import oracle
+go, out, reps, paths = sys.argv[2], sys.argv[3], int(sys.argv[4]), sys.argv[5:]
+open(out + ".ready", "w").close()
+t_end = time.time() + 120
+while not os.path.exists(go):
+    if time.time() > t_end:
+        sys.exit(3)
+    time.sleep(0.001)
+t0 = time.perf_counter()
+oracle.ref_cpuonly_time(paths, reps)
+with open(out, "w") as f:
+    f.write(repr(time.perf_counter() - t0))
+"""
+
+
+def cpu_all_cores(encs, workers: int, reps: int):
+    """The reference CPUonly on every image of the batch (restart-interval
+    files aside), `reps` times each, split over `workers` independent
+    processes (greedy by pixels) that start together; Mpix/s over the
+    slowest process's wall time."""
+    import subprocess
+    import oracle
+    if not os.path.exists(os.path.join(oracle.REF_DIR, "libcpuonly.so")):
+        return None
+    tmp = tempfile.mkdtemp(prefix="cjd_bench_mc_")
+    parts = [[] for _ in range(workers)]
+    load = [0] * workers
+    total_px = 0
+    for i in sorted(range(len(encs)), key=lambda i: -encs[i].spec.pixels):
+        e = encs[i]
+        if e.spec.restart_interval:
+            continue
+        path = os.path.join(tmp, f"{i}.jpg")
+        with open(path, "wb") as f:
+            f.write(e.data)
+        k = load.index(min(load))
+        parts[k].append(path)
+        load[k] += e.spec.pixels
+        total_px += e.spec.pixels * reps
+    go = os.path.join(tmp, "go")
+    root = os.path.dirname(os.path.abspath(__file__))
+    procs = []
+    for k, ps in enumerate(parts):
+        if ps:
+            out = os.path.join(tmp, f"w{k}")
+            procs.append((out, subprocess.Popen([sys.executable, "-c", _CPU_WORKER, root, go, out, str(reps)] + ps)))
+    t_end = time.time() + 120
+    while not all(os.path.exists(o + ".ready") for o, _ in procs):
+        if time.time() > t_end or any(p.poll() not in (None, 0) for _, p in procs):
+            for _, p in procs:
+                p.kill()
+            raise SystemExit("bench.py: CPU workers did not start")
+        time.sleep(0.01)
+    open(go, "w").close()
+    for _, p in procs:
+        if p.wait(timeout=300) != 0:
+            raise SystemExit("bench.py: a CPU worker failed")
+    wall = max(float(open(o).read()) for o, _ in procs)
+    for name in os.listdir(tmp):
+        os.unlink(os.path.join(tmp, name))
+    os.rmdir(tmp)
+    return {"value": round(total_px / wall / 1e6, 2), "unit": "Mpix/s", "cores": len(procs),
+            "sample": f"the batch's {sum(len(ps) for ps in parts)} images without restart intervals, "
+                      f"{reps} decodes each, over {len(procs)} processes of the reference CPUonly "
+                      f"started together, {wall:.2f} s wall"}
+
+
 def serve_bench(args):
     """Serving loop on the request mix: the mixed-batch distribution plus 5%
     8192x8192 4:2:0 q85 images.  Peak = a burst of requests decoded flat out;
@@ -167,6 +239,8 @@ def main():
                     help="decoders of the batch taking the timed steps in turn, one stream each")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="processes for the all-cores CPU baseline (the host's share of cores; 1 = off)")
     ap.add_argument("--plain", action="store_true",
                     help="also run the plain form (autotune, then one image per launch) and its latencies")
     ap.add_argument("--serve", action="store_true", help="also run the serving-loop load test")
@@ -349,6 +423,10 @@ def main():
         log(f"serving: {serving}")
 
     cpu = None if args.no_cpu else cpu_baseline(encs, args.cpu_seconds)
+    if cpu is not None and args.cpu_workers > 1:
+        cpu["all_cores"] = cpu_all_cores(encs, args.cpu_workers, 4)
+        if cpu["all_cores"]:
+            log(f"cpu baseline, {cpu['all_cores']['cores']} processes: {cpu['all_cores']['value']} Mpix/s")
     if cpu:
         log(f"cpu baseline: {cpu}")
 
