@@ -13,7 +13,9 @@ namespace cjd {
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kPixItemsPerWG = 4;  // consecutive items per workgroup
+// Consecutive items per workgroup: 1 measured fastest (many small workgroups
+// interleave best with an entropy launch running beside them).
+constexpr int kPixItemsPerWG = 1;
 
 // B operands of one wave: [table slot][K half], hi and lo parts.
 struct MfmaB {
