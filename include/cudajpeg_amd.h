@@ -138,10 +138,11 @@ int cjd_batch_stage_async(cjd_batch *b, int n, const uint8_t *const *files, cons
 /* Enqueue the decode of the staged batch on `stream` (asynchronous):
  * entropy decode (replaces decodeScanGPU, src/decodeScanGPU.cu:123) and the
  * fused pixel stage (replaces iDCT_resample_colourTransform,
- * src/pixelTransformGPU.cu:621).  Two launches, nothing else: the batch's
- * scratch (work-item ticket counter, per-image status, lookback records) is
- * not cleared between decodes, since every entropy launch tags what it writes
- * there with an epoch of its own.  Decodes of one batch are ordered by the
+ * src/pixelTransformGPU.cu:621).  The entropy and pixel launches alone (in
+ * parts, see cjd_batch_set_overlap): the batch's scratch (work-item ticket
+ * counter, per-image status, lookback records) is not cleared between
+ * decodes, since every entropy launch tags what it writes there with an
+ * epoch of its own.  Decodes of one batch are ordered by the
  * caller (one stream, or events); different batches are independent. */
 int cjd_batch_decode(cjd_batch *b, void *stream);
 
