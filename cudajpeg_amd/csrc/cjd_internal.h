@@ -171,6 +171,8 @@ enum : uint32_t {
 };
 
 enum IdctMode : int { kIdctInt = 0, kIdctFloat = 1, kIdctMfma = 2 };
+// 16-byte MFMA table operands per image: 3 tables x 2 K halves x (hi, lo) x 256 lanes.
+constexpr int kMfmaTabVecs = 3 * 2 * 2 * 256;
 
 // One pixel workgroup: `count` consecutive MCUs of one MCU row of an image.
 struct PixItem {

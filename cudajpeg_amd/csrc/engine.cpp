@@ -63,6 +63,7 @@ struct cjd_batch {
   Plan plan;
 
   DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_luts, d_qtabs, d_coefs, d_out, d_scratch;
+  DevBuf d_btab;  // MFMA table operands per image (k_mfma_tables at staging)
   DevBuf d_ebuf, d_recs;  // tiled decoder: entry regions and block records
   // The pixel kernel reads the block records; the dense coefficient buffer
   // is filled from them (k_expand) only when a caller asks for it.
@@ -249,6 +250,10 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   if (!e) e = b->d_ebuf.ensure((size_t)b->plan.entry_words * 4);
   if (!e) e = b->d_recs.ensure((size_t)b->plan.total_blocks * sizeof(BlockRec));
   if (!e) e = upload(b->d_qtabs, b->plan.qtabs, s);
+  if (!e) e = b->d_btab.ensure(b->plan.images.size() * (size_t)kMfmaTabVecs * 16);
+  if (!e && launch_mfma_tables(b->d_images.as<ImageDesc>(), (uint32_t)b->plan.images.size(), b->d_qtabs.as<uint16_t>(),
+                               b->d_btab.as<uint4>(), s) != hipSuccess)
+    e = CJD_KERNEL_LAUNCH_ERROR;
   if (!e) e = b->d_coefs.ensure((size_t)b->plan.total_blocks * 128);
   if (!e) e = b->d_out.ensure(b->plan.out_bytes);
   // scratch (layout at cjd_batch): a new allocation is cleared whole and
@@ -328,7 +333,8 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
 static hipError_t pixel_range(cjd_batch* b, uint32_t first, uint32_t count, hipStream_t s) {
   return launch_pixel(b->idct_mode, b->d_images.as<ImageDesc>(), b->d_pix.as<PixItem>() + first, count,
                       b->d_recs.as<BlockRec>(), b->d_ebuf.as<uint32_t>(), (uint64_t)b->plan.entry_words,
-                      b->d_coefs.as<int16_t>(), b->d_qtabs.as<uint16_t>(), b->d_out.as<uint8_t>(), s);
+                      b->d_coefs.as<int16_t>(), b->d_qtabs.as<uint16_t>(), b->d_btab.as<uint4>(),
+                      b->d_out.as<uint8_t>(), s);
 }
 
 static int enqueue_pixel(cjd_batch* b, hipStream_t s, bool timed) {

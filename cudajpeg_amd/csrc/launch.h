@@ -17,8 +17,11 @@ hipError_t launch_expand(const BlockRec* recs, const uint32_t* ebuf, uint64_t ew
 hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                                  uint32_t nsegs, const HuffTable* tables, int16_t* coefs, BlockRec* recs,
                                  uint32_t* status, uint32_t epoch, hipStream_t s);
+// MFMA table operands of every image (kMfmaTabVecs uint4 per image).
+hipError_t launch_mfma_tables(const ImageDesc* images, uint32_t nimages, const uint16_t* qtabs, uint4* btab,
+                              hipStream_t s);
 hipError_t launch_pixel(int mode, const ImageDesc* images, const PixItem* items, uint32_t nitems,
                         const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, const int16_t* coefs,
-                        const uint16_t* qtabs, uint8_t* out, hipStream_t s);
+                        const uint16_t* qtabs, const uint4* btab, uint8_t* out, hipStream_t s);
 
 }  // namespace cjd
