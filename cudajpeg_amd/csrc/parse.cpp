@@ -99,6 +99,9 @@ void parse_dht(ParseState& st) {
       if (tuple + count > block_end) PFAIL(CJD_SYNTAX_ERROR);
       remain -= count << (16 - l);
       if (remain < 0) PFAIL(CJD_SYNTAX_ERROR);
+      // HUFFVAL holds at most 256 symbols (T.81 B.2.4.2); the Kraft sum
+      // alone admits up to 2040 codes of lengths 9..16
+      if (n + count > 256) PFAIL(CJD_SYNTAX_ERROR);
       for (int i = 0; i < count; ++i) vals[n++] = *tuple++;
     }
     memcpy(f.huff_bits[id], bits, 16);

@@ -110,6 +110,7 @@ static int parse_dht(orc_jpeg *j, const uint8_t *pos, const uint8_t *end) {
       if (tuple + count > block_end) FAIL(ORC_SYNTAX_ERROR);
       remain -= count << (16 - l);
       if (remain < 0) FAIL(ORC_SYNTAX_ERROR);
+      if (n + count > 256) FAIL(ORC_SYNTAX_ERROR); /* HUFFVAL holds at most 256 symbols */
       for (int i = 0; i < count; i++) j->dht_vals[id][n++] = *tuple++;
     }
     j->dht_present[id] = 1;

@@ -123,3 +123,71 @@ def test_mixed_distribution_shape():
     assert areas.min() >= 200 * 200 and areas.max() <= 4200 * 3200
     frac420 = np.mean([s.sampling == "420" for s in specs])
     assert 0.55 < frac420 < 0.85
+
+
+def _oversized_dht(data):
+    """`data` with its first DHT replaced by one whose Kraft sum holds (255
+    codes at each length 9..16) but which has 2040 symbols, more than the 256
+    a HUFFVAL table can hold."""
+    for i, m, L in _segments(data):
+        if m == 0xC4:
+            body = bytes([0x00]) + bytes([0] * 8 + [255] * 8) + (bytes(range(256)) * 8)[:2040]
+            seg = b"\xff\xc4" + (2 + len(body)).to_bytes(2, "big") + body
+            return data[:i] + seg + data[i + 2 + L:]
+    raise AssertionError("no DHT")
+
+
+def test_dht_with_more_than_256_symbols_is_a_syntax_error():
+    bad = _oversized_dht(BASE)
+    e, _ = _frame(bad)
+    assert e == 1
+    assert oracle.parse(bad).error == 1
+
+
+def test_parser_under_address_sanitizer(tmp_path):
+    """parse.cpp built for the host with AddressSanitizer parses the
+    oversized DHT, the malformed headers above and a valid file without an
+    out-of-bounds access."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("no host C++ compiler")
+    drv = tmp_path / "drv.cpp"
+    drv.write_text(r'''
+#include <stdio.h>
+#include <stdlib.h>
+#include <vector>
+#include "parse_internal.h"
+int main(int argc, char** argv) {
+  for (int a = 1; a < argc; ++a) {
+    FILE* f = fopen(argv[a], "rb");
+    std::vector<unsigned char> d;
+    int c;
+    while ((c = fgetc(f)) != EOF) d.push_back((unsigned char)c);
+    fclose(f);
+    cjd::ParseState st;
+    cjd_frame_info fi;
+    printf("%d\n", cjd::parse_until_scan(st, d.data(), d.size(), &fi));
+  }
+  return 0;
+}
+''')
+    csrc = os.path.join(ROOT, "cudajpeg_amd", "csrc")
+    exe = tmp_path / "drv"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
+                    "-I" + csrc, str(drv), os.path.join(csrc, "parse.cpp"), "-o", str(exe)], check=True)
+    cases = [(_oversized_dht(BASE), 1), (BASE, 0), (BASE[:-2] + b"\x00\x00", 5),
+             (_patch(BASE, 0xC4, 0, 0x20), 1), (BASE[:200], None)]
+    paths = []
+    for k, (d, _) in enumerate(cases):
+        p = tmp_path / f"c{k}.jpg"
+        p.write_bytes(d)
+        paths.append(str(p))
+    r = subprocess.run([str(exe)] + paths, capture_output=True, text=True,
+                       env={**os.environ, "ASAN_OPTIONS": "detect_leaks=0"})
+    assert r.returncode == 0, r.stderr[-2000:]
+    codes = [int(x) for x in r.stdout.split()]
+    assert len(codes) == len(cases)
+    for (d, want), got in zip(cases, codes):
+        if want is not None:
+            assert got == want
