@@ -21,7 +21,8 @@ OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 EXPORTS = [
     "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
     "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_overlap", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
-    "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_finish", "cjd_batch_capture",
+    "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_set_wait_limit", "cjd_batch_protocol_redos",
+    "cjd_batch_finish", "cjd_batch_capture",
     "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_kernel_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
     "cjd_batch_download", "cjd_batch_totals", "cjd_error_name",
     "cjd_server_create", "cjd_server_submit", "cjd_server_poll", "cjd_server_release", "cjd_server_now_ns",
@@ -99,6 +100,8 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_decode_scan": ([vp, vp], i32),
         "cjd_batch_transform": ([vp, vp], i32),
         "cjd_batch_fallbacks": ([vp], i32),
+        "cjd_batch_set_wait_limit": ([vp, ctypes.c_int64], i32),
+        "cjd_batch_protocol_redos": ([vp], i32),
         "cjd_batch_finish": ([vp, vp, vp], i32),
         "cjd_batch_capture": ([vp, vp], i32),
         "cjd_batch_replay": ([vp, vp], i32),

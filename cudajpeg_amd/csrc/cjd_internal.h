@@ -143,9 +143,12 @@ constexpr uint16_t kRecDense = 0x8000;
 // clear them between decodes: a record or status word of an older launch
 // simply does not match.  The status word of a record is
 // lookback_word(epoch, 1) once its aggregate is published and
-// lookback_word(epoch, 2) with its inclusive state (written last).
+// lookback_word(epoch, 2) with its inclusive state (written last), or
+// lookback_word(epoch, 3) when the item gave up (a wait ran out of time or
+// the chain was broken): its successors give up too, and the image is decoded
+// again by the serial decoder.
 struct LookbackRec {
-  int32_t status;         // lookback_word(epoch, 1 aggregate | 2 inclusive)
+  int32_t status;         // lookback_word(epoch, 1 aggregate | 2 inclusive | 3 failed)
   uint32_t lead_p, lead_s;
   uint32_t agg_p, agg_s;
   int32_t agg_c[4];
@@ -155,6 +158,19 @@ struct LookbackRec {
 };
 
 constexpr uint32_t kEpochMax = 0xFFFFFFu;
+
+// Head of a batch's scratch: for each of the (at most kLaunchCounters)
+// entropy launches of one decode, a 64-bit work-item ticket counter (words
+// 2j, 2j+1) and a 64-bit counter of work items finished (words 8+2j, 9+2j).
+// Neither is ever cleared between decodes: launch j of every decode takes its
+// tickets from counter j, so launches never share one, and after the decodes
+// both counters of launch j equal the tickets the host handed out to it.
+constexpr int kLaunchCounters = 4;
+constexpr int kScratchHead = 16;
+
+// Default limit of one lookback wait, in ticks of the 100 MHz real-time
+// counter (200 ms: far above the longest wait of a valid decode).
+constexpr uint64_t kDefaultWaitTicks = 20000000ull;
 inline constexpr int32_t lookback_word(uint32_t epoch, int st) { return (int32_t)(epoch << 2 | (uint32_t)st); }
 
 // Per-image status bits.  A status word holds (epoch << 8) | bits; a word of

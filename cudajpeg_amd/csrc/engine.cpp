@@ -69,17 +69,20 @@ struct cjd_batch {
   // is filled from them (k_expand) only when a caller asks for it.
   bool coefs_stale = false;
   hipStream_t last_stream = nullptr;
-  // scratch: the 64-bit work-item ticket counter (words 0-1 of a 4-word
-  // head), then status_words per-image status words, then the lookback
-  // records.  Nothing in it is cleared between decodes: the ticket counter
-  // only grows (a launch takes its nitems tickets from ticket_base on) and the
-  // status words and records are tagged with the launch's epoch.
+  // scratch: a head of kScratchHead words (per launch j of a decode, a 64-bit
+  // work-item ticket counter and a 64-bit counter of finished work items),
+  // then status_words per-image status words, then the lookback records.
+  // Nothing in it is cleared between decodes: the counters only grow (launch j
+  // takes its nitems tickets from ticket_base[j] on) and the status words and
+  // records are tagged with the launch's epoch.
   size_t scratch_words = 0, status_words = 0;
-  uint64_t ticket_base = 0;
+  uint64_t ticket_base[kLaunchCounters] = {};
   uint32_t epoch = 0;          // of the last entropy launch
-  bool ticket_unknown = false;  // a launch failed: the counter is cleared before the next
-  std::vector<uint32_t> h_status;  // status bits of the last decode (host)
+  bool ticket_unknown = false;  // a launch failed: the counters are cleared before the next
+  uint64_t wait_ticks = kDefaultWaitTicks;  // limit of one lookback wait
+  std::vector<uint32_t> h_status;  // status words of the last decode (host), after the head
   int64_t fallbacks = 0;
+  int64_t protocol_redos = 0;  // checks after a decode that found a counter off and redid the batch
 
   // Decode in parts (cjd_batch_set_overlap): the batch's images are cut into
   // `parts` runs at work-item boundaries split_item[j] / pixel-item
@@ -262,19 +265,19 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   // counter and the epochs go on.  The clear is ordered on the staging stream
   // like the uploads.
   b->status_words = ((size_t)n + 3) & ~(size_t)3;
-  b->scratch_words = 4 + b->status_words + b->plan.items.size() * (sizeof(LookbackRec) / 4);
+  b->scratch_words = kScratchHead + b->status_words + b->plan.items.size() * (sizeof(LookbackRec) / 4);
   if (!e) {
     const bool fresh = !b->d_scratch.p || b->d_scratch.cap < b->scratch_words * 4;
     e = b->d_scratch.ensure(b->scratch_words * 4);
     if (!e && fresh) {
-      b->ticket_base = 0;
+      for (auto& t : b->ticket_base) t = 0;
       b->epoch = 0;
     }
-    const size_t from = fresh ? 0 : 4;
+    const size_t from = fresh ? 0 : kScratchHead;
     if (!e && launch_clear(b->d_scratch.as<uint32_t>() + from, b->scratch_words - from, s) != hipSuccess)
       e = CJD_KERNEL_LAUNCH_ERROR;
   }
-  b->h_status.assign(b->status_words, 0);
+  b->h_status.assign(kScratchHead + b->status_words, 0);
   plan_split(b);
   return e;
 }
@@ -286,11 +289,11 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
 static int next_epoch(cjd_batch* b, hipStream_t s, uint32_t* epoch) {
   if (b->ticket_unknown) {
     if (launch_clear(b->d_scratch.as<uint32_t>(), b->scratch_words, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-    b->ticket_base = 0;
+    for (auto& t : b->ticket_base) t = 0;
     b->epoch = 0;
     b->ticket_unknown = false;
   } else if (b->epoch >= kEpochMax) {
-    if (launch_clear(b->d_scratch.as<uint32_t>() + 4, b->scratch_words - 4, s) != hipSuccess)
+    if (launch_clear(b->d_scratch.as<uint32_t>() + kScratchHead, b->scratch_words - kScratchHead, s) != hipSuccess)
       return CJD_KERNEL_LAUNCH_ERROR;
     b->epoch = 0;
   }
@@ -298,10 +301,17 @@ static int next_epoch(cjd_batch* b, hipStream_t s, uint32_t* epoch) {
   return 0;
 }
 
-static uint32_t* status_of(cjd_batch* b) { return b->d_scratch.as<uint32_t>() + 4; }
+static uint32_t* status_of(cjd_batch* b) { return b->d_scratch.as<uint32_t>() + kScratchHead; }
+
+// Ticket and finished-item counters of launch j of a decode.
+static unsigned long long* ticket_of(cjd_batch* b, int j) {
+  return reinterpret_cast<unsigned long long*>(b->d_scratch.as<uint32_t>()) + j;
+}
+static unsigned long long* done_of(cjd_batch* b, int j) {
+  return reinterpret_cast<unsigned long long*>(b->d_scratch.as<uint32_t>() + 8) + j;
+}
 
 static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
-  uint32_t* scratch = b->d_scratch.as<uint32_t>();
   uint32_t* status = status_of(b);
   LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
   const uint32_t nitems = (uint32_t)b->plan.items.size();
@@ -317,9 +327,9 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
   else {
     he = launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                         b->d_items.as<WorkItem>(), 0, nitems, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
-                        b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
-                        reinterpret_cast<unsigned long long*>(scratch), b->ticket_base, epoch, s);
-    b->ticket_base += nitems;
+                        b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb, ticket_of(b, 0),
+                        done_of(b, 0), b->ticket_base[0], epoch, b->wait_ticks, s);
+    b->ticket_base[0] += nitems;
     b->ticket_unknown = he != hipSuccess;
     if (he == hipSuccess && timed && hipEventRecord(b->ev[4], s) != hipSuccess) he = hipErrorUnknown;
     b->coefs_stale = true;
@@ -358,7 +368,6 @@ static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed) {
     for (auto& e : b->ex)
       if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
   }
-  uint32_t* scratch = b->d_scratch.as<uint32_t>();
   uint32_t* status = status_of(b);
   LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
@@ -372,9 +381,9 @@ static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed) {
     const hipError_t he =
         launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                        b->d_items.as<WorkItem>(), i0, i1, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
-                       b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
-                       reinterpret_cast<unsigned long long*>(scratch), b->ticket_base, epoch, s);
-    b->ticket_base += i1 - i0;
+                       b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb, ticket_of(b, j),
+                       done_of(b, j), b->ticket_base[j], epoch, b->wait_ticks, s);
+    b->ticket_base[j] += i1 - i0;
     if (he != hipSuccess) {
       b->ticket_unknown = true;
       return CJD_KERNEL_LAUNCH_ERROR;
@@ -406,16 +415,34 @@ static int enqueue(cjd_batch* b, hipStream_t s, bool timed) {
 // whose tile links did not resolve; map status bits to reference error codes.
 // `redo` receives the images that were decoded again.
 static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo) {
-  if (b->status_words && hipMemcpyAsync(b->h_status.data(), status_of(b), b->status_words * 4,
-                                        hipMemcpyDeviceToHost, s) != hipSuccess)
+  // the counters of the head and the status words, in one copy
+  if (hipMemcpyAsync(b->h_status.data(), b->d_scratch.p, b->h_status.size() * 4, hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
     return CJD_DEVICE_MEM_ERROR;
   if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  // After the decodes, launch j's ticket counter and finished-item counter
+  // both equal the tickets handed out to it: every work group took one ticket
+  // and every work item finished once.  Anything else means some item was not
+  // decoded (its records may be stale): every image is decoded again by the
+  // serial decoder, and the counters start again at the next decode.
+  bool redo_all = false;
+  if (b->entropy_mode == CJD_ENTROPY_TILED) {
+    uint64_t head[kScratchHead / 2];
+    memcpy(head, b->h_status.data(), sizeof(head));
+    for (int j = 0; j < kLaunchCounters; ++j)
+      if (head[j] != b->ticket_base[j] || head[kLaunchCounters + j] != b->ticket_base[j]) redo_all = true;
+    if (redo_all) {
+      b->ticket_unknown = true;
+      ++b->protocol_redos;
+    }
+  }
   const uint32_t epoch = b->epoch;
-  for (uint32_t& w : b->h_status) w = status_bits(w, epoch);
+  uint32_t* words = b->h_status.data() + kScratchHead;
+  for (size_t i = 0; i < b->status_words; ++i) words[i] = status_bits(words[i], epoch);
   for (size_t i = 0; i < b->plan.recs.size(); ++i) {
     ImageRec& r = b->plan.recs[i];
     if (r.error) continue;
-    const uint32_t st = b->h_status[i];
+    const uint32_t st = words[i] | (redo_all ? kStatusUnresolved : 0u);
     if (st & (kStatusUnresolved | kStatusInconsistent)) {
       // a launch of its own, so its status word starts without bits
       uint32_t e2 = 0, w = 0;
@@ -427,11 +454,11 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
         return CJD_KERNEL_LAUNCH_ERROR;
       if (hipMemcpyAsync(&w, status + i, 4, hipMemcpyDeviceToHost, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
       if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-      b->h_status[i] = status_bits(w, e2) | kStatusFallback;
+      words[i] = status_bits(w, e2) | kStatusFallback;
       ++b->fallbacks;
       if (redo) redo->push_back(i);
     }
-    if (b->h_status[i] & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
+    if (words[i] & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
   }
   return 0;
 }
@@ -485,6 +512,14 @@ extern "C" int cjd_batch_finish(cjd_batch* b, void* stream, int* errors) {
 }
 
 extern "C" int cjd_batch_fallbacks(cjd_batch* b) { return b ? (int)b->fallbacks : -1; }
+
+extern "C" int cjd_batch_set_wait_limit(cjd_batch* b, int64_t microseconds) {
+  if (!b || microseconds < 0 || microseconds > (int64_t)1 << 40) return CJD_PROGRAMMER_ERROR;
+  b->wait_ticks = (uint64_t)microseconds * 100u;  // 100 MHz real-time counter
+  return 0;
+}
+
+extern "C" int cjd_batch_protocol_redos(cjd_batch* b) { return b ? (int)b->protocol_redos : -1; }
 
 extern "C" int cjd_batch_timings(cjd_batch* b, float* ms_entropy, float* ms_pixel, float* ms_total) {
   if (!b || !b->timed) return CJD_PROGRAMMER_ERROR;

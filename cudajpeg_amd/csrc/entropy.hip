@@ -96,21 +96,23 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
                                                  const uint16_t* __restrict__ luts, uint32_t* __restrict__ ebuf,
                                                  BlockRec* __restrict__ recs, uint32_t* __restrict__ status,
                                                  LookbackRec* __restrict__ lb,
-                                                 unsigned long long* __restrict__ ticket, uint64_t ticket_base,
-                                                 uint32_t epoch) {
+                                                 unsigned long long* __restrict__ ticket,
+                                                 unsigned long long* __restrict__ done, uint64_t ticket_base,
+                                                 uint32_t epoch, uint64_t wait_ticks) {
   __shared__ EntShared sh;
   const int tid = threadIdx.x;
   // Work items are taken in ticket order, so every predecessor an item waits
   // for in the lookback has already started.  The ticket counter is never
-  // cleared: a launch over items [item0, item_end) takes that many tickets
-  // from ticket_base on.
+  // cleared and belongs to this launch of the decode alone: a launch over
+  // items [item0, item_end) takes that many tickets from ticket_base on.
   if (tid == 0) {
     sh.item = item0 + (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
     sh.lovf = 0;
+    sh.look_ok = 1;
   }
   __syncthreads();
   const uint32_t item_id = sh.item;
-  if (item_id >= item_end) return;
+  if (item_id < item0 || item_id >= item_end) return;
   EntCtx c;
   c.init(words, images, segs, tables, ebuf, items[item_id]);
   {
@@ -136,37 +138,41 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   LookbackRec* me = lb + item_id;
   if (tid == 0 && c.lead) {
     ent_aggregate(sh, c, me);
-    st_status(&me->status, lookback_word(epoch, 1));
+    st_status(&me->status, lookback_word(epoch, kLookAgg));
     // decoupled lookback: compose predecessors' aggregates until one has its
-    // inclusive state
-    LookAcc acc;
-    acc.reset();
-    uint32_t q_item = c.it.prev_item, p = 0, s = 0;
-    int32_t cc[4] = {0, 0, 0, 0};
-    for (;;) {
-      const LookbackRec* r = lb + q_item;
-      // a record of an older launch does not match this epoch: wait for it
-      int st;
-      while (((uint32_t)(st = ld_status(&r->status)) >> 2) != epoch) __builtin_amdgcn_s_sleep(2);
-      if ((st & 3) == 2) {
-        if (!acc.apply(*r, p, s, cc)) {
-          // an assumption on the way does not hold: take the immediate
-          // predecessor's inclusive state
-          const LookbackRec* pr = lb + c.it.prev_item;
-          while (ld_status(&pr->status) != lookback_word(epoch, 2)) __builtin_amdgcn_s_sleep(2);
-          acc.reset();
-          acc.apply(*pr, p, s, cc);
-        }
-        break;
+    // inclusive state.  A record of an older launch does not match this
+    // epoch; every wait ends, after wait_ticks of the real-time counter at
+    // most, and then the item gives up.
+    auto wait = [&](uint32_t q, bool incl) -> int {
+      const int32_t* w = &lb[q].status;
+      uint64_t t0 = 0;
+      for (bool first = true;; first = false) {
+        const int st = ld_status(w);
+        if (((uint32_t)st >> 2) == epoch && (!incl || (st & 3) >= kLookIncl)) return st;
+        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        if (first) t0 = now;
+        else if (now - t0 >= wait_ticks) return -1;
+        __builtin_amdgcn_s_sleep(2);
       }
-      acc.absorb(*r);
-      q_item = items[q_item].prev_item;
+    };
+    uint32_t p = 0, s = 0;
+    int32_t cc[4] = {0, 0, 0, 0};
+    if (lookback_walk(lb, items, c.sg.first_item, item_id, c.it.prev_item, wait, p, s, cc)) {
+      sh.entry_p = p;
+      sh.entry_s = s;
+      for (int q = 0; q < 4; ++q) sh.entry_c[q] = cc[q];
+    } else {
+      // successors give up at this record; the host decodes the image again
+      sh.look_ok = 0;
+      status_or(status + c.im->error_index, kStatusUnresolved, epoch);
+      st_status(&me->status, lookback_word(epoch, kLookFail));
     }
-    sh.entry_p = p;
-    sh.entry_s = s;
-    for (int q = 0; q < 4; ++q) sh.entry_c[q] = cc[q];
   }
   __syncthreads();
+  if (!sh.look_ok) {
+    if (tid == 0) atomicAdd(done, 1ull);
+    return;
+  }
   if (ent_entry_differs(sh, c)) {  // uniform: the chain is redone from the true entry
     if (tid == 0) {
       sh.xp[0] = sh.entry_p;
@@ -182,7 +188,8 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   __syncthreads();
   if (tid == 0) {
     ent_inclusive(sh, c, me);
-    st_status(&me->status, lookback_word(epoch, 2));
+    st_status(&me->status, lookback_word(epoch, kLookIncl));
+    atomicAdd(done, 1ull);  // read by the host after the launch
   }
   ent_records(sh, tid, c, recs, status, epoch);
 }
@@ -266,10 +273,11 @@ hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s) {
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                           const WorkItem* items, uint32_t item0, uint32_t item_end, const HuffTable* tables,
                           const uint16_t* luts, uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb,
-                          unsigned long long* ticket, uint64_t ticket_base, uint32_t epoch, hipStream_t s) {
+                          unsigned long long* ticket, unsigned long long* done, uint64_t ticket_base, uint32_t epoch,
+                          uint64_t wait_ticks, hipStream_t s) {
   if (item_end <= item0) return hipSuccess;
   hipLaunchKernelGGL(k_entropy, dim3(item_end - item0), dim3(kWG), 0, s, words, images, segs, items, item0, item_end,
-                     tables, luts, ebuf, recs, status, lb, ticket, ticket_base, epoch);
+                     tables, luts, ebuf, recs, status, lb, ticket, done, ticket_base, epoch, wait_ticks);
   return hipGetLastError();
 }
 

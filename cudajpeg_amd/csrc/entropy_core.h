@@ -316,6 +316,7 @@ struct EntShared {
   uint32_t item;
   int32_t nchg;                    // tiles relinked in the current fix-up round
   uint32_t lovf;                   // a link needed more entries than its tile has bits
+  uint32_t look_ok;                // the lookback resolved the item's entry
   uint32_t entry_p, entry_s;       // true state at the exit of the lead-in tile
   int32_t entry_c[4];              //   and the segment's cumulative counts there
 };
@@ -919,6 +920,47 @@ struct LookAcc {
   }
 };
 
+
+// Lookback states of a record (the low 2 bits of its status word): an
+// aggregate is published, the inclusive state is published, or the item
+// failed (a wait gave up or a chain index left the segment's items; the
+// item's image is marked unresolved and decoded again by the serial decoder).
+constexpr int kLookAgg = 1, kLookIncl = 2, kLookFail = 3;
+
+// The lookback walk of item `self`, whose segment's items are [first, self]
+// and whose immediate predecessor is `prev`: compose the predecessors'
+// aggregates back to one that has its inclusive state.  wait(q, incl)
+// returns record q's status word once it holds a state of this launch (an
+// inclusive or failed one when incl), or -1 when the wait gave up; the
+// record's fields are read only after it returned.  Every step moves to a
+// lower item of the same segment, so a walk ends after at most self - first
+// steps.  False: the entry cannot be resolved here (a wait gave up, a
+// predecessor failed, or a chain index is out of range).
+template <class Wait>
+CJD_HD inline bool lookback_walk(const LookbackRec* lb, const WorkItem* items, uint32_t first, uint32_t self,
+                                 uint32_t prev, Wait&& wait, uint32_t& p, uint32_t& s, int32_t* cc) {
+  LookAcc acc;
+  acc.reset();
+  if (prev < first || prev >= self) return false;
+  uint32_t q = prev;
+  for (;;) {
+    const int st = wait(q, false);
+    if (st < 0 || (st & 3) == kLookFail) return false;
+    if ((st & 3) == kLookIncl) {
+      if (acc.apply(lb[q], p, s, cc)) return true;
+      // an assumption on the way does not hold: take the immediate
+      // predecessor's inclusive state
+      const int sp = wait(prev, true);
+      if (sp < 0 || (sp & 3) != kLookIncl) return false;
+      acc.reset();
+      return acc.apply(lb[prev], p, s, cc);
+    }
+    acc.absorb(lb[q]);
+    const uint32_t nq = items[q].prev_item;
+    if (nq < first || nq >= q) return false;
+    q = nq;
+  }
+}
 
 // Aggregate of this item (tc scanned: relative cumulative counts): assumes
 // the item's entry is the speculative exit of its lead-in tile.

@@ -11,7 +11,8 @@ hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s);
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                           const WorkItem* items, uint32_t item0, uint32_t item_end, const HuffTable* tables,
                           const uint16_t* luts, uint32_t* ebuf, BlockRec* recs, uint32_t* status, LookbackRec* lb,
-                          unsigned long long* ticket, uint64_t ticket_base, uint32_t epoch, hipStream_t s);
+                          unsigned long long* ticket, unsigned long long* done, uint64_t ticket_base, uint32_t epoch,
+                          uint64_t wait_ticks, hipStream_t s);
 hipError_t launch_expand(const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, int16_t* coefs, uint64_t nblocks,
                          hipStream_t s);
 hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,

@@ -143,6 +143,16 @@ class BatchDecoder:
     def fallbacks(self) -> int:
         return int(L.lib().cjd_batch_fallbacks(self.h))
 
+    def set_wait_limit(self, microseconds: int) -> None:
+        """Limit of one lookback wait of the tiled decoder (0 = give up unless
+        the predecessor has published; results do not depend on it)."""
+        _check(L.lib().cjd_batch_set_wait_limit(self.h, int(microseconds)), "set_wait_limit")
+
+    def protocol_redos(self) -> int:
+        """Checks after a decode that found a work item never decoded (the
+        whole batch was then decoded again serially); 0 in a correct run."""
+        return int(L.lib().cjd_batch_protocol_redos(self.h))
+
     def timings(self) -> tuple[float, float, float]:
         a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         _check(L.lib().cjd_batch_timings(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "timings: ")

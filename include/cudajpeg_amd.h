@@ -157,6 +157,21 @@ int cjd_batch_transform(cjd_batch *b, void *stream);
 /* Number of images the serial decoder has redone since the batch was created. */
 int cjd_batch_fallbacks(cjd_batch *b);
 
+/* Limit of one wait of the tiled decoder's lookback for a predecessor work
+ * item, in microseconds (default 200000; 0 = give up unless the predecessor
+ * has already published).  A work item whose wait runs out marks its image
+ * unresolved and its successors give up in turn; cjd_batch_finish then
+ * decodes that image again with the serial decoder, so results do not depend
+ * on the limit, and every entropy launch ends.  No counterpart in the
+ * reference, whose scan decode is serial (src/decodeScanGPU.cu:435-451). */
+int cjd_batch_set_wait_limit(cjd_batch *b, int64_t microseconds);
+
+/* Number of checks after a decode (cjd_batch_finish, cjd_batch_decode_scan)
+ * that found a work-item counter off -- some work item was never decoded --
+ * and so decoded every image of the batch again with the serial decoder.
+ * Zero in a correct run.  No counterpart in the reference. */
+int cjd_batch_protocol_redos(cjd_batch *b);
+
 /* Wait for the decode, run the serial entropy decoder for any image whose
  * tile links did not resolve, and report per-image reference error codes. */
 int cjd_batch_finish(cjd_batch *b, void *stream, int *errors);

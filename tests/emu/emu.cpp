@@ -43,6 +43,8 @@ struct EntRun {
   std::vector<std::unique_ptr<EntShared>> shs;
   std::vector<EntCtx> ctx;
   std::vector<char> done;
+  std::vector<char> failed;  // the item gave up in the lookback (no records)
+  bool no_wait = false;      // a wait for an unpublished state gives up at once
   int fixes = 0;  // links decoded again in fix-up rounds (entries off the speculative chain)
 };
 
@@ -101,30 +103,32 @@ void count_all(EntShared& sh, const EntCtx& c) {
 }
 
 // Lookback, entry application and inclusive publication of item i, resolving
-// predecessors first only when the kernel would have waited for them.
+// predecessors first only when the kernel would have waited for them (or, with
+// no_wait, giving up as a wait that ran out of time does).
 void resolve(EntRun& R, size_t i) {
   if (R.done[i]) return;
   EntShared& sh = *R.shs[i];
   const EntCtx& c = R.ctx[i];
   if (c.lead) {
-    LookAcc acc;
-    acc.reset();
-    uint32_t q = c.it.prev_item, p = 0, s = 0;
-    int32_t cc[4] = {0, 0, 0, 0};
-    for (;;) {
-      // the kernel spins until a record of its own epoch is published
-      if (((uint32_t)R.lb[q].status >> 2) != kEpoch) resolve(R, q);
-      const LookbackRec& r = R.lb[q];
-      if (r.status == lookback_word(kEpoch, 2)) {
-        if (!acc.apply(r, p, s, cc)) {
-          resolve(R, c.it.prev_item);
-          acc.reset();
-          acc.apply(R.lb[c.it.prev_item], p, s, cc);
-        }
-        break;
+    auto wait = [&](uint32_t q, bool incl) -> int {
+      auto ready = [&] {
+        const int st = R.lb[q].status;
+        return ((uint32_t)st >> 2) == kEpoch && (!incl || (st & 3) >= kLookIncl);
+      };
+      if (!ready()) {
+        if (R.no_wait) return -1;
+        resolve(R, q);
       }
-      acc.absorb(r);
-      q = R.p->items[q].prev_item;
+      return R.lb[q].status;
+    };
+    uint32_t p = 0, s = 0;
+    int32_t cc[4] = {0, 0, 0, 0};
+    if (!lookback_walk(R.lb.data(), R.p->items.data(), c.sg.first_item, (uint32_t)i, c.it.prev_item, wait, p, s,
+                       cc)) {
+      R.lb[i].status = lookback_word(kEpoch, kLookFail);
+      R.failed[i] = 1;
+      R.done[i] = 1;
+      return;
     }
     sh.entry_p = p;
     sh.entry_s = s;
@@ -139,7 +143,7 @@ void resolve(EntRun& R, size_t i) {
   }
   for (int t = 0; t < kWG; ++t) ent_absolute(sh, t, c);
   ent_inclusive(sh, c, &R.lb[i]);
-  R.lb[i].status = lookback_word(kEpoch, 2);
+  R.lb[i].status = lookback_word(kEpoch, kLookIncl);
   R.done[i] = 1;
 }
 
@@ -176,6 +180,8 @@ void run_entropy(Emu& E, int order) {
   R.shs.resize(ni);
   R.ctx.resize(ni);
   R.done.assign(ni, 0);
+  R.failed.assign(ni, 0);
+  R.no_wait = order == 2;
   // speculative pass, links, fix-up, counts, scan and aggregate publication
   // for every item
   for (size_t i = 0; i < ni; ++i) {
@@ -205,13 +211,21 @@ void run_entropy(Emu& E, int order) {
     }
   }
   // segment-first items resolve from the known start; then the rest in
-  // order (predecessors inclusive) or backwards (predecessors aggregate-only)
-  for (size_t i = 0; i < ni; ++i)
-    if (!R.ctx[i].lead) resolve(R, i);
+  // order (predecessors inclusive) or backwards (predecessors aggregate-only).
+  // order 2: backwards with no waiting and the segment-first items last, so
+  // every item after a segment's first gives up (a failed lookback)
+  if (order != 2)
+    for (size_t i = 0; i < ni; ++i)
+      if (!R.ctx[i].lead) resolve(R, i);
   for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
   E.fixes = R.fixes;
-  for (size_t i = 0; i < ni; ++i)
+  for (size_t i = 0; i < ni; ++i) {
+    if (R.failed[i]) {
+      status_or(E.status.data() + R.ctx[i].im->error_index, kStatusUnresolved, kEpoch);
+      continue;
+    }
     for (int t = 0; t < kWG; ++t) ent_records(*R.shs[i], t, R.ctx[i], E.recs.data(), E.status.data(), kEpoch);
+  }
   E.coefs_stale = true;
 }
 

@@ -143,3 +143,22 @@ def test_emulated_long_code_tables(tile_bits, item_tiles):
     for i, (s, e) in enumerate(zip(specs, encs)):
         assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
 
+
+
+def test_emulated_lookback_give_up_is_redone_serially():
+    # Every wait gives up at once and segment-first items resolve last (order
+    # 2): every work item after the first of its segment fails its lookback,
+    # publishes the failed state, and its image is marked unresolved; the host
+    # decodes those images again with the serial decoder, and the results stay
+    # bit-exact.  Images with one work item per segment are not affected.
+    encs = [synth.encode(s, truth=True) for s in HARD]
+    r = EmuResult([e.data for e in encs], 0, 2048, False, 2, False, 4)
+    assert not r.errors.any()
+    # every image without restart markers has segments of several items (the
+    # restart-interval image's segments of 7 MCUs are one item each)
+    assert r.fallbacks == sum(1 for s in HARD if not s.restart_interval)
+    for i, (s, e) in enumerate(zip(HARD, encs)):
+        assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
+    small = synth.encode(synth.ImageSpec(64, 48, "420", 75, 6.0, 1), truth=True)
+    r = EmuResult([small.data], 0, 4096, False, 2)
+    assert r.fallbacks == 0 and np.array_equal(r.get(0, 1, small.coefs.shape, np.int16), small.coefs)

@@ -1,0 +1,145 @@
+"""GPU parity at the benchmark configurations' full sizes, through the C ABI.
+
+C1 512x512 4:2:0 q75; C2 3840x2160 4:2:0 q90 with 10% random tiles; C3
+1024x768 gray q50, 2048x1536 4:4:4 q95, 1918x1078 4:2:0 q85; C4 the
+256-image mixed batch decoded as one staged batch with the benchmark's
+entropy parameters (8192-bit tiles, 128-tile work items, decode in 2 parts).
+
+Bar: coefficients bit-exact with the encoder's (which the oracle's scan
+decode reproduces); integer-iDCT pixels bit-exact with the oracle; FP32 and
+MFMA planar samples within 1 of the integer and of the FP64 iDCT, MFMA at
+least 99.9% equal to FP64 (the reference's iDCT is integer, so no float form
+can match it bit for bit); RGB from a float form within 3 of the integer
+form's (chroma gains 359/256 and 454/256 amplify a 1-LSB chroma step).
+
+Deviation statistics (max / mean |d| / exact fraction, planar and RGB,
+against FP64 and the integer iDCT) are collected per configuration and, when
+CJD_RECORD_DIR is set, written there as deviation.json.
+"""
+import json
+import os
+import threading
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import oracle
+from cudajpeg_amd import IDCT_FLOAT, IDCT_INT, IDCT_MFMA, BatchDecoder, synth
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {IDCT_INT: "int", IDCT_FLOAT: "fp32", IDCT_MFMA: "mfma"}
+RECORD = {}
+_LOCK = threading.Lock()
+
+
+def _dev(got, ref):
+    d = np.abs(got.astype(np.int32) - ref.astype(np.int32))
+    return {"max": int(d.max()), "mean": float(d.mean()), "exact": float((d == 0).mean()), "n": int(d.size)}
+
+
+def _merge(acc, st):
+    if not acc:
+        return dict(st)
+    n = acc["n"] + st["n"]
+    return {"max": max(acc["max"], st["max"]), "mean": (acc["mean"] * acc["n"] + st["mean"] * st["n"]) / n,
+            "exact": (acc["exact"] * acc["n"] + st["exact"] * st["n"]) / n, "n": n}
+
+
+def _record(config, form, what, st):
+    with _LOCK:
+        key = RECORD.setdefault(config, {}).setdefault(form, {})
+        key[what] = _merge(key.get(what), st)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _write_record():
+    yield
+    out = os.environ.get("CJD_RECORD_DIR")
+    if out and RECORD:
+        os.makedirs(out, exist_ok=True)
+        with open(os.path.join(out, "deviation.json"), "w") as f:
+            json.dump(RECORD, f, indent=1, sort_keys=True)
+
+
+def _check_image(config, form, enc, coefs, pixels, planes):
+    """Compares one decoded image with the encoder's coefficients and the
+    oracle; returns a failure message or None (runs on a worker thread)."""
+    s = enc.spec
+    if not np.array_equal(coefs, enc.coefs):
+        return f"{s}: coefficients differ from the encoder's"
+    p = oracle.parse(enc.data)
+    pl_int = oracle.planes(p, enc.coefs, 0)
+    rgb_int = oracle.colour(p, pl_int)
+    if form == "int":
+        if not np.array_equal(pixels, rgb_int):
+            return f"{s}: integer-iDCT pixels differ from the oracle's"
+        return None
+    pl_f64 = oracle.planes(p, enc.coefs, 1)
+    rgb_f64 = oracle.colour(p, pl_f64)
+    for c in range(p.num_channels):
+        # whole planes: the padding past the component's width and height
+        # holds decoded edge blocks too
+        st_f = _dev(planes[c], pl_f64[c])
+        st_i = _dev(planes[c], pl_int[c])
+        _record(config, form, "planar_vs_fp64", st_f)
+        _record(config, form, "planar_vs_int", st_i)
+        if st_f["max"] > 1 or st_i["max"] > 1:
+            return f"{s} comp {c}: planar deviation {st_f['max']} vs FP64, {st_i['max']} vs integer"
+        if form == "mfma" and st_f["exact"] <= 0.999:
+            return f"{s} comp {c}: MFMA exact fraction vs FP64 {st_f['exact']:.5f}"
+    st = _dev(pixels, rgb_int)
+    _record(config, form, "rgb_vs_int", st)
+    _record(config, form, "rgb_vs_fp64", _dev(pixels, rgb_f64))
+    if st["max"] > 3:
+        return f"{s}: RGB deviation {st['max']} vs the integer form"
+    return None
+
+
+def _decode_and_check(config, specs, idct, tile_bits=0, item_tiles=0, parts=2):
+    encs = synth.encode_many(specs, truth=True)
+    form = FORMS[idct]
+    b = BatchDecoder(idct, tile_bits=tile_bits, planes=idct != IDCT_INT)
+    b.set_overlap(parts)
+    if item_tiles:
+        b.set_tuning(0, item_tiles)
+    print(f"case {config} {form}: {len(encs)} image(s), {sum(s.pixels for s in specs) / 1e6:.1f} Mpix, "
+          f"tile {tile_bits or 4096} x {item_tiles or 256}, parts {parts}", flush=True)
+    assert not b.stage([e.data for e in encs]).any()
+    b.decode()
+    errs = b.finish()
+    assert not errs.any(), np.flatnonzero(errs)
+    assert b.fallbacks() == 0 and b.protocol_redos() == 0
+    futs = []
+    with ThreadPoolExecutor(16) as ex:
+        for i, e in enumerate(encs):
+            nc = e.spec.ncomp
+            planes = [b.plane(i, c) for c in range(nc)] if idct != IDCT_INT else None
+            futs.append(ex.submit(_check_image, config, form, e, b.coefficients(i), b.pixels(i), planes))
+        bad = [m for m in (f.result() for f in futs) if m]
+    b.close()
+    assert not bad, bad[:8]
+
+
+@pytest.mark.parametrize("idct", [IDCT_INT, IDCT_FLOAT, IDCT_MFMA])
+def test_c1_512_420(gpu, idct):
+    _decode_and_check("C1 512x512 4:2:0 q75", [synth.config_plain_512()], idct)
+
+
+@pytest.mark.parametrize("idct", [IDCT_INT, IDCT_FLOAT, IDCT_MFMA])
+def test_c2_4k_random_tiles(gpu, idct):
+    _decode_and_check("C2 3840x2160 4:2:0 q90 random tiles", [synth.config_4k()], idct)
+
+
+@pytest.mark.parametrize("idct", [IDCT_INT, IDCT_FLOAT, IDCT_MFMA])
+def test_c3_shapes(gpu, idct):
+    for s in synth.config_shapes():
+        _decode_and_check(f"C3 {s.width}x{s.height} {s.sampling} q{s.quality}", [s], idct)
+
+
+@pytest.mark.parametrize("idct", [IDCT_INT, IDCT_MFMA])
+def test_c4_mixed_batch_one_launch(gpu, idct):
+    # the benchmark's batch and parameters (bench.py defaults)
+    _decode_and_check("C4 256 mixed images", synth.config_mixed(256, 4000), idct, tile_bits=8192, item_tiles=128,
+                      parts=2)
