@@ -5,7 +5,7 @@ in 256x256..4096x3072, aspect 1:1/4:3/16:9, 70% 4:2:0 / 20% 4:4:4 / 10% gray,
 quality U{60..95}, noise U[2,12] -- with the scans already resident in HBM
 and the RGB/gray frames left in HBM.
 
-Reported:
+Reported (all by default; --no-plain, --no-serve, --no-cpu skip sections):
   value              batched form, Mpixel/s (W*H): each decode runs in
                      --parts runs of images (2): each run's pixel kernel on a
                      second stream beside the next runs' entropy decodes,
@@ -15,19 +15,24 @@ Reported:
   kernel_ms          device time per kernel (HIP events on the decode stream,
                      the kernels run one after the other)
   roofline           dominant kernel, achieved algorithmic GB/s vs 8 TB/s;
-                     traffic = its HBM bytes per launch from a committed
+                     traffic = its HBM bytes per launch from the committed
                      FETCH_SIZE/WRITE_SIZE summary of this workload and these
                      kernels, null when there is none
+  plain_mpix_s       plain form: one image per launch (entropy + pixel
+                     kernels), FP32 VALU iDCT, per-size entropy parameters
+                     from a run-time autotune; ratio = batched / plain;
+                     p50/p99_ms = per-image decode latency of the plain form
+                     (enqueue to stream synchronised)
+  serving            the serving loop on the serving mix (mixed batch + 5%
+                     8192x8192 4:2:0 q85): peak rate, then Poisson arrivals at
+                     each --serve-rates rate and at 70% of peak: p50/p99/p99.9
+                     latency (submit to frame ready in device memory) and
+                     sustained Mpixel/s
   cpu_baseline       the reference CPUonly decoder (built from its sources
-                     into oracle/_ref) on a bounded sample, one host core;
-                     all_cores: the whole batch over --cpu-workers (16)
-                     processes started together
-With --plain (off by default):
-  plain_mpix_s       plain form: one image per launch, FP32 VALU iDCT, with
-                     per-size entropy parameters from a run-time autotune
-  ratio              batched / plain
-  p50/p99_ms         per-image decode latency of the plain form
-With --serve (off by default): the serving loop at 70% of its peak rate.
+                     into oracle/_ref) on a bounded sample, one host core
+                     (a worker process pinned to one CPU); all_cores: the
+                     whole batch over --cpu-workers (16) processes started
+                     together; the host CPU model is stated
 """
 from __future__ import annotations
 
@@ -60,41 +65,77 @@ def check(errs, what: str) -> None:
                          + ", ".join(f"#{i} code {int(errs[i])}" for i in bad[:16]))
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+_PINNED_WORKER = r"""
+import json, os, sys, time
+sys.path.insert(0, sys.argv[1])
+core, budget, kind, paths = int(sys.argv[2]), float(sys.argv[3]), sys.argv[4], sys.argv[5:]
+os.sched_setaffinity(0, {core})
+import oracle
+times = []
+for p in paths:
+    if kind == "reference":
+        t = float(oracle.ref_cpuonly_time([p], 1)[0, 0])
+    else:
+        data = open(p, "rb").read()
+        t0 = time.perf_counter()
+        err, _ = oracle.decode(data, 0)
+        t = time.perf_counter() - t0
+        assert err == 0
+    times.append(t)
+    if sum(times) >= budget:
+        break
+print(json.dumps(times))
+"""
+
+
 def cpu_baseline(encs, budget_s: float):
     """Reference CPUonly (oracle/_ref) or, if it was not built, the oracle's
-    C restatement, on a bounded sample of the same workload, single thread."""
+    C restatement, on a bounded sample of the same workload, in a worker
+    process pinned to one CPU (taskset -c <cpu>)."""
+    import subprocess
     import oracle
     kind = "reference" if os.path.exists(os.path.join(oracle.REF_DIR, "libcpuonly.so")) else "port"
     tmp = tempfile.mkdtemp(prefix="cjd_bench_")
-    done_px, done_t, n = 0, 0.0, 0
     order = sorted(range(len(encs)), key=lambda i: encs[i].spec.pixels)
-    # a spread of sizes: every 8th image by area, then fill the time budget
-    sample = order[::8] + [i for i in order if i % 8]
+    # a spread of sizes: every 8th image by area, then the others
+    sample = [i for i in order[::8] + [i for i in order if i % 8]
+              if not (kind == "reference" and encs[i].spec.restart_interval)]
+    paths = []
     for i in sample:
-        e = encs[i]
-        if kind == "reference" and e.spec.restart_interval:
-            continue
         path = os.path.join(tmp, f"{i}.jpg")
         with open(path, "wb") as f:
-            f.write(e.data)
-        if kind == "reference":
-            t = float(oracle.ref_cpuonly_time([path], 1)[0, 0])
-        else:
-            t0 = time.perf_counter()
-            err, _ = oracle.decode(e.data, 0)
-            t = time.perf_counter() - t0
-            assert err == 0
+            f.write(encs[i].data)
+        paths.append(path)
+    core = min(os.sched_getaffinity(0))
+    out = subprocess.run([sys.executable, "-c", _PINNED_WORKER, ROOT, str(core), str(budget_s), kind] + paths,
+                         capture_output=True, text=True, timeout=600)
+    for path in paths:
         os.unlink(path)
-        done_px += e.spec.pixels
-        done_t += t
-        n += 1
-        if done_t >= budget_s:
-            break
     os.rmdir(tmp)
-    return {"value": round(done_px / done_t / 1e6, 2), "unit": "Mpix/s", "cores": 1, "kind": kind,
+    if out.returncode:
+        raise SystemExit(f"bench.py: CPU baseline worker failed: {out.stderr[-500:]}")
+    times = json.loads(out.stdout.strip().splitlines()[-1])
+    n = len(times)
+    px = sum(encs[i].spec.pixels for i in sample[:n])
+    done_t = sum(times)
+    return {"value": round(px / done_t / 1e6, 2), "unit": "Mpix/s", "cores": 1, "kind": kind,
+            "cpu_model": cpu_model(), "pinned_cpu": core,
+            "p50_ms": round(float(np.percentile(times, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(times, 99)) * 1e3, 3),
             "sample": f"{n} of the 256 mixed images (spread of sizes), decoded once each by the "
                       f"{'reference CPUonly openJPG' if kind == 'reference' else 'oracle C restatement'}, "
-                      f"file in page cache, {done_t:.1f} s total, {done_px / 1e6:.0f} Mpix"}
+                      f"file in page cache, one process pinned to CPU {core}, {done_t:.1f} s total, "
+                      f"{px / 1e6:.0f} Mpix"}
 
 
 _CPU_WORKER = r"""
@@ -168,10 +209,12 @@ def cpu_all_cores(encs, workers: int, reps: int):
 
 
 def serve_bench(args):
-    """Serving loop on the request mix: the mixed-batch distribution plus 5%
-    8192x8192 4:2:0 q85 images.  Peak = a burst of requests decoded flat out;
-    then Poisson arrivals at 70% of that rate for --serve-seconds, with
-    latency from submit to the frame ready in device memory."""
+    """Serving loop on the serving mix: the mixed-batch distribution plus 5%
+    8192x8192 4:2:0 q85 images (a pool of 252 files sampled with
+    replacement).  Peak = a burst of requests decoded flat out; then Poisson
+    arrivals at each of --serve-rates and at 70% of peak, each for about
+    --serve-seconds (at most 6000 requests); latency from submit to the frame
+    ready in device memory, sustained Mpixel/s = pixels completed / wall."""
     from cudajpeg_amd import Server, synth
     rng = np.random.default_rng(args.seed + 1)
     specs = synth.config_mixed(240, args.seed + 1000)
@@ -187,30 +230,40 @@ def serve_bench(args):
     n_peak = 2 * len(files)
     burst = [files[k % len(files)] for k in range(n_peak)]
     lat, err, wall = srv.replay(burst, np.zeros(n_peak, np.int64))
-    assert not err.any(), err
+    check(err, "serving burst")
     peak_req_s = n_peak / (wall / 1e9)
     peak_mpix_s = float(sum(px[k % len(files)] for k in range(n_peak)) / (wall / 1e9) / 1e6)
-    # 70% of peak, Poisson arrivals
-    rate = 0.7 * peak_req_s
-    n = max(64, int(rate * args.serve_seconds))
-    arrival = np.cumsum(rng.exponential(1e9 / rate, n)).astype(np.int64)
-    reqs = [files[int(k)] for k in rng.integers(0, len(files), n)]
-    lat, err, wall = srv.replay(reqs, arrival)
+
+    def run(rate, seconds):
+        n = int(min(6000, max(64, rate * seconds)))
+        arrival = np.cumsum(rng.exponential(1e9 / rate, n)).astype(np.int64)
+        pick = rng.integers(0, len(files), n)
+        lat, err, wall = srv.replay([files[int(k)] for k in pick], arrival)
+        check(err, f"serving at {rate:.0f} req/s")
+        ms = lat / 1e6
+        return {"offered_req_s": round(rate, 1), "requests": n, "seconds": round(wall / 1e9, 3),
+                "sustained_mpix_s": round(float(px[pick].sum()) / (wall / 1e9) / 1e6, 1),
+                "p50_ms": round(float(np.percentile(ms, 50)), 3), "p99_ms": round(float(np.percentile(ms, 99)), 3),
+                "p999_ms": round(float(np.percentile(ms, 99.9)), 3)}
+
+    sweep = []
+    for rate in args.serve_rates:
+        sweep.append(run(rate, args.serve_seconds))
+        log(f"serving {rate:.0f} req/s: {sweep[-1]}")
+    at70 = run(0.7 * peak_req_s, max(args.serve_seconds, 3.0))
+    log(f"serving at 70% of peak: {at70}")
     srv.close()
-    assert not err.any(), err
-    ms = lat / 1e6
-    return {"request_mix": "240 mixed-batch images + 12 8192x8192 4:2:0 q85 (5%)", "peak_req_s": round(peak_req_s, 1),
-            "peak_mpix_s": round(peak_mpix_s, 1), "load_frac": 0.7, "offered_req_s": round(rate, 1),
-            "requests": n, "seconds": round(wall / 1e9, 2), "p50_ms": round(float(np.percentile(ms, 50)), 3),
-            "p99_ms": round(float(np.percentile(ms, 99)), 3), "p999_ms": round(float(np.percentile(ms, 99.9)), 3),
-            "max_batch": 64, "max_wait_us": 2000}
+    return {"request_mix": "240 mixed-batch images + 12 8192x8192 4:2:0 q85 (5%), sampled with replacement",
+            "peak_req_s": round(peak_req_s, 1), "peak_mpix_s": round(peak_mpix_s, 1),
+            "at_70pct_of_peak": at70, "sweep": sweep, "max_batch": 64, "max_wait_us": 2000,
+            "latency": "submit to frame ready in device memory (host monotonic clock)"}
 
 
 def measured_traffic(kernel: str, args) -> int | None:
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this same workload (profiles/hbm_traffic_mixed256.md, counters
     scaled by a copy kernel of known size); None for any other workload."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed256_entries.md")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed256.md")
     if (args.images != 256 or args.seed != 4000 or (args.tile_bits, args.item_tiles) != (8192, 128)
             or not os.path.exists(path)):
         return None
@@ -241,12 +294,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=16,
                     help="processes for the all-cores CPU baseline (the host's share of cores; 1 = off)")
-    ap.add_argument("--plain", action="store_true",
-                    help="also run the plain form (autotune, then one image per launch) and its latencies")
-    ap.add_argument("--serve", action="store_true", help="also run the serving-loop load test")
-    ap.add_argument("--no-plain", action="store_true", help="the default; kept for older command lines")
-    ap.add_argument("--no-serve", action="store_true", help="the default; kept for older command lines")
-    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain form (autotune, then one image per launch) and its latencies")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop load test")
+    ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
+                    default=[500.0, 1000.0, 2000.0, 4000.0, 8000.0], help="Poisson rates of the sweep, req/s")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per swept rate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (reads MIB, writes MIB) "
                          "three times: the reference point for scaling FETCH_SIZE/WRITE_SIZE counters")
@@ -383,7 +436,7 @@ def main():
     plain_mpix = None
     lat = []
     tune_table = None
-    if args.plain and not args.no_plain:
+    if not args.no_plain:
         # per-image entropy parameters for one-image launches, autotuned per
         # scan-size bucket on a few images of the batch
         from cudajpeg_amd.autotune import autotune
@@ -418,7 +471,7 @@ def main():
         for d in per:
             d.close()
 
-    serving = serve_bench(args) if args.serve and not args.no_serve else None
+    serving = None if args.no_serve else serve_bench(args)
     if serving:
         log(f"serving: {serving}")
 

@@ -22,6 +22,7 @@ EXPORTS = [
     "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
     "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_overlap", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
     "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_set_wait_limit", "cjd_batch_protocol_redos",
+    "cjd_batch_status",
     "cjd_batch_finish", "cjd_batch_capture",
     "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_kernel_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
     "cjd_batch_download", "cjd_batch_totals", "cjd_error_name",
@@ -102,6 +103,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_fallbacks": ([vp], i32),
         "cjd_batch_set_wait_limit": ([vp, ctypes.c_int64], i32),
         "cjd_batch_protocol_redos": ([vp], i32),
+        "cjd_batch_status": ([vp, i32, ctypes.POINTER(ctypes.c_uint32)], i32),
         "cjd_batch_finish": ([vp, vp, vp], i32),
         "cjd_batch_capture": ([vp, vp], i32),
         "cjd_batch_replay": ([vp, vp], i32),

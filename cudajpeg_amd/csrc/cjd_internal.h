@@ -154,7 +154,7 @@ struct LookbackRec {
   int32_t agg_c[4];
   uint32_t incl_p, incl_s;
   int32_t incl_c[4];
-  int32_t pad_[3];
+  int32_t pad_[3];  // 72 bytes
 };
 
 constexpr uint32_t kEpochMax = 0xFFFFFFu;

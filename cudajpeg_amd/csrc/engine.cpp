@@ -439,11 +439,19 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
   const uint32_t epoch = b->epoch;
   uint32_t* words = b->h_status.data() + kScratchHead;
   for (size_t i = 0; i < b->status_words; ++i) words[i] = status_bits(words[i], epoch);
+  // Every image the tiled decoder could not resolve, or for which it reports
+  // a syntax error or a truncated scan, is decoded again by the serial
+  // decoder (the reference's order), whose verdict is the one reported: a
+  // tiled error is never reported unless the serial decoder agrees.
+  const uint32_t redo_bits = b->entropy_mode == CJD_ENTROPY_TILED
+                                 ? (kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated)
+                                 : 0u;
   for (size_t i = 0; i < b->plan.recs.size(); ++i) {
     ImageRec& r = b->plan.recs[i];
+    r.decode_error = 0;
     if (r.error) continue;
     const uint32_t st = words[i] | (redo_all ? kStatusUnresolved : 0u);
-    if (st & (kStatusUnresolved | kStatusInconsistent)) {
+    if (st & redo_bits) {
       // a launch of its own, so its status word starts without bits
       uint32_t e2 = 0, w = 0;
       if (int e = next_epoch(b, s, &e2)) return e;
@@ -458,7 +466,7 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
       ++b->fallbacks;
       if (redo) redo->push_back(i);
     }
-    if (words[i] & (kStatusSyntax | kStatusTruncated)) r.error = CJD_SYNTAX_ERROR;
+    if (words[i] & (kStatusSyntax | kStatusTruncated)) r.decode_error = CJD_SYNTAX_ERROR;
   }
   return 0;
 }
@@ -507,7 +515,7 @@ extern "C" int cjd_batch_finish(cjd_batch* b, void* stream, int* errors) {
   }
   if (!redo.empty() && hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   if (errors)
-    for (size_t i = 0; i < b->plan.recs.size(); ++i) errors[i] = b->plan.recs[i].error;
+    for (size_t i = 0; i < b->plan.recs.size(); ++i) errors[i] = b->plan.recs[i].code();
   return 0;
 }
 
@@ -520,6 +528,13 @@ extern "C" int cjd_batch_set_wait_limit(cjd_batch* b, int64_t microseconds) {
 }
 
 extern "C" int cjd_batch_protocol_redos(cjd_batch* b) { return b ? (int)b->protocol_redos : -1; }
+
+extern "C" int cjd_batch_status(cjd_batch* b, int i, uint32_t* bits) {
+  if (!b || !bits || i < 0 || (size_t)i >= b->plan.recs.size() || (size_t)i >= b->status_words)
+    return CJD_PROGRAMMER_ERROR;
+  *bits = b->h_status[kScratchHead + (size_t)i];
+  return 0;
+}
 
 extern "C" int cjd_batch_timings(cjd_batch* b, float* ms_entropy, float* ms_pixel, float* ms_total) {
   if (!b || !b->timed) return CJD_PROGRAMMER_ERROR;
@@ -548,13 +563,13 @@ extern "C" int cjd_batch_kernel_timings(cjd_batch* b, float* ms_entropy_kernel, 
 extern "C" int cjd_batch_image_info(cjd_batch* b, int i, cjd_frame_info* info) {
   if (!b || i < 0 || (size_t)i >= b->plan.recs.size() || !info) return CJD_PROGRAMMER_ERROR;
   *info = b->plan.recs[(size_t)i].f;
-  return b->plan.recs[(size_t)i].error;
+  return b->plan.recs[(size_t)i].code();
 }
 
 extern "C" int cjd_batch_device_ptr(cjd_batch* b, int i, int what, void** ptr, size_t* bytes) {
   if (!b || i < 0 || (size_t)i >= b->plan.recs.size()) return CJD_PROGRAMMER_ERROR;
   const ImageRec& r = b->plan.recs[(size_t)i];
-  if (r.error) return r.error;
+  if (r.code()) return r.code();
   size_t n = 0;
   void* p = nullptr;
   if (what == CJD_OUT_PIXELS) {

@@ -22,6 +22,30 @@ __device__ __forceinline__ void st_status(int32_t* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Lookback records are read and written word by word with agent-scope
+// atomics: a record is written by one work group and read by others, on other
+// XCDs, whose L2 caches are separate; an agent-scope acquire does not drop
+// every line an L2 holds, so a plain load after it could return a copy of the
+// line older than the writer's release (seen as a wrong entry state: syntax
+// and truncation errors on valid scans, now and then, under concurrent
+// decodes).  Atomic accesses at agent scope are coherent across the XCDs.
+__device__ __forceinline__ LookbackRec ld_record(const LookbackRec* r) {
+  LookbackRec v{};
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(r);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+  for (int k = kRecAgg0; k < kRecIncl1; ++k)
+    dst[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return v;
+}
+__device__ __forceinline__ void st_record_words(LookbackRec* r, const LookbackRec& v, int w0, int w1) {
+  uint32_t* dst = reinterpret_cast<uint32_t*>(r);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&v);
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k)
+    if (k >= w0 && k < w1) __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ void k_clear_u32(uint32_t* __restrict__ p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0;
@@ -43,6 +67,7 @@ __device__ __forceinline__ void run_links(EntShared& sh, int tid, const EntCtx& 
       int j;
       if (!task_round(sh, tid, c, T, L, j)) task_put(sh, atomicAdd(&sh.nnext, 1), j, L);
     }
+    ent_drain();  // the round's stream and DC-list stores, read by other waves after the barrier
     __syncthreads();
     if (tid == 0) sh.ntask = sh.nnext;
   }
@@ -128,6 +153,7 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   }
   __syncthreads();
   ent_spec(sh, tid, c);
+  ent_drain();  // checkpoints and lists, read by other waves' links after the barrier
   __syncthreads();
   queue_links(sh, tid, ent_phase2(sh, tid, c));
   run_links(sh, tid, c);
@@ -136,8 +162,10 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   __syncthreads();
   scan_counts(sh, tid, c);
   LookbackRec* me = lb + item_id;
+  LookbackRec mine;
   if (tid == 0 && c.lead) {
-    ent_aggregate(sh, c, me);
+    ent_aggregate(sh, c, mine);
+    st_record_words(me, mine, kRecAgg0, kRecAgg1);
     st_status(&me->status, lookback_word(epoch, kLookAgg));
     // decoupled lookback: compose predecessors' aggregates until one has its
     // inclusive state.  A record of an older launch does not match this
@@ -157,7 +185,8 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
     };
     uint32_t p = 0, s = 0;
     int32_t cc[4] = {0, 0, 0, 0};
-    if (lookback_walk(lb, items, c.sg.first_item, item_id, c.it.prev_item, wait, p, s, cc)) {
+    auto load = [&](uint32_t q) { return ld_record(lb + q); };
+    if (lookback_walk(items, c.sg.first_item, item_id, c.it.prev_item, wait, load, p, s, cc)) {
       sh.entry_p = p;
       sh.entry_s = s;
       for (int q = 0; q < 4; ++q) sh.entry_c[q] = cc[q];
@@ -187,7 +216,8 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   ent_absolute(sh, tid, c);
   __syncthreads();
   if (tid == 0) {
-    ent_inclusive(sh, c, me);
+    ent_inclusive(sh, c, mine);
+    st_record_words(me, mine, kRecIncl0, kRecIncl1);
     st_status(&me->status, lookback_word(epoch, kLookIncl));
     atomicAdd(done, 1ull);  // read by the host after the launch
   }

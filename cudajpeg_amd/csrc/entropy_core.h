@@ -31,6 +31,7 @@
 // streams.  An invalid code ends the block (an invalid DC code consumes one
 // bit), so every trajectory advances.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -246,7 +247,30 @@ CJD_HD inline void status_or(uint32_t* p, uint32_t v, uint32_t epoch) {
       break;
   }
 #else
-  *p = ((*p >> 8) == epoch ? *p : epoch << 8) | v;
+  uint32_t old = __atomic_load_n(p, __ATOMIC_RELAXED);
+  for (;;) {
+    const uint32_t nv = ((old >> 8) == epoch ? old : epoch << 8) | v;
+    if (nv == old || __atomic_compare_exchange_n(p, &old, nv, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) break;
+  }
+#endif
+}
+
+// Words of the entry buffer that one wave of a work group writes and another
+// reads after a barrier (checkpoints, DC lists) are read with agent-scope
+// atomic loads, which go to the L2 instead of the CU's L1, and every lane
+// drains its outstanding stores (ent_drain) before such a barrier: the
+// compiler's barrier waits only for LDS, so without the drain a store can
+// still be in flight when another wave passes the barrier and reads the word.
+CJD_HD inline uint32_t ld_shared_word(const uint32_t* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+  return *p;
+#endif
+}
+CJD_HD inline void ent_drain() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_s_waitcnt(0);  // every outstanding memory operation of the wave
 #endif
 }
 
@@ -590,8 +614,8 @@ CJD_HD inline bool lane_run(EntShared& sh, int tid, const EntCtx& c, Lane& L, ui
   uint32_t thr = L.m < M && t1 < end ? t1 : end;
   uint32_t want_p = ~0u, want_i = ~0u;
   if (kLink && L.m < nck) {
-    want_p = ck[2 * L.m];
-    want_i = ck[2 * L.m + 1];
+    want_p = ld_shared_word(ck + 2 * L.m);
+    want_i = ld_shared_word(ck + 2 * L.m + 1);
   }
   joined = 0;
   L.hit = false;
@@ -621,8 +645,8 @@ CJD_HD inline bool lane_run(EntShared& sh, int tid, const EntCtx& c, Lane& L, ui
       const uint32_t tn = b0 + (uint32_t)(L.m + 1) * C;
       thr = L.m < M && tn < end ? tn : end;
       if (kLink && L.m < nck) {
-        want_p = ck[2 * L.m];
-        want_i = ck[2 * L.m + 1];
+        want_p = ld_shared_word(ck + 2 * L.m);
+        want_i = ld_shared_word(ck + 2 * L.m + 1);
       }
       if (L.pos >= thr) {
         lane_restore(L, true, c.ebuf);
@@ -710,12 +734,32 @@ CJD_HD inline void ent_spec(EntShared& sh, int tid, const EntCtx& c) {
 // occupy one wave instead of every wave of the workgroup.
 constexpr int kLinkBudget = 16;
 
+// The join point's steps and DC entries are those of the speculative pass up
+// to the checkpoint, so they cannot exceed the pass's totals; a checkpoint
+// word that says otherwise (not what this pass wrote) leaves the tile without
+// speculative blocks and the image to the serial decoder, so that no block
+// count ever goes negative.
+CJD_HD inline void set_lovf(EntShared& sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  sh.lovf = 1;
+#else
+  __atomic_store_n(&sh.lovf, 1u, __ATOMIC_RELAXED);
+#endif
+}
+
 CJD_HD inline void link_joined(EntShared& sh, int j, uint32_t ps, uint32_t pdn, uint32_t info) {
+  uint32_t js = (info >> 4) & 0x3FFF, jdn = info >> 18, jslot = info & 15;
+  if (js > sh.ss[j] || jdn > sh.sdn[j]) {
+    set_lovf(sh);
+    js = sh.ss[j];
+    jdn = sh.sdn[j];
+    jslot = 0;
+  }
   sh.ps[j] = (uint16_t)ps;
   sh.pdn[j] = (uint16_t)pdn;
-  sh.js[j] = (uint16_t)((info >> 4) & 0x3FFF);
-  sh.jdn[j] = (uint16_t)(info >> 18);
-  sh.jslot[j] = (uint8_t)(info & 15);
+  sh.js[j] = (uint16_t)js;
+  sh.jdn[j] = (uint16_t)jdn;
+  sh.jslot[j] = (uint8_t)jslot;
   sh.xp[j] = sh.sxp[j];
   sh.xs[j] = sh.sxs[j];
 }
@@ -770,9 +814,11 @@ CJD_HD inline bool task_round(EntShared& sh, int tid, const EntCtx& c, const Tas
   uint32_t* ck = c.ckpts(j);
   int joined;
   if (!lane_run<true>(sh, tid, c, L, t * c.S, c.tile_end(t), ck, sh.nck[j], joined, kLinkBudget)) return false;
-  if (L.ovf) sh.lovf = 1;  // under two bits per symbol over a whole tile: left to the serial decoder
+  // under two bits per symbol over a whole tile: left to the serial decoder
+  // (several lanes may set the flag)
+  if (L.ovf) set_lovf(sh);
   if (joined) {
-    link_joined(sh, j, L.ovf ? 0 : L.s, L.ovf ? 0 : L.dn, ck[2 * (joined - 1) + 1]);
+    link_joined(sh, j, L.ovf ? 0 : L.s, L.ovf ? 0 : L.dn, ld_shared_word(ck + 2 * (joined - 1) + 1));
   } else {
     sh.ps[j] = (uint16_t)(L.ovf ? 0 : L.s);
     sh.pdn[j] = (uint16_t)(L.ovf ? 0 : L.dn);
@@ -827,19 +873,8 @@ template <class F>
 CJD_HD inline void dcl_for(const uint32_t* dcl, uint32_t d0, uint32_t d1, F&& f) {
   for (uint32_t g = d0 & ~15u; g < d1; g += 16) {
     uint32_t v[16];
-#if defined(__HIP_DEVICE_COMPILE__)
-    const uint4* p = reinterpret_cast<const uint4*>(dcl + g);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint4 x = p[q];
-      v[4 * q] = x.x;
-      v[4 * q + 1] = x.y;
-      v[4 * q + 2] = x.z;
-      v[4 * q + 3] = x.w;
-    }
-#else
-    for (int q = 0; q < 16; ++q) v[q] = dcl[g + q];
-#endif
+    for (int q = 0; q < 16; ++q) v[q] = ld_shared_word(dcl + g + q);
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       if (g + q >= d0 && g + q < d1) f(v[q]);
@@ -857,7 +892,7 @@ CJD_HD inline void ent_count(EntShared& sh, int tid, const EntCtx& c) {
         add_dc(q + 1, c.comp(slot), (int16_t)(e & 0xFFFFu));
         slot = slot + 1 == c.nh ? 0 : slot + 1;
       });
-      q[0] += (int32_t)(d1 - d0);
+      q[0] += d1 > d0 ? (int32_t)(d1 - d0) : 0;
     });
   }
   for (int k = 0; k < 4; ++k) sh.u.c.cnt[tid][k] = q[k];
@@ -931,14 +966,17 @@ constexpr int kLookAgg = 1, kLookIncl = 2, kLookFail = 3;
 // and whose immediate predecessor is `prev`: compose the predecessors'
 // aggregates back to one that has its inclusive state.  wait(q, incl)
 // returns record q's status word once it holds a state of this launch (an
-// inclusive or failed one when incl), or -1 when the wait gave up; the
-// record's fields are read only after it returned.  Every step moves to a
-// lower item of the same segment, so a walk ends after at most self - first
-// steps.  False: the entry cannot be resolved here (a wait gave up, a
-// predecessor failed, or a chain index is out of range).
-template <class Wait>
-CJD_HD inline bool lookback_walk(const LookbackRec* lb, const WorkItem* items, uint32_t first, uint32_t self,
-                                 uint32_t prev, Wait&& wait, uint32_t& p, uint32_t& s, int32_t* cc) {
+// inclusive or failed one when incl), or -1 when the wait gave up; load(q)
+// returns record q's fields and is called only after wait(q) returned (on
+// the device both read with agent-scope atomics: the records are written by
+// work groups on other XCDs, each with an L2 of its own, and a plain load
+// could hit a line that this XCD's L2 holds from before the write).  Every
+// step moves to a lower item of the same segment, so a walk ends after at
+// most self - first steps.  False: the entry cannot be resolved here (a wait
+// gave up, a predecessor failed, or a chain index is out of range).
+template <class Wait, class Load>
+CJD_HD inline bool lookback_walk(const WorkItem* items, uint32_t first, uint32_t self, uint32_t prev, Wait&& wait,
+                                 Load&& load, uint32_t& p, uint32_t& s, int32_t* cc) {
   LookAcc acc;
   acc.reset();
   if (prev < first || prev >= self) return false;
@@ -947,37 +985,45 @@ CJD_HD inline bool lookback_walk(const LookbackRec* lb, const WorkItem* items, u
     const int st = wait(q, false);
     if (st < 0 || (st & 3) == kLookFail) return false;
     if ((st & 3) == kLookIncl) {
-      if (acc.apply(lb[q], p, s, cc)) return true;
+      if (acc.apply(load(q), p, s, cc)) return true;
       // an assumption on the way does not hold: take the immediate
       // predecessor's inclusive state
       const int sp = wait(prev, true);
       if (sp < 0 || (sp & 3) != kLookIncl) return false;
       acc.reset();
-      return acc.apply(lb[prev], p, s, cc);
+      return acc.apply(load(prev), p, s, cc);
     }
-    acc.absorb(lb[q]);
+    acc.absorb(load(q));
     const uint32_t nq = items[q].prev_item;
     if (nq < first || nq >= q) return false;
     q = nq;
   }
 }
 
+// Words of a LookbackRec: the aggregate's [kRecAgg0, kRecAgg1), the
+// inclusive state's [kRecIncl0, kRecIncl1); the status word is word 0.
+constexpr int kRecWords = (int)(sizeof(LookbackRec) / 4);
+constexpr int kRecAgg0 = 1, kRecAgg1 = 9, kRecIncl0 = 9, kRecIncl1 = 15;
+static_assert(offsetof(LookbackRec, lead_p) == 4 * kRecAgg0 && offsetof(LookbackRec, incl_p) == 4 * kRecIncl0 &&
+                  offsetof(LookbackRec, pad_) == 4 * kRecIncl1 && kRecWords >= kRecIncl1,
+              "lookback record layout");
+
 // Aggregate of this item (tc scanned: relative cumulative counts): assumes
 // the item's entry is the speculative exit of its lead-in tile.
-CJD_HD inline void ent_aggregate(const EntShared& sh, const EntCtx& c, LookbackRec* me) {
-  me->lead_p = sh.sxp[0];
-  me->lead_s = sh.sxs[0];
-  me->agg_p = sh.xp[c.J - 1];
-  me->agg_s = sh.xs[c.J - 1];
-  for (int q = 0; q < 4; ++q) me->agg_c[q] = sh.u.c.tc[c.J - 1][q];
+CJD_HD inline void ent_aggregate(const EntShared& sh, const EntCtx& c, LookbackRec& me) {
+  me.lead_p = sh.sxp[0];
+  me.lead_s = sh.sxs[0];
+  me.agg_p = sh.xp[c.J - 1];
+  me.agg_s = sh.xs[c.J - 1];
+  for (int q = 0; q < 4; ++q) me.agg_c[q] = sh.u.c.tc[c.J - 1][q];
 }
 
 // Inclusive state of this item: the true exit of its last tile and the
 // segment's cumulative counts there (tc absolute).
-CJD_HD inline void ent_inclusive(const EntShared& sh, const EntCtx& c, LookbackRec* me) {
-  me->incl_p = sh.xp[c.J - 1];
-  me->incl_s = sh.xs[c.J - 1];
-  for (int q = 0; q < 4; ++q) me->incl_c[q] = sh.u.c.tc[c.J - 1][q];
+CJD_HD inline void ent_inclusive(const EntShared& sh, const EntCtx& c, LookbackRec& me) {
+  me.incl_p = sh.xp[c.J - 1];
+  me.incl_s = sh.xs[c.J - 1];
+  for (int q = 0; q < 4; ++q) me.incl_c[q] = sh.u.c.tc[c.J - 1][q];
 }
 
 // After the lookback: whether the true entry differs from the assumed one
@@ -997,6 +1043,8 @@ CJD_HD inline void ent_absolute(EntShared& sh, int tid, const EntCtx& c) {
 CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, BlockRec* recs, uint32_t* status,
                                uint32_t epoch) {
   if (tid >= c.J || (c.lead && tid == 0)) return;
+  // only blocks of this segment get records (A in [first_block, seg_end)),
+  // whatever the counts say
   const int64_t seg_end = c.sg.first_block + (int64_t)c.sg.nblocks;
   int64_t A = c.sg.first_block + (tid > 0 ? sh.u.c.tc[tid - 1][0] : 0);
   int32_t pred[3] = {0, 0, 0};
@@ -1013,7 +1061,7 @@ CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, Bl
       const uint32_t st = e >> 17;
       const int comp = c.comp(slot);
       add_dc(pred, comp, (int16_t)(e & 0xFFFFu));
-      if (A < seg_end) {
+      if (A >= c.sg.first_block && A < seg_end) {
         BlockRec r;
         r.eidx = (uint32_t)entry_word(base, W, st + 1);
         r.nw = (uint16_t)((sn - st - 1) | ((W / 16 - 1) << 8));

@@ -14,7 +14,9 @@
 namespace cjd {
 
 struct ImageRec {
-  int error = 0;
+  int error = 0;         // parse / staging result (reference error code)
+  int decode_error = 0;  // result of the last decode (reset by each check)
+  int code() const { return error ? error : decode_error; }
   cjd_frame_info f{};
   int64_t pixels = 0;
   uint64_t out_offset = 0, out_bytes = 0;

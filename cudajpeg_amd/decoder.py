@@ -148,6 +148,13 @@ class BatchDecoder:
         the predecessor has published; results do not depend on it)."""
         _check(L.lib().cjd_batch_set_wait_limit(self.h, int(microseconds)), "set_wait_limit")
 
+    def status(self, i: int) -> int:
+        """Status bits of image i after the last finish (1 syntax, 2 truncated,
+        4 unresolved, 8 entries did not fit, 0x100 decoded again serially)."""
+        v = ctypes.c_uint32()
+        _check(L.lib().cjd_batch_status(self.h, i, ctypes.byref(v)), "status")
+        return int(v.value)
+
     def protocol_redos(self) -> int:
         """Checks after a decode that found a work item never decoded (the
         whole batch was then decoded again serially); 0 in a correct run."""

@@ -172,6 +172,14 @@ int cjd_batch_set_wait_limit(cjd_batch *b, int64_t microseconds);
  * Zero in a correct run.  No counterpart in the reference. */
 int cjd_batch_protocol_redos(cjd_batch *b);
 
+/* Status bits of image i after the last cjd_batch_finish / decode_scan: 1 an
+ * invalid code or coefficient index (SYNTAX_ERROR), 2 the scan ended before
+ * the frame's blocks, 4 the tiled decoder could not resolve the image, 8 the
+ * image's entries did not fit the tiled decoder's streams, 0x100 the serial
+ * decoder decoded the image again (the bits are then the serial decoder's,
+ * with 0x100).  For diagnostics; no counterpart in the reference. */
+int cjd_batch_status(cjd_batch *b, int i, uint32_t *bits);
+
 /* Wait for the decode, run the serial entropy decoder for any image whose
  * tile links did not resolve, and report per-image reference error codes. */
 int cjd_batch_finish(cjd_batch *b, void *stream, int *errors);

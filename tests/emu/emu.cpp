@@ -123,8 +123,8 @@ void resolve(EntRun& R, size_t i) {
     };
     uint32_t p = 0, s = 0;
     int32_t cc[4] = {0, 0, 0, 0};
-    if (!lookback_walk(R.lb.data(), R.p->items.data(), c.sg.first_item, (uint32_t)i, c.it.prev_item, wait, p, s,
-                       cc)) {
+    auto load = [&](uint32_t q) { return R.lb[q]; };
+    if (!lookback_walk(R.p->items.data(), c.sg.first_item, (uint32_t)i, c.it.prev_item, wait, load, p, s, cc)) {
       R.lb[i].status = lookback_word(kEpoch, kLookFail);
       R.failed[i] = 1;
       R.done[i] = 1;
@@ -142,7 +142,7 @@ void resolve(EntRun& R, size_t i) {
     }
   }
   for (int t = 0; t < kWG; ++t) ent_absolute(sh, t, c);
-  ent_inclusive(sh, c, &R.lb[i]);
+  ent_inclusive(sh, c, R.lb[i]);
   R.lb[i].status = lookback_word(kEpoch, kLookIncl);
   R.done[i] = 1;
 }
@@ -166,10 +166,19 @@ void expand(Emu& E) {
   }
 }
 
+// Entry buffer left by the previous emulated launch (carry-over mode): a
+// launch must not depend on what an earlier batch left in its regions.
+std::vector<uint32_t> g_carry;
+bool g_carry_on = false;
+uint64_t g_lds_seed = 0;  // nonzero: every work group's LDS starts as random bytes
+int g_plant_ck = 0;       // nonzero: checkpoint words of every g_plant_ck-th tile claim too many DC entries
+
 void run_entropy(Emu& E, int order) {
   Plan& p = E.plan;
   const size_t ni = p.items.size();
   E.ebuf.assign((size_t)p.entry_words, 0x7EADBEEFu);
+  if (g_carry_on)
+    for (size_t k = 0; k < E.ebuf.size() && k < g_carry.size(); ++k) E.ebuf[k] = g_carry[k];
   EntRun R;
   R.p = &p;
   // records left by an older launch (epoch kEpoch - 1, inclusive), which the
@@ -187,7 +196,19 @@ void run_entropy(Emu& E, int order) {
   for (size_t i = 0; i < ni; ++i) {
     R.shs[i].reset(new EntShared);
     EntShared& sh = *R.shs[i];
-    memset(&sh, 0, sizeof(sh));
+    // the work group's LDS as a previous occupant of the CU left it
+    if (g_lds_seed) {
+      uint64_t x = g_lds_seed * 0x9E3779B97F4A7C15ull + i;
+      uint8_t* b = reinterpret_cast<uint8_t*>(&sh);
+      for (size_t k = 0; k < sizeof(sh); ++k) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        b[k] = (uint8_t)x;
+      }
+    } else {
+      memset(&sh, 0, sizeof(sh));
+    }
     EntCtx& c = R.ctx[i];
     c.init(p.words.data(), p.images.data(), p.segs.data(), p.tables.data(), E.ebuf.data(), p.items[i]);
     memcpy(sh.lut, &p.luts[(size_t)c.sg.image * kLutEntries], sizeof(sh.lut));
@@ -198,6 +219,10 @@ void run_entropy(Emu& E, int order) {
       sh.tsel[2 * k + 1] = (uint16_t)(c.im->slot_dc[k] * kL1Size);
     }
     for (int t = 0; t < kWG; ++t) ent_spec(sh, t, c);
+    if (g_plant_ck)  // failure injection: checkpoint words this pass did not write
+      for (int t = 0; t < c.J; ++t)
+        if ((c.tfirst + (uint32_t)t) % (uint32_t)g_plant_ck == (uint32_t)g_plant_ck / 2)
+          for (int m = 0; m < sh.nck[t]; ++m) c.ckpts(t)[2 * m + 1] |= 0x3FFFu << 18;
     std::vector<char> task(kWG);
     for (int t = 0; t < kWG; ++t) task[(size_t)t] = ent_phase2(sh, t, c);
     queue_links(sh, task);
@@ -206,7 +231,7 @@ void run_entropy(Emu& E, int order) {
     count_all(sh, c);
     scan_counts(sh, c);
     if (c.lead) {
-      ent_aggregate(sh, c, &R.lb[i]);
+      ent_aggregate(sh, c, R.lb[i]);
       R.lb[i].status = lookback_word(kEpoch, 1);
     }
   }
@@ -219,6 +244,7 @@ void run_entropy(Emu& E, int order) {
       if (!R.ctx[i].lead) resolve(R, i);
   for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
   E.fixes = R.fixes;
+  if (g_carry_on) g_carry = E.ebuf;
   for (size_t i = 0; i < ni; ++i) {
     if (R.failed[i]) {
       status_or(E.status.data() + R.ctx[i].im->error_index, kStatusUnresolved, kEpoch);
@@ -296,7 +322,9 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
     if (r.error) continue;
     uint32_t& st = E->bits[(size_t)i];
     st = status_bits(E->status[(size_t)i], kEpoch);
-    if (st & (kStatusUnresolved | kStatusInconsistent)) {
+    // as the engine: unresolved images and tiled syntax / truncation errors
+    // are decoded again serially, and the serial verdict is reported
+    if (!serial && (st & (kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated))) {
       // the serial redo is a launch of its own (next epoch)
       ++E->fallbacks;
       for (uint32_t s = 0; s < r.seg_count; ++s)
@@ -355,5 +383,19 @@ void emu_status(void* h, uint32_t* out, int n) {
 }
 
 void emu_free(void* h) { delete static_cast<Emu*>(h); }
+
+// Failure injection: after each speculative pass the checkpoint words of every
+// `every`-th tile claim more DC entries than the pass decoded (0 = off).
+void emu_plant_ck(int every) { g_plant_ck = every; }
+
+// Initial LDS of every emulated work group: random bytes from `seed` (0 = zeroed).
+void emu_lds_seed(unsigned long long seed) { g_lds_seed = seed; }
+
+// Carry-over mode: every launch starts with the entry buffer the previous
+// launch left (on = 1), instead of a fixed fill; 0 turns it off and forgets.
+void emu_carry(int on) {
+  g_carry_on = on != 0;
+  if (!on) g_carry.clear();
+}
 
 }  // extern "C"

@@ -2,6 +2,8 @@
 compiled for the CPU and run lane by lane, against the oracle: coefficient
 parity of the tile/link/lookback entropy decoder and pixel parity of the fused
 pixel stage, without a GPU."""
+import sys
+
 import numpy as np
 import pytest
 
@@ -162,3 +164,75 @@ def test_emulated_lookback_give_up_is_redone_serially():
     small = synth.encode(synth.ImageSpec(64, 48, "420", 75, 6.0, 1), truth=True)
     r = EmuResult([small.data], 0, 4096, False, 2)
     assert r.fallbacks == 0 and np.array_equal(r.get(0, 1, small.coefs.shape, np.int16), small.coefs)
+
+
+def test_emulated_inconsistent_checkpoints_stay_inside_their_image():
+    # Failure injection on the CPU: checkpoint words that claim more DC
+    # entries than their speculative pass decoded (as a word this pass never
+    # wrote would).  A link that joins at one of them must not produce a
+    # negative block count -- which would make the record phase write block
+    # records below its image's first block, into other images' records or
+    # outside the buffer -- but must leave the image to the serial decoder:
+    # every image of the batch still decodes bit-exact.
+    import ctypes
+    from tests.emu import lib
+    L = lib()
+    L.emu_plant_ck.argtypes = [ctypes.c_int]
+    encs = [synth.encode(s, truth=True) for s in HARD]
+    L.emu_plant_ck(5)
+    try:
+        r = EmuResult([e.data for e in encs], 0, 2048, False, 0, False, 8)
+    finally:
+        L.emu_plant_ck(0)
+    assert not r.errors.any()
+    assert r.fallbacks > 0
+    for i, (s, e) in enumerate(zip(HARD, encs)):
+        assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
+
+
+_TSAN_CHILD = r"""
+import ctypes, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from cudajpeg_amd import synth
+L = ctypes.CDLL(sys.argv[2])
+L.emt_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                      ctypes.c_void_p]
+specs = [synth.ImageSpec(487, 487, "420", 88, 9.0, 77), synth.ImageSpec(400, 300, "444", 95, 12.0, 15)]
+encs = [synth.encode(s, truth=True) for s in specs]
+n = len(encs)
+keep = [ctypes.create_string_buffer(e.data, len(e.data)) for e in encs]
+ptrs = (ctypes.c_void_p * n)(*[ctypes.cast(b, ctypes.c_void_p) for b in keep])
+sizes = (ctypes.c_size_t * n)(*[len(e.data) for e in encs])
+for tile, items in ((2048, 8), (4096, 256)):
+    outs = [np.zeros_like(e.coefs) for e in encs]
+    optr = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+    bits = np.zeros(n, np.uint32)
+    L.emt_run(n, ptrs, sizes, tile, items, optr, bits.ctypes.data)
+    for e, o, b in zip(encs, outs, bits):
+        print("exact", int(b == 0 and np.array_equal(o, e.coefs)), flush=True)
+"""
+
+
+def test_kernel_body_has_no_lane_races_under_thread_sanitizer():
+    # The body of k_entropy run with one host thread per lane and a barrier
+    # wherever the kernel has one (tests/emu/emu_threads.cpp), built with
+    # ThreadSanitizer: lanes really run at the same time between barriers, so
+    # an unsynchronised conflicting access between lanes (LDS state or entry
+    # buffer) is reported.  Items of 8 tiles (many links, fix-ups and
+    # lookbacks) and of 256 tiles; results must be exact as well.
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu")
+    so = os.path.join(here, "libcjd_emt_tsan.so")
+    subprocess.run(["make", "-s", "-C", here], check=True)
+    rt = subprocess.run(["g++", "-print-file-name=libtsan.so"], capture_output=True, text=True).stdout.strip()
+    if not os.path.isfile(rt):
+        pytest.skip("no ThreadSanitizer runtime")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = {**os.environ, "LD_PRELOAD": rt, "TSAN_OPTIONS": "report_signal_unsafe=0 halt_on_error=0 exitcode=0"}
+    r = subprocess.run([sys.executable, "-c", _TSAN_CHILD, root, so], capture_output=True, text=True, env=env,
+                       timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
+    assert r.stdout.count("exact 1") == 4, r.stdout

@@ -1,0 +1,42 @@
+"""Batches decoded at the same time on several streams: the tiled decoder's
+lookback across work groups (on any of the XCDs) must resolve every image's
+entry states, whatever the timing.  Items of 16 tiles of 2048 bits give long
+lookback chains; four batches of the serving mix (with 8192x8192 outliers) on
+four streams, three times; every image's coefficients equal the encoder's,
+with no error, no serial redo and no undecoded work item."""
+import numpy as np
+import pytest
+import torch
+
+from cudajpeg_amd import IDCT_MFMA, BatchDecoder, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_concurrent_batches_resolve_every_image(gpu):
+    rng = np.random.default_rng(7007)
+    specs = synth.config_mixed(48, 7000)
+    specs += [synth.ImageSpec(8192, 8192, "420", 85, float(rng.uniform(2, 12)), 7100 + k) for k in range(4)]
+    specs = [specs[i] for i in rng.permutation(len(specs))]
+    encs = synth.encode_many(specs, truth=True)
+    groups = [list(range(k, len(encs), 4)) for k in range(4)]
+    decs = [BatchDecoder(IDCT_MFMA, tile_bits=2048) for _ in groups]
+    streams = [torch.cuda.Stream() for _ in groups]
+    for d in decs:
+        d.set_tuning(0, 16)
+    for rep in range(3):
+        print("case concurrent batches rep", rep, flush=True)
+        for d, g in zip(decs, groups):
+            assert not d.stage([encs[i].data for i in g]).any()
+        for d, st in zip(decs, streams):
+            d.decode(st)
+        for d, st, g in zip(decs, streams, groups):
+            errs = d.finish(st)
+            bad = [(g[i], hex(d.status(i))) for i in np.flatnonzero(errs)]
+            assert not bad, (rep, bad)
+        for d, g in zip(decs, groups):
+            assert d.fallbacks() == 0 and d.protocol_redos() == 0
+            for i, k in enumerate(g):
+                assert np.array_equal(d.coefficients(i), encs[k].coefs), (rep, k, specs[k])
+    for d in decs:
+        d.close()
