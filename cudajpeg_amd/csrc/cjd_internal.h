@@ -159,14 +159,16 @@ struct LookbackRec {
 
 constexpr uint32_t kEpochMax = 0xFFFFFFu;
 
-// Head of a batch's scratch: for each of the (at most kLaunchCounters)
-// entropy launches of one decode, a 64-bit work-item ticket counter (words
-// 2j, 2j+1) and a 64-bit counter of work items finished (words 8+2j, 9+2j).
-// Neither is ever cleared between decodes: launch j of every decode takes its
-// tickets from counter j, so launches never share one, and after the decodes
-// both counters of launch j equal the tickets the host handed out to it.
-constexpr int kLaunchCounters = 4;
-constexpr int kScratchHead = 16;
+// Head of a batch's scratch: for each kind j of entropy launch (j < 4: run j
+// of a decode in parts; kRedoCounter: one image decoded again alone), a 64-bit
+// work-item ticket counter (words 2j, 2j+1) and a 64-bit counter of work items
+// finished (words 2 kLaunchCounters + 2j, + 1).  Neither is ever cleared
+// between decodes: launch kind j always takes its tickets from counter j, so
+// launches never share one, and after the decodes both counters of kind j
+// equal the tickets the host handed out to it.
+constexpr int kLaunchCounters = 8;
+constexpr int kRedoCounter = 4;
+constexpr int kScratchHead = 4 * kLaunchCounters;
 
 // Default limit of one lookback wait, in ticks of the 100 MHz real-time
 // counter (200 ms: far above the longest wait of a valid decode).
@@ -184,7 +186,11 @@ enum : uint32_t {
   kStatusUnresolved = 4u,    // the decoder state could not be resolved; rerun with the serial decoder
   kStatusInconsistent = 8u,  // the tiled decoder could not hold the image's entries (invalid stream); rerun serially
   kStatusFallback = 0x100u,  // host: the image was decoded again by the serial decoder
+  kStatusRelaunch = 0x200u,  // host: the image was decoded again by the tiled decoder alone
 };
+// host: bits 16-23 of an image's status after a redo hold the first tiled
+// launch's bits that led to it
+constexpr int kStatusFirstShift = 16;
 
 enum IdctMode : int { kIdctInt = 0, kIdctFloat = 1, kIdctMfma = 2 };
 // 16-byte MFMA table operands per image: 3 tables x 2 K halves x (hi, lo) x 256 lanes.

@@ -27,6 +27,10 @@ namespace {
 // Decode in parts (cjd_batch_set_overlap): the most runs, and the default.
 constexpr int kMaxParts = 4;
 constexpr int kDefaultParts = 2;
+// Solo relaunches of the tiled decoder for an image before the serial decoder
+// (which decodes a segment on one lane: tens of seconds for an 8192x8192 scan
+// without restart markers).
+constexpr int kRelaunches = 3;
 
 // Grow-only device buffer (src/format.cu:81-110 policy).
 struct DevBuf {
@@ -308,7 +312,7 @@ static unsigned long long* ticket_of(cjd_batch* b, int j) {
   return reinterpret_cast<unsigned long long*>(b->d_scratch.as<uint32_t>()) + j;
 }
 static unsigned long long* done_of(cjd_batch* b, int j) {
-  return reinterpret_cast<unsigned long long*>(b->d_scratch.as<uint32_t>() + 8) + j;
+  return reinterpret_cast<unsigned long long*>(b->d_scratch.as<uint32_t>() + 2 * kLaunchCounters) + j;
 }
 
 static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
@@ -440,29 +444,59 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
   uint32_t* words = b->h_status.data() + kScratchHead;
   for (size_t i = 0; i < b->status_words; ++i) words[i] = status_bits(words[i], epoch);
   // Every image the tiled decoder could not resolve, or for which it reports
-  // a syntax error or a truncated scan, is decoded again by the serial
-  // decoder (the reference's order), whose verdict is the one reported: a
-  // tiled error is never reported unless the serial decoder agrees.
-  const uint32_t redo_bits = b->entropy_mode == CJD_ENTROPY_TILED
-                                 ? (kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated)
-                                 : 0u;
+  // a syntax error or a truncated scan, is decoded again: first by the tiled
+  // decoder in a launch of its own over the image's work items alone, then,
+  // if that still reports anything, by the serial decoder (the reference's
+  // order), whose verdict is the one reported -- a tiled error is never
+  // reported unless the serial decoder agrees.  Each redo launch has an epoch
+  // of its own, so the image's status word starts without bits.
+  const bool tiled = b->entropy_mode == CJD_ENTROPY_TILED;
+  const uint32_t redo_bits =
+      tiled ? (kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated) : 0u;
+  uint32_t* status = status_of(b);
+  auto read_status = [&](size_t i, uint32_t epoch_of, uint32_t& bits) -> int {
+    uint32_t w = 0;
+    if (hipMemcpyAsync(&w, status + i, 4, hipMemcpyDeviceToHost, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+    if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+    bits = status_bits(w, epoch_of);
+    return 0;
+  };
   for (size_t i = 0; i < b->plan.recs.size(); ++i) {
     ImageRec& r = b->plan.recs[i];
     r.decode_error = 0;
     if (r.error) continue;
     const uint32_t st = words[i] | (redo_all ? kStatusUnresolved : 0u);
     if (st & redo_bits) {
-      // a launch of its own, so its status word starts without bits
-      uint32_t e2 = 0, w = 0;
-      if (int e = next_epoch(b, s, &e2)) return e;
-      uint32_t* status = status_of(b);
-      if (launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(),
-                                b->d_segs.as<SegmentDesc>() + r.seg_first, r.seg_count, b->d_tables.as<HuffTable>(),
-                                b->d_coefs.as<int16_t>(), b->d_recs.as<BlockRec>(), status, e2, s) != hipSuccess)
-        return CJD_KERNEL_LAUNCH_ERROR;
-      if (hipMemcpyAsync(&w, status + i, 4, hipMemcpyDeviceToHost, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
-      if (hipStreamSynchronize(s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
-      words[i] = status_bits(w, e2) | kStatusFallback;
+      const uint32_t first = (st & 0xFFu) << kStatusFirstShift;
+      bool redone = false;
+      uint32_t e2 = 0, bits = 0;
+      for (int attempt = 0; attempt < kRelaunches && r.item_count && !redone; ++attempt) {
+        if (int e = next_epoch(b, s, &e2)) return e;
+        LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
+        const hipError_t he = launch_entropy(
+            b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
+            b->d_items.as<WorkItem>(), r.item_first, r.item_first + r.item_count, b->d_tables.as<HuffTable>(),
+            b->d_luts.as<uint16_t>(), b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb,
+            ticket_of(b, kRedoCounter), done_of(b, kRedoCounter), b->ticket_base[kRedoCounter], e2, b->wait_ticks, s);
+        b->ticket_base[kRedoCounter] += r.item_count;
+        if (he != hipSuccess) {
+          b->ticket_unknown = true;
+          return CJD_KERNEL_LAUNCH_ERROR;
+        }
+        if (int e = read_status(i, e2, bits)) return e;
+        redone = !(bits & redo_bits);
+        if (redone) words[i] = bits | kStatusRelaunch | first;
+      }
+      if (!redone) {
+        if (int e = next_epoch(b, s, &e2)) return e;
+        if (launch_entropy_serial(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(),
+                                  b->d_segs.as<SegmentDesc>() + r.seg_first, r.seg_count, b->d_tables.as<HuffTable>(),
+                                  b->d_coefs.as<int16_t>(), b->d_recs.as<BlockRec>(), status, e2, s) != hipSuccess)
+          return CJD_KERNEL_LAUNCH_ERROR;
+        if (int e = read_status(i, e2, bits)) return e;
+        words[i] = bits | kStatusFallback | first;
+      }
+      b->coefs_stale = true;
       ++b->fallbacks;
       if (redo) redo->push_back(i);
     }

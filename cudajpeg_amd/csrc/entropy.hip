@@ -67,33 +67,38 @@ __device__ __forceinline__ void run_links(EntShared& sh, int tid, const EntCtx& 
       int j;
       if (!task_round(sh, tid, c, T, L, j)) task_put(sh, atomicAdd(&sh.nnext, 1), j, L);
     }
-    ent_drain();  // the round's stream and DC-list stores, read by other waves after the barrier
     __syncthreads();
     if (tid == 0) sh.ntask = sh.nnext;
   }
 }
 
-// Queue a link task for tile tid when `need` (all lanes call this).
-__device__ __forceinline__ void queue_links(EntShared& sh, int tid, bool need) {
-  if (tid == 0) sh.ntask = 0;
+// Queue a link task for tile tid when `task`; sh.nneed counts the lanes with
+// `need` (all lanes call this; both counts are read after the next barrier).
+__device__ __forceinline__ void queue_links(EntShared& sh, int tid, bool task, bool need = false) {
+  if (tid == 0) {
+    sh.ntask = 0;
+    sh.nneed = 0;
+  }
   __syncthreads();
-  if (need) task_new(sh, atomicAdd(&sh.ntask, 1), tid);
+  if (task) task_new(sh, atomicAdd(&sh.ntask, 1), tid);
+  if (need) atomicAdd(&sh.nneed, 1);
 }
 
 // Fix-up rounds (all lanes): relink every tile whose entry changed, until no
-// tile changes.
+// tile changes.  The "did any tile change" decision is a count in LDS taken
+// beside the task count, not a library work-group reduction.
 __device__ __forceinline__ void fix_rounds(EntShared& sh, int tid, const EntCtx& c) {
   for (;;) {
     uint32_t p = 0, s = 0;
     const bool need = ent_fix_need(sh, tid, c, p, s);
     __syncthreads();  // reads of the previous state done
     const bool task = need && link_begin(sh, c, tid, p, s);
-    queue_links(sh, tid, task);
+    queue_links(sh, tid, task, need);
     __syncthreads();
-    const int n = sh.ntask;
+    const int n = sh.ntask, changed = sh.nneed;
     if (!n) {
       // a relink may have joined at once without a task: the exits moved, check again
-      if (!__syncthreads_or(need)) break;
+      if (!changed) break;
       continue;
     }
     run_links(sh, tid, c);
@@ -153,7 +158,6 @@ __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ wo
   }
   __syncthreads();
   ent_spec(sh, tid, c);
-  ent_drain();  // checkpoints and lists, read by other waves' links after the barrier
   __syncthreads();
   queue_links(sh, tid, ent_phase2(sh, tid, c));
   run_links(sh, tid, c);

@@ -255,25 +255,6 @@ CJD_HD inline void status_or(uint32_t* p, uint32_t v, uint32_t epoch) {
 #endif
 }
 
-// Words of the entry buffer that one wave of a work group writes and another
-// reads after a barrier (checkpoints, DC lists) are read with agent-scope
-// atomic loads, which go to the L2 instead of the CU's L1, and every lane
-// drains its outstanding stores (ent_drain) before such a barrier: the
-// compiler's barrier waits only for LDS, so without the drain a store can
-// still be in flight when another wave passes the barrier and reads the word.
-CJD_HD inline uint32_t ld_shared_word(const uint32_t* p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#else
-  return *p;
-#endif
-}
-CJD_HD inline void ent_drain() {
-#if defined(__HIP_DEVICE_COMPILE__)
-  __builtin_amdgcn_s_waitcnt(0);  // every outstanding memory operation of the wave
-#endif
-}
-
 // ------------------------------------------------------ tiled decoder --
 //
 // Lookup entry of a code of `len` bits for symbol `sym` (layout in
@@ -338,7 +319,7 @@ struct EntShared {
   } u;
   int32_t ntask, nnext;
   uint32_t item;
-  int32_t nchg;                    // tiles relinked in the current fix-up round
+  int32_t nneed;                   // tiles whose entry changed in the current fix-up round
   uint32_t lovf;                   // a link needed more entries than its tile has bits
   uint32_t look_ok;                // the lookback resolved the item's entry
   uint32_t entry_p, entry_s;       // true state at the exit of the lead-in tile
@@ -614,8 +595,8 @@ CJD_HD inline bool lane_run(EntShared& sh, int tid, const EntCtx& c, Lane& L, ui
   uint32_t thr = L.m < M && t1 < end ? t1 : end;
   uint32_t want_p = ~0u, want_i = ~0u;
   if (kLink && L.m < nck) {
-    want_p = ld_shared_word(ck + 2 * L.m);
-    want_i = ld_shared_word(ck + 2 * L.m + 1);
+    want_p = ck[2 * L.m];
+    want_i = ck[2 * L.m + 1];
   }
   joined = 0;
   L.hit = false;
@@ -645,8 +626,8 @@ CJD_HD inline bool lane_run(EntShared& sh, int tid, const EntCtx& c, Lane& L, ui
       const uint32_t tn = b0 + (uint32_t)(L.m + 1) * C;
       thr = L.m < M && tn < end ? tn : end;
       if (kLink && L.m < nck) {
-        want_p = ld_shared_word(ck + 2 * L.m);
-        want_i = ld_shared_word(ck + 2 * L.m + 1);
+        want_p = ck[2 * L.m];
+        want_i = ck[2 * L.m + 1];
       }
       if (L.pos >= thr) {
         lane_restore(L, true, c.ebuf);
@@ -818,7 +799,7 @@ CJD_HD inline bool task_round(EntShared& sh, int tid, const EntCtx& c, const Tas
   // (several lanes may set the flag)
   if (L.ovf) set_lovf(sh);
   if (joined) {
-    link_joined(sh, j, L.ovf ? 0 : L.s, L.ovf ? 0 : L.dn, ld_shared_word(ck + 2 * (joined - 1) + 1));
+    link_joined(sh, j, L.ovf ? 0 : L.s, L.ovf ? 0 : L.dn, ck[2 * (joined - 1) + 1]);
   } else {
     sh.ps[j] = (uint16_t)(L.ovf ? 0 : L.s);
     sh.pdn[j] = (uint16_t)(L.ovf ? 0 : L.dn);
@@ -873,8 +854,19 @@ template <class F>
 CJD_HD inline void dcl_for(const uint32_t* dcl, uint32_t d0, uint32_t d1, F&& f) {
   for (uint32_t g = d0 & ~15u; g < d1; g += 16) {
     uint32_t v[16];
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4* p = reinterpret_cast<const uint4*>(dcl + g);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = ld_shared_word(dcl + g + q);
+    for (int q = 0; q < 4; ++q) {
+      const uint4 x = p[q];
+      v[4 * q] = x.x;
+      v[4 * q + 1] = x.y;
+      v[4 * q + 2] = x.z;
+      v[4 * q + 3] = x.w;
+    }
+#else
+    for (int q = 0; q < 16; ++q) v[q] = dcl[g + q];
+#endif
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       if (g + q >= d0 && g + q < d1) f(v[q]);

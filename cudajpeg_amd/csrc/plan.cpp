@@ -471,6 +471,8 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     r.desc = (uint32_t)w.desc;
     r.seg_first = (uint32_t)w.seg0;
     r.seg_count = (uint32_t)w.nseg;
+    r.item_first = (uint32_t)w.item0;
+    r.item_count = (uint32_t)w.nitems;
     r.pix_first = (uint32_t)w.pix0;
     r.pix_count = (uint32_t)w.npix;
     r.pixels = (int64_t)f.width * f.height;

@@ -24,6 +24,7 @@ struct ImageRec {
   int64_t coef_base = 0;
   uint32_t desc = 0;  // ImageDesc index (valid when error == 0)
   uint32_t seg_first = 0, seg_count = 0;
+  uint32_t item_first = 0, item_count = 0;  // entropy work items (contiguous)
   uint32_t pix_first = 0, pix_count = 0;
 };
 

@@ -154,7 +154,10 @@ int cjd_batch_decode(cjd_batch *b, void *stream);
 int cjd_batch_decode_scan(cjd_batch *b, void *stream);
 int cjd_batch_transform(cjd_batch *b, void *stream);
 
-/* Number of images the serial decoder has redone since the batch was created. */
+/* Number of images decoded again since the batch was created: images the
+ * tiled decoder could not resolve or reported an error for are decoded again
+ * by the tiled decoder alone and, if that still reports anything, by the
+ * serial decoder (cjd_batch_status tells which). */
 int cjd_batch_fallbacks(cjd_batch *b);
 
 /* Limit of one wait of the tiled decoder's lookback for a predecessor work
@@ -175,9 +178,11 @@ int cjd_batch_protocol_redos(cjd_batch *b);
 /* Status bits of image i after the last cjd_batch_finish / decode_scan: 1 an
  * invalid code or coefficient index (SYNTAX_ERROR), 2 the scan ended before
  * the frame's blocks, 4 the tiled decoder could not resolve the image, 8 the
- * image's entries did not fit the tiled decoder's streams, 0x100 the serial
- * decoder decoded the image again (the bits are then the serial decoder's,
- * with 0x100).  For diagnostics; no counterpart in the reference. */
+ * image's entries did not fit the tiled decoder's streams; after a redo, 0x200
+ * the tiled decoder decoded the image again alone or 0x100 the serial decoder
+ * did (bits 0-7 are then that decode's), and bits 16-23 hold the first
+ * launch's bits that led to the redo.  For diagnostics; no counterpart in the
+ * reference. */
 int cjd_batch_status(cjd_batch *b, int i, uint32_t *bits);
 
 /* Wait for the decode, run the serial entropy decoder for any image whose

@@ -197,7 +197,9 @@ void run_entropy(Emu& E, int order) {
     R.shs[i].reset(new EntShared);
     EntShared& sh = *R.shs[i];
     // the work group's LDS as a previous occupant of the CU left it
-    if (g_lds_seed) {
+    if (g_lds_seed == ~0ull) {
+      // left uninitialised (a memory-sanitizer build reports any read of it)
+    } else if (g_lds_seed) {
       uint64_t x = g_lds_seed * 0x9E3779B97F4A7C15ull + i;
       uint8_t* b = reinterpret_cast<uint8_t*>(&sh);
       for (size_t k = 0; k < sizeof(sh); ++k) {
@@ -209,6 +211,8 @@ void run_entropy(Emu& E, int order) {
     } else {
       memset(&sh, 0, sizeof(sh));
     }
+    sh.lovf = 0;  // what the kernel's lane 0 sets before the first barrier
+    sh.look_ok = 1;
     EntCtx& c = R.ctx[i];
     c.init(p.words.data(), p.images.data(), p.segs.data(), p.tables.data(), E.ebuf.data(), p.items[i]);
     memcpy(sh.lut, &p.luts[(size_t)c.sg.image * kLutEntries], sizeof(sh.lut));
@@ -388,7 +392,8 @@ void emu_free(void* h) { delete static_cast<Emu*>(h); }
 // `every`-th tile claim more DC entries than the pass decoded (0 = off).
 void emu_plant_ck(int every) { g_plant_ck = every; }
 
-// Initial LDS of every emulated work group: random bytes from `seed` (0 = zeroed).
+// Initial LDS of every emulated work group: random bytes from `seed` (0 =
+// zeroed, ~0 = left uninitialised).
 void emu_lds_seed(unsigned long long seed) { g_lds_seed = seed; }
 
 // Carry-over mode: every launch starts with the entry buffer the previous

@@ -25,21 +25,9 @@ namespace {
 
 struct Group {
   pthread_barrier_t bar;
-  unsigned orv = 0;  // __syncthreads_or accumulator
   Group() { pthread_barrier_init(&bar, nullptr, kWG); }
   ~Group() { pthread_barrier_destroy(&bar); }
   void sync() { pthread_barrier_wait(&bar); }
-  bool sync_or(bool v) {
-    sync();
-    if (v) __atomic_fetch_or(&orv, 1u, __ATOMIC_RELAXED);
-    sync();
-    const bool r = __atomic_load_n(&orv, __ATOMIC_RELAXED) != 0;
-    sync();
-    if (r) {
-      // every lane has read the result; lane 0 clears it for the next call
-    }
-    return r;
-  }
 };
 
 struct Launch {
@@ -65,16 +53,19 @@ void run_links(Group& g, EntShared& sh, int tid, const EntCtx& c) {
       int j;
       if (!task_round(sh, tid, c, T, L, j)) task_put(sh, __atomic_fetch_add(&sh.nnext, 1, __ATOMIC_RELAXED), j, L);
     }
-    ent_drain();
     g.sync();
     if (tid == 0) sh.ntask = sh.nnext;
   }
 }
 
-void queue_links(Group& g, EntShared& sh, int tid, bool need) {
-  if (tid == 0) sh.ntask = 0;
+void queue_links(Group& g, EntShared& sh, int tid, bool task, bool need = false) {
+  if (tid == 0) {
+    sh.ntask = 0;
+    sh.nneed = 0;
+  }
   g.sync();
-  if (need) task_new(sh, __atomic_fetch_add(&sh.ntask, 1, __ATOMIC_RELAXED), tid);
+  if (task) task_new(sh, __atomic_fetch_add(&sh.ntask, 1, __ATOMIC_RELAXED), tid);
+  if (need) __atomic_fetch_add(&sh.nneed, 1, __ATOMIC_RELAXED);
 }
 
 void fix_rounds(Group& g, EntShared& sh, int tid, const EntCtx& c) {
@@ -83,12 +74,11 @@ void fix_rounds(Group& g, EntShared& sh, int tid, const EntCtx& c) {
     const bool need = ent_fix_need(sh, tid, c, p, s);
     g.sync();
     const bool task = need && link_begin(sh, c, tid, p, s);
-    queue_links(g, sh, tid, task);
+    queue_links(g, sh, tid, task, need);
     g.sync();
-    const int n = sh.ntask;
+    const int n = sh.ntask, changed = sh.nneed;
     if (!n) {
-      if (tid == 0) __atomic_store_n(&g.orv, 0u, __ATOMIC_RELAXED);
-      if (!g.sync_or(need)) break;
+      if (!changed) break;
       continue;
     }
     run_links(g, sh, tid, c);
@@ -125,7 +115,6 @@ void lane_main(Group& g, EntShared& sh, const Launch& X, uint32_t item_id, int t
   }
   g.sync();
   ent_spec(sh, tid, c);
-  ent_drain();
   g.sync();
   queue_links(g, sh, tid, ent_phase2(sh, tid, c));
   run_links(g, sh, tid, c);
