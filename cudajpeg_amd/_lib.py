@@ -25,7 +25,7 @@ EXPORTS = [
     "cjd_batch_status",
     "cjd_batch_finish", "cjd_batch_capture",
     "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_kernel_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
-    "cjd_batch_download", "cjd_batch_totals", "cjd_error_name",
+    "cjd_batch_download", "cjd_batch_totals", "cjd_decode_noalloc", "cjd_noalloc_scratch_bytes", "cjd_error_name",
     "cjd_server_create", "cjd_server_submit", "cjd_server_poll", "cjd_server_release", "cjd_server_now_ns",
     "cjd_server_replay", "cjd_server_destroy",
     "newJPGReader", "delJPGReader", "openJPG", "openJPGMem", "writeJPG", "printError", "read16", "skipBlock",
@@ -113,6 +113,8 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_device_ptr": ([vp, i32, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)], i32),
         "cjd_batch_download": ([vp, i32, i32, vp, sz], i32),
         "cjd_batch_totals": ([vp, vp, vp, vp, vp, vp], i32),
+        "cjd_decode_noalloc": ([i32, ctypes.c_char_p, sz, i32, vp, sz, vp, ctypes.POINTER(sz)], i32),
+        "cjd_noalloc_scratch_bytes": ([], sz),
         "cjd_error_name": ([i32], ctypes.c_char_p),
         "cjd_server_create": ([vp, ctypes.POINTER(ServerOptions), ctypes.POINTER(vp)], i32),
         "cjd_server_submit": ([vp, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_uint64)], i32),

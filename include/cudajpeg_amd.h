@@ -220,6 +220,28 @@ int cjd_batch_download(cjd_batch *b, int i, int what, void *host, size_t cap);
 int cjd_batch_totals(cjd_batch *b, int64_t *images, int64_t *pixels, int64_t *scan_bytes, int64_t *blocks,
                      int64_t *out_bytes);
 
+/* Allocation-free decode of one image (no counterpart in the reference: for
+ * callers that cannot allocate device memory -- tight serving loops, harnesses
+ * that forbid hipMalloc).  The same path and kernels as cjd_batch_stage +
+ * cjd_batch_decode + cjd_batch_finish for one file (parse, unstuff, tiled
+ * entropy decode with the serial redo, fused pixel stage), but every device
+ * buffer of the decode lives in one static device array of
+ * cjd_noalloc_scratch_bytes() bytes (256 MiB) that each GPU the library runs
+ * on holds until the process ends; nothing is allocated on the device, and
+ * host-side planner buffers are reused from call to call.  `data`/`size` is
+ * the file in host memory; the RGB (W*H*3) or gray (W*H) frame is written to
+ * the caller's device buffer `frame` (frame_cap bytes, 16-byte aligned).  Ordered on `stream`
+ * and synchronised before returning.  Returns the reference error code;
+ * CJD_OOM_ERROR when the decode needs more scratch than the array holds
+ * (*scratch_needed, if not NULL, gets the bytes it needs: about 60 x the
+ * entropy-coded bytes plus 136 bytes per 8x8 block).  Every call on a device
+ * shares that device's array; calls on one device are serialised by the
+ * library (each holds the device's array until its decode has finished), and
+ * the array's contents after a call are not defined. */
+int cjd_decode_noalloc(int device, const uint8_t *data, size_t size, int idct_mode, void *frame, size_t frame_cap,
+                       void *stream, size_t *scratch_needed);
+size_t cjd_noalloc_scratch_bytes(void);
+
 /* Human-readable name of a reference error code (src/format.cu:62-78). */
 const char *cjd_error_name(int code);
 
