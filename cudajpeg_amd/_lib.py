@@ -26,7 +26,7 @@ EXPORTS = [
     "cjd_batch_finish", "cjd_batch_capture",
     "cjd_batch_replay", "cjd_batch_timings", "cjd_batch_kernel_timings", "cjd_batch_image_info", "cjd_batch_device_ptr",
     "cjd_batch_download", "cjd_batch_totals", "cjd_decode_noalloc", "cjd_noalloc_scratch_bytes", "cjd_error_name",
-    "cjd_server_create", "cjd_server_submit", "cjd_server_poll", "cjd_server_release", "cjd_server_now_ns",
+    "cjd_server_create", "cjd_server_submit", "cjd_server_poll", "cjd_server_release", "cjd_server_stats", "cjd_server_now_ns",
     "cjd_server_replay", "cjd_server_destroy",
     "newJPGReader", "delJPGReader", "openJPG", "openJPGMem", "writeJPG", "printError", "read16", "skipBlock",
     "decodeSOF", "decodeDHT", "decodeDQT", "decodeDRI", "decodeScanGPU", "iDCT_resample_colourTransform",
@@ -120,6 +120,7 @@ def lib() -> ctypes.CDLL:
         "cjd_server_submit": ([vp, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_uint64)], i32),
         "cjd_server_poll": ([vp, ctypes.POINTER(Completion), i32, i32], i32),
         "cjd_server_release": ([vp, ctypes.c_uint64], i32),
+        "cjd_server_stats": ([vp] + [ctypes.POINTER(ctypes.c_int64)] * 5, i32),
         "cjd_server_now_ns": ([], ctypes.c_int64),
         "cjd_server_replay": ([vp, i32, vp, vp, vp, vp, vp, vp], i32),
         "cjd_server_destroy": ([vp], None),

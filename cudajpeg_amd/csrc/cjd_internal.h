@@ -187,6 +187,7 @@ enum : uint32_t {
   kStatusInconsistent = 8u,  // the tiled decoder could not hold the image's entries (invalid stream); rerun serially
   kStatusFallback = 0x100u,  // host: the image was decoded again by the serial decoder
   kStatusRelaunch = 0x200u,  // host: the image was decoded again by the tiled decoder alone
+  kStatusOrdered = 0x400u,   // host: ... by the tiled decoder in ordered launches (no lookback wait)
 };
 // host: bits 16-23 of an image's status after a redo hold the first tiled
 // launch's bits that led to it

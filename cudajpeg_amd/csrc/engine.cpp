@@ -27,10 +27,12 @@ namespace {
 // Decode in parts (cjd_batch_set_overlap): the most runs, and the default.
 constexpr int kMaxParts = 4;
 constexpr int kDefaultParts = 2;
-// Solo relaunches of the tiled decoder for an image before the serial decoder
-// (which decodes a segment on one lane: tens of seconds for an 8192x8192 scan
-// without restart markers).
-constexpr int kRelaunches = 3;
+// Redo of an image the tiled decoder could not decode: kRelaunches launches
+// of the tiled decoder over the image's work items alone, then kOrdered
+// ordered redos (below), then the serial decoder, which decodes a segment on
+// one lane (tens of seconds for an 8192x8192 scan without restart markers).
+constexpr int kRelaunches = 1;
+constexpr int kOrdered = 2;
 
 // Grow-only device buffer (src/format.cu:81-110 policy).
 struct DevBuf {
@@ -418,6 +420,34 @@ static int enqueue(cjd_batch* b, hipStream_t s, bool timed) {
 // Wait for the entropy stage; decode again with the serial decoder every image
 // whose tile links did not resolve; map status bits to reference error codes.
 // `redo` receives the images that were decoded again.
+// Ordered redo of image r with the tiled decoder: its work items in launches
+// [a, e) chosen so that no item waits in the lookback for an item of its own
+// launch (every item of the launch either starts its segment or follows an
+// item of an earlier launch), all with one epoch, one after another on s.  A
+// lookback then finds its predecessor's inclusive state at once: there is no
+// wait between work groups, at the cost of one launch per item of a segment's
+// run (about 100 launches of a few microseconds for an 8192x8192 scan).
+static int relaunch_ordered(cjd_batch* b, const ImageRec& r, uint32_t epoch, hipStream_t s) {
+  uint32_t* status = status_of(b);
+  LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
+  const uint32_t end = r.item_first + r.item_count;
+  for (uint32_t a = r.item_first; a < end;) {
+    const uint32_t e = ordered_launch_end(b->plan.items, a, end);
+    const hipError_t he = launch_entropy(
+        b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
+        b->d_items.as<WorkItem>(), a, e, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
+        b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb, ticket_of(b, kRedoCounter),
+        done_of(b, kRedoCounter), b->ticket_base[kRedoCounter], epoch, b->wait_ticks, s);
+    b->ticket_base[kRedoCounter] += e - a;
+    if (he != hipSuccess) {
+      b->ticket_unknown = true;
+      return CJD_KERNEL_LAUNCH_ERROR;
+    }
+    a = e;
+  }
+  return 0;
+}
+
 static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo) {
   // the counters of the head and the status words, in one copy
   if (hipMemcpyAsync(b->h_status.data(), b->d_scratch.p, b->h_status.size() * 4, hipMemcpyDeviceToHost, s) !=
@@ -445,11 +475,12 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
   for (size_t i = 0; i < b->status_words; ++i) words[i] = status_bits(words[i], epoch);
   // Every image the tiled decoder could not resolve, or for which it reports
   // a syntax error or a truncated scan, is decoded again: first by the tiled
-  // decoder in a launch of its own over the image's work items alone, then,
-  // if that still reports anything, by the serial decoder (the reference's
-  // order), whose verdict is the one reported -- a tiled error is never
-  // reported unless the serial decoder agrees.  Each redo launch has an epoch
-  // of its own, so the image's status word starts without bits.
+  // decoder in a launch of its own over the image's work items alone, then in
+  // ordered launches (relaunch_ordered), then, if those still report
+  // anything, by the serial decoder (the reference's order), whose verdict is
+  // the one reported -- a tiled error is never reported unless the serial
+  // decoder agrees.  Each redo has an epoch of its own, so the image's status
+  // word starts without bits.
   const bool tiled = b->entropy_mode == CJD_ENTROPY_TILED;
   const uint32_t redo_bits =
       tiled ? (kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated) : 0u;
@@ -486,6 +517,13 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
         if (int e = read_status(i, e2, bits)) return e;
         redone = !(bits & redo_bits);
         if (redone) words[i] = bits | kStatusRelaunch | first;
+      }
+      for (int attempt = 0; attempt < kOrdered && r.item_count && !redone; ++attempt) {
+        if (int e = next_epoch(b, s, &e2)) return e;
+        if (int e = relaunch_ordered(b, r, e2, s)) return e;
+        if (int e = read_status(i, e2, bits)) return e;
+        redone = !(bits & redo_bits);
+        if (redone) words[i] = bits | kStatusOrdered | first;
       }
       if (!redone) {
         if (int e = next_epoch(b, s, &e2)) return e;

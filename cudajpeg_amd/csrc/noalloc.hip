@@ -167,8 +167,10 @@ extern "C" int cjd_decode_noalloc(int device, const uint8_t* data, size_t size, 
   if (head[0] != nitems || head[kLaunchCounters] != nitems) st |= kStatusUnresolved;
   // as cjd_batch_finish: an unresolved image, or a tiled syntax or
   // truncation error, is decoded again by the tiled decoder in a launch of
-  // its own (epoch 2, the redo counter) and, if that still reports anything,
-  // by the serial decoder (epoch 3), whose verdict is reported
+  // its own (epoch 2, the redo counter), then in ordered launches (epoch 3,
+  // the next counter: no work item waits for another of its launch) and, if
+  // that still reports anything, by the serial decoder (epoch 4), whose
+  // verdict is reported
   const uint32_t redo_bits = kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated;
   auto read_status = [&](uint32_t epoch_of) -> int {
     if (hipMemcpyAsync(d.status + kScratchHead, status, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -183,8 +185,18 @@ extern "C" int cjd_decode_noalloc(int device, const uint8_t* data, size_t size, 
     if (int e = read_status(2)) return e;
   }
   if (st & redo_bits) {
-    ok(launch_entropy_serial(words, images, segs, r.seg_count, tables, coefs, recs, status, 3, s));
+    for (uint32_t a = 0, base = 0; a < nitems;) {
+      const uint32_t e = ordered_launch_end(p.items, a, nitems);
+      ok(launch_entropy(words, images, segs, items, a, e, tables, luts, ebuf, recs, status, lb,
+                        ticket + kRedoCounter + 1, done + kRedoCounter + 1, base, 3, kDefaultWaitTicks, s));
+      base += e - a;
+      a = e;
+    }
     if (int e = read_status(3)) return e;
+  }
+  if (st & redo_bits) {
+    ok(launch_entropy_serial(words, images, segs, r.seg_count, tables, coefs, recs, status, 4, s));
+    if (int e = read_status(4)) return e;
   }
   if (st & (kStatusSyntax | kStatusTruncated)) return CJD_SYNTAX_ERROR;
   ok(launch_pixel(idct_mode, images, pix, npix, recs, ebuf, (uint64_t)p.entry_words, coefs, qtabs, btab, out, s));

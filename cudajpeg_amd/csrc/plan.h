@@ -55,6 +55,16 @@ struct Plan {
   uint64_t out_bytes = 0;
 };
 
+// End of the ordered-redo launch of the tiled decoder that starts at work item
+// a (of items [a, end)): the longest run [a, e) in which no item waits in the
+// lookback for another item of the run -- each either starts its segment
+// (t0 == 0) or follows an item before a.
+inline uint32_t ordered_launch_end(const std::vector<WorkItem>& items, uint32_t a, uint32_t end) {
+  uint32_t e = a + 1;
+  while (e < end && !(items[e].t0 > 0 && items[e].prev_item >= a)) ++e;
+  return e;
+}
+
 // Build the plan for n files; errors[i] gets each file's parse/stage result.
 void plan_batch(Plan& p, int n, const uint8_t* const* files, const size_t* sizes, int* errors);
 void build_table(const uint8_t bits[16], const uint8_t vals[256], HuffTable& t);

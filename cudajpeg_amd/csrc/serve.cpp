@@ -50,6 +50,7 @@ struct Slot {
   std::vector<Request*> reqs;
   int held = 0;       // completed requests of this slot not yet released
   bool busy = false;  // staged / in flight / holding frames
+  int64_t redone_seen = 0;  // cjd_batch_fallbacks of b after its last finish
 };
 
 }  // namespace
@@ -68,10 +69,12 @@ struct cjd_server {
   std::vector<Slot> slots;
   uint64_t next_id = 1;
   uint64_t admitted_bytes = 0;
+  int64_t n_submitted = 0, n_completed = 0, n_redone = 0;
   bool stop = false;
   std::thread batcher, completer;
 
   void complete_locked(Request* r) {
+    ++n_completed;
     r->done = true;
     if (!r->t_ready) r->t_ready = mono_ns();
     completed.push_back(r);
@@ -167,6 +170,7 @@ struct cjd_server {
       std::vector<int> errs((size_t)n, 0);
       const int e = cjd_batch_finish(sl.b, sl.stream, errs.data());
       const int64_t tr = mono_ns();
+      const int64_t redone = cjd_batch_fallbacks(sl.b);  // read by the thread that owns the batch now
       for (int i = 0; i < n; ++i) {
         Request* r = sl.reqs[(size_t)i];
         r->t_ready = tr;
@@ -177,6 +181,8 @@ struct cjd_server {
         }
       }
       lk.lock();
+      n_redone += redone - sl.redone_seen;
+      sl.redone_seen = redone;
       sl.held = n;
       for (Request* r : sl.reqs) complete_locked(r);
     }
@@ -231,6 +237,7 @@ extern "C" int cjd_server_submit(cjd_server* s, const uint8_t* data, size_t size
   r->t_arrival = mono_ns();
   std::lock_guard<std::mutex> lk(s->mu);
   r->id = s->next_id++;
+  ++s->n_submitted;
   s->live[r->id] = r;
   s->pending.push_back(r);
   s->cv_work.notify_one();
@@ -261,6 +268,18 @@ extern "C" int cjd_server_poll(cjd_server* s, cjd_completion* out, int max, int 
     c.t_ready_ns = r->t_ready;
   }
   return n;
+}
+
+extern "C" int cjd_server_stats(cjd_server* s, int64_t* submitted, int64_t* completed, int64_t* pending,
+                                int64_t* inflight, int64_t* redone) {
+  if (!s) return CJD_PROGRAMMER_ERROR;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (submitted) *submitted = s->n_submitted;
+  if (completed) *completed = s->n_completed;
+  if (pending) *pending = (int64_t)s->pending.size();
+  if (inflight) *inflight = (int64_t)s->inflight.size();
+  if (redone) *redone = s->n_redone;
+  return 0;
 }
 
 extern "C" int cjd_server_release(cjd_server* s, uint64_t id) {

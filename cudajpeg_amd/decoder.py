@@ -43,12 +43,13 @@ def parse(data: bytes) -> tuple[int, L.FrameInfo]:
 class Context:
     def __init__(self, device: int = 0):
         self.h = ctypes.c_void_p()
+        self._destroy = L.lib().cjd_context_destroy  # see BatchDecoder.__init__
         _check(L.lib().cjd_context_create(device, ctypes.byref(self.h)), "context: ")
 
     def close(self):
-        if self.h:
-            L.lib().cjd_context_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
 
     def __del__(self):
         self.close()
@@ -84,6 +85,9 @@ class BatchDecoder:
                  planes: bool = False, ctx: Context | None = None):
         self.ctx = ctx or default_context()
         self.h = ctypes.c_void_p()
+        # kept here: at interpreter exit the module's globals may be gone
+        # before this object's finaliser runs
+        self._destroy = L.lib().cjd_batch_destroy
         _check(L.lib().cjd_batch_create(self.ctx.h, ctypes.byref(self.h)), "batch: ")
         self.set_options(idct, entropy, tile_bits, planes)
         self.n = 0
@@ -150,7 +154,9 @@ class BatchDecoder:
 
     def status(self, i: int) -> int:
         """Status bits of image i after the last finish (1 syntax, 2 truncated,
-        4 unresolved, 8 entries did not fit, 0x100 decoded again serially)."""
+        4 unresolved, 8 entries did not fit; after a redo 0x200 decoded again
+        by the tiled decoder alone, 0x400 in ordered launches, 0x100 by the
+        serial decoder, and bits 16-23 the first launch's bits)."""
         v = ctypes.c_uint32()
         _check(L.lib().cjd_batch_status(self.h, i, ctypes.byref(v)), "status")
         return int(v.value)
@@ -205,9 +211,9 @@ class BatchDecoder:
         return self._download(i, L.OUT_PLANE0 + comp, (f.comp_plane_height[comp], f.comp_stride[comp]), np.uint8)
 
     def close(self):
-        if self.h:
-            L.lib().cjd_batch_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
 
     def __del__(self):
         self.close()

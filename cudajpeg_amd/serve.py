@@ -62,6 +62,12 @@ class Server:
     def release(self, rid: int) -> None:
         _check(L.lib().cjd_server_release(self.h, rid), "release: ")
 
+    def stats(self) -> dict:
+        """Counters of the loop: submitted, completed, pending, inflight, redone."""
+        v = [ctypes.c_int64() for _ in range(5)]
+        _check(L.lib().cjd_server_stats(self.h, *[ctypes.byref(x) for x in v]), "stats: ")
+        return dict(zip(["submitted", "completed", "pending", "inflight", "redone"], [x.value for x in v]))
+
     def frame_tensor(self, c: Completion):
         """Zero-copy torch view (uint8, H x W x channels) of a completed
         frame in device memory; valid until the request is released."""
@@ -71,9 +77,10 @@ class Server:
         """Copy a completed frame to host memory."""
         return self.frame_tensor(c).cpu().numpy()
 
-    def replay(self, files: list[bytes], arrival_ns: np.ndarray):
+    def replay(self, files: list[bytes], arrival_ns: np.ndarray, progress=None, every_s: float = 2.0):
         """Submit file k at start + arrival_ns[k] from a native thread, release
-        each frame as it completes.  Returns (latency_ns, errors, wall_ns)."""
+        each frame as it completes.  Returns (latency_ns, errors, wall_ns).
+        progress(stats) is called every every_s seconds while it runs."""
         n = len(files)
         keep = [ctypes.create_string_buffer(f, len(f)) for f in files]
         ptrs = (ctypes.c_void_p * n)(*[ctypes.cast(b, ctypes.c_void_p) for b in keep])
@@ -82,8 +89,24 @@ class Server:
         lat = np.zeros(n, np.int64)
         err = np.zeros(n, np.int32)
         wall = ctypes.c_int64()
-        _check(L.lib().cjd_server_replay(self.h, n, ptrs, sizes, arr.ctypes.data, lat.ctypes.data, err.ctypes.data,
-                                         ctypes.byref(wall)), "replay: ")
+        rc = [0]
+
+        def run():
+            rc[0] = L.lib().cjd_server_replay(self.h, n, ptrs, sizes, arr.ctypes.data, lat.ctypes.data,
+                                              err.ctypes.data, ctypes.byref(wall))
+
+        if progress is None:
+            run()
+        else:
+            import threading
+            t = threading.Thread(target=run)
+            t.start()
+            while True:
+                t.join(every_s)
+                if not t.is_alive():
+                    break
+                progress(self.stats())
+        _check(rc[0], "replay: ")
         return lat, err, wall.value
 
     def close(self) -> None:

@@ -156,8 +156,9 @@ int cjd_batch_transform(cjd_batch *b, void *stream);
 
 /* Number of images decoded again since the batch was created: images the
  * tiled decoder could not resolve or reported an error for are decoded again
- * by the tiled decoder alone and, if that still reports anything, by the
- * serial decoder (cjd_batch_status tells which). */
+ * by the tiled decoder alone, then by the tiled decoder in ordered launches
+ * (no work item waits for another of its launch) and, if that still reports
+ * anything, by the serial decoder (cjd_batch_status tells which). */
 int cjd_batch_fallbacks(cjd_batch *b);
 
 /* Limit of one wait of the tiled decoder's lookback for a predecessor work
@@ -179,9 +180,10 @@ int cjd_batch_protocol_redos(cjd_batch *b);
  * invalid code or coefficient index (SYNTAX_ERROR), 2 the scan ended before
  * the frame's blocks, 4 the tiled decoder could not resolve the image, 8 the
  * image's entries did not fit the tiled decoder's streams; after a redo, 0x200
- * the tiled decoder decoded the image again alone or 0x100 the serial decoder
- * did (bits 0-7 are then that decode's), and bits 16-23 hold the first
- * launch's bits that led to the redo.  For diagnostics; no counterpart in the
+ * the tiled decoder decoded the image again alone, 0x400 it did so in ordered
+ * launches, or 0x100 the serial decoder did (bits 0-7 are then that
+ * decode's), and bits 16-23 hold the first launch's bits that led to the
+ * redo.  For diagnostics; no counterpart in the
  * reference. */
 int cjd_batch_status(cjd_batch *b, int i, uint32_t *bits);
 
@@ -285,6 +287,12 @@ int cjd_server_submit(cjd_server *s, const uint8_t *data, size_t size, uint64_t 
 /* Up to max completions, waiting up to wait_us for the first; returns the count. */
 int cjd_server_poll(cjd_server *s, cjd_completion *out, int max, int wait_us);
 int cjd_server_release(cjd_server *s, uint64_t id);
+/* Counters of the loop since it was created: requests submitted and
+ * completed, requests waiting for a batch, batches launched and not yet
+ * finished, and images the slots' batches decoded again (cjd_batch_fallbacks
+ * summed).  Any pointer may be NULL.  For monitoring. */
+int cjd_server_stats(cjd_server *s, int64_t *submitted, int64_t *completed, int64_t *pending, int64_t *inflight,
+                     int64_t *redone);
 /* Host monotonic clock of the loop's timestamps, in ns. */
 int64_t cjd_server_now_ns(void);
 /* Replays an arrival trace: file k is submitted at start + arrival_ns[k]

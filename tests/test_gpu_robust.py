@@ -24,8 +24,10 @@ def test_lookback_wait_limit_does_not_change_results(gpu, wait_us):
     # Work items of 4 tiles of 2048 bits: many items per segment, so items
     # wait for their predecessors in the lookback.  With a limit of 0 an item
     # gives up whenever its predecessor has not published yet (forced from
-    # the host); its image is then decoded again by the serial decoder.
-    # Results are bit-exact either way and no work item goes undecoded.
+    # the host); its image is then decoded again, alone, and then in ordered
+    # launches, where no item waits: that redo resolves every image, so the
+    # serial decoder is never needed.  Results are bit-exact either way and
+    # no work item goes undecoded.
     encs = [synth.encode(s, truth=True) for s in HARD]
     b = BatchDecoder(IDCT_INT, tile_bits=2048)
     b.set_tuning(0, 4)
@@ -38,6 +40,8 @@ def test_lookback_wait_limit_does_not_change_results(gpu, wait_us):
         for i, (s, e) in enumerate(zip(HARD, encs)):
             assert np.array_equal(b.coefficients(i), e.coefs), s
             assert np.array_equal(b.pixels(i), _ref(e)), s
+            assert not b.status(i) & 0x100, (s, hex(b.status(i)))  # no serial redo
+        print("status", [hex(b.status(i)) for i in range(len(HARD))], flush=True)
     print("fallbacks", b.fallbacks(), flush=True)
     assert b.protocol_redos() == 0
     if wait_us:
