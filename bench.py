@@ -227,11 +227,12 @@ def serve_bench(args):
     files = [e.data for e in synth.encode_many(specs)]
     px = np.array([s.pixels for s in specs], np.float64)
     srv = Server(max_batch=64, max_wait_us=2000)
-    # warm-up and peak: a burst of 8 x the pool (long enough that the first
-    # batch's staging and the last one's tail are a small part of it)
-    srv.replay(files[:64], np.zeros(64, np.int64))
+    # peak: a burst of 8 x the pool (long enough that the first batch's
+    # staging and the last one's tail are a small part of it), after the same
+    # burst once untimed, in which every slot grows its buffers to the mix
     n_peak = 8 * len(files)
     burst = [files[k % len(files)] for k in range(n_peak)]
+    check(srv.replay(burst, np.zeros(n_peak, np.int64))[1], "serving warm-up burst")
     lat, err, wall = srv.replay(burst, np.zeros(n_peak, np.int64), progress=lambda st: log(f"serving burst: {st}"))
     check(err, "serving burst")
     peak_req_s = n_peak / (wall / 1e9)

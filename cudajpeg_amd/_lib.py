@@ -29,7 +29,8 @@ EXPORTS = [
     "cjd_server_create", "cjd_server_submit", "cjd_server_poll", "cjd_server_release", "cjd_server_stats", "cjd_server_now_ns",
     "cjd_server_replay", "cjd_server_destroy",
     "newJPGReader", "delJPGReader", "openJPG", "openJPGMem", "writeJPG", "printError", "read16", "skipBlock",
-    "decodeSOF", "decodeDHT", "decodeDQT", "decodeDRI", "decodeScanGPU", "iDCT_resample_colourTransform",
+    "decodeSOF", "decodeDHT", "decodeDQT", "decodeDRI", "decodeScanGPU", "decodeScanCPU",
+    "iDCT_resample_colourTransform",
 ]
 
 
@@ -130,6 +131,8 @@ def lib() -> ctypes.CDLL:
         "openJPGMem": ([vp, ctypes.c_char_p, sz], i32),
         "writeJPG": ([vp, ctypes.c_char_p], None),
         "printError": ([vp], None),
+        "decodeScanGPU": ([vp], None),
+        "decodeScanCPU": ([vp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

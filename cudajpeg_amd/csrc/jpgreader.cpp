@@ -56,15 +56,21 @@ void run_parser(JPGReader* jpg, F fn, const char* func) {
   jpg->pos = const_cast<unsigned char*>(st.pos);
 }
 
-int grow(unsigned char** p, unsigned int* cap, size_t need) {
-  if (*p && *cap >= need) return 0;
-  free(*p);
-  *p = static_cast<unsigned char*>(malloc(need ? need : 1));
-  if (!*p) {
-    *cap = 0;
-    return OOM_ERROR;
+// Grow-only host buffer of `need` elements (the ensureMemSize policy of
+// src/format.cu:81-110): size = need, max_size = what is allocated.
+template <class M>
+int grow(M* m, size_t need) {
+  if (need > 0xFFFFFFFFu) return OOM_ERROR;
+  if (!m->mem || m->max_size < need) {
+    free(m->mem);
+    m->mem = static_cast<decltype(m->mem)>(malloc((need ? need : 1) * sizeof(*m->mem)));
+    m->max_size = m->mem ? (unsigned int)need : 0u;
+    if (!m->mem) {
+      m->size = 0;
+      return OOM_ERROR;
+    }
   }
-  *cap = (unsigned int)need;
+  m->size = (unsigned int)need;
   return 0;
 }
 
@@ -102,11 +108,11 @@ void delJPGReader(JPGReader* r) {
     delete e;
   }
   free(r->pixels);
-  free(r->file_buf);
+  free(r->file_buf.mem);
   for (int i = 0; i < 3; ++i) {
-    free(r->channels[i].working_space);
-    free(r->channels[i].raw_pixels);
-    free(r->channels[i].pixels);
+    free(r->channels[i].working_space.mem);
+    free(r->channels[i].raw_pixels.mem);
+    free(r->channels[i].pixels.mem);
   }
   free(r);
 }
@@ -152,8 +158,7 @@ void decodeSOF(JPGReader* jpg) {
       jpg->max_pixels_size = (int)n;
     }
   } else {
-    ColourChannel* c = &jpg->channels[0];
-    if (grow(&c->pixels, &c->pixels_size, (size_t)f.width * f.height)) RTHROW(OOM_ERROR);
+    if (grow(&jpg->channels[0].pixels, (size_t)f.width * f.height)) RTHROW(OOM_ERROR);
   }
 }
 
@@ -169,16 +174,19 @@ void decodeDRI(JPGReader* jpg) {
   if (!jpg->error) jpg->restart_interval = eng(jpg)->f.restart_interval;
 }
 
-// include/decodeScanGPU.h:6 -- SOS header, then the whole scan on the GPU.
-void decodeScanGPU(JPGReader* jpg) {
-  run_parser(jpg, cjd::parse_sos, __func__);
+// SOS header, then the whole scan on the GPU with the given entropy decoder;
+// working_space is filled when `coefficients`.  The clock() ticks it takes
+// are added to jpg->time (src/decodeScanGPU.cu:154, 224-225).
+static void decode_scan(JPGReader* jpg, int entropy_mode, bool coefficients) {
+  const clock_t t0 = clock();
+  run_parser(jpg, cjd::parse_sos, "decodeScan");
   if (jpg->error) return;
   ReaderEngine* e = eng(jpg);
   for (int i = 0; i < jpg->num_channels; ++i) {
     jpg->channels[i].dc_id = e->f.dc_id[i];
     jpg->channels[i].ac_id = e->f.ac_id[i];
   }
-  int err = cjd_batch_set_options(e->batch, jpg->idct_mode, jpg->entropy_mode, 0, jpg->keep_planes);
+  int err = cjd_batch_set_options(e->batch, jpg->idct_mode, entropy_mode, 0, jpg->keep_planes);
   if (err) RTHROW(err);
   const uint8_t* files[1] = {jpg->buf};
   const size_t sizes[1] = {(size_t)(jpg->end - jpg->buf) + 2};
@@ -192,7 +200,7 @@ void decodeScanGPU(JPGReader* jpg) {
   ferr = cjd_batch_image_info(e->batch, 0, &info);
   if (ferr) RTHROW(ferr);
   e->staged = true;
-  if (jpg->keep_coefficients) {
+  if (coefficients) {
     // dequantised working_space in the reference layout (src/decodeScanCPU.cu:78-96):
     // per component, 64 ints per block in decode order, natural order within a block
     std::vector<int16_t> q((size_t)info.num_blocks * 64);
@@ -205,14 +213,8 @@ void decodeScanGPU(JPGReader* jpg) {
     int* wpos[3] = {nullptr, nullptr, nullptr};
     for (int c = 0; c < jpg->num_channels; ++c) {
       ColourChannel* ch = &jpg->channels[c];
-      const size_t n = (size_t)ch->stride * ch->plane_height;
-      if (!ch->working_space || ch->working_space_size < n) {
-        free(ch->working_space);
-        ch->working_space = static_cast<int*>(malloc(n * sizeof(int)));
-        if (!ch->working_space) RTHROW(OOM_ERROR);
-        ch->working_space_size = (unsigned int)n;
-      }
-      wpos[c] = ch->working_space;
+      if (grow(&ch->working_space, (size_t)ch->stride * ch->plane_height)) RTHROW(OOM_ERROR);
+      wpos[c] = ch->working_space.mem;
     }
     size_t blk = 0;
     for (int m = 0; m < info.mcus_x * info.mcus_y; ++m)
@@ -226,7 +228,18 @@ void decodeScanGPU(JPGReader* jpg) {
       }
   }
   jpg->pos = jpg->end;
+  jpg->time += clock() - t0;
 }
+
+// include/decodeScanGPU.h:6 -- the scan with the reader's entropy decoder
+// (the tiled decoder by default).
+void decodeScanGPU(JPGReader* jpg) { decode_scan(jpg, jpg->entropy_mode, jpg->keep_coefficients != 0); }
+
+// include/decodeScanCPU.h:6 -- the reference's serial scan decoder, whose
+// result is working_space (src/decodeScanCPU.cu:98-152): here the serial
+// decoder on the device (one lane per restart segment, the reference's
+// order), and working_space is always filled.
+void decodeScanCPU(JPGReader* jpg) { decode_scan(jpg, CJD_ENTROPY_SERIAL, true); }
 
 // include/pixelTransformGPU.h:17 -- fused dequant + iDCT + upsample + colour.
 void iDCT_resample_colourTransform(JPGReader* jpg) {
@@ -236,15 +249,15 @@ void iDCT_resample_colourTransform(JPGReader* jpg) {
   if (err) RTHROW(err);
   if (hipStreamSynchronize(nullptr) != hipSuccess) RTHROW(CUDA_KERNEL_LAUNCH_ERROR);
   const size_t n = (size_t)jpg->width * jpg->height * (jpg->num_channels == 3 ? 3 : 1);
-  unsigned char* dst = jpg->num_channels == 3 ? jpg->pixels : jpg->channels[0].pixels;
+  unsigned char* dst = jpg->num_channels == 3 ? jpg->pixels : jpg->channels[0].pixels.mem;
   err = cjd_batch_download(e->batch, 0, CJD_OUT_PIXELS, dst, n);
   if (err) RTHROW(err);
   if (jpg->keep_planes) {
     for (int c = 0; c < jpg->num_channels; ++c) {
       ColourChannel* ch = &jpg->channels[c];
       const size_t pn = (size_t)ch->stride * ch->plane_height;
-      if (grow(&ch->raw_pixels, &ch->raw_pixels_size, pn)) RTHROW(OOM_ERROR);
-      err = cjd_batch_download(e->batch, 0, CJD_OUT_PLANE0 + c, ch->raw_pixels, pn);
+      if (grow(&ch->raw_pixels, pn)) RTHROW(OOM_ERROR);
+      err = cjd_batch_download(e->batch, 0, CJD_OUT_PLANE0 + c, ch->raw_pixels.mem, pn);
       if (err) RTHROW(err);
     }
   }
@@ -252,8 +265,9 @@ void iDCT_resample_colourTransform(JPGReader* jpg) {
 
 // src/format.cu:160-212 on a file already in jpg->file_buf.
 static int run_marker_loop(JPGReader* jpg, size_t size) {
-  unsigned char* buf = jpg->file_buf;
+  unsigned char* buf = jpg->file_buf.mem;
   jpg->buf = buf;
+  jpg->time = 0;
   jpg->error = NO_ERROR;
   jpg->error_line = 0;
   jpg->error_file = jpg->error_func = nullptr;
@@ -296,8 +310,8 @@ static int run_marker_loop(JPGReader* jpg, size_t size) {
 
 int openJPGMem(JPGReader* jpg, const unsigned char* data, size_t size) {
   if (!jpg || !data) return FILE_ERROR;
-  if (grow(&jpg->file_buf, &jpg->file_buf_size, size + 16)) return OOM_ERROR;
-  memcpy(jpg->file_buf, data, size);
+  if (grow(&jpg->file_buf, size + 16)) return OOM_ERROR;
+  memcpy(jpg->file_buf.mem, data, size);
   return run_marker_loop(jpg, size);
 }
 
@@ -307,11 +321,11 @@ int openJPG(JPGReader* jpg, const char* filename) {
   fseek(f, 0, SEEK_END);
   const long size = ftell(f);
   fseek(f, 0, SEEK_SET);
-  if (size <= 0 || grow(&jpg->file_buf, &jpg->file_buf_size, (size_t)size + 16)) {
+  if (size <= 0 || grow(&jpg->file_buf, (size_t)size + 16)) {
     fclose(f);
     return size <= 0 ? FILE_ERROR : OOM_ERROR;
   }
-  const size_t got = fread(jpg->file_buf, 1, (size_t)size, f);
+  const size_t got = fread(jpg->file_buf.mem, 1, (size_t)size, f);
   fclose(f);
   if (got != (size_t)size) return FILE_ERROR;
   return run_marker_loop(jpg, (size_t)size);
@@ -325,7 +339,7 @@ void writeJPG(JPGReader* jpg, const char* filename) {
     return;
   }
   fprintf(f, "P%d\n%d %d\n255\n", jpg->num_channels > 1 ? 6 : 5, jpg->width, jpg->height);
-  fwrite(jpg->num_channels == 1 ? jpg->channels[0].pixels : jpg->pixels, 1,
+  fwrite(jpg->num_channels == 1 ? jpg->channels[0].pixels.mem : jpg->pixels, 1,
          (size_t)jpg->width * jpg->height * jpg->num_channels, f);
   fclose(f);
 }

@@ -233,13 +233,19 @@ def decode(data: bytes, idct: int = L.IDCT_INT) -> np.ndarray:
 
 
 # ------------------------------------------------ reference-style reader --
+class _ManagedUCharMem(ctypes.Structure):
+    _fields_ = [("mem", ctypes.POINTER(ctypes.c_uint8)), ("size", ctypes.c_uint), ("max_size", ctypes.c_uint)]
+
+
+class _ManagedIntMem(ctypes.Structure):
+    _fields_ = [("mem", ctypes.POINTER(ctypes.c_int)), ("size", ctypes.c_uint), ("max_size", ctypes.c_uint)]
+
+
 class _ColourChannel(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int), ("dq_id", ctypes.c_int), ("ac_id", ctypes.c_int), ("dc_id", ctypes.c_int),
                 ("width", ctypes.c_int), ("height", ctypes.c_int), ("samples_x", ctypes.c_int),
                 ("samples_y", ctypes.c_int), ("stride", ctypes.c_int), ("plane_height", ctypes.c_int),
-                ("working_space", ctypes.POINTER(ctypes.c_int)), ("working_space_size", ctypes.c_uint),
-                ("raw_pixels", ctypes.POINTER(ctypes.c_uint8)), ("raw_pixels_size", ctypes.c_uint),
-                ("pixels", ctypes.POINTER(ctypes.c_uint8)), ("pixels_size", ctypes.c_uint)]
+                ("working_space", _ManagedIntMem), ("raw_pixels", _ManagedUCharMem), ("pixels", _ManagedUCharMem)]
 
 
 class _JPGReader(ctypes.Structure):
@@ -251,10 +257,10 @@ class _JPGReader(ctypes.Structure):
                 ("channels", _ColourChannel * 3),
                 ("pixels", ctypes.POINTER(ctypes.c_uint8)), ("max_pixels_size", ctypes.c_int),
                 ("dq_tables", (ctypes.c_uint8 * 64) * 4),
-                ("file_buf", ctypes.c_void_p), ("file_buf_size", ctypes.c_uint),
+                ("file_buf", _ManagedUCharMem),
                 ("idct_mode", ctypes.c_int), ("entropy_mode", ctypes.c_int),
                 ("keep_coefficients", ctypes.c_int), ("keep_planes", ctypes.c_int),
-                ("engine", ctypes.c_void_p),
+                ("engine", ctypes.c_void_p), ("time", ctypes.c_long),
                 ("error", ctypes.c_int), ("error_line", ctypes.c_int),
                 ("error_file", ctypes.c_char_p), ("error_func", ctypes.c_char_p)]
 
@@ -296,15 +302,20 @@ class JPGReader:
         r = self.raw
         if r.num_channels == 3:
             return np.ctypeslib.as_array(r.pixels, (r.height, r.width, 3)).copy()
-        return np.ctypeslib.as_array(r.channels[0].pixels, (r.height, r.width, 1)).copy()
+        return np.ctypeslib.as_array(r.channels[0].pixels.mem, (r.height, r.width, 1)).copy()
 
     def working_space(self, comp: int) -> np.ndarray:
         ch = self.raw.channels[comp]
-        return np.ctypeslib.as_array(ch.working_space, (ch.stride * ch.plane_height,)).copy()
+        return np.ctypeslib.as_array(ch.working_space.mem, (ch.working_space.size,)).copy()
 
     def raw_pixels(self, comp: int) -> np.ndarray:
         ch = self.raw.channels[comp]
-        return np.ctypeslib.as_array(ch.raw_pixels, (ch.plane_height, ch.stride)).copy()
+        return np.ctypeslib.as_array(ch.raw_pixels.mem, (ch.plane_height, ch.stride)).copy()
+
+    @property
+    def time(self) -> int:
+        """clock() ticks of the last open's scan decode."""
+        return int(self.raw.time)
 
     def write(self, path: str) -> None:
         L.lib().writeJPG(self._p, path.encode())

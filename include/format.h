@@ -10,11 +10,15 @@
  *    instead of filling 65536-entry LUTs (src/format.cu:371-391);
  *  - coefficients stay on the device; `keep_coefficients` copies the
  *    dequantised working_space back in the reference layout for inspection;
- *  - openJPGMem decodes a file already in memory.
+ *  - openJPGMem decodes a file already in memory;
+ *  - decodeScanCPU, the reference's serial scan decoder, runs the serial
+ *    decoder on the device (one lane per restart segment) and fills
+ *    working_space; there is no host decode path.
  */
 #ifndef CUDAJPEG_AMD_FORMAT_H
 #define CUDAJPEG_AMD_FORMAT_H
 #include <stddef.h>
+#include <time.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -30,6 +34,18 @@ extern "C" {
 #define CUDA_KERNEL_LAUNCH_ERROR 6
 #define PROGRAMMER_ERROR -1
 
+/* include/format.h:36-52: a host buffer, `size` elements in use of
+ * `max_size` allocated (grow-only). */
+typedef struct _ManagedUCharMem {
+  unsigned char *mem;
+  unsigned int size, max_size;
+} ManagedUCharMem;
+
+typedef struct _ManagedIntMem {
+  int *mem;
+  unsigned int size, max_size;
+} ManagedIntMem;
+
 /* include/format.h:55-69 */
 typedef struct _ColourChannel {
   int id;
@@ -37,12 +53,9 @@ typedef struct _ColourChannel {
   int width, height;
   int samples_x, samples_y, stride;
   int plane_height;
-  int *working_space;              /* host dequantised coefficients (keep_coefficients) */
-  unsigned int working_space_size;
-  unsigned char *raw_pixels;       /* host planar samples (keep_planes), stride x plane_height */
-  unsigned int raw_pixels_size;
-  unsigned char *pixels;           /* grayscale output, width x height */
-  unsigned int pixels_size;
+  ManagedIntMem working_space;     /* host dequantised coefficients (keep_coefficients, decodeScanCPU) */
+  ManagedUCharMem raw_pixels;      /* host planar samples (keep_planes), stride x plane_height */
+  ManagedUCharMem pixels;          /* grayscale output, width x height */
 } ColourChannel;
 
 /* include/format.h:72-103 */
@@ -57,14 +70,17 @@ typedef struct _JPGReader {
   unsigned char *pixels;           /* RGB output, width x height x 3 */
   int max_pixels_size;
   unsigned char dq_tables[4][64];
-  unsigned char *file_buf;
-  unsigned int file_buf_size;
+  ManagedUCharMem file_buf;
   /* options */
   int idct_mode;                   /* CJD_IDCT_INT / _FLOAT / _MFMA */
   int entropy_mode;                /* CJD_ENTROPY_TILED / _SERIAL */
   int keep_coefficients;
   int keep_planes;
   void *engine;                    /* device context, batch and parsed frame */
+  /* clock() ticks of the last openJPG's scan decode (include/format.h:100;
+   * set to 0 by openJPG, src/format.cu:174, and accumulated by the scan
+   * decode, src/decodeScanGPU.cu:154, 224-225) */
+  clock_t time;
   /* error handling (include/format.h:22-27, :100-102) */
   int error, error_line;
   const char *error_file, *error_func;
@@ -83,6 +99,7 @@ void decodeDHT(JPGReader *jpg);                                /* src/format.cu:
 void decodeDQT(JPGReader *jpg);                                /* src/format.cu:424 */
 void decodeDRI(JPGReader *jpg);                                /* src/format.cu:404 */
 void decodeScanGPU(JPGReader *jpg);                            /* include/decodeScanGPU.h:6 */
+void decodeScanCPU(JPGReader *jpg);                            /* include/decodeScanCPU.h:6 */
 void iDCT_resample_colourTransform(JPGReader *jpg);            /* include/pixelTransformGPU.h:17 */
 
 #ifdef __cplusplus
