@@ -247,6 +247,7 @@ def serve_bench(args):
         lat, err, wall = srv.replay([files[int(k)] for k in pick], arrival,
                                     progress=lambda st: log(f"serving at {rate:.0f} req/s: {st}"))
         check(err, f"serving at {rate:.0f} req/s")
+        log(f"serving at {rate:.0f} req/s, loop counters so far: {srv.stats()}")
         ms = lat / 1e6
         return {"offered_req_s": round(rate, 1), "above_peak": bool(rate >= peak_req_s), "requests": n,
                 "seconds": round(wall / 1e9, 3),

@@ -66,6 +66,13 @@ class ServerOptions(ctypes.Structure):
                 ("slots", ctypes.c_int), ("idct_mode", ctypes.c_int), ("tile_bits", ctypes.c_int)]
 
 
+class ServerCounters(ctypes.Structure):
+    """cjd_server_counters"""
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "pending", "inflight", "redone", "batches",
+                                              "stage_ns", "launch_ns", "finish_ns", "slot_wait_ns", "device_ns",
+                                              "device_max_ns", "turnaround_ns")]
+
+
 class Completion(ctypes.Structure):
     """cjd_completion"""
     _fields_ = [("id", ctypes.c_uint64), ("error", ctypes.c_int), ("batch_size", ctypes.c_int),
@@ -121,7 +128,7 @@ def lib() -> ctypes.CDLL:
         "cjd_server_submit": ([vp, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_uint64)], i32),
         "cjd_server_poll": ([vp, ctypes.POINTER(Completion), i32, i32], i32),
         "cjd_server_release": ([vp, ctypes.c_uint64], i32),
-        "cjd_server_stats": ([vp] + [ctypes.POINTER(ctypes.c_int64)] * 5, i32),
+        "cjd_server_stats": ([vp, ctypes.POINTER(ServerCounters)], i32),
         "cjd_server_now_ns": ([], ctypes.c_int64),
         "cjd_server_replay": ([vp, i32, vp, vp, vp, vp, vp, vp], i32),
         "cjd_server_destroy": ([vp], None),

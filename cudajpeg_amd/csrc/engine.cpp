@@ -60,6 +60,31 @@ struct DevBuf {
   }
 };
 
+// Grow-only pinned host buffer: the staging area of a batch's uploads, so
+// that each upload is one DMA on the caller's stream (from pageable memory
+// the runtime copies through small staging buffers of its own, one after
+// another, and the call returns only when they are done).
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t bytes) {
+    if (bytes <= cap && p) return true;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t n = std::max<size_t>(bytes + bytes / 8, 4096);  // room to grow
+    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    cap = n;
+    return true;
+  }
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 }  // namespace
 
 struct cjd_batch {
@@ -70,6 +95,13 @@ struct cjd_batch {
 
   DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_luts, d_qtabs, d_coefs, d_out, d_scratch;
   DevBuf d_btab;  // MFMA table operands per image (k_mfma_tables at staging)
+  // Pinned staging: the planner writes the scan words into h_words, the
+  // descriptors and tables are packed into h_meta, and every upload is a DMA
+  // from them.  up_ev follows the last stage's uploads: the next stage waits
+  // for it before writing into these buffers again.
+  PinnedBuf h_words, h_meta;
+  hipEvent_t up_ev = nullptr;
+  bool up_pending = false;
   DevBuf d_ebuf, d_recs;  // tiled decoder: entry regions and block records
   // The pixel kernel reads the block records; the dense coefficient buffer
   // is filled from them (k_expand) only when a caller asks for it.
@@ -108,6 +140,8 @@ struct cjd_batch {
   bool timed = false;
 
   ~cjd_batch() {
+    if (up_pending) (void)hipEventSynchronize(up_ev);  // no DMA may still read the pinned buffers
+    if (up_ev) (void)hipEventDestroy(up_ev);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : ex)
@@ -138,6 +172,13 @@ extern "C" int cjd_batch_create(cjd_context* ctx, cjd_batch** out) {
       delete b;
       return CJD_DEVICE_MEM_ERROR;
     }
+  if (hipEventCreateWithFlags(&b->up_ev, hipEventDisableTiming) != hipSuccess) {
+    delete b;
+    return CJD_DEVICE_MEM_ERROR;
+  }
+  b->plan.word_store = [b](size_t n) -> uint32_t* {
+    return b->h_words.ensure(n * 4) ? static_cast<uint32_t*>(b->h_words.p) : nullptr;
+  };
   *out = b;
   return 0;
 }
@@ -181,14 +222,36 @@ extern "C" int cjd_batch_set_tune_table(cjd_batch* b, int n, const cjd_tune_entr
   return 0;
 }
 
-template <class T>
-static int upload(DevBuf& d, const std::vector<T>& v, hipStream_t s) {
-  const size_t bytes = v.size() * sizeof(T);
+static int upload_bytes(DevBuf& d, const void* src, size_t bytes, hipStream_t s) {
   if (int e = d.ensure(bytes)) return e;
-  if (bytes && hipMemcpyAsync(d.p, v.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess)
-    return CJD_DEVICE_MEM_ERROR;
+  if (bytes && hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
   return 0;
 }
+
+// The plan's descriptor and table arrays, each uploaded from its copy in the
+// pinned meta buffer (256-byte aligned), or from the vector itself when no
+// pinned memory could be had.
+struct MetaPacker {
+  cjd_batch* b;
+  size_t total = 0;
+  template <class T>
+  void count(const std::vector<T>& v) {
+    total += (v.size() * sizeof(T) + 255) & ~(size_t)255;
+  }
+  size_t off = 0;
+  template <class T>
+  int put(DevBuf& d, const std::vector<T>& v, bool pinned, hipStream_t s) {
+    const size_t bytes = v.size() * sizeof(T);
+    const void* src = v.data();
+    if (pinned && bytes) {
+      uint8_t* dst = static_cast<uint8_t*>(b->h_meta.p) + off;
+      memcpy(dst, v.data(), bytes);
+      src = dst;
+      off += (bytes + 255) & ~(size_t)255;
+    }
+    return upload_bytes(d, src, bytes, s);
+  }
+};
 
 // Runs of the decode in parts: cut j (1 .. parts-1) at the first image
 // boundary at or after (j+1)/(parts+1) of the work items, where both work
@@ -246,19 +309,38 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   if (!b || n < 0 || (n && (!files || !sizes))) return CJD_PROGRAMMER_ERROR;
   hipStream_t s = (hipStream_t)stream;
   if (hipSetDevice(b->ctx->device) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+  // the last stage's DMAs read the pinned buffers the planner writes next
+  if (b->up_pending) {
+    if (hipEventSynchronize(b->up_ev) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+    b->up_pending = false;
+  }
   plan_batch(b->plan, n, files, sizes, errors);
+  const Plan& P = b->plan;
+  MetaPacker mp{b};
+  mp.count(P.images);
+  mp.count(P.segs);
+  mp.count(P.items);
+  mp.count(P.pix);
+  mp.count(P.tables);
+  mp.count(P.luts);
+  mp.count(P.qtabs);
+  const bool pinned = b->h_meta.ensure(mp.total);
   int e = 0;
-  if (!e) e = upload(b->d_words, b->plan.words, s);
-  if (!e) e = upload(b->d_images, b->plan.images, s);
-  if (!e) e = upload(b->d_segs, b->plan.segs, s);
-  if (!e) e = upload(b->d_items, b->plan.items, s);
-  if (!e) e = upload(b->d_pix, b->plan.pix, s);
-  if (!e) e = upload(b->d_tables, b->plan.tables, s);
-  if (!e) e = upload(b->d_luts, b->plan.luts, s);
+  if (!e) e = upload_bytes(b->d_words, P.wptr, P.nwords * 4, s);
+  if (!e) e = mp.put(b->d_images, P.images, pinned, s);
+  if (!e) e = mp.put(b->d_segs, P.segs, pinned, s);
+  if (!e) e = mp.put(b->d_items, P.items, pinned, s);
+  if (!e) e = mp.put(b->d_pix, P.pix, pinned, s);
+  if (!e) e = mp.put(b->d_tables, P.tables, pinned, s);
+  if (!e) e = mp.put(b->d_luts, P.luts, pinned, s);
   if (!e && b->plan.entry_words >= ((int64_t)1 << 32)) e = CJD_OOM_ERROR;  // records hold 32-bit entry indexes
   if (!e) e = b->d_ebuf.ensure((size_t)b->plan.entry_words * 4);
   if (!e) e = b->d_recs.ensure((size_t)b->plan.total_blocks * sizeof(BlockRec));
-  if (!e) e = upload(b->d_qtabs, b->plan.qtabs, s);
+  if (!e) e = mp.put(b->d_qtabs, P.qtabs, pinned, s);
+  if (!e) {
+    if (hipEventRecord(b->up_ev, s) != hipSuccess) e = CJD_DEVICE_MEM_ERROR;
+    else b->up_pending = true;
+  }
   if (!e) e = b->d_btab.ensure(b->plan.images.size() * (size_t)kMfmaTabVecs * 16);
   if (!e && launch_mfma_tables(b->d_images.as<ImageDesc>(), (uint32_t)b->plan.images.size(), b->d_qtabs.as<uint16_t>(),
                                b->d_btab.as<uint4>(), s) != hipSuccess)

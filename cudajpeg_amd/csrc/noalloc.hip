@@ -70,7 +70,7 @@ size_t take(size_t& at, size_t bytes) {
 Layout layout_of(const Plan& p) {
   Layout L{};
   size_t at = 0;
-  L.words = take(at, p.words.size() * 4);
+  L.words = take(at, p.nwords * 4);
   L.images = take(at, p.images.size() * sizeof(ImageDesc));
   L.segs = take(at, p.segs.size() * sizeof(SegmentDesc));
   L.items = take(at, p.items.size() * sizeof(WorkItem));
@@ -144,7 +144,7 @@ extern "C" int cjd_decode_noalloc(int device, const uint8_t* data, size_t size, 
   auto ok = [&](hipError_t x) {
     if (he == hipSuccess) he = x;
   };
-  ok(put(B, L.words, p.words, s));
+  if (p.nwords) ok(hipMemcpyAsync(B + L.words, p.wptr, p.nwords * 4, hipMemcpyHostToDevice, s));
   ok(put(B, L.images, p.images, s));
   ok(put(B, L.segs, p.segs, s));
   ok(put(B, L.items, p.items, s));

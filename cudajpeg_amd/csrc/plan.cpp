@@ -292,7 +292,7 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
   // words: 0xFF padding, then the unstuffed data written straight in
   std::vector<size_t> dst_off((size_t)w.nseg);
   int64_t wo = w.word0;
-  uint32_t* words = b->words.data();
+  uint32_t* words = b->wptr;
   for (int64_t s = 0; s < w.nseg; ++s) {
     const size_t dlen = w.bounds[s + 1] - w.bounds[s];
     const int64_t nw = seg_words(dlen);
@@ -488,7 +488,14 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
   b.tables.resize((size_t)nimg * 4);
   b.luts.resize((size_t)nimg * kLutEntries);
   b.qtabs.resize((size_t)nimg * 4 * 64);
-  b.words.resize((size_t)nwords);
+  b.nwords = (size_t)nwords;
+  b.wptr = b.word_store ? b.word_store(b.nwords) : nullptr;
+  if (b.wptr) {
+    b.words.clear();
+  } else {
+    b.words.resize(b.nwords);
+    b.wptr = b.words.data();
+  }
   // pass 3: fill, per image in parallel
   Pool::get().run(n, [&](int i) {
     ImageRec& r = b.recs[(size_t)i];

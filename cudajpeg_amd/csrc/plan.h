@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 #include "../../include/cudajpeg_amd.h"
@@ -49,7 +50,14 @@ struct Plan {
   std::vector<HuffTable> tables;
   std::vector<uint16_t> luts;    // kLutEntries per image
   std::vector<uint16_t> qtabs;   // natural order, 64 per table
-  std::vector<uint32_t> words;   // unstuffed segments, word aligned, 0xFF padded
+  // Unstuffed segments, word aligned, 0xFF padded: nwords words at wptr.
+  // When word_store is set, plan_batch asks it for the storage (pinned host
+  // memory in the engine, so that the upload is one DMA); when it is unset or
+  // returns nullptr, the words go to the `words` vector.
+  std::function<uint32_t*(size_t)> word_store;
+  std::vector<uint32_t> words;
+  uint32_t* wptr = nullptr;
+  size_t nwords = 0;
   int64_t total_blocks = 0, total_pixels = 0, scan_bytes = 0;
   int64_t entry_words = 0;       // entry buffer of the tiled decoder (regions, checkpoints, sinks)
   uint64_t out_bytes = 0;

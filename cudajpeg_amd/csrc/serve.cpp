@@ -70,6 +70,8 @@ struct cjd_server {
   uint64_t next_id = 1;
   uint64_t admitted_bytes = 0;
   int64_t n_submitted = 0, n_completed = 0, n_redone = 0;
+  int64_t n_batches = 0, t_stage = 0, t_launch = 0, t_finish = 0, t_slot_wait = 0;
+  int64_t t_device = 0, t_device_max = 0, t_turnaround = 0;
   bool stop = false;
   std::thread batcher, completer;
 
@@ -104,7 +106,9 @@ struct cjd_server {
           break;
         }
       if (si < 0) {  // every slot in flight or holding frames
+        const int64_t t0 = mono_ns();
         cv_work.wait(lk);
+        t_slot_wait += mono_ns() - t0;
         continue;
       }
       // admission by decoded-frame bytes, in arrival order
@@ -141,9 +145,11 @@ struct cjd_server {
         files[(size_t)i] = take[(size_t)i]->data.data();
         sizes[(size_t)i] = take[(size_t)i]->data.size();
       }
+      const int64_t ts = mono_ns();
       int e = cjd_batch_stage_async(sl.b, n, files.data(), sizes.data(), errs.data(), sl.stream);
       const int64_t tl = mono_ns();
       if (!e) e = cjd_batch_decode(sl.b, sl.stream);
+      const int64_t te = mono_ns();
       for (int i = 0; i < n; ++i) {
         Request* r = take[(size_t)i];
         r->t_launch = tl;
@@ -151,6 +157,9 @@ struct cjd_server {
         r->error = e ? e : errs[(size_t)i];
       }
       lk.lock();
+      ++n_batches;
+      t_stage += tl - ts;
+      t_launch += te - tl;
       inflight.push_back(si);
       cv_inflight.notify_one();
     }
@@ -168,9 +177,14 @@ struct cjd_server {
       lk.unlock();
       const int n = (int)sl.reqs.size();
       std::vector<int> errs((size_t)n, 0);
+      const int64_t tf = mono_ns();
       const int e = cjd_batch_finish(sl.b, sl.stream, errs.data());
       const int64_t tr = mono_ns();
       const int64_t redone = cjd_batch_fallbacks(sl.b);  // read by the thread that owns the batch now
+      float ms_ent = 0.f, ms_pix = 0.f, ms_all = 0.f;
+      if (!e) (void)cjd_batch_timings(sl.b, &ms_ent, &ms_pix, &ms_all);
+      const int64_t dev_ns = (int64_t)((double)ms_all * 1e6);
+      const int64_t t_launched = n ? sl.reqs[0]->t_launch : tr;
       for (int i = 0; i < n; ++i) {
         Request* r = sl.reqs[(size_t)i];
         r->t_ready = tr;
@@ -181,6 +195,10 @@ struct cjd_server {
         }
       }
       lk.lock();
+      t_finish += tr - tf;
+      t_device += dev_ns;
+      t_device_max = std::max(t_device_max, dev_ns);
+      t_turnaround += tr - t_launched;
       n_redone += redone - sl.redone_seen;
       sl.redone_seen = redone;
       sl.held = n;
@@ -270,15 +288,22 @@ extern "C" int cjd_server_poll(cjd_server* s, cjd_completion* out, int max, int 
   return n;
 }
 
-extern "C" int cjd_server_stats(cjd_server* s, int64_t* submitted, int64_t* completed, int64_t* pending,
-                                int64_t* inflight, int64_t* redone) {
-  if (!s) return CJD_PROGRAMMER_ERROR;
+extern "C" int cjd_server_stats(cjd_server* s, cjd_server_counters* out) {
+  if (!s || !out) return CJD_PROGRAMMER_ERROR;
   std::lock_guard<std::mutex> lk(s->mu);
-  if (submitted) *submitted = s->n_submitted;
-  if (completed) *completed = s->n_completed;
-  if (pending) *pending = (int64_t)s->pending.size();
-  if (inflight) *inflight = (int64_t)s->inflight.size();
-  if (redone) *redone = s->n_redone;
+  out->submitted = s->n_submitted;
+  out->completed = s->n_completed;
+  out->pending = (int64_t)s->pending.size();
+  out->inflight = (int64_t)s->inflight.size();
+  out->redone = s->n_redone;
+  out->batches = s->n_batches;
+  out->stage_ns = s->t_stage;
+  out->launch_ns = s->t_launch;
+  out->finish_ns = s->t_finish;
+  out->slot_wait_ns = s->t_slot_wait;
+  out->device_ns = s->t_device;
+  out->device_max_ns = s->t_device_max;
+  out->turnaround_ns = s->t_turnaround;
   return 0;
 }
 
@@ -333,13 +358,15 @@ extern "C" int cjd_server_replay(cjd_server* s, int n, const uint8_t* const* fil
       for (int64_t now = mono_ns(); now < target; now = mono_ns())
         if (target - now > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(target - now - 100000));
       uint64_t id = 0;
-      std::lock_guard<std::mutex> lk(mu);  // held across submit: the id is indexed before it can complete
+      // not under the index lock: a submit parses and copies the file, and
+      // the completion loop below must not wait for it to release frames
       const int e = cjd_server_submit(s, files[k], sizes[k], &id);
       if (e) {
         errors[k] = e;
         latency_ns[k] = 0;
         rejected.fetch_add(1);
       } else {
+        std::lock_guard<std::mutex> lk(mu);
         index[id] = k;
       }
     }
@@ -351,10 +378,19 @@ extern "C" int cjd_server_replay(cjd_server* s, int n, const uint8_t* const* fil
     const int m = cjd_server_poll(s, buf.data(), (int)buf.size(), 1000);
     for (int i = 0; i < m; ++i) {
       const cjd_completion& c = buf[(size_t)i];
-      int k;
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        k = index[c.id];
+      // the id is indexed right after its submit returns; a completion can
+      // be polled just before that
+      int k = -1;
+      while (k < 0) {
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          auto it = index.find(c.id);
+          if (it != index.end()) {
+            k = it->second;
+            index.erase(it);
+          }
+        }
+        if (k < 0) std::this_thread::yield();
       }
       latency_ns[k] = c.t_ready_ns - c.t_arrival_ns;
       errors[k] = c.error;

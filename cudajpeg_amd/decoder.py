@@ -112,15 +112,22 @@ class BatchDecoder:
         arr = (L.TuneEntry * max(1, len(table)))(*[L.TuneEntry(*t[:4], 0) for t in table])
         _check(L.lib().cjd_batch_set_tune_table(self.h, len(table), arr), "tune table: ")
 
-    def stage(self, files: list[bytes]) -> np.ndarray:
+    def stage(self, files: list[bytes], stream=None, asynchronous: bool = False) -> np.ndarray:
+        """Parse, unstuff and plan the files and upload them (the files are
+        read during the call, not after).  asynchronous: the uploads are
+        ordered on `stream` and not waited for (cjd_batch_stage_async)."""
         n = len(files)
-        self._bufs = [ctypes.create_string_buffer(f, len(f)) for f in files]
-        ptrs = (ctypes.c_void_p * n)(*[ctypes.cast(b, ctypes.c_void_p) for b in self._bufs])
+        keep = [ctypes.c_char_p(bytes(f)) for f in files]  # points at each bytes object's own buffer
+        ptrs = (ctypes.c_void_p * n)(*[ctypes.cast(k, ctypes.c_void_p) for k in keep])
         sizes = (ctypes.c_size_t * n)(*[len(f) for f in files])
         self.errors = np.zeros(n, np.int32)
-        _check(L.lib().cjd_batch_stage(self.h, n, ptrs, sizes, self.errors.ctypes.data), "stage: ")
+        if asynchronous:
+            _check(L.lib().cjd_batch_stage_async(self.h, n, ptrs, sizes, self.errors.ctypes.data,
+                                                 _stream_handle(stream)), "stage: ")
+        else:
+            _check(L.lib().cjd_batch_stage(self.h, n, ptrs, sizes, self.errors.ctypes.data), "stage: ")
+        del keep
         self.n = n
-        self._bufs = None
         return self.errors.copy()
 
     def decode(self, stream=None) -> None:

@@ -63,10 +63,12 @@ class Server:
         _check(L.lib().cjd_server_release(self.h, rid), "release: ")
 
     def stats(self) -> dict:
-        """Counters of the loop: submitted, completed, pending, inflight, redone."""
-        v = [ctypes.c_int64() for _ in range(5)]
-        _check(L.lib().cjd_server_stats(self.h, *[ctypes.byref(x) for x in v]), "stats: ")
-        return dict(zip(["submitted", "completed", "pending", "inflight", "redone"], [x.value for x in v]))
+        """Counters of the loop (cjd_server_counters): requests submitted,
+        completed, pending and in flight, images redone, batches, and host
+        ns spent staging, launching, finishing and waiting for a slot."""
+        c = L.ServerCounters()
+        _check(L.lib().cjd_server_stats(self.h, ctypes.byref(c)), "stats: ")
+        return {k: getattr(c, k) for k, _ in L.ServerCounters._fields_}
 
     def frame_tensor(self, c: Completion):
         """Zero-copy torch view (uint8, H x W x channels) of a completed

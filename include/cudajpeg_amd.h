@@ -287,12 +287,21 @@ int cjd_server_submit(cjd_server *s, const uint8_t *data, size_t size, uint64_t 
 /* Up to max completions, waiting up to wait_us for the first; returns the count. */
 int cjd_server_poll(cjd_server *s, cjd_completion *out, int max, int wait_us);
 int cjd_server_release(cjd_server *s, uint64_t id);
-/* Counters of the loop since it was created: requests submitted and
- * completed, requests waiting for a batch, batches launched and not yet
- * finished, and images the slots' batches decoded again (cjd_batch_fallbacks
- * summed).  Any pointer may be NULL.  For monitoring. */
-int cjd_server_stats(cjd_server *s, int64_t *submitted, int64_t *completed, int64_t *pending, int64_t *inflight,
-                     int64_t *redone);
+/* Counters of the loop since it was created, for monitoring: requests
+ * submitted and completed, requests waiting for a batch, batches launched and
+ * not yet finished, images the slots' batches decoded again
+ * (cjd_batch_fallbacks summed), batches launched, and host time (ns) the
+ * batcher spent staging and launching batches, the completion thread spent
+ * in cjd_batch_finish, and the batcher waited for a free slot; then the
+ * batches' device time (ns, cjd_batch_timings of each decode summed, and its
+ * largest value) and the time from a batch's launch to its completion
+ * (summed). */
+typedef struct {
+  int64_t submitted, completed, pending, inflight, redone;
+  int64_t batches, stage_ns, launch_ns, finish_ns, slot_wait_ns;
+  int64_t device_ns, device_max_ns, turnaround_ns;
+} cjd_server_counters;
+int cjd_server_stats(cjd_server *s, cjd_server_counters *out);
 /* Host monotonic clock of the loop's timestamps, in ns. */
 int64_t cjd_server_now_ns(void);
 /* Replays an arrival trace: file k is submitted at start + arrival_ns[k]

@@ -214,7 +214,7 @@ void run_entropy(Emu& E, int order) {
     sh.lovf = 0;  // what the kernel's lane 0 sets before the first barrier
     sh.look_ok = 1;
     EntCtx& c = R.ctx[i];
-    c.init(p.words.data(), p.images.data(), p.segs.data(), p.tables.data(), E.ebuf.data(), p.items[i]);
+    c.init(p.wptr, p.images.data(), p.segs.data(), p.tables.data(), E.ebuf.data(), p.items[i]);
     memcpy(sh.lut, &p.luts[(size_t)c.sg.image * kLutEntries], sizeof(sh.lut));
     memcpy(sh.dz, kDeZigZag, 64);
     for (int k = 0; k < 4; ++k) canon_fill(sh.canon[k], c.tabs[k]);
@@ -315,7 +315,7 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
   E->status.assign((size_t)n, (kEpoch - 1) << 8 | 0xFFu);
   if (serial) {
     for (auto& s : E->plan.segs)
-      ent_serial_segment(E->plan.words.data(), E->plan.images.data(), s, E->plan.tables.data(), E->coefs.data(),
+      ent_serial_segment(E->plan.wptr, E->plan.images.data(), s, E->plan.tables.data(), E->coefs.data(),
                          E->recs.data(), E->status.data(), kEpoch);
   } else {
     run_entropy(*E, order);
@@ -332,7 +332,7 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
       // the serial redo is a launch of its own (next epoch)
       ++E->fallbacks;
       for (uint32_t s = 0; s < r.seg_count; ++s)
-        ent_serial_segment(E->plan.words.data(), E->plan.images.data(), E->plan.segs[r.seg_first + s],
+        ent_serial_segment(E->plan.wptr, E->plan.images.data(), E->plan.segs[r.seg_first + s],
                            E->plan.tables.data(), E->coefs.data(), E->recs.data(), E->status.data(), kEpoch + 1);
       st = status_bits(E->status[(size_t)i], kEpoch + 1) | kStatusFallback;
     }

@@ -101,7 +101,7 @@ void scan_counts(Group& g, EntShared& sh, int tid, const EntCtx& c) {
 void lane_main(Group& g, EntShared& sh, const Launch& X, uint32_t item_id, int tid) {
   Plan& P = *X.p;
   EntCtx c;
-  c.init(P.words.data(), P.images.data(), P.segs.data(), P.tables.data(), X.ebuf->data(), P.items[item_id]);
+  c.init(P.wptr, P.images.data(), P.segs.data(), P.tables.data(), X.ebuf->data(), P.items[item_id]);
   if (tid == 0) {
     sh.lovf = 0;
     sh.look_ok = 1;
