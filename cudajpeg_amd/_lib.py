@@ -15,12 +15,13 @@ NO_ERROR, SYNTAX_ERROR, UNSUPPORTED_ERROR, OOM_ERROR = 0, 1, 2, 3
 DEVICE_MEM_ERROR, FILE_ERROR, KERNEL_LAUNCH_ERROR, PROGRAMMER_ERROR = 4, 5, 6, -1
 IDCT_INT, IDCT_FLOAT, IDCT_MFMA = 0, 1, 2
 ENTROPY_TILED, ENTROPY_SERIAL = 0, 1
+UNSTUFF_HOST, UNSTUFF_DEVICE = 0, 1
 OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 
 # Every symbol include/cudajpeg_amd.h and include/format.h declare.
 EXPORTS = [
     "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
-    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_overlap", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
+    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_overlap", "cjd_batch_set_unstuff", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
     "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_set_wait_limit", "cjd_batch_protocol_redos",
     "cjd_batch_status",
     "cjd_batch_finish", "cjd_batch_capture",
@@ -102,6 +103,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_set_options": ([vp, i32, i32, i32, i32], i32),
         "cjd_batch_set_tuning": ([vp, i32, i32], i32),
         "cjd_batch_set_overlap": ([vp, i32], i32),
+        "cjd_batch_set_unstuff": ([vp, i32], i32),
         "cjd_batch_set_tune_table": ([vp, i32, ctypes.POINTER(TuneEntry)], i32),
         "cjd_batch_stage": ([vp, i32, vp, vp, vp], i32),
         "cjd_batch_stage_async": ([vp, i32, vp, vp, vp, vp], i32),

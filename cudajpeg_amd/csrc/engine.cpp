@@ -99,7 +99,11 @@ struct cjd_batch {
   // descriptors and tables are packed into h_meta, and every upload is a DMA
   // from them.  up_ev follows the last stage's uploads: the next stage waits
   // for it before writing into these buffers again.
-  PinnedBuf h_words, h_meta;
+  PinnedBuf h_words, h_meta, h_raw;
+  // Device unstuffing (cjd_batch_set_unstuff): the stuffed scans and their
+  // chunk states go up instead of the words, and k_unstuff writes the words.
+  int unstuff = CJD_UNSTUFF_DEVICE;
+  DevBuf d_raw, d_chunks;
   hipEvent_t up_ev = nullptr;
   bool up_pending = false;
   DevBuf d_ebuf, d_recs;  // tiled decoder: entry regions and block records
@@ -178,6 +182,9 @@ extern "C" int cjd_batch_create(cjd_context* ctx, cjd_batch** out) {
   }
   b->plan.word_store = [b](size_t n) -> uint32_t* {
     return b->h_words.ensure(n * 4) ? static_cast<uint32_t*>(b->h_words.p) : nullptr;
+  };
+  b->plan.raw_store = [b](size_t n) -> uint8_t* {
+    return b->h_raw.ensure(n) ? static_cast<uint8_t*>(b->h_raw.p) : nullptr;
   };
   *out = b;
   return 0;
@@ -291,6 +298,12 @@ static void plan_split(cjd_batch* b) {
   b->nsplit = n + 1;
 }
 
+extern "C" int cjd_batch_set_unstuff(cjd_batch* b, int mode) {
+  if (!b || (mode != CJD_UNSTUFF_HOST && mode != CJD_UNSTUFF_DEVICE)) return CJD_PROGRAMMER_ERROR;
+  b->unstuff = mode;
+  return 0;
+}
+
 extern "C" int cjd_batch_set_overlap(cjd_batch* b, int parts) {
   if (!b || parts < 0 || parts > kMaxParts) return CJD_PROGRAMMER_ERROR;
   b->parts = parts == 0 ? 1 : parts;
@@ -314,9 +327,11 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
     if (hipEventSynchronize(b->up_ev) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
     b->up_pending = false;
   }
+  b->plan.unstuff_device = b->unstuff == CJD_UNSTUFF_DEVICE;
   plan_batch(b->plan, n, files, sizes, errors);
   const Plan& P = b->plan;
   MetaPacker mp{b};
+  mp.count(P.chunks);
   mp.count(P.images);
   mp.count(P.segs);
   mp.count(P.items);
@@ -326,9 +341,19 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   mp.count(P.qtabs);
   const bool pinned = b->h_meta.ensure(mp.total);
   int e = 0;
-  if (!e) e = upload_bytes(b->d_words, P.wptr, P.nwords * 4, s);
+  if (P.unstuff_device) {
+    if (!e) e = b->d_words.ensure(P.nwords * 4);
+    if (!e) e = upload_bytes(b->d_raw, P.rptr, P.nraw, s);
+    if (!e) e = mp.put(b->d_chunks, P.chunks, pinned, s);
+  } else {
+    if (!e) e = upload_bytes(b->d_words, P.wptr, P.nwords * 4, s);
+  }
   if (!e) e = mp.put(b->d_images, P.images, pinned, s);
   if (!e) e = mp.put(b->d_segs, P.segs, pinned, s);
+  if (!e && P.unstuff_device &&
+      launch_unstuff(b->d_chunks.as<UnstuffChunk>(), (uint32_t)P.chunks.size(), b->d_raw.as<uint8_t>(),
+                     b->d_segs.as<SegmentDesc>(), b->d_words.as<uint32_t>(), P.nwords, s) != hipSuccess)
+    e = CJD_KERNEL_LAUNCH_ERROR;
   if (!e) e = mp.put(b->d_items, P.items, pinned, s);
   if (!e) e = mp.put(b->d_pix, P.pix, pinned, s);
   if (!e) e = mp.put(b->d_tables, P.tables, pinned, s);

@@ -6,6 +6,12 @@
 
 namespace cjd {
 
+struct UnstuffChunk;
+// Fill the batch's scan words with 0xFF, then unstuff every chunk of stuffed
+// scan bytes into them (unstuff.hip).
+hipError_t launch_unstuff(const UnstuffChunk* chunks, uint32_t nchunks, const uint8_t* raw, const SegmentDesc* segs,
+                          uint32_t* words, uint64_t nwords, hipStream_t s);
+
 hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s);
 // Tiled entropy decode of work items [item0, item_end) (whole segments).
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,

@@ -83,64 +83,82 @@ void build_lut(const uint8_t bits[16], const uint8_t vals[256], bool ac, uint16_
   }
 }
 
-namespace {
-
 // src/decodeScanGPU.cu:11-24 (remove 0x00 after 0xFF, keep RSTn, drop other
 // marker bytes) with the scan split at RSTn when a restart interval is set.
 // Runs between 0xFF bytes are found with memchr and block-copied.  With
 // dst == nullptr only the segment bounds (in unstuffed bytes, segment k =
 // [bounds[k], bounds[k+1])) are computed; otherwise segment k is written to
-// dst + dst_off[k].  Returns false on an out-of-sequence RSTn
-// (src/decodeScanGPU.cu:480-482).
+// dst + dst_off[k].  With chunks, the starting state of every chunk of
+// kUnstuffChunk stuffed bytes is appended for the device unstuffer (src
+// relative to the scan, seg relative to the image's first segment).  Returns
+// false on an out-of-sequence RSTn (src/decodeScanGPU.cu:480-482).
 bool unstuff_segments(const uint8_t* src, size_t len, bool split, std::vector<size_t>* bounds, uint8_t* dst,
-                      const size_t* dst_off) {
+                      const size_t* dst_off, std::vector<UnstuffChunk>* chunks) {
   if (bounds) bounds->assign(1, 0);
+  if (chunks) chunks->clear();
   size_t o = 0, s = 0, seg = 0;
   uint8_t* out = dst ? dst + dst_off[0] : nullptr;
   int next_rst = 0;
+  size_t q = 0;  // start of the next chunk
+  // chunk at q: its first output byte is byte dst_at of segment seg
+  auto chunk = [&](size_t dst_at, uint32_t skip) {
+    UnstuffChunk c{};
+    c.src = q;
+    c.len = (uint32_t)std::min<size_t>(kUnstuffChunk, len - q);
+    c.seg = (uint32_t)seg;
+    c.dst = (uint32_t)dst_at;
+    c.flags = skip | (split ? kChunkSplit : 0u) | (q + c.len == len ? kChunkScanEnd : 0u);
+    chunks->push_back(c);
+    q += kUnstuffChunk;
+  };
   while (s < len) {
     const void* ff = memchr(src + s, 0xFF, len - s);
     const size_t e = ff ? (size_t)(static_cast<const uint8_t*>(ff) - src) : len;
+    if (chunks)
+      while (q < e) chunk(o + (q - s), 0);
     if (out) memcpy(out + o, src + s, e - s);
     o += e - s;
     s = e;
     if (s >= len) break;
-    ++s;  // the 0xFF
-    const uint8_t nx = s < len ? src[s] : 0xFF;
-    ++s;
+    const size_t p = s;  // the pair (p, p + 1)
+    if (chunks && q == p) chunk(o, 0);
+    const uint8_t nx = p + 1 < len ? src[p + 1] : 0xFF;
+    s = p + 2;
     if (nx == 0x00) {
       if (out) out[o] = 0xFF;
       ++o;
-      continue;
-    }
-    if ((nx & 0xF8) == 0xD0) {
-      if (split) {
-        if ((nx & 7) != next_rst) return false;
-        next_rst = (next_rst + 1) & 7;
-        if (bounds) bounds->push_back(bounds->back() + o);
-        ++seg;
-        if (dst) out = dst + dst_off[seg];
-        o = 0;
-        continue;
-      }
+    } else if ((nx & 0xF8) == 0xD0 && split) {
+      if ((nx & 7) != next_rst) return false;
+      next_rst = (next_rst + 1) & 7;
+      if (bounds) bounds->push_back(bounds->back() + o);
+      ++seg;
+      if (dst) out = dst + dst_off[seg];
+      o = 0;
+    } else if ((nx & 0xF8) == 0xD0) {
       if (out) {
         out[o] = 0xFF;
         out[o + 1] = nx;
       }
       o += 2;
-      continue;
+    } else {
+      if (out) out[o] = 0xFF;
+      ++o;
     }
-    if (out) out[o] = 0xFF;
-    ++o;
+    // a chunk that starts at the pair's second byte
+    if (chunks && q == p + 1 && p + 1 < len) chunk(o, kChunkSkipFirst);
   }
   if (bounds) bounds->push_back(bounds->back() + o);
   return true;
 }
 
+namespace {
+
 // Sizes of one image's share of the plan (pass 1) and its offsets in the
 // batch (pass 2).
 struct ImageWork {
   std::vector<size_t> bounds;    // unstuffed segment bounds
+  std::vector<UnstuffChunk> chunks;  // device unstuffing: the scan's chunks (image-relative)
+  int64_t raw0 = 0, chunk0 = 0;  // device unstuffing: offsets in the raw buffer and the chunk list
   uint32_t S = 4096, C = 512, items = kWG;  // entropy parameters for this image
   int64_t nseg = 0, nitems = 0, npix = 0, nwords = 0, nent = 0;
   int64_t seg0 = 0, item0 = 0, pix0 = 0, word0 = 0, desc = 0, coef0 = 0, ent0 = 0;
@@ -193,7 +211,7 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
   const cjd_frame_info& f = r.f;
   if (f.blocks_per_mcu > kMaxSlots) return CJD_UNSUPPORTED_ERROR;
   if (!unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, &w.bounds, nullptr,
-                        nullptr))
+                        nullptr, b.unstuff_device ? &w.chunks : nullptr))
     return CJD_SYNTAX_ERROR;
   const int64_t nmcu = (int64_t)f.mcus_x * f.mcus_y;
   int64_t nseg_expected = 1;
@@ -289,19 +307,30 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
   d.error_index = index;
   b->images[(size_t)w.desc] = d;
 
-  // words: 0xFF padding, then the unstuffed data written straight in
+  // words: 0xFF padding, then the unstuffed data written straight in; or,
+  // for the device unstuffer, the stuffed scan and its chunk states
   std::vector<size_t> dst_off((size_t)w.nseg);
   int64_t wo = w.word0;
   uint32_t* words = b->wptr;
   for (int64_t s = 0; s < w.nseg; ++s) {
     const size_t dlen = w.bounds[s + 1] - w.bounds[s];
     const int64_t nw = seg_words(dlen);
-    std::fill(words + wo, words + wo + nw, 0xFFFFFFFFu);
+    if (words) std::fill(words + wo, words + wo + nw, 0xFFFFFFFFu);
     dst_off[(size_t)s] = (size_t)wo * 4;
     wo += nw;
   }
-  unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, nullptr,
-                   reinterpret_cast<uint8_t*>(words), dst_off.data());
+  if (b->unstuff_device) {
+    memcpy(b->rptr + w.raw0, file + f.scan_offset, (size_t)f.scan_length);
+    for (size_t k = 0; k < w.chunks.size(); ++k) {
+      UnstuffChunk c = w.chunks[k];
+      c.src += (uint64_t)w.raw0;
+      c.seg += (uint32_t)w.seg0;
+      b->chunks[(size_t)w.chunk0 + k] = c;
+    }
+  } else {
+    unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, nullptr,
+                     reinterpret_cast<uint8_t*>(words), dst_off.data(), nullptr);
+  }
 
   // segments, tiles and entropy work items: one lane per tile; an item after
   // the first also decodes its lead-in tile
@@ -435,7 +464,7 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     r.error = image_pass1(b, r, work[(size_t)i], files[i], sizes[i]);
   });
   // pass 2: offsets, in image order
-  int64_t nimg = 0, nseg = 0, nitems = 0, npix = 0, nwords = 0, nblocks = 0, nent = 0;
+  int64_t nimg = 0, nseg = 0, nitems = 0, npix = 0, nwords = 0, nblocks = 0, nent = 0, nraw = 0, nchunks = 0;
   b.total_pixels = b.scan_bytes = 0;
   b.out_bytes = 0;
   for (int i = 0; i < n; ++i) {
@@ -452,6 +481,10 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     w.coef0 = nblocks;
     w.ent0 = nent;
     nent += w.nent;
+    w.raw0 = nraw;
+    nraw += (f.scan_length + 15) & ~(int64_t)15;
+    w.chunk0 = nchunks;
+    nchunks += (int64_t)w.chunks.size();
     nseg += w.nseg;
     nitems += w.nitems;
     npix += w.npix;
@@ -489,12 +522,25 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
   b.luts.resize((size_t)nimg * kLutEntries);
   b.qtabs.resize((size_t)nimg * 4 * 64);
   b.nwords = (size_t)nwords;
-  b.wptr = b.word_store ? b.word_store(b.nwords) : nullptr;
-  if (b.wptr) {
+  b.chunks.resize((size_t)nchunks);
+  b.nraw = b.unstuff_device ? (size_t)nraw : 0;
+  b.wptr = nullptr;
+  b.rptr = nullptr;
+  if (b.unstuff_device) {
     b.words.clear();
+    b.rptr = b.raw_store ? b.raw_store(b.nraw) : nullptr;
+    if (!b.rptr) {
+      b.raw.resize(b.nraw);
+      b.rptr = b.raw.data();
+    }
   } else {
-    b.words.resize(b.nwords);
-    b.wptr = b.words.data();
+    b.wptr = b.word_store ? b.word_store(b.nwords) : nullptr;
+    if (b.wptr) {
+      b.words.clear();
+    } else {
+      b.words.resize(b.nwords);
+      b.wptr = b.words.data();
+    }
   }
   // pass 3: fill, per image in parallel
   Pool::get().run(n, [&](int i) {

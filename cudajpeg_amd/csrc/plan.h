@@ -11,6 +11,7 @@
 
 #include "../../include/cudajpeg_amd.h"
 #include "cjd_internal.h"
+#include "unstuff_core.h"
 
 namespace cjd {
 
@@ -58,6 +59,17 @@ struct Plan {
   std::vector<uint32_t> words;
   uint32_t* wptr = nullptr;
   size_t nwords = 0;
+  // Device unstuffing (unstuff_device = 1): the host leaves the words to the
+  // device (wptr stays null) and gives it instead every image's stuffed scan
+  // bytes (nraw bytes at rptr, each scan 16-byte aligned; storage from
+  // raw_store as for the words) and the starting state of every chunk of
+  // kUnstuffChunk scan bytes.
+  int unstuff_device = 0;
+  std::function<uint8_t*(size_t)> raw_store;
+  std::vector<uint8_t> raw;
+  uint8_t* rptr = nullptr;
+  size_t nraw = 0;
+  std::vector<UnstuffChunk> chunks;
   int64_t total_blocks = 0, total_pixels = 0, scan_bytes = 0;
   int64_t entry_words = 0;       // entry buffer of the tiled decoder (regions, checkpoints, sinks)
   uint64_t out_bytes = 0;
@@ -72,6 +84,12 @@ inline uint32_t ordered_launch_end(const std::vector<WorkItem>& items, uint32_t 
   while (e < end && !(items[e].t0 > 0 && items[e].prev_item >= a)) ++e;
   return e;
 }
+
+// src/decodeScanGPU.cu:11-24 on one scan (plan.cpp): the unstuffed segment
+// bounds, the unstuffed segments written at dst + dst_off[k], and/or the
+// chunk states of the device unstuffer; false on an out-of-sequence RSTn.
+bool unstuff_segments(const uint8_t* src, size_t len, bool split, std::vector<size_t>* bounds, uint8_t* dst,
+                      const size_t* dst_off, std::vector<UnstuffChunk>* chunks);
 
 // Build the plan for n files; errors[i] gets each file's parse/stage result.
 void plan_batch(Plan& p, int n, const uint8_t* const* files, const size_t* sizes, int* errors);

@@ -101,6 +101,11 @@ class BatchDecoder:
         work item; 0 = defaults; applies from the next stage()."""
         _check(L.lib().cjd_batch_set_tuning(self.h, ck_bits, item_tiles), "tuning: ")
 
+    def set_unstuff(self, mode: int) -> None:
+        """UNSTUFF_DEVICE (default): scans unstuffed by a kernel at staging;
+        UNSTUFF_HOST: by the host planner.  From the next stage."""
+        _check(L.lib().cjd_batch_set_unstuff(self.h, mode), "unstuff: ")
+
     def set_overlap(self, parts: int = 2) -> None:
         """Decode in parts (default 2; 1 = off): each run's pixel kernel runs
         beside the next runs' entropy decodes, on a second stream."""

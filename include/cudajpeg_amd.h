@@ -97,6 +97,16 @@ int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int til
  * cjd_batch_stage. */
 int cjd_batch_set_tuning(cjd_batch *b, int ck_bits, int item_tiles);
 
+/* Where the scans are unstuffed (the 0x00 after each 0xFF removed and the
+ * scan split at RSTn: src/decodeScanGPU.cu:11-24, which the reference does on
+ * the host): CJD_UNSTUFF_DEVICE (default) uploads each file's stuffed scan
+ * and unstuffs it with a kernel on the staging stream; CJD_UNSTUFF_HOST
+ * unstuffs in the host planner and uploads the words.  Results are identical.
+ * Applies from the next stage.  No counterpart in the reference. */
+#define CJD_UNSTUFF_HOST 0
+#define CJD_UNSTUFF_DEVICE 1
+int cjd_batch_set_unstuff(cjd_batch *b, int mode);
+
 /* Decode in parts (no counterpart in the reference; 2 parts by default): for
  * a batch of at least 64 entropy work items, cjd_batch_decode cuts the images
  * into `parts` runs of about equal work (whole images) and launches the runs'

@@ -22,6 +22,12 @@ def lib():
         L.emu_get.restype = ctypes.c_long
         L.emu_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.emu_free.argtypes = [ctypes.c_void_p]
+        L.emu_unstuff_files.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        L.emu_unstuff_files.restype = ctypes.c_long
+        L.emu_unstuff_bytes.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_size_t]
+        L.emu_unstuff_bytes.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -52,3 +58,30 @@ class EmuResult:
         if getattr(self, "h", None):
             lib().emu_free(self.h)
             self.h = None
+
+
+def unstuff_files(files):
+    """(host words, emulated device words) of the batch, or the error code."""
+    n = len(files)
+    keep = [ctypes.c_char_p(bytes(f)) for f in files]
+    ptrs = (ctypes.c_void_p * n)(*[ctypes.cast(k, ctypes.c_void_p) for k in keep])
+    sizes = (ctypes.c_size_t * n)(*[len(f) for f in files])
+    cap = sum(len(f) for f in files) + 4096 * n + 200000  # words: data / 4 plus 100 per restart segment, at most
+    h = np.zeros(cap, np.uint32)
+    d = np.zeros(cap, np.uint32)
+    errors = np.zeros(n, np.int32)
+    r = lib().emu_unstuff_files(n, ptrs, sizes, h.ctypes.data, d.ctypes.data, cap, errors.ctypes.data)
+    if r < 0:
+        return r
+    return h[:r], d[:r]
+
+
+def unstuff_bytes(data: bytes, split: bool):
+    """(host words, emulated device words) of one raw scan, or the error code."""
+    cap = len(data) // 4 + 100 * (len(data) // 2 + 2) + 64  # at most one segment per RSTn pair
+    h = np.zeros(cap, np.uint32)
+    d = np.zeros(cap, np.uint32)
+    r = lib().emu_unstuff_bytes(data, len(data), int(split), h.ctypes.data, d.ctypes.data, cap)
+    if r < 0:
+        return r
+    return h[:r], d[:r]

@@ -16,6 +16,7 @@
 #include "entropy_core.h"
 #include "pixel_core.h"
 #include "plan.h"
+#include "unstuff_core.h"
 
 using namespace cjd;
 
@@ -294,6 +295,54 @@ void run_pixel(Emu& E, int mode, size_t first, size_t count) {
   }
 }
 
+// k_unstuff (unstuff.hip) lane by lane, a barrier between phases: every
+// chunk of stuffed scan bytes into the word buffer (already 0xFF-filled).
+void emu_k_unstuff(const UnstuffChunk* chunks, size_t nchunks, const uint8_t* raw, const SegmentDesc* segs,
+                   size_t nsegs, uint8_t* out, size_t out_bytes, bool* oob) {
+  std::unique_ptr<UnstuffShared> shp(new UnstuffShared);
+  UnstuffShared& sh = *shp;
+  std::vector<uint8_t> fn(kUnstuffLanes);
+  std::vector<uint32_t> brk(kUnstuffLanes), tail(kUnstuffLanes);
+  std::vector<char> consumed(kUnstuffLanes);
+  for (size_t ci = 0; ci < nchunks; ++ci) {
+    const UnstuffChunk& c = chunks[ci];
+    memset(&sh, 0x5A, sizeof(sh));
+    for (int l = 0; l < kUnstuffLanes; ++l) unstuff_load(sh, l, c, raw);
+    for (int l = 0; l < kUnstuffLanes; ++l) unstuff_carry(sh, l, c);
+    for (int d = 1; d < kUnstuffLanes; d <<= 1) {
+      for (int l = 0; l < kUnstuffLanes; ++l) fn[(size_t)l] = unstuff_fn_read(sh, l, d);
+      for (int l = 0; l < kUnstuffLanes; ++l) sh.fn[l] = fn[(size_t)l];
+    }
+    for (int l = 0; l < kUnstuffLanes; ++l) consumed[(size_t)l] = unstuff_carry_in(sh, l, c);
+    for (int l = 0; l < kUnstuffLanes; ++l) unstuff_count(sh, l, c, consumed[(size_t)l] != 0);
+    for (int d = 1; d < kUnstuffLanes; d <<= 1) {
+      for (int l = 0; l < kUnstuffLanes; ++l) unstuff_count_read(sh, l, d, brk[(size_t)l], tail[(size_t)l]);
+      for (int l = 0; l < kUnstuffLanes; ++l) {
+        sh.brk[l] = brk[(size_t)l];
+        sh.tail[l] = tail[(size_t)l];
+      }
+    }
+    // byte stores and segment reads, bounds-checked
+    struct Out {
+      uint8_t* p;
+      size_t n;
+      bool* oob;
+      uint8_t dummy = 0;
+      uint8_t& operator[](uint64_t k) {
+        if (k < n) return p[k];
+        *oob = true;
+        return dummy;
+      }
+    } o{out, out_bytes, oob};
+    auto base = [&](uint32_t sgi) -> uint64_t {
+      if (sgi < nsegs) return segs[sgi].word_base * 4u;
+      *oob = true;
+      return 0;
+    };
+    for (int l = 0; l < kUnstuffLanes; ++l) unstuff_write(sh, l, c, consumed[(size_t)l] != 0, o, base);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -387,6 +436,56 @@ void emu_status(void* h, uint32_t* out, int n) {
 }
 
 void emu_free(void* h) { delete static_cast<Emu*>(h); }
+
+// Device unstuffing of a batch against the host's: the batch planned both
+// ways, the device plan's chunks run through the emulated k_unstuff into a
+// 0xFF-filled word buffer.  Copies both word buffers out (cap words each);
+// returns the word count, -1 when the plans disagree on it or cap is short,
+// -2 when a byte store or segment read would leave its buffer.
+long emu_unstuff_files(int n, const uint8_t* const* files, const size_t* sizes, uint32_t* out_host, uint32_t* out_dev,
+                       size_t cap, int* errors) {
+  Plan a, d;
+  d.unstuff_device = 1;
+  plan_batch(a, n, files, sizes, errors);
+  plan_batch(d, n, files, sizes, nullptr);
+  if (a.nwords != d.nwords || a.nwords > cap || d.wptr != nullptr) return -1;
+  std::vector<uint32_t> w(d.nwords, 0xFFFFFFFFu);
+  bool oob = false;
+  emu_k_unstuff(d.chunks.data(), d.chunks.size(), d.rptr, d.segs.data(), d.segs.size(),
+                reinterpret_cast<uint8_t*>(w.data()), w.size() * 4, &oob);
+  if (oob) return -2;
+  memcpy(out_host, a.wptr, a.nwords * 4);
+  memcpy(out_dev, w.data(), w.size() * 4);
+  return (long)a.nwords;
+}
+
+// The same on one raw scan (any bytes), with segments laid out as the
+// planner lays them out: -3 on an out-of-sequence RSTn (both refuse it).
+long emu_unstuff_bytes(const uint8_t* src, size_t len, int split, uint32_t* out_host, uint32_t* out_dev, size_t cap) {
+  std::vector<size_t> bounds;
+  std::vector<UnstuffChunk> chunks;
+  if (!unstuff_segments(src, len, split != 0, &bounds, nullptr, nullptr, &chunks)) return -3;
+  const size_t nseg = bounds.size() - 1;
+  std::vector<SegmentDesc> segs(nseg);
+  std::vector<size_t> dst_off(nseg);
+  size_t nw = 0;
+  for (size_t k = 0; k < nseg; ++k) {
+    const size_t dlen = bounds[k + 1] - bounds[k];
+    segs[k].word_base = nw;
+    dst_off[k] = nw * 4;
+    nw += (((dlen + 3) / 4 + kSegPadWords) + 3) & ~(size_t)3;
+  }
+  if (nw > cap) return -1;
+  std::vector<uint32_t> h(nw, 0xFFFFFFFFu), w(nw, 0xFFFFFFFFu);
+  unstuff_segments(src, len, split != 0, nullptr, reinterpret_cast<uint8_t*>(h.data()), dst_off.data(), nullptr);
+  bool oob = false;
+  emu_k_unstuff(chunks.data(), chunks.size(), src, segs.data(), segs.size(), reinterpret_cast<uint8_t*>(w.data()),
+                w.size() * 4, &oob);
+  if (oob) return -2;
+  memcpy(out_host, h.data(), nw * 4);
+  memcpy(out_dev, w.data(), nw * 4);
+  return (long)nw;
+}
 
 // Failure injection: after each speculative pass the checkpoint words of every
 // `every`-th tile claim more DC entries than the pass decoded (0 = off).

@@ -236,3 +236,63 @@ def test_kernel_body_has_no_lane_races_under_thread_sanitizer():
     assert r.returncode == 0, r.stderr[-3000:]
     assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
     assert r.stdout.count("exact 1") == 4, r.stdout
+
+
+def test_emulated_device_unstuff_equals_host_on_files(encoded):
+    # the device unstuffer (unstuff_core.h, emulated lane by lane) writes the
+    # batch's word buffer exactly as the host planner does, padding included,
+    # for every sampling, restart intervals and random tiles, and files large
+    # enough for many chunks
+    from tests.emu import unstuff_files
+    files = [e.data for e in encoded] + [synth.encode(synth.ImageSpec(1200, 900, "420", 95, 12.0, 77)).data,
+                                         synth.encode(synth.ImageSpec(700, 500, "422", 90, 9.0, 78,
+                                                                      restart_interval=2)).data]
+    r = unstuff_files(files)
+    assert not isinstance(r, int), r
+    host, dev = r
+    assert len(host) > 4 * 4096 // 4
+    assert np.array_equal(host, dev)
+
+
+def _adversarial_scan(rng, n, split):
+    """Random bytes with many 0xFF: pairs with 0x00, runs of 0xFF, other marker
+    bytes, RSTn in sequence (split) or anywhere, some at chunk boundaries."""
+    out = bytearray()
+    rst = 0
+    while len(out) < n:
+        r = rng.random()
+        if r < 0.55:
+            out += bytes(rng.integers(0, 255, int(rng.integers(1, 64)), dtype=np.uint8))  # no 0xFF
+        elif r < 0.75:
+            out += b"\xff\x00"
+        elif r < 0.85:
+            out += b"\xff" * int(rng.integers(1, 9))
+        elif r < 0.92:
+            out += bytes([0xFF, 0xD0 + rst])
+            rst = (rst + 1) & 7
+        else:
+            out += bytes([0xFF, int(rng.integers(1, 0xCF))])
+        if rng.random() < 0.05:  # pad to just before / at / after a chunk boundary
+            k = 4096 - len(out) % 4096 + int(rng.integers(-2, 3))
+            out += b"\x11" * max(0, k)
+    return bytes(out[:n])
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_emulated_device_unstuff_equals_host_on_adversarial_bytes(split):
+    from tests.emu import unstuff_bytes
+    rng = np.random.default_rng(9100 + split)
+    done = 0
+    for case in range(60):
+        n = int(rng.choice([1, 2, 15, 16, 17, 4095, 4096, 4097, 8191, 8192, 8193, int(rng.integers(1, 40000))]))
+        data = _adversarial_scan(rng, n, split)
+        if case % 7 == 3:
+            data = data[:-1] + b"\xff"  # a 0xFF as the scan's last byte
+        r = unstuff_bytes(data, split)
+        if r == -3:  # out-of-sequence RSTn after truncation: both refuse
+            continue
+        assert not isinstance(r, int), (case, n, r)
+        host, dev = r
+        assert np.array_equal(host, dev), (case, n, int(np.flatnonzero(host != dev)[0]))
+        done += 1
+    assert done >= 40

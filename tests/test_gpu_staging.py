@@ -1,4 +1,5 @@
-"""Staging through pinned host buffers (cjd_batch_stage / _stage_async).
+"""Staging through pinned host buffers (cjd_batch_stage / _stage_async), and
+device-side unstuffing against the host's.
 
 A batch staged asynchronously on one stream while another batch decodes on a
 second stream, then decoded on the first, gives exactly the frames and
@@ -11,6 +12,7 @@ import pytest
 import torch
 
 from cudajpeg_amd import IDCT_INT, BatchDecoder, synth
+from cudajpeg_amd._lib import UNSTUFF_DEVICE, UNSTUFF_HOST
 
 pytestmark = pytest.mark.gpu
 
@@ -60,3 +62,29 @@ def test_async_staging_equals_sync(gpu):
         assert np.array_equal(px[i], refs["small"][0][i]), i
     d.close()
     other.close()
+
+
+def test_device_unstuff_equals_host(gpu):
+    # the same batch staged with the scans unstuffed by the host planner and
+    # by k_unstuff: identical coefficients and frames, restart intervals and
+    # random tiles included (the words themselves are checked against the
+    # host's in the CPU emulator of k_unstuff)
+    specs = synth.config_mixed(40, 8400) + [
+        synth.ImageSpec(1000, 700, "422", 80, 6.0, 8401, restart_interval=7),
+        synth.ImageSpec(640, 480, "444", 95, 12.0, 8402, random_tile_frac=0.5),
+        synth.ImageSpec(333, 211, "gray", 90, 8.0, 8403, restart_interval=1)]
+    encs = synth.encode_many(specs, truth=True)
+    out = {}
+    for mode in (UNSTUFF_HOST, UNSTUFF_DEVICE):
+        d = BatchDecoder(IDCT_INT)
+        d.set_unstuff(mode)
+        print("case unstuff mode", mode, flush=True)
+        assert not d.stage([e.data for e in encs]).any()
+        d.decode()
+        assert not d.finish().any()
+        out[mode] = _frames(d, len(encs))
+        d.close()
+    for i, e in enumerate(encs):
+        assert np.array_equal(out[UNSTUFF_DEVICE][1][i], e.coefs), (i, specs[i])
+        assert np.array_equal(out[UNSTUFF_DEVICE][1][i], out[UNSTUFF_HOST][1][i]), i
+        assert np.array_equal(out[UNSTUFF_DEVICE][0][i], out[UNSTUFF_HOST][0][i]), i
