@@ -313,6 +313,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain form (autotune, then one image per launch) and its latencies")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop load test")
+    ap.add_argument("--no-graph", action="store_true", help="skip the timing of graph replays of the decode")
     ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
                     default=[500.0, 1000.0, 2000.0, 4000.0, 8000.0], help="Poisson rates of the sweep, req/s")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per swept rate")
@@ -432,6 +433,22 @@ def main():
             kexp.append(kx)
         check(b.finish(stream), f"timed steps, decoder 0 (fallbacks {b.fallbacks()})")
         b.set_overlap(args.parts)
+        # the same steps as replays of the decode captured as one HIP graph
+        t_graph = None
+        if not args.no_graph:
+            b.capture(stream)
+            for _ in range(args.warmup):
+                b.replay(stream)
+            check(b.finish(stream), "graph warm-up")
+            torch.cuda.synchronize()
+            t_start = time.perf_counter()
+            for _ in range(args.steps):
+                b.replay(stream)
+            stream.synchronize()
+            torch.cuda.synchronize()
+            t_graph = (time.perf_counter() - t_start) / args.steps
+            check(b.finish(stream), "graph replays")
+            log(f"graph replays: {t_graph * 1e3:.2f} ms/step ({total_px / t_graph / 1e9:.1f} Gpix/s)")
     batched_mpix = total_px / t_batched / 1e6
     # frames back to host memory (the reader API's pixels buffer)
     t1 = time.perf_counter()
@@ -517,6 +534,7 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(t_batched * 1e3, 3),
         "ms_per_step_one_in_flight": round(t_single * 1e3, 3),
+        "ms_per_step_graph": None if t_graph is None else round(t_graph * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

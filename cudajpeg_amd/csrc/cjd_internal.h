@@ -157,7 +157,11 @@ struct LookbackRec {
   int32_t pad_[3];  // 72 bytes
 };
 
-constexpr uint32_t kEpochMax = 0xFFFFFFu;
+// Epochs of direct launches run 1 .. kEpochMax; kGraphEpoch is the fixed
+// epoch of every entropy launch of a captured decode (whose first kernel
+// clears the counters, status words and record states instead).
+constexpr uint32_t kEpochMax = 0xFFFFFEu;
+constexpr uint32_t kGraphEpoch = 0xFFFFFFu;
 
 // Head of a batch's scratch: for each kind j of entropy launch (j < 4: run j
 // of a decode in parts; kRedoCounter: one image decoded again alone), a 64-bit

@@ -126,6 +126,15 @@ struct cjd_batch {
   int64_t fallbacks = 0;
   int64_t protocol_redos = 0;  // checks after a decode that found a counter off and redid the batch
 
+  // Captured decode (cjd_batch_capture): valid while graph_signature(b)
+  // equals the one taken at capture.  After a replay the head's counters of
+  // launch j hold graph_counts[j] and the status words kGraphEpoch's bits.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> graph_sig;
+  uint64_t graph_counts[kLaunchCounters] = {};
+  bool replayed = false;
+
   // Decode in parts (cjd_batch_set_overlap): the batch's images are cut into
   // `parts` runs at work-item boundaries split_item[j] / pixel-item
   // boundaries split_pix[j] (j = 0 .. nsplit, whole images); the entropy
@@ -144,6 +153,8 @@ struct cjd_batch {
   bool timed = false;
 
   ~cjd_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (up_pending) (void)hipEventSynchronize(up_ev);  // no DMA may still read the pinned buffers
     if (up_ev) (void)hipEventDestroy(up_ev);
     for (auto& e : ev)
@@ -424,13 +435,28 @@ static unsigned long long* done_of(cjd_batch* b, int j) {
   return reinterpret_cast<unsigned long long*>(b->d_scratch.as<uint32_t>() + 2 * kLaunchCounters) + j;
 }
 
-static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
+// Epoch of a decode's entropy launches: the next one for a direct decode;
+// for a captured one the fixed kGraphEpoch, after a kernel that clears what
+// the decode's launches count and tag.
+static int decode_epoch(cjd_batch* b, hipStream_t s, bool captured, uint32_t* epoch) {
+  if (!captured) {
+    b->replayed = false;  // a direct decode after a replay is checked as a direct one
+    return next_epoch(b, s, epoch);
+  }
+  *epoch = kGraphEpoch;
+  return launch_decode_begin(b->d_scratch.as<uint32_t>(), (uint32_t)b->status_words,
+                             (uint32_t)b->plan.items.size(), s) == hipSuccess
+             ? 0
+             : CJD_KERNEL_LAUNCH_ERROR;
+}
+
+static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed, bool captured = false) {
   uint32_t* status = status_of(b);
   LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
   const uint32_t nitems = (uint32_t)b->plan.items.size();
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   uint32_t epoch = 0;
-  if (int e = next_epoch(b, s, &epoch)) return e;
+  if (int e = decode_epoch(b, s, captured, &epoch)) return e;
   if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   hipError_t he;
   if (b->entropy_mode == CJD_ENTROPY_SERIAL)
@@ -441,9 +467,13 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed) {
     he = launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                         b->d_items.as<WorkItem>(), 0, nitems, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
                         b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb, ticket_of(b, 0),
-                        done_of(b, 0), b->ticket_base[0], epoch, b->wait_ticks, s);
-    b->ticket_base[0] += nitems;
-    b->ticket_unknown = he != hipSuccess;
+                        done_of(b, 0), captured ? 0 : b->ticket_base[0], epoch, b->wait_ticks, s);
+    if (captured) {
+      b->graph_counts[0] = nitems;
+    } else {
+      b->ticket_base[0] += nitems;
+      b->ticket_unknown = he != hipSuccess;
+    }
     if (he == hipSuccess && timed && hipEventRecord(b->ev[4], s) != hipSuccess) he = hipErrorUnknown;
     b->coefs_stale = true;
   }
@@ -471,21 +501,25 @@ static int enqueue_pixel(cjd_batch* b, hipStream_t s, bool timed) {
 // the last run's pixel kernel on s; s waits for aux at the end, so the decode
 // is complete when s is.  The entropy launches share the epoch (their
 // images, work items and records are disjoint) and take consecutive tickets.
-static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed) {
-  const int k = b->nsplit;
-  if (!b->aux) {
-    if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess) {
-      b->aux = nullptr;
-      return CJD_DEVICE_MEM_ERROR;
-    }
-    for (auto& e : b->ex)
-      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+static int ensure_aux(cjd_batch* b) {
+  if (b->aux) return 0;
+  if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess) {
+    b->aux = nullptr;
+    return CJD_DEVICE_MEM_ERROR;
   }
+  for (auto& e : b->ex)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return CJD_DEVICE_MEM_ERROR;
+  return 0;
+}
+
+static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed, bool captured = false) {
+  const int k = b->nsplit;
+  if (int e = ensure_aux(b)) return e;
   uint32_t* status = status_of(b);
   LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   uint32_t epoch = 0;
-  if (int e = next_epoch(b, s, &epoch)) return e;
+  if (int e = decode_epoch(b, s, captured, &epoch)) return e;
   if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   b->coefs_stale = true;
   b->last_stream = s;
@@ -495,10 +529,14 @@ static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed) {
         launch_entropy(b->d_words.as<uint32_t>(), b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(),
                        b->d_items.as<WorkItem>(), i0, i1, b->d_tables.as<HuffTable>(), b->d_luts.as<uint16_t>(),
                        b->d_ebuf.as<uint32_t>(), b->d_recs.as<BlockRec>(), status, lb, ticket_of(b, j),
-                       done_of(b, j), b->ticket_base[j], epoch, b->wait_ticks, s);
-    b->ticket_base[j] += i1 - i0;
+                       done_of(b, j), captured ? 0 : b->ticket_base[j], epoch, b->wait_ticks, s);
+    if (captured) {
+      b->graph_counts[j] = i1 - i0;
+    } else {
+      b->ticket_base[j] += i1 - i0;
+    }
     if (he != hipSuccess) {
-      b->ticket_unknown = true;
+      if (!captured) b->ticket_unknown = true;
       return CJD_KERNEL_LAUNCH_ERROR;
     }
     if (j + 1 < k) {
@@ -518,10 +556,27 @@ static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed) {
   return 0;
 }
 
-static int enqueue(cjd_batch* b, hipStream_t s, bool timed) {
-  if (b->nsplit > 1 && b->entropy_mode == CJD_ENTROPY_TILED) return enqueue_split(b, s, timed);
-  int e = enqueue_entropy(b, s, timed);
+static int enqueue(cjd_batch* b, hipStream_t s, bool timed, bool captured = false) {
+  if (b->nsplit > 1 && b->entropy_mode == CJD_ENTROPY_TILED) return enqueue_split(b, s, timed, captured);
+  int e = enqueue_entropy(b, s, timed, captured);
   return e ? e : enqueue_pixel(b, s, timed);
+}
+
+// Everything a captured decode bakes into its launches: buffers, sizes,
+// runs, modes and the wait limit.  A replay is valid while this is unchanged.
+static std::vector<uint64_t> graph_signature(const cjd_batch* b) {
+  std::vector<uint64_t> v = {
+      (uint64_t)b->d_words.p,  (uint64_t)b->d_images.p, (uint64_t)b->d_segs.p,    (uint64_t)b->d_items.p,
+      (uint64_t)b->d_pix.p,    (uint64_t)b->d_tables.p, (uint64_t)b->d_luts.p,    (uint64_t)b->d_qtabs.p,
+      (uint64_t)b->d_coefs.p,  (uint64_t)b->d_out.p,    (uint64_t)b->d_scratch.p, (uint64_t)b->d_btab.p,
+      (uint64_t)b->d_ebuf.p,   (uint64_t)b->d_recs.p,   (uint64_t)b->aux,         b->status_words,
+      b->plan.items.size(),    b->plan.pix.size(),      b->plan.segs.size(),      (uint64_t)b->plan.entry_words,
+      (uint64_t)b->idct_mode,  (uint64_t)b->entropy_mode, b->wait_ticks,          (uint64_t)b->nsplit};
+  for (int j = 0; j <= kMaxParts; ++j) {
+    v.push_back(b->split_item[j]);
+    v.push_back(b->split_pix[j]);
+  }
+  return v;
 }
 
 // Wait for the entropy stage; decode again with the serial decoder every image
@@ -567,17 +622,21 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
   // decoded (its records may be stale): every image is decoded again by the
   // serial decoder, and the counters start again at the next decode.
   bool redo_all = false;
+  const bool replayed = b->replayed;
+  b->replayed = false;
   if (b->entropy_mode == CJD_ENTROPY_TILED) {
     uint64_t head[kScratchHead / 2];
     memcpy(head, b->h_status.data(), sizeof(head));
-    for (int j = 0; j < kLaunchCounters; ++j)
-      if (head[j] != b->ticket_base[j] || head[kLaunchCounters + j] != b->ticket_base[j]) redo_all = true;
-    if (redo_all) {
-      b->ticket_unknown = true;
-      ++b->protocol_redos;
+    for (int j = 0; j < kLaunchCounters; ++j) {
+      const uint64_t want = replayed ? b->graph_counts[j] : b->ticket_base[j];
+      if (head[j] != want || head[kLaunchCounters + j] != want) redo_all = true;
     }
+    if (redo_all) ++b->protocol_redos;
   }
-  const uint32_t epoch = b->epoch;
+  // after a replay the counters no longer follow ticket_base, nor the words
+  // the epochs: the next direct launch (a redo below included) clears them
+  if (redo_all || replayed) b->ticket_unknown = true;
+  const uint32_t epoch = replayed ? kGraphEpoch : b->epoch;
   uint32_t* words = b->h_status.data() + kScratchHead;
   for (size_t i = 0; i < b->status_words; ++i) words[i] = status_bits(words[i], epoch);
   // Every image the tiled decoder could not resolve, or for which it reports
@@ -669,17 +728,55 @@ extern "C" int cjd_batch_decode(cjd_batch* b, void* stream) {
   return enqueue(b, (hipStream_t)stream, true);
 }
 
-// Every entropy launch takes its epoch and ticket base as kernel arguments, so
-// a captured graph would replay the first decode's values: capture is refused.
-extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
-  (void)stream;
-  return b ? CJD_UNSUPPORTED_ERROR : CJD_PROGRAMMER_ERROR;
+static void drop_graph(cjd_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->graph_sig.clear();
 }
 
+// The decode as one HIP graph: k_decode_begin, the entropy launches with the
+// fixed kGraphEpoch and ticket base 0, and the pixel launches (the decode in
+// parts with its second stream joined in by events).  Nothing is run.
+extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
+  if (!b || !stream || b->plan.recs.empty() || !b->d_scratch.p) return CJD_PROGRAMMER_ERROR;
+  hipStream_t s = (hipStream_t)stream;
+  drop_graph(b);
+  if (b->nsplit > 1)
+    if (int e = ensure_aux(b)) return e;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  const int e = enqueue(b, s, false, true);
+  hipGraph_t g = nullptr;
+  const hipError_t he = hipStreamEndCapture(s, &g);
+  if (e || he != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return e ? e : CJD_KERNEL_LAUNCH_ERROR;
+  }
+  if (hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+    b->gexec = nullptr;
+    (void)hipGraphDestroy(g);
+    return CJD_KERNEL_LAUNCH_ERROR;
+  }
+  b->graph = g;
+  b->graph_sig = graph_signature(b);
+  return 0;
+}
+
+// One replay of the captured decode on `stream`; cjd_batch_finish then checks
+// it like a direct decode.  UNSUPPORTED_ERROR when the batch was re-staged or
+// re-configured so that the graph's launches no longer match it.
 extern "C" int cjd_batch_replay(cjd_batch* b, void* stream) {
-  (void)b;
-  (void)stream;
-  return CJD_PROGRAMMER_ERROR;  // nothing is ever captured
+  if (!b || !b->gexec) return CJD_PROGRAMMER_ERROR;
+  if (graph_signature(b) != b->graph_sig) return CJD_UNSUPPORTED_ERROR;
+  hipStream_t s = (hipStream_t)stream;
+  if (hipGraphLaunch(b->gexec, s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
+  b->replayed = true;
+  b->ticket_unknown = true;  // the replay reset the counters: a direct launch after it clears first
+  b->timed = false;
+  b->coefs_stale = true;
+  b->last_stream = s;
+  return 0;
 }
 
 extern "C" int cjd_batch_finish(cjd_batch* b, void* stream, int* errors) {

@@ -46,6 +46,19 @@ __device__ __forceinline__ void st_record_words(LookbackRec* r, const LookbackRe
     if (k >= w0 && k < w1) __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// First kernel of a captured decode: zero the launch counters, the status
+// words and the status word of every lookback record, so that the decode's
+// entropy launches can take ticket base 0 and the fixed kGraphEpoch on every
+// replay (a record that shows no state of this replay is waited for).
+__global__ void k_decode_begin(uint32_t* __restrict__ scratch, uint32_t nstatus, uint32_t nrec) {
+  const uint32_t words = kScratchHead + nstatus;
+  const uint32_t n = words + nrec;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    if (k < words) scratch[k] = 0u;
+    else scratch[words + (uint64_t)(k - words) * kRecWords] = 0u;
+  }
+}
+
 __global__ void k_clear_u32(uint32_t* __restrict__ p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0;
@@ -294,6 +307,14 @@ __global__ void k_entropy_serial(const uint32_t* __restrict__ words, const Image
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsegs) return;
   ent_serial_segment(words, images, segs[s], tables, coefs, recs, status, epoch);
+}
+
+hipError_t launch_decode_begin(uint32_t* scratch, uint32_t nstatus, uint32_t nrec, hipStream_t s) {
+  const uint32_t n = kScratchHead + nstatus + nrec;
+  uint32_t blocks = (n + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_decode_begin, dim3(blocks), dim3(256), 0, s, scratch, nstatus, nrec);
+  return hipGetLastError();
 }
 
 hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s) {

@@ -13,6 +13,8 @@ hipError_t launch_unstuff(const UnstuffChunk* chunks, uint32_t nchunks, const ui
                           uint32_t* words, uint64_t nwords, hipStream_t s);
 
 hipError_t launch_clear(uint32_t* p, uint64_t nwords, hipStream_t s);
+// First kernel of a captured decode (entropy.hip: k_decode_begin).
+hipError_t launch_decode_begin(uint32_t* scratch, uint32_t nstatus, uint32_t nrec, hipStream_t s);
 // Tiled entropy decode of work items [item0, item_end) (whole segments).
 hipError_t launch_entropy(const uint32_t* words, const ImageDesc* images, const SegmentDesc* segs,
                           const WorkItem* items, uint32_t item0, uint32_t item_end, const HuffTable* tables,

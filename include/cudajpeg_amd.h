@@ -201,11 +201,18 @@ int cjd_batch_status(cjd_batch *b, int i, uint32_t *bits);
  * tile links did not resolve, and report per-image reference error codes. */
 int cjd_batch_finish(cjd_batch *b, void *stream, int *errors);
 
-/* Graph capture of a batch's decode: refused (CJD_UNSUPPORTED_ERROR).  Each
- * entropy launch takes its epoch and ticket base as kernel arguments, so a
- * captured graph would replay the first decode's values.  replay returns
- * CJD_PROGRAMMER_ERROR (nothing is captured).  A decode is two launches
- * (cjd_batch_decode), so a graph would save little. */
+/* The batch's decode captured as one HIP graph on `stream` (a created
+ * stream, not the null stream; nothing runs), and replays of it on any
+ * stream.  The graph's first kernel clears the batch's launch counters,
+ * status words and lookback states, so every replay's entropy launches take
+ * ticket base 0 and one fixed epoch.  cjd_batch_finish checks a replay and
+ * redoes images exactly as after cjd_batch_decode (direct launches).  A
+ * replay returns CJD_UNSUPPORTED_ERROR once the batch no longer matches the
+ * graph (re-staged with other sizes or into reallocated buffers, or other
+ * options, parts or wait limit): capture again then.  Re-staging files of the
+ * same shape (a size bucket) keeps the graph valid.  No counterpart in the
+ * reference, which launches its kernels one by one (src/decodeScanGPU.cu:
+ * 156-220). */
 int cjd_batch_capture(cjd_batch *b, void *stream);
 int cjd_batch_replay(cjd_batch *b, void *stream);
 

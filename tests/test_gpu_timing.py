@@ -24,16 +24,15 @@ def test_kernel_timings_split_the_entropy_stage(gpu):
     b.close()
 
 
-def test_graph_capture_is_refused(gpu):
-    # a decode's launches take per-launch arguments (epoch, ticket base), so
-    # capture is refused with the reference's UNSUPPORTED_ERROR; replay has
-    # nothing to run; the batch stays usable for direct launches
+def test_graph_capture_needs_a_stream_and_a_graph(gpu):
+    # capture on the null stream is refused (PROGRAMMER_ERROR), a replay with
+    # nothing captured too; the batch stays usable for direct launches
     from cudajpeg_amd import DecodeError
     b = BatchDecoder(IDCT_INT)
     assert not b.stage([synth.encode(synth.ImageSpec(64, 48, "420", 75, 6.0, 1)).data]).any()
     with pytest.raises(DecodeError) as exc:
         b.capture(None)
-    assert exc.value.code == 2
+    assert exc.value.code == -1
     with pytest.raises(DecodeError):
         b.replay(None)
     b.decode()
