@@ -280,12 +280,13 @@ def measured_traffic(kernel: str, args) -> int | None:
     summary of this same workload (profiles/hbm_traffic_mixed256.md, counters
     scaled by a copy kernel of known size); None for any other workload."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_mixed256.md")
-    if (args.images != 256 or args.seed != 4000 or (args.tile_bits, args.item_tiles) != (8192, 128)
-            or not os.path.exists(path)):
+    if args.images != 256 or args.seed != 4000 or not os.path.exists(path):
         return None
     text = open(path).read()
-    blob = text[text.rfind("```json") + 7:text.rfind("```")]
-    return json.loads(blob)["traffic_bytes"].get(kernel)
+    doc = json.loads(text[text.rfind("```json") + 7:text.rfind("```")])
+    if doc.get("entropy_params") != args.entropy_params:
+        return None  # the summary is of another tiling
+    return doc["traffic_bytes"].get(kernel)
 
 
 def main():
@@ -295,11 +296,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--images", type=int, default=256)
     ap.add_argument("--seed", type=int, default=4000)
-    # the batched form's entropy parameters: 8192-bit tiles in 128-tile work
-    # items measured fastest for this batch (the library default, 4096 x 256,
-    # suits one-image launches)
-    ap.add_argument("--tile-bits", type=int, default=8192)
-    ap.add_argument("--item-tiles", type=int, default=128, help="tiles per entropy work item (0 = library default)")
+    # the batched form's entropy parameters: by default the per-bucket table
+    # the batched autotuner measured (cudajpeg_amd/tune/mi355x_batched.json);
+    # --autotune measures one now; --tile-bits / --item-tiles set one tiling
+    # for every image instead
+    ap.add_argument("--tile-bits", type=int, default=0, help="one tile length for every image (0 = the table)")
+    ap.add_argument("--item-tiles", type=int, default=0, help="tiles per entropy work item for every image "
+                                                               "(0 = the table)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="measure the per-bucket table of the batched form now (autotune_batched) and use it")
     ap.add_argument("--ck-bits", type=int, default=0, help="checkpoint spacing of the tiled decoder (0 = default)")
     ap.add_argument("--batched-idct", default="mfma", choices=["int", "float", "mfma"])
     ap.add_argument("--parts", type=int, default=2,
@@ -355,10 +360,29 @@ def main():
 
     idct = {"int": IDCT_INT, "float": IDCT_FLOAT, "mfma": IDCT_MFMA}[args.batched_idct]
     # ---------------------------------------------------------------- batched
-    b = BatchDecoder(idct, tile_bits=args.tile_bits)
-    b.set_overlap(args.parts)
-    if args.item_tiles or args.ck_bits:
-        b.set_tuning(args.ck_bits, args.item_tiles)
+    from cudajpeg_amd.autotune import BATCHED_TABLE_MI355X, autotune_batched, load_table
+    table = []
+    if args.tile_bits or args.item_tiles:
+        args.entropy_params = f"uniform {args.tile_bits or 4096} x {args.item_tiles or 256}"
+    elif args.autotune:
+        t_at = time.time()
+        table = autotune_batched(files, idct, args.parts, reps=5)
+        args.entropy_params = "autotuned now: " + json.dumps([list(t) for t in table])
+        log(f"batched autotune ({time.time() - t_at:.1f} s): {table}")
+    else:
+        table = load_table(BATCHED_TABLE_MI355X)
+        args.entropy_params = "table " + os.path.relpath(BATCHED_TABLE_MI355X, ROOT)
+
+    def batched_decoder():
+        d = BatchDecoder(idct, tile_bits=args.tile_bits)
+        d.set_overlap(args.parts)
+        if args.item_tiles or args.ck_bits:
+            d.set_tuning(args.ck_bits, args.item_tiles)
+        if table:
+            d.set_tune_table(table)
+        return d
+
+    b = batched_decoder()
     check(b.stage(files), "stage")
     # host side of the boundary: parse + unstuff + plan + upload of the scans
     def time_stage():
@@ -397,10 +421,7 @@ def main():
         t_single = (time.perf_counter() - t_start) / args.steps
         decs, streams = [b], [stream]
         for _ in range(args.in_flight - 1):
-            d = BatchDecoder(idct, tile_bits=args.tile_bits)
-            d.set_overlap(args.parts)
-            if args.item_tiles or args.ck_bits:
-                d.set_tuning(args.ck_bits, args.item_tiles)
+            d = batched_decoder()
             check(d.stage(files), f"stage of decoder {len(decs)}")
             decs.append(d)
             streams.append(torch.cuda.Stream())
@@ -545,7 +566,7 @@ def main():
                    "scan_mb": round(tot["scan_bytes"] / 1e6, 1), "blocks": tot["blocks"],
                    "batched_idct": args.batched_idct, "plain_idct": "float", "parallelism": "1 GPU",
                    "launch": "direct", "parts": args.parts, "in_flight": n_fl,
-                   "tile_bits": args.tile_bits or 4096, "item_tiles": args.item_tiles or 256},
+                   "entropy_params": args.entropy_params},
         "pcie_inclusive": {"stage_ms": round(stage_ms, 2), "stage_ms_host_unstuff": round(stage_ms_host, 2),
                            "download_ms": round(d2h_ms, 2),
                            "mpix_s_with_stage": round(total_px / ((stage_ms + t_batched * 1e3) * 1e-3) / 1e6, 1),

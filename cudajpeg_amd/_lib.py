@@ -58,7 +58,7 @@ class FrameInfo(ctypes.Structure):
 class TuneEntry(ctypes.Structure):
     """cjd_tune_entry"""
     _fields_ = [("max_scan_bytes", ctypes.c_uint64), ("tile_bits", ctypes.c_int), ("ck_bits", ctypes.c_int),
-                ("item_tiles", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("item_tiles", ctypes.c_int), ("max_bits_per_block", ctypes.c_int)]
 
 
 class ServerOptions(ctypes.Structure):

@@ -1,4 +1,4 @@
-"""Autotuner of the tiled entropy decoder's parameters per scan-size bucket.
+"""Autotuners of the tiled entropy decoder's parameters per image bucket.
 
 For each bucket of sample images (by entropy-coded scan bytes) it times the
 candidate (tile bits, tiles per work item) pairs on the GPU and keeps the
@@ -9,7 +9,13 @@ fastest:
                      is time per pixel.
 The result is a table for BatchDecoder.set_tune_table / cjd_batch_set_tune_table:
 rows (max_scan_bytes, tile_bits, ck_bits, item_tiles), the checkpoint spacing
-left at its default (tile_bits / 8).
+left at its default (tile_bits / 16).
+
+autotune_batched tunes the batched form as a whole: buckets by scan size and
+density (scan bits per 8x8 block), and the objective is the decode time of
+the whole batch (all buckets share its launches), minimised by coordinate
+descent over the buckets from the best uniform setting; rows
+(max_scan_bytes, tile_bits, 0, item_tiles, max_bits_per_block).
 """
 from __future__ import annotations
 
@@ -70,11 +76,78 @@ def autotune(files: list[bytes], mode: str = "latency", idct: int = L.IDCT_FLOAT
     return table
 
 
+BATCH_SIZES = (128 << 10, 512 << 10, 2 << 20, (1 << 63) - 1)
+BATCH_DENSITIES = (40, 96, 0)  # bits per block; 0 = any
+BATCH_CANDIDATES = tuple((s, it) for s in (4096, 6144, 8192) for it in (64, 128, 256))
+
+
+def _batch_time(b, files, table, reps):
+    b.set_tune_table(table)
+    if b.stage(files).any():
+        return float("inf")
+    t = []
+    for _ in range(reps + 1):
+        b.decode()
+        b.finish()
+        t.append(b.timings()[2])
+    return float(np.median(t[1:]))
+
+
+def autotune_batched(files: list[bytes], idct: int = L.IDCT_MFMA, parts: int = 2, reps: int = 5,
+                     candidates=BATCH_CANDIDATES, sizes=BATCH_SIZES, densities=BATCH_DENSITIES, min_gain=0.003,
+                     log=None) -> list[tuple]:
+    """Per-bucket (tile bits, tiles per item) of the batched decode of `files`,
+    measured on the whole batch; returns the tune table (buckets without
+    images are left out)."""
+    info = [parse(f)[1] for f in files]
+    keys = []
+    for f in info:
+        bpb = f.scan_length * 8 / max(1, f.num_blocks)
+        size = next(s for s in sizes if f.scan_length <= s)
+        dens = next(d for d in densities if d == 0 or bpb <= d)
+        keys.append((size, dens))
+    buckets = sorted(set(keys), key=lambda k: (k[0], k[1] or 1 << 30))
+    b = BatchDecoder(idct)
+    b.set_overlap(parts)
+
+    def table_of(choice):
+        return [(k[0], choice[k][0], 0, choice[k][1], k[1]) for k in buckets]
+
+    best_t, best = float("inf"), None
+    for c in candidates:  # the best uniform setting first
+        choice = {k: c for k in buckets}
+        t = _batch_time(b, files, table_of(choice), reps)
+        if log:
+            log(f"autotune batched uniform {c}: {t:.3f} ms")
+        if t < best_t:
+            best_t, best = t, choice
+    for k in buckets:  # then one bucket at a time
+        for c in candidates:
+            if c == best[k]:
+                continue
+            choice = dict(best)
+            choice[k] = c
+            t = _batch_time(b, files, table_of(choice), reps)
+            if t < best_t * (1 - min_gain):
+                if log:
+                    log(f"autotune batched bucket {k} -> {c}: {t:.3f} ms (was {best_t:.3f})")
+                best_t, best = t, choice
+    b.close()
+    return table_of(best)
+
+
 def save_table(table, path: str) -> None:
     with open(path, "w") as f:
         json.dump([list(t) for t in table], f)
 
 
 def load_table(path: str) -> list[tuple]:
+    """A table saved by save_table, or a {"table": [...]} document."""
     with open(path) as f:
-        return [tuple(t) for t in json.load(f)]
+        doc = json.load(f)
+    rows = doc["table"] if isinstance(doc, dict) else doc
+    return [tuple(t) for t in rows]
+
+
+# the batched-form table measured on MI355X (see its description)
+BATCHED_TABLE_MI355X = __file__.rsplit("/", 1)[0] + "/tune/mi355x_batched.json"

@@ -231,11 +231,16 @@ extern "C" int cjd_batch_set_tune_table(cjd_batch* b, int n, const cjd_tune_entr
   for (int i = 0; i < n; ++i) {
     const cjd_tune_entry& x = entries[i];
     if (x.tile_bits < 512 || x.tile_bits > 8192 || (x.tile_bits & 31) || x.item_tiles < 2 || x.item_tiles > kWG ||
-        x.ck_bits < 0 || (x.ck_bits && x.ck_bits < 64))
+        x.ck_bits < 0 || (x.ck_bits && x.ck_bits < 64) || x.max_bits_per_block < 0)
       return CJD_PROGRAMMER_ERROR;
-    t.push_back(TuneEntry{x.max_scan_bytes, (uint32_t)x.tile_bits, (uint32_t)x.ck_bits, (uint32_t)x.item_tiles});
+    t.push_back(TuneEntry{x.max_scan_bytes, (uint32_t)x.tile_bits, (uint32_t)x.ck_bits, (uint32_t)x.item_tiles,
+                          (uint32_t)x.max_bits_per_block});
   }
-  std::sort(t.begin(), t.end(), [](const TuneEntry& a, const TuneEntry& c) { return a.max_scan_bytes < c.max_scan_bytes; });
+  // by size, then density (0 = any density last)
+  auto dens = [](const TuneEntry& a) { return a.max_bpb ? a.max_bpb : ~0u; };
+  std::sort(t.begin(), t.end(), [&](const TuneEntry& a, const TuneEntry& c) {
+    return a.max_scan_bytes != c.max_scan_bytes ? a.max_scan_bytes < c.max_scan_bytes : dens(a) < dens(c);
+  });
   b->plan.tune = t;
   return 0;
 }

@@ -226,8 +226,9 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
   w.S = b.tile_bits;
   w.C = b.ck_bits;
   w.items = b.item_tiles;
+  const uint64_t bpb = f.num_blocks ? (uint64_t)f.scan_length * 8 / (uint64_t)f.num_blocks : 0;
   for (const TuneEntry& t : b.tune)
-    if ((uint64_t)f.scan_length <= t.max_scan_bytes) {
+    if ((uint64_t)f.scan_length <= t.max_scan_bytes && (!t.max_bpb || bpb <= t.max_bpb)) {
       w.S = t.tile_bits;
       w.C = t.ck_bits;
       w.items = t.item_tiles;

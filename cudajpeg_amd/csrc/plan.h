@@ -31,10 +31,12 @@ struct ImageRec {
 };
 
 // Entropy-decoder parameters for images whose scan is at most max_scan_bytes
-// (the first matching entry of Plan::tune applies).
+// and at most max_bpb scan bits per block (0 = any); the first matching entry
+// of Plan::tune applies (entries sorted by size, then density).
 struct TuneEntry {
   uint64_t max_scan_bytes;
   uint32_t tile_bits, ck_bits, item_tiles;
+  uint32_t max_bpb = 0;
 };
 
 struct Plan {

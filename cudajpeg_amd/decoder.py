@@ -112,9 +112,11 @@ class BatchDecoder:
         _check(L.lib().cjd_batch_set_overlap(self.h, int(parts)), "overlap: ")
 
     def set_tune_table(self, table) -> None:
-        """Per-image entropy parameters by scan size: a list of
-        (max_scan_bytes, tile_bits, ck_bits, item_tiles)."""
-        arr = (L.TuneEntry * max(1, len(table)))(*[L.TuneEntry(*t[:4], 0) for t in table])
+        """Per-image entropy parameters by scan size and density: a list of
+        (max_scan_bytes, tile_bits, ck_bits, item_tiles[, max_bits_per_block]),
+        max_bits_per_block 0 or absent = any density."""
+        arr = (L.TuneEntry * max(1, len(table)))(*[L.TuneEntry(*t[:4], int(t[4]) if len(t) > 4 else 0)
+                                                    for t in table])
         _check(L.lib().cjd_batch_set_tune_table(self.h, len(table), arr), "tune table: ")
 
     def stage(self, files: list[bytes], stream=None, asynchronous: bool = False) -> np.ndarray:

@@ -120,15 +120,17 @@ int cjd_batch_set_unstuff(cjd_batch *b, int mode);
  * effect at once for the staged batch. */
 int cjd_batch_set_overlap(cjd_batch *b, int parts);
 
-/* Per-image tuning by scan size (no counterpart in the reference): an image
- * whose entropy-coded scan has at most max_scan_bytes bytes uses the first
- * such entry's tile length, checkpoint spacing (0 = default) and tiles per
- * work item; images past every entry use the batch options.  n = 0 clears
- * the table.  Entries come from an autotuner (cudajpeg_amd.autotune).  Takes
- * effect at the next cjd_batch_stage. */
+/* Per-image tuning by scan size and density (no counterpart in the
+ * reference): an image whose entropy-coded scan has at most max_scan_bytes
+ * bytes and at most max_bits_per_block scan bits per 8x8 block (0 = any)
+ * uses the matching entry with the smallest max_scan_bytes, then the
+ * smallest max_bits_per_block: its tile length, checkpoint spacing
+ * (0 = default) and tiles per work item.  Images past every entry use the
+ * batch options.  n = 0 clears the table.  Entries come from an autotuner
+ * (cudajpeg_amd.autotune).  Takes effect at the next cjd_batch_stage. */
 typedef struct {
   uint64_t max_scan_bytes;
-  int tile_bits, ck_bits, item_tiles, reserved;
+  int tile_bits, ck_bits, item_tiles, max_bits_per_block;
 } cjd_tune_entry;
 int cjd_batch_set_tune_table(cjd_batch *b, int n, const cjd_tune_entry *entries);
 

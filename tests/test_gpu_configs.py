@@ -97,13 +97,15 @@ def _check_image(config, form, enc, coefs, pixels, planes):
     return None
 
 
-def _decode_and_check(config, specs, idct, tile_bits=0, item_tiles=0, parts=2):
+def _decode_and_check(config, specs, idct, tile_bits=0, item_tiles=0, parts=2, table=None):
     encs = synth.encode_many(specs, truth=True)
     form = FORMS[idct]
     b = BatchDecoder(idct, tile_bits=tile_bits, planes=idct != IDCT_INT)
     b.set_overlap(parts)
     if item_tiles:
         b.set_tuning(0, item_tiles)
+    if table:
+        b.set_tune_table(table)
     print(f"case {config} {form}: {len(encs)} image(s), {sum(s.pixels for s in specs) / 1e6:.1f} Mpix, "
           f"tile {tile_bits or 4096} x {item_tiles or 256}, parts {parts}", flush=True)
     assert not b.stage([e.data for e in encs]).any()
@@ -143,6 +145,10 @@ def test_c3_shapes(gpu, idct):
 
 @pytest.mark.parametrize("idct", [IDCT_INT, IDCT_MFMA])
 def test_c4_mixed_batch_one_launch(gpu, idct):
-    # the benchmark's batch and parameters (bench.py defaults)
-    _decode_and_check("C4 256 mixed images", synth.config_mixed(256, 4000), idct, tile_bits=8192, item_tiles=128,
-                      parts=2)
+    # the benchmark's batch and parameters (bench.py defaults: the measured
+    # per-bucket table of the batched form, 2 parts), and one uniform tiling
+    from cudajpeg_amd.autotune import BATCHED_TABLE_MI355X, load_table
+    _decode_and_check("C4 256 mixed images", synth.config_mixed(256, 4000), idct, parts=2,
+                      table=load_table(BATCHED_TABLE_MI355X))
+    _decode_and_check("C4 256 mixed images, 8192 x 128", synth.config_mixed(256, 4000), idct, tile_bits=8192,
+                      item_tiles=128, parts=2)
