@@ -471,6 +471,34 @@ def main():
             check(b.finish(stream), "graph replays")
             log(f"graph replays: {t_graph * 1e3:.2f} ms/step ({total_px / t_graph / 1e9:.1f} Gpix/s)")
     batched_mpix = total_px / t_batched / 1e6
+    # staging pipelined with decoding: two decoders of the batch in turn, each
+    # on its own stream; a decoder's next stage (host planning + DMA + device
+    # unstuffing, ordered on its stream) runs while the other decodes: files
+    # in host memory to frames in HBM, PCIe included
+    t_pipe = None
+    if args.steps >= 2:
+        pdecs = [batched_decoder() for _ in range(2)]
+        pstreams = [torch.cuda.Stream() for _ in range(2)]
+        for d, st in zip(pdecs, pstreams):  # buffers grown before the timed loop
+            check(d.stage(files, st, asynchronous=True), "pipelined stage")
+            d.decode(st)
+            check(d.finish(st), "pipelined warm-up")
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            d, st = pdecs[k % 2], pstreams[k % 2]
+            if k >= 2:
+                check(d.finish(st), "pipelined decode")
+            check(d.stage(files, st, asynchronous=True), "pipelined stage")
+            d.decode(st)
+        for k in range(max(0, args.steps - 2), args.steps):
+            check(pdecs[k % 2].finish(pstreams[k % 2]), "pipelined decode")
+        torch.cuda.synchronize()
+        t_pipe = (time.perf_counter() - t_start) / args.steps
+        for d in pdecs:
+            d.close()
+        log(f"staging pipelined with decoding: {t_pipe * 1e3:.2f} ms per batch "
+            f"({total_px / t_pipe / 1e9:.1f} Gpix/s, files in host memory to frames in HBM)")
     # frames back to host memory (the reader API's pixels buffer)
     t1 = time.perf_counter()
     for i in range(len(files)):
@@ -568,6 +596,8 @@ def main():
                    "launch": "direct", "parts": args.parts, "in_flight": n_fl,
                    "entropy_params": args.entropy_params},
         "pcie_inclusive": {"stage_ms": round(stage_ms, 2), "stage_ms_host_unstuff": round(stage_ms_host, 2),
+                           "ms_per_batch_staging_pipelined": None if t_pipe is None else round(t_pipe * 1e3, 3),
+                           "mpix_s_staging_pipelined": None if t_pipe is None else round(total_px / t_pipe / 1e6, 1),
                            "download_ms": round(d2h_ms, 2),
                            "mpix_s_with_stage": round(total_px / ((stage_ms + t_batched * 1e3) * 1e-3) / 1e6, 1),
                            "mpix_s_with_stage_and_download":

@@ -16,12 +16,13 @@ DEVICE_MEM_ERROR, FILE_ERROR, KERNEL_LAUNCH_ERROR, PROGRAMMER_ERROR = 4, 5, 6, -
 IDCT_INT, IDCT_FLOAT, IDCT_MFMA = 0, 1, 2
 ENTROPY_TILED, ENTROPY_SERIAL = 0, 1
 UNSTUFF_HOST, UNSTUFF_DEVICE = 0, 1
+EXT_DQT16 = 1
 OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 
 # Every symbol include/cudajpeg_amd.h and include/format.h declare.
 EXPORTS = [
-    "cjd_parse", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
-    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_overlap", "cjd_batch_set_unstuff", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
+    "cjd_parse", "cjd_parse_ext", "cjd_context_create", "cjd_context_destroy", "cjd_batch_create", "cjd_batch_destroy",
+    "cjd_batch_set_options", "cjd_batch_set_tuning", "cjd_batch_set_overlap", "cjd_batch_set_unstuff", "cjd_batch_set_extensions", "cjd_batch_set_tune_table", "cjd_batch_stage", "cjd_batch_stage_async", "cjd_batch_decode", "cjd_batch_decode_scan",
     "cjd_batch_transform", "cjd_batch_fallbacks", "cjd_batch_set_wait_limit", "cjd_batch_protocol_redos",
     "cjd_batch_status",
     "cjd_batch_finish", "cjd_batch_capture",
@@ -52,6 +53,7 @@ class FrameInfo(ctypes.Structure):
         ("huff_vals", (ctypes.c_uint8 * 256) * 4),
         ("scan_offset", ctypes.c_uint64), ("scan_length", ctypes.c_uint64),
         ("error_line", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("dq_tables16", (ctypes.c_uint16 * 64) * 4),
     ]
 
 
@@ -96,6 +98,7 @@ def lib() -> ctypes.CDLL:
     FP = ctypes.POINTER(FrameInfo)
     sig = {
         "cjd_parse": ([ctypes.c_char_p, sz, FP], i32),
+        "cjd_parse_ext": ([ctypes.c_char_p, sz, i32, FP], i32),
         "cjd_context_create": ([i32, ctypes.POINTER(vp)], i32),
         "cjd_context_destroy": ([vp], None),
         "cjd_batch_create": ([vp, ctypes.POINTER(vp)], i32),
@@ -104,6 +107,7 @@ def lib() -> ctypes.CDLL:
         "cjd_batch_set_tuning": ([vp, i32, i32], i32),
         "cjd_batch_set_overlap": ([vp, i32], i32),
         "cjd_batch_set_unstuff": ([vp, i32], i32),
+        "cjd_batch_set_extensions": ([vp, i32], i32),
         "cjd_batch_set_tune_table": ([vp, i32, ctypes.POINTER(TuneEntry)], i32),
         "cjd_batch_stage": ([vp, i32, vp, vp, vp], i32),
         "cjd_batch_stage_async": ([vp, i32, vp, vp, vp, vp], i32),

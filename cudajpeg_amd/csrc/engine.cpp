@@ -314,6 +314,12 @@ static void plan_split(cjd_batch* b) {
   b->nsplit = n + 1;
 }
 
+extern "C" int cjd_batch_set_extensions(cjd_batch* b, int ext) {
+  if (!b || (ext & ~CJD_EXT_DQT16)) return CJD_PROGRAMMER_ERROR;
+  b->plan.ext = ext;
+  return 0;
+}
+
 extern "C" int cjd_batch_set_unstuff(cjd_batch* b, int mode) {
   if (!b || (mode != CJD_UNSTUFF_HOST && mode != CJD_UNSTUFF_DEVICE)) return CJD_PROGRAMMER_ERROR;
   b->unstuff = mode;

@@ -44,6 +44,7 @@ void set_error(JPGReader* jpg, int code, int line, const char* file, const char*
 template <class F>
 void run_parser(JPGReader* jpg, F fn, const char* func) {
   cjd::ParseState st;
+  st.ext = jpg->extensions;
   st.buf = jpg->buf;
   st.pos = jpg->pos;
   st.end = jpg->end;
@@ -187,6 +188,7 @@ static void decode_scan(JPGReader* jpg, int entropy_mode, bool coefficients) {
     jpg->channels[i].ac_id = e->f.ac_id[i];
   }
   int err = cjd_batch_set_options(e->batch, jpg->idct_mode, entropy_mode, 0, jpg->keep_planes);
+  if (!err) err = cjd_batch_set_extensions(e->batch, jpg->extensions);
   if (err) RTHROW(err);
   const uint8_t* files[1] = {jpg->buf};
   const size_t sizes[1] = {(size_t)(jpg->end - jpg->buf) + 2};
@@ -219,7 +221,7 @@ static void decode_scan(JPGReader* jpg, int entropy_mode, bool coefficients) {
     size_t blk = 0;
     for (int m = 0; m < info.mcus_x * info.mcus_y; ++m)
       for (int c = 0; c < jpg->num_channels; ++c) {
-        const unsigned char* dq = jpg->dq_tables[jpg->channels[c].dq_id];
+        const uint16_t* dq = e->f.dq_tables16[jpg->channels[c].dq_id];
         for (int k = 0; k < jpg->channels[c].samples_x * jpg->channels[c].samples_y; ++k, ++blk) {
           const int16_t* src = &q[blk * 64];
           for (int z = 0; z < 64; ++z) wpos[c][zz[z]] = (int)src[zz[z]] * dq[z];

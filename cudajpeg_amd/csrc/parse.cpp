@@ -123,16 +123,26 @@ void parse_dri(ParseState& st) {
 }
 
 // src/format.cu:424-442
+// src/format.cu:424-442; with CJD_EXT_DQT16 a table may also have 16-bit
+// entries (Pq = 1, T.81 B.2.4.1: 128 bytes, big-endian).
 void parse_dqt(ParseState& st) {
   const unsigned block_len = read16(st.pos);
   const uint8_t* block_end = st.pos + block_len;
   if (block_end > st.end) PFAIL(CJD_SYNTAX_ERROR);
   const uint8_t* pos = st.pos + 2;
+  const bool wide_ok = (st.ext & CJD_EXT_DQT16) != 0;
   while (pos + 65 <= block_end) {
     const uint8_t id = pos[0];
-    if (id & 0xFC) PFAIL(CJD_SYNTAX_ERROR);
-    memcpy(st.f->dq_tables[id], pos + 1, 64);
-    pos += 65;
+    const bool wide = wide_ok && (id & 0xF0) == 0x10;
+    if ((id & 0xFC) && !wide) PFAIL(CJD_SYNTAX_ERROR);
+    if ((id & 0x0C) || (wide && pos + 129 > block_end)) PFAIL(CJD_SYNTAX_ERROR);
+    const int t = id & 3;
+    for (int k = 0; k < 64; ++k) {
+      const uint16_t v = wide ? (uint16_t)((pos[1 + 2 * k] << 8) | pos[2 + 2 * k]) : pos[1 + k];
+      st.f->dq_tables16[t][k] = v;
+      st.f->dq_tables[t][k] = (uint8_t)(v > 255 ? 255 : v);
+    }
+    pos += wide ? 129 : 65;
   }
   if (pos != block_end) PFAIL(CJD_SYNTAX_ERROR);
   st.pos = block_end;
@@ -163,9 +173,10 @@ void parse_sos(ParseState& st) {
 
 // src/format.cu:113-215 without the file read; stops after the SOS header
 // (or at the EOI) and leaves st.pos there.
-int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame_info* f) {
+int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame_info* f, int ext) {
   memset(f, 0, sizeof(*f));
   st.f = f;
+  st.ext = ext;
   st.buf = data;
   st.error = 0;
   st.error_line = 0;
@@ -217,13 +228,17 @@ int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame
 
 }  // namespace cjd
 
-extern "C" int cjd_parse(const uint8_t* data, size_t size, cjd_frame_info* out) {
+extern "C" int cjd_parse_ext(const uint8_t* data, size_t size, int ext, cjd_frame_info* out) {
   cjd::ParseState st;
-  int e = cjd::parse_until_scan(st, data, size, out);
+  int e = cjd::parse_until_scan(st, data, size, out, ext);
   if (!e && !st.at_scan) e = CJD_SYNTAX_ERROR;  // EOI before any scan
   if (!e && !st.have_sof) e = CJD_SYNTAX_ERROR;
   out->error_line = st.error_line;
   return e;
+}
+
+extern "C" int cjd_parse(const uint8_t* data, size_t size, cjd_frame_info* out) {
+  return cjd_parse_ext(data, size, 0, out);
 }
 
 extern "C" const char* cjd_error_name(int code) {

@@ -17,6 +17,7 @@ struct ParseState {
   int error_line = 0;
   bool have_sof = false;
   bool at_scan = false;
+  int ext = 0;  // CJD_EXT_* accepted
 };
 
 void parse_sof(ParseState& st);
@@ -24,6 +25,6 @@ void parse_dht(ParseState& st);
 void parse_dri(ParseState& st);
 void parse_dqt(ParseState& st);
 void parse_sos(ParseState& st);
-int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame_info* f);
+int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame_info* f, int ext = 0);
 
 }  // namespace cjd

@@ -205,7 +205,7 @@ inline void for_items(uint32_t nt, uint32_t per_first, uint32_t per_next, F&& f)
 // Pass 1 (per image, in parallel): parse, find the segments, count.
 int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, size_t size) {
   ParseState st;
-  int e = parse_until_scan(st, file, size, &r.f);
+  int e = parse_until_scan(st, file, size, &r.f, b.ext);
   if (!e && (!st.at_scan || !st.have_sof)) e = CJD_SYNTAX_ERROR;
   if (e) return e;
   const cjd_frame_info& f = r.f;
@@ -299,7 +299,7 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
   for (int t = 0; t < 4; ++t) {
     build_table(f.huff_bits[t], f.huff_vals[t], b->tables[(size_t)(w.desc * 4 + t)]);
     uint16_t* q = &b->qtabs[(size_t)(w.desc * 4 + t) * 64];
-    for (int k = 0; k < 64; ++k) q[kZZ[k]] = f.dq_tables[t][k];
+    for (int k = 0; k < 64; ++k) q[kZZ[k]] = f.dq_tables16[t][k];
   }
   d.coef_base = w.coef0;
   d.nblocks = f.num_blocks;

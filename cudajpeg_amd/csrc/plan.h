@@ -45,6 +45,7 @@ struct Plan {
   uint32_t item_tiles = kWG;      // tiles per entropy work item (2 .. kWG)
   std::vector<TuneEntry> tune;    // per scan-size bucket overrides of the three above
   int want_planes = 0;
+  int ext = 0;                    // CJD_EXT_* the parser accepts
   std::vector<ImageRec> recs;
   std::vector<ImageDesc> images;
   std::vector<SegmentDesc> segs;

@@ -203,15 +203,21 @@ static const uint8_t k_ac_chr_vals[162] = {
     0xd2, 0xd3, 0xd4, 0xd5, 0xd6, 0xd7, 0xd8, 0xd9, 0xda, 0xe2, 0xe3, 0xe4, 0xe5, 0xe6, 0xe7,
     0xe8, 0xe9, 0xea, 0xf2, 0xf3, 0xf4, 0xf5, 0xf6, 0xf7, 0xf8, 0xf9, 0xfa};
 
-void synth_quant_table(int quality, int chroma, uint8_t out_natural[64]) {
+void synth_quant_table16(int quality, int chroma, int maxv, uint16_t out_natural[64]) {
   if (quality < 1) quality = 1;
   if (quality > 100) quality = 100;
   int scale = quality < 50 ? 5000 / quality : 200 - 2 * quality;
   const uint8_t *base = chroma ? k_chr_q : k_lum_q;
   for (int i = 0; i < 64; i++) {
     long v = ((long)base[i] * scale + 50) / 100;
-    out_natural[i] = (uint8_t)(v < 1 ? 1 : v > 255 ? 255 : v);
+    out_natural[i] = (uint16_t)(v < 1 ? 1 : v > maxv ? maxv : v);
   }
+}
+
+void synth_quant_table(int quality, int chroma, uint8_t out_natural[64]) {
+  uint16_t t[64];
+  synth_quant_table16(quality, chroma, 255, t);
+  for (int i = 0; i < 64; i++) out_natural[i] = (uint8_t)t[i];
 }
 
 /* Encoder-side code tables (T.81 C.2 canonical construction). */
@@ -288,7 +294,7 @@ static void cos_init(void) {
       g_cos[u][x] = (u == 0 ? sqrt(0.125) : 0.5) * cos((2 * x + 1) * u * M_PI / 16.0);
   g_cos_ready = 1;
 }
-static void fdct_quant(const float blk[64], const uint8_t qnat[64], int16_t out[64]) {
+static void fdct_quant(const float blk[64], const uint16_t qnat[64], int16_t out[64]) {
   double tmp[64];
   for (int y = 0; y < 8; y++)
     for (int u = 0; u < 8; u++) {
@@ -311,7 +317,7 @@ static void fdct_quant(const float blk[64], const uint8_t qnat[64], int16_t out[
 /* quant table of component c: Y 0, Cb 1, Cr 1 (or 2 with three tables) */
 static int comp_qtab(const synth_params *p, int c) { return c == 0 ? 0 : (c == 2 && p->qtables == 3 ? 2 : 1); }
 
-static void put_header(bw_t *w, const synth_params *p, uint8_t q[3][64]) {
+static void put_header(bw_t *w, const synth_params *p, uint16_t q[3][64]) {
   const int C = p->ncomp == 1 ? 1 : 3;
   raw(w, 0xFF); raw(w, 0xD8);
   /* APP0 JFIF */
@@ -320,8 +326,13 @@ static void put_header(bw_t *w, const synth_params *p, uint8_t q[3][64]) {
   raw(w, 1); raw(w, 1); raw(w, 0); raw16(w, 1); raw16(w, 1); raw(w, 0); raw(w, 0);
   /* DQT, zigzag order */
   for (int t = 0; t < (C == 1 ? 1 : (p->qtables == 3 ? 3 : 2)); t++) {
-    raw(w, 0xFF); raw(w, 0xDB); raw16(w, 67); raw(w, (uint8_t)t);
-    for (int k = 0; k < 64; k++) raw(w, q[t][k_zz[k]]);
+    if (p->dqt16) {  /* Pq = 1: 16-bit entries, big-endian */
+      raw(w, 0xFF); raw(w, 0xDB); raw16(w, 131); raw(w, (uint8_t)(0x10 | t));
+      for (int k = 0; k < 64; k++) raw16(w, q[t][k_zz[k]]);
+    } else {
+      raw(w, 0xFF); raw(w, 0xDB); raw16(w, 67); raw(w, (uint8_t)t);
+      for (int k = 0; k < 64; k++) raw(w, (uint8_t)q[t][k_zz[k]]);
+    }
   }
   /* SOF0 */
   raw(w, 0xFF); raw(w, 0xC0); raw16(w, 8 + 3 * C); raw(w, 8);
@@ -371,10 +382,11 @@ long synth_encode(const synth_params *p, const uint8_t *pix, uint8_t *out, long 
   cos_init();
   const int hmax = hs[0], vmax = vs[0];
   const int nbx = (W + 8 * hmax - 1) / (8 * hmax), nby = (H + 8 * vmax - 1) / (8 * vmax);
-  uint8_t q[3][64];
-  synth_quant_table(p->quality, 0, q[0]);
-  synth_quant_table(p->quality, 1, q[1]);
-  synth_quant_table(p->quality > 20 ? p->quality - 15 : p->quality + 15, 1, q[2]);
+  uint16_t q[3][64];
+  const int qmax = p->dqt16 ? 65535 : 255;
+  synth_quant_table16(p->quality, 0, qmax, q[0]);
+  synth_quant_table16(p->quality, 1, qmax, q[1]);
+  synth_quant_table16(p->quality > 20 ? p->quality - 15 : p->quality + 15, 1, qmax, q[2]);
 
   /* component planes, padded to whole MCUs by edge replication */
   uint8_t *plane[3] = {0, 0, 0};

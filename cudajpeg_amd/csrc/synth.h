@@ -18,6 +18,7 @@ typedef struct {
   double random_tile_frac;  /* fraction of 64x64 tiles replaced by uniform noise */
   int32_t qtables;          /* quant tables for YCbCr: 2 (Y | CbCr) or 3 (Y | Cb | Cr) */
   int32_t huff;             /* AC Huffman tables: 0 Annex K, 1 long codes (60 of 11 bits, 92 of 16) */
+  int32_t dqt16;            /* 1: quant tables written with 16-bit entries (Pq = 1), values up to 65535 */
 } synth_params;
 
 /* W*H*ncomp interleaved pixels; returns 0 on success. */
@@ -31,6 +32,8 @@ long synth_encode(const synth_params *p, const uint8_t *pixels, uint8_t *out, lo
 long synth_make(const synth_params *p, uint8_t *out, long cap, int16_t *coefs, uint32_t *block_bits);
 long synth_num_blocks(const synth_params *p);
 void synth_quant_table(int quality, int chroma, uint8_t out_natural[64]);
+/* The same scaled table clamped to [1, maxv] (255 for 8-bit tables). */
+void synth_quant_table16(int quality, int chroma, int maxv, uint16_t out_natural[64]);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,8 @@ __global__ void __launch_bounds__(kUnstuffLanes) k_unstuff(const UnstuffChunk* _
   const UnstuffChunk c = chunks[blockIdx.x];
   unstuff_load(sh, lane, c, raw);
   __syncthreads();
+  if (lane == 0) unstuff_load_next(sh, c, raw);
+  __syncthreads();
   unstuff_carry(sh, lane, c);
   __syncthreads();
   for (int d = 1; d < kUnstuffLanes; d <<= 1) {
@@ -42,7 +44,16 @@ __global__ void __launch_bounds__(kUnstuffLanes) k_unstuff(const UnstuffChunk* _
     sh.tail[lane] = tail;
     __syncthreads();
   }
-  unstuff_write(sh, lane, c, consumed, out, [&](uint32_t s) { return segs[s].word_base * 4u; });
+  if (unstuff_single_run(sh)) {
+    // gathered in LDS, written as aligned words
+    const uint64_t dst0 = segs[c.seg].word_base * 4u + c.dst;
+    const uint32_t lead = (uint32_t)(dst0 & 15u);
+    unstuff_gather(sh, lane, c, consumed, lead);
+    __syncthreads();
+    unstuff_flush(sh, lane, sh.tail[kUnstuffLanes - 1], lead, dst0, out);
+  } else {
+    unstuff_write(sh, lane, c, consumed, out, [&](uint32_t s) { return segs[s].word_base * 4u; });
+  }
 }
 
 hipError_t launch_unstuff(const UnstuffChunk* chunks, uint32_t nchunks, const uint8_t* raw, const SegmentDesc* segs,

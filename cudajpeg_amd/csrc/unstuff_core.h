@@ -51,17 +51,41 @@ struct UnstuffShared {
   uint8_t fn[kUnstuffLanes];         // carry function of each lane, then the composed prefix
   uint32_t brk[kUnstuffLanes];       // RSTn splits in the lane (then inclusive prefix)
   uint32_t tail[kUnstuffLanes];      // output bytes after the lane's last split, or all when none (then prefix)
-  uint8_t bytes[kUnstuffChunk + 16];  // the chunk, then the byte after it (0xFF at the scan's end)
+  alignas(16) uint8_t bytes[kUnstuffChunk + 16];  // the chunk, then the byte after it (0xFF at the scan's end)
+  // a chunk with no RSTn split: its output bytes gathered here (the chunk's
+  // output starts at byte `lead` of the buffer, the alignment of its first
+  // destination byte), then written out as aligned words
+  alignas(16) uint8_t outb[kUnstuffChunk + 16];
 };
 
 // Phase 0 (lane): the chunk's bytes into LDS, 16 per lane; lane 0 adds the
-// byte that follows the chunk.
+// byte that follows the chunk.  The raw buffer holds each scan at a 16-byte
+// aligned offset and chunks start 4096 bytes apart within it, so on the
+// device a lane's 16 bytes are one aligned 16-byte load (the last lane's may
+// run past the chunk into the scan's padding; those bytes are zeroed).
 CJD_UHD inline void unstuff_load(UnstuffShared& sh, int lane, const UnstuffChunk& c, const uint8_t* raw) {
+  const uint32_t k0 = (uint32_t)(lane * kUnstuffLaneBytes);
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint4 v = make_uint4(0u, 0u, 0u, 0u);
+  if (k0 < c.len) v = *reinterpret_cast<const uint4*>(raw + c.src + k0);
+  if (k0 + kUnstuffLaneBytes > c.len) {  // zero the bytes past the chunk
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (int i = 0; i < 16; ++i)
+      if (k0 + (uint32_t)i >= c.len) w[i >> 2] &= ~(0xFFu << (8 * (i & 3)));
+    v = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  *reinterpret_cast<uint4*>(&sh.bytes[k0]) = v;
+#else
   for (int i = 0; i < kUnstuffLaneBytes; ++i) {
-    const uint32_t k = (uint32_t)(lane * kUnstuffLaneBytes + i);
+    const uint32_t k = k0 + (uint32_t)i;
     sh.bytes[k] = k < c.len ? raw[c.src + k] : 0u;
   }
-  if (lane == 0) sh.bytes[c.len] = (c.flags & kChunkScanEnd) ? 0xFFu : raw[c.src + c.len];
+#endif
+}
+
+// After phase 0's barrier (lane 0): the byte that follows the chunk.
+CJD_UHD inline void unstuff_load_next(UnstuffShared& sh, const UnstuffChunk& c, const uint8_t* raw) {
+  sh.bytes[c.len] = (c.flags & kChunkScanEnd) ? 0xFFu : raw[c.src + c.len];
 }
 
 // Whether byte k of the chunk is a 0xFF that begins a pair, given whether it
@@ -193,6 +217,43 @@ CJD_UHD inline void unstuff_write(const UnstuffShared& sh, int lane, const Unstu
         pos = 0;
         have_base = false;
       });
+}
+
+// Whether the chunk splits at no RSTn (after the count scan): its output is
+// then one run of sh.tail[last] bytes from (c.seg, c.dst).
+CJD_UHD inline bool unstuff_single_run(const UnstuffShared& sh) { return sh.brk[kUnstuffLanes - 1] == 0; }
+
+// Phase 3 of a chunk without splits (lane): the lane's output bytes into
+// sh.outb at lead + their place in the run (lead = the run's destination
+// byte address mod 16).
+CJD_UHD inline void unstuff_gather(UnstuffShared& sh, int lane, const UnstuffChunk& c, bool consumed, uint32_t lead) {
+  uint32_t pos = lead + (lane > 0 ? sh.tail[lane - 1] : 0u);
+  unstuff_walk_lane(
+      sh, lane, c, consumed, [&](uint8_t v) { sh.outb[pos++] = v; }, [&]() {});
+}
+
+// Phase 4 (lane, after a barrier): the run written from sh.outb, aligned
+// 16-byte words where the whole word is the run's, bytes at the two ends.
+// dst = the run's first destination byte (16-byte aligned dst - lead).
+template <class Out>
+CJD_UHD inline void unstuff_flush(const UnstuffShared& sh, int lane, uint32_t n, uint32_t lead, uint64_t dst0,
+                                  Out&& out) {
+  // bytes [lead, lead + n) of outb go to out[dst0 - lead + j] for j in that range
+  const uint64_t base = dst0 - lead;  // 16-byte aligned
+  const uint32_t end = lead + n;
+  for (uint32_t q = (uint32_t)lane; q * 16u < end; q += kUnstuffLanes) {
+    const uint32_t j0 = q * 16u;
+    if (j0 >= lead && j0 + 16u <= end) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      *reinterpret_cast<uint4*>(&out[base + j0]) = *reinterpret_cast<const uint4*>(&sh.outb[j0]);
+#else
+      for (uint32_t j = j0; j < j0 + 16u; ++j) out[base + j] = sh.outb[j];
+#endif
+    } else {
+      for (uint32_t j = j0; j < j0 + 16u; ++j)
+        if (j >= lead && j < end) out[base + j] = sh.outb[j];
+    }
+  }
 }
 
 }  // namespace cjd

@@ -33,10 +33,11 @@ def _check(code: int, what: str = "") -> None:
         raise DecodeError(code, what)
 
 
-def parse(data: bytes) -> tuple[int, L.FrameInfo]:
-    """Host marker/table parse of one file (cjd_parse)."""
+def parse(data: bytes, ext: int = 0) -> tuple[int, L.FrameInfo]:
+    """Host marker/table parse of one file (cjd_parse, or cjd_parse_ext with
+    the extensions `ext`)."""
     info = L.FrameInfo()
-    e = L.lib().cjd_parse(data, len(data), ctypes.byref(info))
+    e = L.lib().cjd_parse_ext(data, len(data), ext, ctypes.byref(info))
     return e, info
 
 
@@ -100,6 +101,10 @@ class BatchDecoder:
         """Tiled entropy decoder: checkpoint spacing in bits and tiles per
         work item; 0 = defaults; applies from the next stage()."""
         _check(L.lib().cjd_batch_set_tuning(self.h, ck_bits, item_tiles), "tuning: ")
+
+    def set_extensions(self, ext: int) -> None:
+        """Extensions (EXT_*) accepted at the next stage; 0 = the reference's parsing."""
+        _check(L.lib().cjd_batch_set_extensions(self.h, ext), "extensions: ")
 
     def set_unstuff(self, mode: int) -> None:
         """UNSTUFF_DEVICE (default): scans unstuffed by a kernel at staging;
@@ -273,7 +278,7 @@ class _JPGReader(ctypes.Structure):
                 ("dq_tables", (ctypes.c_uint8 * 64) * 4),
                 ("file_buf", _ManagedUCharMem),
                 ("idct_mode", ctypes.c_int), ("entropy_mode", ctypes.c_int),
-                ("keep_coefficients", ctypes.c_int), ("keep_planes", ctypes.c_int),
+                ("keep_coefficients", ctypes.c_int), ("keep_planes", ctypes.c_int), ("extensions", ctypes.c_int),
                 ("engine", ctypes.c_void_p), ("time", ctypes.c_long),
                 ("error", ctypes.c_int), ("error_line", ctypes.c_int),
                 ("error_file", ctypes.c_char_p), ("error_func", ctypes.c_char_p)]
@@ -285,7 +290,7 @@ class JPGReader:
     gray frame is in `pixels`."""
 
     def __init__(self, idct: int = L.IDCT_INT, entropy: int = L.ENTROPY_TILED, keep_coefficients: bool = False,
-                 keep_planes: bool = False):
+                 keep_planes: bool = False, extensions: int = 0):
         p = L.lib().newJPGReader()
         if not p:
             raise DecodeError(L.OOM_ERROR, "newJPGReader: ")
@@ -295,6 +300,7 @@ class JPGReader:
         r.entropy_mode = entropy
         r.keep_coefficients = int(keep_coefficients)
         r.keep_planes = int(keep_planes)
+        r.extensions = int(extensions)
 
     @property
     def raw(self) -> _JPGReader:

@@ -34,6 +34,7 @@ class _Params(ctypes.Structure):
         ("random_tile_frac", ctypes.c_double),
         ("qtables", ctypes.c_int32),
         ("huff", ctypes.c_int32),
+        ("dqt16", ctypes.c_int32),
     ]
 
 
@@ -77,12 +78,13 @@ class ImageSpec:
     restart_interval: int = 0
     qtables: int = 2               # YCbCr: 2 (Y | CbCr) or 3 (Y | Cb | Cr) quant tables
     huff: int = 0                  # AC Huffman tables: 0 Annex K, 1 long codes (60 of 11 bits)
+    dqt16: int = 0                 # 1: 16-bit quant tables (Pq = 1), entries up to 65535
 
     def _params(self) -> _Params:
         ncomp, h, v = SAMPLING[self.sampling]
         return _Params(self.seed, self.width, self.height, ncomp, h, v, self.quality,
                        self.restart_interval, self.sigma, self.amp_total, self.random_tile_frac,
-                       self.qtables, self.huff)
+                       self.qtables, self.huff, self.dqt16)
 
     @property
     def ncomp(self) -> int:

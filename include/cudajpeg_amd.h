@@ -63,7 +63,16 @@ typedef struct {
   uint64_t scan_length;
   int32_t error_line;                   /* source line of the failed check, 0 if none */
   int32_t pad_;
+  uint16_t dq_tables16[4][64];          /* the same tables with 16-bit entries (CJD_EXT_DQT16), zigzag */
 } cjd_frame_info;
+
+/* Extensions beyond the reference decoder (off by default: the reference's
+ * behaviour and error codes):
+ *   CJD_EXT_DQT16  16-bit quantisation tables (DQT Pq = 1, which
+ *                  src/format.cu:432 rejects with SYNTAX_ERROR); dq_tables
+ *                  then holds the entries clamped to 255, dq_tables16 the
+ *                  values. */
+#define CJD_EXT_DQT16 1
 
 typedef struct cjd_context cjd_context;
 typedef struct cjd_batch cjd_batch;
@@ -73,6 +82,8 @@ typedef struct cjd_batch cjd_batch;
  * :242-354, decodeDHT :357-400, decodeDRI :404-421, decodeDQT :424-442 and the
  * SOS header check of src/decodeScanGPU.cu:124-141).  Stops at the scan. */
 int cjd_parse(const uint8_t *data, size_t size, cjd_frame_info *out);
+/* cjd_parse with the extensions `ext` (CJD_EXT_*) accepted. */
+int cjd_parse_ext(const uint8_t *data, size_t size, int ext, cjd_frame_info *out);
 
 /* Device context (replaces the device half of newJPGReader / delJPGReader,
  * src/format.cu:12-59). */
@@ -103,6 +114,10 @@ int cjd_batch_set_tuning(cjd_batch *b, int ck_bits, int item_tiles);
  * and unstuffs it with a kernel on the staging stream; CJD_UNSTUFF_HOST
  * unstuffs in the host planner and uploads the words.  Results are identical.
  * Applies from the next stage.  No counterpart in the reference. */
+/* Extensions (CJD_EXT_*) the batch accepts in the files it stages; 0 (the
+ * default) parses as the reference does.  From the next stage. */
+int cjd_batch_set_extensions(cjd_batch *b, int ext);
+
 #define CJD_UNSTUFF_HOST 0
 #define CJD_UNSTUFF_DEVICE 1
 int cjd_batch_set_unstuff(cjd_batch *b, int mode);

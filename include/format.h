@@ -76,6 +76,7 @@ typedef struct _JPGReader {
   int entropy_mode;                /* CJD_ENTROPY_TILED / _SERIAL */
   int keep_coefficients;
   int keep_planes;
+  int extensions;                  /* CJD_EXT_* accepted beyond the reference (0 = none) */
   void *engine;                    /* device context, batch and parsed frame */
   /* clock() ticks of the last openJPG's scan decode (include/format.h:100;
    * set to 0 by openJPG, src/format.cu:174, and accumulated by the scan

@@ -54,6 +54,7 @@ class _Jpeg(ctypes.Structure):
         ("dht_vals", (ctypes.c_uint8 * 256) * 4),
         ("dht_present", ctypes.c_int * 4),
         ("scan", ctypes.c_void_p), ("scan_len", ctypes.c_size_t), ("have_scan", ctypes.c_int),
+        ("ext", ctypes.c_int), ("dq16", (ctypes.c_uint16 * 64) * 4),
     ]
 
 
@@ -76,6 +77,7 @@ def lib():
         L = ctypes.CDLL(path)
         P = ctypes.POINTER(_Jpeg)
         L.orc_parse.argtypes = [ctypes.c_char_p, ctypes.c_size_t, P]
+        L.orc_parse_ext.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, P]
         L.orc_decode_scan.argtypes = [P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_planes.argtypes = [P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.orc_colour.argtypes = [P, ctypes.c_void_p, ctypes.c_void_p]
@@ -87,6 +89,9 @@ def lib():
         L.orc_decode_file.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.orc_decode_file_ext.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -107,16 +112,19 @@ class Parsed:
     _data: bytes | None = None
 
 
-def parse(data: bytes) -> Parsed:
+EXT_DQT16 = 1  # ORC_EXT_DQT16: 16-bit quantisation tables (beyond the reference)
+
+
+def parse(data: bytes, ext: int = 0) -> Parsed:
     j = _Jpeg()
-    e = lib().orc_parse(data, len(data), ctypes.byref(j))
+    e = lib().orc_parse_ext(data, len(data), ext, ctypes.byref(j))
     if e:
         return Parsed(e)
     chans = [dict(samples_x=c.samples_x, samples_y=c.samples_y, stride=c.stride,
                   plane_height=c.plane_height, width=c.width, height=c.height, dq_id=c.dq_id,
                   dc_id=c.dc_id, ac_id=c.ac_id)
              for c in list(j.ch)[: j.num_channels]]
-    dq = np.array([list(t) for t in j.dq], np.uint8)
+    dq = np.array([list(t) for t in j.dq16], np.uint16)
     return Parsed(0, j.width, j.height, j.num_channels, j.num_blocks_x, j.num_blocks_y,
                   j.restart_interval, j.num_blocks, chans, dq, j, data)
 
@@ -160,15 +168,15 @@ def colour(p: Parsed, pl: list) -> np.ndarray:
     return out
 
 
-def decode(data: bytes, mode: int = 0):
+def decode(data: bytes, mode: int = 0, ext: int = 0):
     """Whole decode: returns (error, HxWx{1,3} uint8)."""
     w, h, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    p = parse(data)
+    p = parse(data, ext)
     if p.error:
         return p.error, None
     out = np.zeros(p.width * p.height * p.num_channels, np.uint8)
-    e = lib().orc_decode_file(data, len(data), mode, out.ctypes.data, out.size,
-                              ctypes.byref(w), ctypes.byref(h), ctypes.byref(c))
+    e = lib().orc_decode_file_ext(data, len(data), ext, mode, out.ctypes.data, out.size,
+                                  ctypes.byref(w), ctypes.byref(h), ctypes.byref(c))
     if e:
         return e, None
     return 0, out.reshape(h.value, w.value, c.value)

@@ -120,7 +120,9 @@ static int parse_dht(orc_jpeg *j, const uint8_t *pos, const uint8_t *end) {
   return 0;
 }
 
-/* src/format.cu:424-442 */
+/* src/format.cu:424-442; with ORC_EXT_DQT16 also tables of 16-bit entries
+ * (T.81 B.2.4.1: Pq = 1, 64 big-endian values).  dq keeps the reference's
+ * 8-bit view (entries clamped to 255), dq16 the values. */
 static int parse_dqt(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const uint8_t **out_end) {
   unsigned block_len = read16(p);
   const uint8_t *block_end = p + block_len;
@@ -128,8 +130,20 @@ static int parse_dqt(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const ui
   const uint8_t *pos = p + 2;
   while (pos + 65 <= block_end) {
     uint8_t id = pos[0];
+    int pq = id >> 4;
+    if (pq == 1 && (j->ext & ORC_EXT_DQT16)) {
+      if ((id & 0x0C) || pos + 129 > block_end) FAIL(ORC_SYNTAX_ERROR);
+      for (int k = 0; k < 64; k++) {
+        unsigned v = (unsigned)pos[1 + 2 * k] << 8 | pos[2 + 2 * k];
+        j->dq16[id & 3][k] = (uint16_t)v;
+        j->dq[id & 3][k] = (uint8_t)(v > 255 ? 255 : v);
+      }
+      pos += 129;
+      continue;
+    }
     if (id & 0xFC) FAIL(ORC_SYNTAX_ERROR);
     memcpy(j->dq[id], pos + 1, 64);
+    for (int k = 0; k < 64; k++) j->dq16[id][k] = pos[1 + k];
     pos += 65;
   }
   if (pos != block_end) FAIL(ORC_SYNTAX_ERROR);
@@ -156,8 +170,11 @@ static int parse_sos(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const ui
 }
 
 /* src/format.cu:113-215 (file read replaced by an in-memory buffer) */
-int orc_parse(const uint8_t *buf, size_t size, orc_jpeg *j) {
+int orc_parse(const uint8_t *buf, size_t size, orc_jpeg *j) { return orc_parse_ext(buf, size, 0, j); }
+
+int orc_parse_ext(const uint8_t *buf, size_t size, int ext, orc_jpeg *j) {
   memset(j, 0, sizeof(*j));
+  j->ext = ext;
   if (size < 4 || buf[0] != 0xFF || buf[1] != 0xD8 || buf[size - 2] != 0xFF || buf[size - 1] != 0xD9)
     FAIL(ORC_FILE_ERROR);
   if (size < 6) FAIL(ORC_SYNTAX_ERROR);
@@ -312,7 +329,7 @@ int orc_decode_scan(const orc_jpeg *jc, int16_t *qcoef, int32_t *work[3], uint32
       for (int c = 0; c < j->num_channels && !err; c++) {
         const orc_channel *ch = &j->ch[c];
         const vlc_t *dct = lut + 65536 * ch->dc_id, *act = lut + 65536 * ch->ac_id;
-        const uint8_t *dq = j->dq[ch->dq_id];
+        const uint16_t *dq = j->dq16[ch->dq_id];
         for (int sy = 0; sy < ch->samples_y && !err; sy++)
           for (int sx = 0; sx < ch->samples_x && !err; sx++) {
             /* src/decodeScanGPU.cu:78-100 (decodeBlock) */
@@ -471,7 +488,7 @@ int orc_planes(const orc_jpeg *j, const int16_t *qcoef, uint8_t *planes[3], int 
     for (int bx = 0; bx < j->num_blocks_x; bx++)
       for (int c = 0; c < j->num_channels; c++) {
         const orc_channel *ch = &j->ch[c];
-        const uint8_t *dq = j->dq[ch->dq_id];
+        const uint16_t *dq = j->dq16[ch->dq_id];
         for (int sy = 0; sy < ch->samples_y; sy++)
           for (int sx = 0; sx < ch->samples_x; sx++, blk++) {
             int32_t deq[64];
@@ -516,8 +533,13 @@ int orc_colour(const orc_jpeg *j, uint8_t *const planes[3], uint8_t *out) {
 
 int orc_decode_file(const uint8_t *file, size_t size, int idct_mode, uint8_t *out, size_t out_cap,
                     int *width, int *height, int *ncomp) {
+  return orc_decode_file_ext(file, size, 0, idct_mode, out, out_cap, width, height, ncomp);
+}
+
+int orc_decode_file_ext(const uint8_t *file, size_t size, int ext, int idct_mode, uint8_t *out, size_t out_cap,
+                        int *width, int *height, int *ncomp) {
   orc_jpeg j;
-  int e = orc_parse(file, size, &j);
+  int e = orc_parse_ext(file, size, ext, &j);
   if (e) return e;
   *width = j.width; *height = j.height; *ncomp = j.num_channels;
   if ((size_t)j.width * j.height * j.num_channels > out_cap) return ORC_OOM_ERROR;

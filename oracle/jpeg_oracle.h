@@ -43,7 +43,13 @@ typedef struct {
   const uint8_t *scan;
   size_t scan_len;
   int have_scan;
+  int ext;                          /* extensions accepted (ORC_EXT_*), set by orc_parse_ext */
+  uint16_t dq16[4][64];             /* the quant tables with 16-bit entries, zigzag order */
 } orc_jpeg;
+
+/* Extensions beyond the reference: 16-bit quantisation tables (DQT Pq = 1,
+ * T.81 B.2.4.1), which src/format.cu:432 rejects. */
+#define ORC_EXT_DQT16 1
 
 /* Parse markers, SOF, DHT, DQT, DRI and the SOS header of an in-memory file
  * (src/format.cu:113-215, :242-442; src/decodeScanGPU.cu:124-141).
@@ -86,6 +92,10 @@ int orc_colour(const orc_jpeg *j, uint8_t *const planes[3], uint8_t *out);
 /* Whole decode of an in-memory file (parse, scan, iDCT, colour). */
 int orc_decode_file(const uint8_t *file, size_t size, int idct_mode, uint8_t *out,
                     size_t out_cap, int *width, int *height, int *ncomp);
+/* orc_parse / orc_decode_file with the extensions `ext` accepted. */
+int orc_parse_ext(const uint8_t *file, size_t size, int ext, orc_jpeg *j);
+int orc_decode_file_ext(const uint8_t *file, size_t size, int ext, int idct_mode, uint8_t *out,
+                        size_t out_cap, int *width, int *height, int *ncomp);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@ def lib():
         L.emu_get.restype = ctypes.c_long
         L.emu_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.emu_free.argtypes = [ctypes.c_void_p]
+        L.emu_ext.argtypes = [ctypes.c_int]
         L.emu_unstuff_files.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         L.emu_unstuff_files.restype = ctypes.c_long

@@ -172,6 +172,7 @@ void expand(Emu& E) {
 std::vector<uint32_t> g_carry;
 bool g_carry_on = false;
 uint64_t g_lds_seed = 0;  // nonzero: every work group's LDS starts as random bytes
+int g_ext = 0;  // CJD_EXT_* accepted by the planner (emu_ext)
 int g_plant_ck = 0;       // nonzero: checkpoint words of every g_plant_ck-th tile claim too many DC entries
 
 void run_entropy(Emu& E, int order) {
@@ -308,6 +309,7 @@ void emu_k_unstuff(const UnstuffChunk* chunks, size_t nchunks, const uint8_t* ra
     const UnstuffChunk& c = chunks[ci];
     memset(&sh, 0x5A, sizeof(sh));
     for (int l = 0; l < kUnstuffLanes; ++l) unstuff_load(sh, l, c, raw);
+    unstuff_load_next(sh, c, raw);
     for (int l = 0; l < kUnstuffLanes; ++l) unstuff_carry(sh, l, c);
     for (int d = 1; d < kUnstuffLanes; d <<= 1) {
       for (int l = 0; l < kUnstuffLanes; ++l) fn[(size_t)l] = unstuff_fn_read(sh, l, d);
@@ -339,7 +341,14 @@ void emu_k_unstuff(const UnstuffChunk* chunks, size_t nchunks, const uint8_t* ra
       *oob = true;
       return 0;
     };
-    for (int l = 0; l < kUnstuffLanes; ++l) unstuff_write(sh, l, c, consumed[(size_t)l] != 0, o, base);
+    if (unstuff_single_run(sh)) {
+      const uint64_t dst0 = base(c.seg) + c.dst;
+      const uint32_t lead = (uint32_t)(dst0 & 15u);
+      for (int l = 0; l < kUnstuffLanes; ++l) unstuff_gather(sh, l, c, consumed[(size_t)l] != 0, lead);
+      for (int l = 0; l < kUnstuffLanes; ++l) unstuff_flush(sh, l, sh.tail[kUnstuffLanes - 1], lead, dst0, o);
+    } else {
+      for (int l = 0; l < kUnstuffLanes; ++l) unstuff_write(sh, l, c, consumed[(size_t)l] != 0, o, base);
+    }
   }
 }
 
@@ -354,6 +363,7 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
   E->plan.ck_bits = (uint32_t)ck_bits;
   if (item_tiles) E->plan.item_tiles = (uint32_t)item_tiles;
   E->plan.want_planes = want_planes;
+  E->plan.ext = g_ext;
   plan_batch(E->plan, n, files, sizes, errors);
   E->coefs.assign((size_t)E->plan.total_blocks * 64, (int16_t)0x5555);
   BlockRec junk{0, 0, 0x5555};  // eidx, nw, dc
@@ -494,6 +504,9 @@ void emu_plant_ck(int every) { g_plant_ck = every; }
 // Initial LDS of every emulated work group: random bytes from `seed` (0 =
 // zeroed, ~0 = left uninitialised).
 void emu_lds_seed(unsigned long long seed) { g_lds_seed = seed; }
+
+// Extensions (CJD_EXT_*) the planner of the next runs accepts.
+void emu_ext(int ext) { g_ext = ext; }
 
 // Carry-over mode: every launch starts with the entry buffer the previous
 // launch left (on = 1), instead of a fixed fill; 0 turns it off and forgets.

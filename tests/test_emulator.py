@@ -296,3 +296,38 @@ def test_emulated_device_unstuff_equals_host_on_adversarial_bytes(split):
         assert np.array_equal(host, dev), (case, n, int(np.flatnonzero(host != dev)[0]))
         done += 1
     assert done >= 40
+
+
+DQT16_SPECS = [synth.ImageSpec(96, 64, "gray", 3, 6.0, 91, dqt16=1),
+               synth.ImageSpec(130, 70, "420", 5, 8.0, 92, dqt16=1, qtables=3),
+               synth.ImageSpec(64, 64, "444", 8, 10.0, 93, dqt16=1, restart_interval=5)]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_emulated_dqt16_extension(mode):
+    # 16-bit quant tables with CJD_EXT_DQT16: coefficients from the encoder,
+    # integer-iDCT planes and pixels equal the oracle's (with the extension),
+    # FP32 and MFMA planes within 1 of its FP64 planes; without the extension
+    # every such file is refused with SYNTAX_ERROR, as by the reference
+    from tests.emu import lib as emu_lib
+    encs = [synth.encode(s, truth=True) for s in DQT16_SPECS]
+    r0 = EmuResult([e.data for e in encs], mode, 4096, True)
+    assert (r0.errors == 1).all()
+    emu_lib().emu_ext(1)
+    try:
+        r = EmuResult([e.data for e in encs], mode, 4096, True)
+    finally:
+        emu_lib().emu_ext(0)
+    assert not r.errors.any()
+    for i, (s, e) in enumerate(zip(DQT16_SPECS, encs)):
+        p = oracle.parse(e.data, oracle.EXT_DQT16)
+        assert np.array_equal(r.get(i, 1, (p.num_blocks, 64), np.int16), e.coefs), s
+        pl = oracle.planes(p, e.coefs, 0 if mode == 0 else 1)
+        for c in range(p.num_channels):
+            got = r.get(i, 2 + c, pl[c].shape, np.uint8).astype(int)
+            if mode == 0:
+                assert np.array_equal(got, pl[c]), (s, c)
+            else:
+                assert np.abs(got - pl[c]).max() <= 1, (s, c)
+        if mode == 0:
+            assert np.array_equal(r.get(i, 0, (s.height, s.width, s.ncomp), np.uint8), oracle.colour(p, pl)), s

@@ -191,3 +191,29 @@ int main(int argc, char** argv) {
     for (d, want), got in zip(cases, codes):
         if want is not None:
             assert got == want
+
+
+def test_dqt16_extension_in_host_parser():
+    # the reference refuses 16-bit quant tables (src/format.cu:432); with
+    # CJD_EXT_DQT16 the parser takes them: dq_tables16 holds the values,
+    # dq_tables the 8-bit view clamped at 255
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle import _dqt16_tables
+    from cudajpeg_amd import _lib as L
+    from cudajpeg_amd.decoder import parse
+    enc = synth.encode(synth.ImageSpec(120, 72, "420", 4, 6.0, 83, dqt16=1))
+    e, _ = parse(enc.data)
+    assert e == 1
+    e, f = parse(enc.data, L.EXT_DQT16)
+    assert e == 0
+    for tq, t in _dqt16_tables(enc.data).items():
+        assert list(f.dq_tables16[tq]) == t
+        assert list(f.dq_tables[tq]) == [min(v, 255) for v in t]
+    # an 8-bit file parses the same either way, dq_tables16 widened
+    enc8 = synth.encode(synth.ImageSpec(64, 48, "420", 75, 6.0, 84))
+    e0, f0 = parse(enc8.data)
+    e1, f1 = parse(enc8.data, L.EXT_DQT16)
+    assert e0 == e1 == 0
+    assert [list(t) for t in f0.dq_tables] == [list(t) for t in f1.dq_tables]
+    assert [list(t) for t in f1.dq_tables16] == [list(t) for t in f1.dq_tables]

@@ -131,3 +131,55 @@ def test_unstuff_rule():
     assert oracle.unstuff(b"\x12\xff\x00\x34") == b"\x12\xff\x34"
     assert oracle.unstuff(b"\xff\xd3\x01") == b"\xff\xd3\x01"
     assert oracle.unstuff(b"\xff\xe1\x01") == b"\xff\x01"
+
+
+def _dqt16_tables(data: bytes):
+    """The file's quant tables read straight from its DQT segments (T.81
+    B.2.4.1), independent of both parsers: {id: 64 values in zigzag order}."""
+    out, pos = {}, 2
+    while data[pos + 1] != 0xDA:
+        n = (data[pos + 2] << 8) | data[pos + 3]
+        if data[pos + 1] == 0xDB:
+            q, end = pos + 4, pos + 2 + n
+            while q < end:
+                pq, tq = data[q] >> 4, data[q] & 15
+                w = 2 if pq else 1
+                out[tq] = [int.from_bytes(data[q + 1 + w * k:q + 1 + w * k + w], "big") for k in range(64)]
+                q += 1 + 64 * w
+        pos += 2 + n
+    return out
+
+
+def test_dqt16_extension_pinned():
+    # 16-bit quant tables (Pq = 1): refused as the reference refuses them
+    # (SYNTAX_ERROR, src/format.cu:432) unless the extension is asked for;
+    # with it, the tables are the file's values and the FP64 planes equal
+    # C^T (coef x q) C + 128 computed here from those values
+    zz = [0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7,
+          14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39,
+          46, 53, 60, 61, 54, 47, 55, 62, 63]
+    C = np.array([[(np.sqrt(0.125) if u == 0 else 0.5) * np.cos((2 * x + 1) * u * np.pi / 16)
+                   for x in range(8)] for u in range(8)])
+    for spec in [synth.ImageSpec(96, 64, "gray", 3, 6.0, 81, dqt16=1),
+                 synth.ImageSpec(80, 48, "444", 5, 8.0, 82, dqt16=1, qtables=3)]:
+        enc = synth.encode(spec, truth=True)
+        assert oracle.parse(enc.data).error == 1
+        p = oracle.parse(enc.data, oracle.EXT_DQT16)
+        assert p.error == 0
+        tabs = _dqt16_tables(enc.data)
+        assert max(max(t) for t in tabs.values()) > 255  # really 16-bit content
+        for tq, t in tabs.items():
+            assert list(p.dq[tq]) == t
+        f64 = oracle.planes(p, enc.coefs, 1)
+        i32 = oracle.planes(p, enc.coefs, 0)
+        # block 0 of component 0 by the closed form
+        q = np.zeros(64)
+        for k in range(64):
+            q[zz[k]] = tabs[p.channels[0]["dq_id"]][k]
+        S = enc.coefs[0].astype(np.float64) * q
+        want = np.clip(np.floor(C.T @ S.reshape(8, 8) @ C + 128.0 + 0.5), 0, 255)
+        assert np.array_equal(f64[0][:8, :8].astype(int), want.astype(int))
+        for c in range(p.num_channels):
+            assert np.abs(f64[c].astype(int) - i32[c]).max() <= 1
+        err, px = oracle.decode(enc.data, 0, oracle.EXT_DQT16)
+        assert err == 0 and px.shape == (spec.height, spec.width, spec.ncomp)
