@@ -216,9 +216,13 @@ def serve_bench(args):
     replacement).  Peak = a burst of requests decoded flat out; then Poisson
     arrivals at each of --serve-rates and at 70% of peak, each for about
     --serve-seconds (at most 6000 requests); latency from submit to the frame
-    ready in device memory, sustained Mpixel/s = pixels completed / wall."""
+    ready in device memory, sustained Mpixel/s = pixels completed / wall.
+    Every phase logs its progress (the loop's counters every 2 s); when no
+    request completes for --serve-stall seconds while some are outstanding,
+    the counters are logged and the process ends (exit status 4)."""
     from cudajpeg_amd import Server, synth
     rng = np.random.default_rng(args.seed + 1)
+    t0 = time.time()
     specs = synth.config_mixed(240, args.seed + 1000)
     specs += [synth.ImageSpec(8192, 8192, "420", 85, float(rng.uniform(2, 12)), args.seed + 2000 + k)
               for k in range(12)]
@@ -226,14 +230,33 @@ def serve_bench(args):
     specs = [specs[i] for i in order]
     files = [e.data for e in synth.encode_many(specs)]
     px = np.array([s.pixels for s in specs], np.float64)
+    log(f"serving pool: {len(files)} files, {sum(map(len, files)) / 1e6:.0f} MB, encoded in {time.time() - t0:.1f} s")
     srv = Server(max_batch=64, max_wait_us=2000)
+    log("serving loop created")
+
+    def watch(what):
+        last = {"completed": -1, "t": time.time()}
+
+        def progress(st):
+            log(f"{what}: {st}")
+            now = time.time()
+            if st["completed"] != last["completed"]:
+                last["completed"], last["t"] = st["completed"], now
+            elif st["submitted"] > st["completed"] and now - last["t"] > args.serve_stall:
+                log(f"{what}: no request completed for {now - last['t']:.0f} s with "
+                    f"{st['submitted'] - st['completed']} outstanding; ending the serving run")
+                os._exit(4)
+        return progress
+
     # peak: a burst of 8 x the pool (long enough that the first batch's
     # staging and the last one's tail are a small part of it), after the same
     # burst once untimed, in which every slot grows its buffers to the mix
     n_peak = 8 * len(files)
     burst = [files[k % len(files)] for k in range(n_peak)]
-    check(srv.replay(burst, np.zeros(n_peak, np.int64))[1], "serving warm-up burst")
-    lat, err, wall = srv.replay(burst, np.zeros(n_peak, np.int64), progress=lambda st: log(f"serving burst: {st}"))
+    lat, err, wall = srv.replay(burst, np.zeros(n_peak, np.int64), progress=watch("serving warm-up burst"))
+    check(err, "serving warm-up burst")
+    log(f"serving warm-up burst: {n_peak} requests in {wall / 1e9:.3f} s")
+    lat, err, wall = srv.replay(burst, np.zeros(n_peak, np.int64), progress=watch("serving burst"))
     check(err, "serving burst")
     peak_req_s = n_peak / (wall / 1e9)
     peak_mpix_s = float(sum(px[k % len(files)] for k in range(n_peak)) / (wall / 1e9) / 1e6)
@@ -245,7 +268,7 @@ def serve_bench(args):
         arrival = np.cumsum(rng.exponential(1e9 / rate, n)).astype(np.int64)
         pick = rng.integers(0, len(files), n)
         lat, err, wall = srv.replay([files[int(k)] for k in pick], arrival,
-                                    progress=lambda st: log(f"serving at {rate:.0f} req/s: {st}"))
+                                    progress=watch(f"serving at {rate:.0f} req/s"))
         check(err, f"serving at {rate:.0f} req/s")
         log(f"serving at {rate:.0f} req/s, loop counters so far: {srv.stats()}")
         ms = lat / 1e6
@@ -268,11 +291,44 @@ def serve_bench(args):
     for rate in sorted(r for r in args.serve_rates if r >= peak_req_s):
         sweep.append(run(rate, args.serve_seconds))
         log(f"serving {rate:.0f} req/s (at or above peak): {sweep[-1]}")
+    counters = srv.stats()
     srv.close()
     return {"request_mix": "240 mixed-batch images + 12 8192x8192 4:2:0 q85 (5%), sampled with replacement",
             "peak_req_s": round(peak_req_s, 1), "peak_mpix_s": round(peak_mpix_s, 1),
             "at_70pct_of_peak": at70, "sweep": sweep, "max_batch": 64, "max_wait_us": 2000,
+            "redone_requests": counters["redone"], "batches": counters["batches"],
             "latency": "submit to frame ready in device memory (host monotonic clock)"}
+
+
+def serve_child(args):
+    """The serving section in a child process of its own (bench.py
+    --serve-only): its progress lines go to this process's stderr, and it is
+    ended after --serve-timeout seconds, so that a serving run that does not
+    finish cannot keep the rest of the bench from reporting."""
+    import subprocess
+    fd, out = tempfile.mkstemp(prefix="cjd_serve_", suffix=".json")
+    os.close(fd)
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only", out, "--seed", str(args.seed),
+           "--serve-rates", ",".join(str(r) for r in args.serve_rates), "--serve-seconds", str(args.serve_seconds),
+           "--serve-stall", str(args.serve_stall)]
+    p = subprocess.Popen(cmd, stdout=sys.stderr)
+    try:
+        rc = p.wait(timeout=args.serve_timeout)
+    except subprocess.TimeoutExpired:
+        p.kill()
+        p.wait()
+        rc = None
+    try:
+        res = json.load(open(out)) if rc == 0 else None
+    except (OSError, ValueError):
+        res = None
+    os.unlink(out)
+    if res is None:
+        why = (f"did not finish within {args.serve_timeout:.0f} s" if rc is None
+               else "stalled (no request completed)" if rc == 4 else f"exited with status {rc}")
+        log(f"serving run {why}")
+        return {"error": f"the serving run {why}; its progress lines are in the bench log"}
+    return res
 
 
 def measured_traffic(kernel: str, args) -> int | None:
@@ -322,6 +378,12 @@ def main():
     ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
                     default=[500.0, 1000.0, 2000.0, 4000.0, 8000.0], help="Poisson rates of the sweep, req/s")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of arrivals per swept rate")
+    ap.add_argument("--serve-timeout", type=float, default=240.0,
+                    help="seconds after which the serving run (a child process) is ended")
+    ap.add_argument("--serve-stall", type=float, default=45.0,
+                    help="seconds without a completed request (some outstanding) that end the serving run")
+    ap.add_argument("--serve-only", metavar="PATH", default=None,
+                    help="only run the serving section and write its result (JSON) to PATH")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (reads MIB, writes MIB) "
                          "three times: the reference point for scaling FETCH_SIZE/WRITE_SIZE counters")
@@ -338,6 +400,11 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
+    if args.serve_only:
+        res = serve_bench(args)
+        with open(args.serve_only, "w") as f:
+            json.dump(res, f)
+        return
     if args.copy_calibration:
         n = args.copy_calibration << 18  # int32 elements
         x = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, device="cuda")
@@ -562,10 +629,6 @@ def main():
         for d in per:
             d.close()
 
-    serving = None if args.no_serve else serve_bench(args)
-    if serving:
-        log(f"serving: {serving}")
-
     cpu = None if args.no_cpu else cpu_baseline(encs, args.cpu_seconds)
     if cpu is not None and args.cpu_workers > 1:
         cpu["all_cores"] = cpu_all_cores(encs, args.cpu_workers, 4)
@@ -573,6 +636,10 @@ def main():
             log(f"cpu baseline, {cpu['all_cores']['cores']} processes: {cpu['all_cores']['value']} Mpix/s")
     if cpu:
         log(f"cpu baseline: {cpu}")
+
+    serving = None if args.no_serve else serve_child(args)
+    if serving:
+        log(f"serving: {serving}")
 
     out = {
         "metric": "Mpix/s batched-MFMA vs plain per-image on 1 MI355X; p50/p99 decode latency",

@@ -64,6 +64,41 @@ __global__ void k_clear_u32(uint32_t* __restrict__ p, uint64_t n) {
     p[i] = 0;
 }
 
+// Wave64 primitives.  A wave's lanes claim slots of an LDS list with one
+// atomic per wave: the ballot of the lanes that want one gives the wave's
+// count (added by the lowest such lane) and each lane's rank among them
+// (mbcnt).  Called by every lane of the wave (the ballot must see them all).
+__device__ __forceinline__ int wave_claim(bool want, int32_t* counter) {
+  const uint64_t m = __ballot(want);
+  if (!m) return 0;  // wave-uniform
+  const uint32_t below =
+      __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  const int leader = __builtin_ctzll(m);
+  int base = 0;
+  if ((int)(threadIdx.x & 63u) == leader) base = atomicAdd(counter, __builtin_popcountll(m));
+  base = __builtin_amdgcn_readlane(base, leader);
+  return base + (int)below;
+}
+
+__device__ __forceinline__ void wave_count(bool pred, int32_t* counter) {
+  const uint64_t m = __ballot(pred);
+  if (m && (int)(threadIdx.x & 63u) == __builtin_ctzll(m)) atomicAdd(counter, __builtin_popcountll(m));
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave in DPP steps: shifts of
+// 1, 2, 4 and 8 lanes within each row of 16 (lanes shifted in read 0), then
+// lane 15 of each row added into the next row (rows 1 and 3) and lane 31
+// into rows 2 and 3.  Each step is one v_add_u32 with a DPP source.
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
+  return x;
+}
+
 // Run the sh.ntask queued link tasks in rounds; after each round the
 // unfinished ones are packed into the lowest lanes (all lanes call this).
 __device__ __forceinline__ void run_links(EntShared& sh, int tid, const EntCtx& c) {
@@ -75,11 +110,11 @@ __device__ __forceinline__ void run_links(EntShared& sh, int tid, const EntCtx& 
     if (tid < n) T = task_get(sh, tid);
     if (tid == 0) sh.nnext = 0;
     __syncthreads();  // every task read before any is put back
-    if (tid < n) {
-      Lane L;
-      int j;
-      if (!task_round(sh, tid, c, T, L, j)) task_put(sh, atomicAdd(&sh.nnext, 1), j, L);
-    }
+    Lane L;
+    int j = 0;
+    const bool keep = tid < n && !task_round(sh, tid, c, T, L, j);
+    const int idx = wave_claim(keep, &sh.nnext);
+    if (keep) task_put(sh, idx, j, L);
     __syncthreads();
     if (tid == 0) sh.ntask = sh.nnext;
   }
@@ -93,8 +128,9 @@ __device__ __forceinline__ void queue_links(EntShared& sh, int tid, bool task, b
     sh.nneed = 0;
   }
   __syncthreads();
-  if (task) task_new(sh, atomicAdd(&sh.ntask, 1), tid);
-  if (need) atomicAdd(&sh.nneed, 1);
+  const int idx = wave_claim(task, &sh.ntask);
+  if (task) task_new(sh, idx, tid);
+  wave_count(need, &sh.nneed);
 }
 
 // Fix-up rounds (all lanes): relink every tile whose entry changed, until no
@@ -118,17 +154,24 @@ __device__ __forceinline__ void fix_rounds(EntShared& sh, int tid, const EntCtx&
   }
 }
 
-// Inclusive prefix sum of the per-tile counts into tc (all lanes).
+// Inclusive prefix sum of the per-tile counts into tc (all lanes): a DPP scan
+// within each wave, then each lane adds the totals of the waves before its
+// own -- one barrier, where a scan over the workgroup in LDS takes two per
+// doubling step.  Each lane reads only its own tile's counts (written by it
+// in ent_count); the barrier at the end publishes tc.
 __device__ __forceinline__ void scan_counts(EntShared& sh, int tid, const EntCtx& c) {
-  ent_scan_init(sh, tid, c);
+  const int w = tid >> 6;
+  int32_t v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = wave_incl_scan(tid < c.J ? sh.u.c.cnt[tid][q] : 0);
+  if ((tid & 63) == 63)
+    for (int q = 0; q < 4; ++q) sh.wsum[w][q] = v[q];
   __syncthreads();
-  for (int d = 1; d < c.J; d <<= 1) {
-    int32_t v[4];
-    ent_scan_read(sh, tid, d, v);
-    __syncthreads();
-    if (tid < c.J) ent_scan_write(sh, tid, v);
-    __syncthreads();
-  }
+  for (int k = 0; k < w; ++k)
+    for (int q = 0; q < 4; ++q) v[q] += sh.wsum[k][q];
+  if (tid < c.J)
+    for (int q = 0; q < 4; ++q) sh.u.c.tc[tid][q] = v[q];
+  __syncthreads();
 }
 
 __global__ void __launch_bounds__(kWG) k_entropy(const uint32_t* __restrict__ words,

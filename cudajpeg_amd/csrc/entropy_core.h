@@ -324,6 +324,7 @@ struct EntShared {
   uint32_t look_ok;                // the lookback resolved the item's entry
   uint32_t entry_p, entry_s;       // true state at the exit of the lead-in tile
   int32_t entry_c[4];              //   and the segment's cumulative counts there
+  int32_t wsum[kWG / 64][4];       // count scan: each wave's totals
 };
 
 // Constants of one work item.
