@@ -408,6 +408,8 @@ struct EntCtx {
 struct Lane {
   uint32_t pos, c;
   uint32_t f0, f1, f2, f3;
+  uint32_t g0, g1, g2, g3;  // the load issued at the last period's top
+  bool greal;               // ... and whether it was the lane's next chunk
   int k;         // -1 when the next symbol is a DC symbol, else the last AC index
   int slot;
   uint32_t s;    // step index
@@ -456,6 +458,7 @@ CJD_HD inline void lane_seek(EntShared& sh, int tid, const EntCtx& c, Lane& L, u
   ld4(c, L.c + 4, a, b, d, e);
   ring_put(sh, tid, (L.c + 4) & 7, a, b, d, e);
   ld4(c, L.c + 8, L.f0, L.f1, L.f2, L.f3);
+  L.greal = false;
 #if defined(__HIP_DEVICE_COMPILE__)
   // a seek is rare: wait here for the chunk in flight, so that the decode
   // loop's own prefetch only ever waits for the load of the previous period
@@ -463,12 +466,28 @@ CJD_HD inline void lane_seek(EntShared& sh, int tid, const EntCtx& c, Lane& L, u
 #endif
 }
 
+// Top of a period: the chunk requested at the last period's top becomes the
+// one in flight, and a lane past word c+3 moves on (the in-flight chunk
+// enters the ring) and requests its next chunk.  A lane moves about once in
+// six periods; the load is issued by every lane every period all the same
+// (a load under a branch made the compiler wait for it at the join), but a
+// lane that did not move reads the segment's first words -- one cache line
+// for the whole wave, where every lane re-reading its own chunk took a line
+// per lane -- and drops them at the next period's top.  A lane moves at
+// most once a period (a period consumes under 128 bits), so the chunk it
+// requested is in place before it can be needed.
 CJD_HD inline void lane_rotate(EntShared& sh, int tid, const EntCtx& c, Lane& L) {
-  if (L.pos >= (L.c + 4) * 32) {
+  L.f0 = L.greal ? L.g0 : L.f0;
+  L.f1 = L.greal ? L.g1 : L.f1;
+  L.f2 = L.greal ? L.g2 : L.f2;
+  L.f3 = L.greal ? L.g3 : L.f3;
+  const bool mv = L.pos >= (L.c + 4) * 32;
+  if (mv) {
     ring_put(sh, tid, L.c & 7, L.f0, L.f1, L.f2, L.f3);
     L.c += 4;
   }
-  ld4(c, L.c + 8, L.f0, L.f1, L.f2, L.f3);  // re-requests the same chunk when nothing moved
+  ld4(c, mv ? L.c + 8 : 0u, L.g0, L.g1, L.g2, L.g3);
+  L.greal = mv;
 }
 
 // The 32 bits at `pos`: rows r, r+1 with r = word of bit pos-1, shifted so
