@@ -754,6 +754,7 @@ extern "C" int cjd_batch_capture(cjd_batch* b, void* stream) {
   if (!b || !stream || b->plan.recs.empty() || !b->d_scratch.p) return CJD_PROGRAMMER_ERROR;
   hipStream_t s = (hipStream_t)stream;
   drop_graph(b);
+  for (auto& g : b->graph_counts) g = 0;  // kinds the captured decode does not launch expect no tickets
   if (b->nsplit > 1)
     if (int e = ensure_aux(b)) return e;
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;

@@ -6,6 +6,36 @@
 
 namespace cjd {
 
+// Buffers and launch constants of the tiled entropy decoder.
+struct EntArgs {
+  const uint32_t* words;
+  const ImageDesc* images;
+  const SegmentDesc* segs;
+  const WorkItem* items;
+  const HuffTable* tables;
+  const uint16_t* luts;
+  uint32_t* ebuf;
+  BlockRec* recs;
+  uint32_t* status;
+  LookbackRec* lb;
+  unsigned long long* done;  // finished work items (read by the host)
+  uint32_t epoch;
+  uint64_t wait_ticks;
+};
+
+// Buffers of the pixel stage.
+struct PixArgs {
+  const ImageDesc* images;
+  const PixItem* items;
+  const BlockRec* recs;
+  const uint32_t* ebuf;
+  uint64_t ewords;
+  const int16_t* coefs;
+  const uint16_t* qtabs;
+  const uint4* btab;
+  uint8_t* out;
+};
+
 struct UnstuffChunk;
 // Fill the batch's scan words with 0xFF, then unstuff every chunk of stuffed
 // scan bytes into them (unstuff.hip).

@@ -108,16 +108,16 @@ int cjd_batch_set_options(cjd_batch *b, int idct_mode, int entropy_mode, int til
  * cjd_batch_stage. */
 int cjd_batch_set_tuning(cjd_batch *b, int ck_bits, int item_tiles);
 
+/* Extensions (CJD_EXT_*) the batch accepts in the files it stages; 0 (the
+ * default) parses as the reference does.  From the next stage. */
+int cjd_batch_set_extensions(cjd_batch *b, int ext);
+
 /* Where the scans are unstuffed (the 0x00 after each 0xFF removed and the
  * scan split at RSTn: src/decodeScanGPU.cu:11-24, which the reference does on
  * the host): CJD_UNSTUFF_DEVICE (default) uploads each file's stuffed scan
  * and unstuffs it with a kernel on the staging stream; CJD_UNSTUFF_HOST
  * unstuffs in the host planner and uploads the words.  Results are identical.
  * Applies from the next stage.  No counterpart in the reference. */
-/* Extensions (CJD_EXT_*) the batch accepts in the files it stages; 0 (the
- * default) parses as the reference does.  From the next stage. */
-int cjd_batch_set_extensions(cjd_batch *b, int ext);
-
 #define CJD_UNSTUFF_HOST 0
 #define CJD_UNSTUFF_DEVICE 1
 int cjd_batch_set_unstuff(cjd_batch *b, int mode);
