@@ -17,6 +17,7 @@ IDCT_INT, IDCT_FLOAT, IDCT_MFMA = 0, 1, 2
 ENTROPY_TILED, ENTROPY_SERIAL = 0, 1
 UNSTUFF_HOST, UNSTUFF_DEVICE = 0, 1
 EXT_DQT16 = 1
+EXT_NONINTERLEAVED = 2
 OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 
 # Every symbol include/cudajpeg_amd.h and include/format.h declare.
@@ -52,7 +53,7 @@ class FrameInfo(ctypes.Structure):
         ("huff_bits", (ctypes.c_uint8 * 16) * 4),
         ("huff_vals", (ctypes.c_uint8 * 256) * 4),
         ("scan_offset", ctypes.c_uint64), ("scan_length", ctypes.c_uint64),
-        ("error_line", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("error_line", ctypes.c_int32), ("num_scans", ctypes.c_int32),
         ("dq_tables16", (ctypes.c_uint16 * 64) * 4),
     ]
 

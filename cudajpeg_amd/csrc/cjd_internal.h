@@ -80,13 +80,21 @@ struct SegmentDesc {
   uint32_t tile_bits;   // tile length S in bits
   uint32_t ntiles;
   uint32_t nblocks;     // blocks the segment must contain
-  int64_t first_block;  // global coefficient-buffer index of its first block
+  int64_t first_block;  // interleaved scan: global coefficient-buffer index of its first block;
+                        // one-component scan: index of its first block in the scan
   uint32_t first_item;  // index of its first work item
   uint32_t ck_bits;     // checkpoint spacing C in bits
   uint32_t nck;         // checkpoints per tile (at C, 2C, ... < S)
   uint32_t rs;          // steps (entries) of a tile's speculative or prefix stream
   uint32_t rd;          // entries of a tile's DC list (per stream)
-  uint32_t pad_;
+  uint32_t tset;        // Huffman table set of its scan: HuffTables 4 tset .. 4 tset + 3, lookup table tset
+  // A one-component scan (CJD_EXT_NONINTERLEAVED, ni = 1) codes component
+  // comp's own block grid, bw blocks a row, in raster order (T.81 A.2.2), one
+  // block per MCU of the scan; its block (bx, by) is the interleaved layout's
+  // slot slot0 + (by % v) h + bx % h of MCU (bx / h, by / v).
+  uint8_t ni, comp, h, v;
+  uint8_t dc_id, ac_id, slot0, pad8_;  // the scan's DC (0..1) and AC (2..3) table ids
+  uint32_t bw;
 };
 
 // One entropy workgroup: tiles [t0, t0 + ntiles) of a segment.  When t0 > 0 the

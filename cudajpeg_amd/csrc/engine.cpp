@@ -315,7 +315,7 @@ static void plan_split(cjd_batch* b) {
 }
 
 extern "C" int cjd_batch_set_extensions(cjd_batch* b, int ext) {
-  if (!b || (ext & ~CJD_EXT_DQT16)) return CJD_PROGRAMMER_ERROR;
+  if (!b || (ext & ~(CJD_EXT_DQT16 | CJD_EXT_NONINTERLEAVED))) return CJD_PROGRAMMER_ERROR;
   b->plan.ext = ext;
   return 0;
 }
@@ -383,6 +383,14 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   if (!e && b->plan.entry_words >= ((int64_t)1 << 32)) e = CJD_OOM_ERROR;  // records hold 32-bit entry indexes
   if (!e) e = b->d_ebuf.ensure((size_t)b->plan.entry_words * 4);
   if (!e) e = b->d_recs.ensure((size_t)b->plan.total_blocks * sizeof(BlockRec));
+  // one-component scans do not code the blocks MCUs pad a component's grid
+  // with (outside the image), and decodes write only coded blocks' records:
+  // those images' records start as zero blocks
+  for (const ImageRec& r : P.recs)
+    if (!e && !r.error && r.f.num_scans > 1 &&
+        launch_clear(reinterpret_cast<uint32_t*>(b->d_recs.as<BlockRec>() + r.coef_base),
+                     (uint64_t)r.f.num_blocks * (sizeof(BlockRec) / 4), s) != hipSuccess)
+      e = CJD_KERNEL_LAUNCH_ERROR;
   if (!e) e = mp.put(b->d_qtabs, P.qtabs, pinned, s);
   if (!e) {
     if (hipEventRecord(b->up_ev, s) != hipSuccess) e = CJD_DEVICE_MEM_ERROR;
@@ -393,6 +401,13 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
                                b->d_btab.as<uint4>(), s) != hipSuccess)
     e = CJD_KERNEL_LAUNCH_ERROR;
   if (!e) e = b->d_coefs.ensure((size_t)b->plan.total_blocks * 128);
+  // images with one-component scans: zero coefficient blocks as well (the
+  // serial decoder writes the coded blocks only)
+  for (const ImageRec& r : P.recs)
+    if (!e && !r.error && r.f.num_scans > 1 &&
+        launch_clear(reinterpret_cast<uint32_t*>(b->d_coefs.as<int16_t>() + r.coef_base * 64),
+                     (uint64_t)r.f.num_blocks * 32, s) != hipSuccess)
+      e = CJD_KERNEL_LAUNCH_ERROR;
   if (!e) e = b->d_out.ensure(b->plan.out_bytes);
   // scratch (layout at cjd_batch): a new allocation is cleared whole and
   // starts the ticket counter again; otherwise the status words and records

@@ -328,6 +328,28 @@ struct EntShared {
 };
 
 // Constants of one work item.
+// A segment's MCU structure: the image's slots for its interleaved scan, one
+// slot (the component's) for a one-component scan.
+CJD_HD inline int seg_nh(const SegmentDesc& sg, const ImageDesc& im) { return sg.ni ? 1 : im.nslots; }
+CJD_HD inline int seg_comp(const SegmentDesc& sg, const ImageDesc& im, int slot) {
+  return sg.ni ? sg.comp : im.slot_comp[slot];
+}
+CJD_HD inline int seg_dc(const SegmentDesc& sg, const ImageDesc& im, int slot) {
+  return sg.ni ? sg.dc_id : im.slot_dc[slot];
+}
+CJD_HD inline int seg_ac(const SegmentDesc& sg, const ImageDesc& im, int slot) {
+  return sg.ni ? sg.ac_id : im.slot_ac[slot];
+}
+// Coefficient-buffer (record) index of block a of the segment's scan (a
+// counts from first_block): itself in an interleaved scan; for a
+// one-component scan, where the interleaved scan holds that block.
+CJD_HD inline int64_t seg_block(const SegmentDesc& sg, const ImageDesc& im, int64_t a) {
+  if (!sg.ni) return a;
+  const int64_t by = a / sg.bw, bx = a - by * sg.bw;
+  return im.coef_base + ((by / sg.v) * im.mcus_x + bx / sg.h) * im.nslots + sg.slot0 + (by % sg.v) * sg.h +
+         bx % sg.h;
+}
+
 struct EntCtx {
   const uint32_t* w;
   uint32_t nwords;
@@ -348,9 +370,9 @@ struct EntCtx {
     it = item;
     sg = segs[item.segment];
     im = images + sg.image;
-    tabs = tables + im->table_base;
+    tabs = tables + (size_t)sg.tset * 4;
     ebuf = entries;
-    nh = im->nslots;
+    nh = seg_nh(sg, *im);
     S = sg.tile_bits;
     w = words + sg.word_base;
     nwords = sg.nwords;
@@ -358,7 +380,7 @@ struct EntCtx {
     J = (int)item.ntiles + lead;
     tfirst = item.t0 - (uint32_t)lead;
     slotinfo = 0;
-    for (int k = 0; k < nh; ++k) slotinfo |= (uint64_t)(im->slot_comp[k] << 2) << (4 * k);
+    for (int k = 0; k < nh; ++k) slotinfo |= (uint64_t)(seg_comp(sg, *im, k) << 2) << (4 * k);
   }
   CJD_HD int comp(int slot) const { return (int)((slotinfo >> (4 * slot + 2)) & 3); }
   CJD_HD uint32_t tile_end(uint32_t t) const { return (t + 1 >= sg.ntiles) ? sg.nbits : (t + 1) * S; }
@@ -1078,7 +1100,7 @@ CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, Bl
         r.eidx = (uint32_t)entry_word(base, W, st + 1);
         r.nw = (uint16_t)((sn - st - 1) | ((W / 16 - 1) << 8));
         r.dc = (int16_t)(comp == 0 ? pred[0] : comp == 1 ? pred[1] : pred[2]);
-        recs[A] = r;
+        recs[seg_block(c.sg, *c.im, A)] = r;
         err = err || (e & 0x10000u);
       }
       ++A;
@@ -1106,7 +1128,8 @@ CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* im
                                       const HuffTable* tables, int16_t* coefs, BlockRec* recs, uint32_t* status,
                                       uint32_t epoch) {
   const ImageDesc& im = images[sg.image];
-  const HuffTable* tab = tables + im.table_base;
+  const HuffTable* tab = tables + (size_t)sg.tset * 4;
+  const int nh = seg_nh(sg, im);
   BitReader br;
   br.seek(words + sg.word_base, sg.nwords, 0);
   int slot = 0, dc[3] = {0, 0, 0};
@@ -1114,22 +1137,23 @@ CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* im
   for (uint32_t b = 0; b < sg.nblocks; ++b) {
     for (int k = 0; k < 64; ++k) blk[k] = 0;
     int d, err = 0;
-    decode_block<true>(br, tab[im.slot_dc[slot]], tab[im.slot_ac[slot]], d, err, blk);
+    decode_block<true>(br, tab[seg_dc(sg, im, slot)], tab[seg_ac(sg, im, slot)], d, err, blk);
     if (err) {
       status_or(status + im.error_index, kStatusSyntax, epoch);
       return;
     }
-    const int comp = im.slot_comp[slot];
+    const int comp = seg_comp(sg, im, slot);
     dc[comp] += d;
     blk[0] = (int16_t)dc[comp];
-    int16_t* g = coefs + (sg.first_block + b) * 64;
+    const int64_t gi = seg_block(sg, im, sg.first_block + b);
+    int16_t* g = coefs + gi * 64;
     for (int k = 0; k < 64; ++k) g[k] = blk[k];
     BlockRec r;
     r.eidx = 0;
     r.nw = kRecDense;
     r.dc = blk[0];
-    recs[sg.first_block + b] = r;
-    if (++slot == im.nslots) slot = 0;
+    recs[gi] = r;
+    if (++slot == nh) slot = 0;
   }
 }
 

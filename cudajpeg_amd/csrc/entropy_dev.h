@@ -167,14 +167,14 @@ __device__ inline void entropy_item(EntShared& sh, int tid, uint32_t item_id, co
   EntCtx c;
   c.init(a.words, a.images, a.segs, a.tables, a.ebuf, a.items[item_id]);
   {
-    const uint4* src = reinterpret_cast<const uint4*>(a.luts + (size_t)c.sg.image * kLutEntries);
+    const uint4* src = reinterpret_cast<const uint4*>(a.luts + (size_t)c.sg.tset * kLutEntries);
     uint4* dst = reinterpret_cast<uint4*>(sh.lut);
     for (int i = tid; i < kLutEntries / 8; i += kWG) dst[i] = src[i];
     if (tid < 64) sh.dz[tid] = kDeZigZag[tid];
     if (tid < 4) canon_fill(sh.canon[tid], c.tabs[tid]);
     if (tid < c.nh) {
-      sh.tsel[2 * tid] = (uint16_t)(c.im->slot_ac[tid] * kL1Size);
-      sh.tsel[2 * tid + 1] = (uint16_t)(c.im->slot_dc[tid] * kL1Size);
+      sh.tsel[2 * tid] = (uint16_t)(seg_ac(c.sg, *c.im, tid) * kL1Size);
+      sh.tsel[2 * tid + 1] = (uint16_t)(seg_dc(c.sg, *c.im, tid) * kL1Size);
     }
   }
   __syncthreads();

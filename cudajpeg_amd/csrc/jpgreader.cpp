@@ -180,12 +180,18 @@ void decodeDRI(JPGReader* jpg) {
 // are added to jpg->time (src/decodeScanGPU.cu:154, 224-225).
 static void decode_scan(JPGReader* jpg, int entropy_mode, bool coefficients) {
   const clock_t t0 = clock();
-  run_parser(jpg, cjd::parse_sos, "decodeScan");
-  if (jpg->error) return;
   ReaderEngine* e = eng(jpg);
-  for (int i = 0; i < jpg->num_channels; ++i) {
-    jpg->channels[i].dc_id = e->f.dc_id[i];
-    jpg->channels[i].ac_id = e->f.ac_id[i];
+  // CJD_EXT_NONINTERLEAVED: a one-component first scan; the batch's planner
+  // parses the file's scans
+  const bool one = (jpg->extensions & CJD_EXT_NONINTERLEAVED) && jpg->num_channels > 1 && jpg->pos + 3 <= jpg->end &&
+                   jpg->pos[2] == 1;
+  if (!one) {
+    run_parser(jpg, cjd::parse_sos, "decodeScan");
+    if (jpg->error) return;
+    for (int i = 0; i < jpg->num_channels; ++i) {
+      jpg->channels[i].dc_id = e->f.dc_id[i];
+      jpg->channels[i].ac_id = e->f.ac_id[i];
+    }
   }
   int err = cjd_batch_set_options(e->batch, jpg->idct_mode, entropy_mode, 0, jpg->keep_planes);
   if (!err) err = cjd_batch_set_extensions(e->batch, jpg->extensions);
@@ -218,10 +224,12 @@ static void decode_scan(JPGReader* jpg, int entropy_mode, bool coefficients) {
       if (grow(&ch->working_space, (size_t)ch->stride * ch->plane_height)) RTHROW(OOM_ERROR);
       wpos[c] = ch->working_space.mem;
     }
+    // the staged file's tables (one-component scans: each component's as
+    // latched at its scan)
     size_t blk = 0;
     for (int m = 0; m < info.mcus_x * info.mcus_y; ++m)
       for (int c = 0; c < jpg->num_channels; ++c) {
-        const uint16_t* dq = e->f.dq_tables16[jpg->channels[c].dq_id];
+        const uint16_t* dq = info.dq_tables16[info.dq_id[c]];
         for (int k = 0; k < jpg->channels[c].samples_x * jpg->channels[c].samples_y; ++k, ++blk) {
           const int16_t* src = &q[blk * 64];
           for (int z = 0; z < 64; ++z) wpos[c][zz[z]] = (int)src[zz[z]] * dq[z];

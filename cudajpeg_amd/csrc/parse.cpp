@@ -171,8 +171,57 @@ void parse_sos(ParseState& st) {
   f.scan_length = (uint64_t)(st.end - st.pos);
 }
 
+// T.81 B.2.3 with Ns = 1 (src/decodeScanGPU.cu:124-131 rejects it): the
+// scan's component, its tables and the restart interval in effect, the
+// component's quant table as it is now; the scan's data runs to the first
+// 0xFF followed by neither 0x00 nor an RSTn (T.81 B.1.1.2, B.1.1.5), at the
+// latest to the EOI.
+void parse_sos_one(ParseState& st) {
+  cjd_frame_info& f = *st.f;
+  const uint8_t* pos = st.pos;
+  const unsigned header_len = read16(pos);
+  if (header_len != 8 || pos + header_len > st.end) PFAIL(CJD_SYNTAX_ERROR);
+  pos += 3;
+  int c = 0;
+  while (c < f.num_channels && f.comp_id[c] != pos[0]) ++c;
+  if (c == f.num_channels || st.nscans >= f.num_channels) PFAIL(CJD_SYNTAX_ERROR);
+  for (int k = 0; k < st.nscans; ++k)
+    if (st.scans[k].comp == c) PFAIL(CJD_SYNTAX_ERROR);  // each component in one scan
+  if (pos[1] & 0xEE) PFAIL(CJD_SYNTAX_ERROR);
+  if (pos[2] || pos[3] != 63 || pos[4]) PFAIL(CJD_UNSUPPORTED_ERROR);
+  ScanInfo& sc = st.scans[st.nscans++];
+  sc.comp = c;
+  sc.dc_id = pos[1] >> 4;
+  sc.ac_id = (pos[1] & 1) | 2;
+  sc.restart_interval = f.restart_interval;
+  memcpy(sc.huff_bits, f.huff_bits, sizeof(sc.huff_bits));
+  memcpy(sc.huff_vals, f.huff_vals, sizeof(sc.huff_vals));
+  memcpy(sc.dq, f.dq_tables16[f.dq_id[c]], sizeof(sc.dq));
+  const uint8_t* d = st.pos + header_len;
+  const uint8_t* e = d;
+  for (;;) {
+    const void* ff = e < st.end ? memchr(e, 0xFF, (size_t)(st.end - e)) : nullptr;
+    if (!ff) {
+      e = st.end;
+      break;
+    }
+    e = static_cast<const uint8_t*>(ff);  // e < st.end: e[1] is in the file
+    if (e[1] != 0x00 && (e[1] & 0xF8) != 0xD0) break;
+    e += 2;
+  }
+  sc.offset = (uint64_t)(d - st.buf);
+  sc.length = (uint64_t)(e - d);
+  if (st.nscans == 1) {
+    f.scan_offset = sc.offset;
+    f.scan_length = sc.length;
+  }
+  st.pos = e;
+}
+
 // src/format.cu:113-215 without the file read; stops after the SOS header
-// (or at the EOI) and leaves st.pos there.
+// (or at the EOI) and leaves st.pos there.  With CJD_EXT_NONINTERLEAVED, a
+// frame of several components whose first scan has one component is parsed
+// to the EOI, scan by scan.
 int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame_info* f, int ext) {
   memset(f, 0, sizeof(*f));
   st.f = f;
@@ -182,6 +231,7 @@ int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame
   st.error_line = 0;
   st.have_sof = false;
   st.at_scan = false;
+  st.nscans = 0;
   if (size < 4 || data[0] != 0xFF || data[1] != 0xD8 || data[size - 2] != 0xFF || data[size - 1] != 0xD9) {
     st.error = CJD_FILE_ERROR;
     st.error_line = __LINE__;
@@ -208,6 +258,15 @@ int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame
       case 0xDB: parse_dqt(st); break;
       case 0xDD: parse_dri(st); break;
       case 0xDA:
+        if ((st.ext & CJD_EXT_NONINTERLEAVED) && f->num_channels > 1 && st.pos + 3 <= st.end && st.pos[2] == 1) {
+          parse_sos_one(st);
+          break;
+        }
+        if (st.nscans) {  // an interleaved scan after one-component scans
+          st.error = CJD_SYNTAX_ERROR;
+          st.error_line = __LINE__;
+          break;
+        }
         parse_sos(st);
         if (!st.error) st.at_scan = true;
         break;
@@ -222,6 +281,26 @@ int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame
     }
     if (st.error || st.at_scan || m == 0xD9) break;
   }
+  if (!st.error && st.nscans) {
+    if (st.nscans != f->num_channels) {  // a component no scan codes
+      st.error = CJD_SYNTAX_ERROR;
+      st.error_line = __LINE__;
+    } else {
+      // each component's quant table as latched at its scan, listed under
+      // table id c (a DQT between scans may have redefined the id it named)
+      st.at_scan = true;
+      for (int k = 0; k < st.nscans; ++k) {
+        const int c = st.scans[k].comp;
+        f->dq_id[c] = c;
+        for (int i = 0; i < 64; ++i) {
+          const uint16_t v = st.scans[k].dq[i];
+          f->dq_tables16[c][i] = v;
+          f->dq_tables[c][i] = (uint8_t)(v > 255 ? 255 : v);
+        }
+      }
+    }
+  }
+  f->num_scans = st.nscans ? st.nscans : 1;
   f->error_line = st.error_line;
   return st.error;
 }

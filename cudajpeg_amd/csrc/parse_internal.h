@@ -8,6 +8,16 @@
 
 namespace cjd {
 
+// A one-component scan (CJD_EXT_NONINTERLEAVED) and the state in effect at
+// its SOS.
+struct ScanInfo {
+  int comp = 0, dc_id = 0, ac_id = 2, restart_interval = 0;
+  uint64_t offset = 0, length = 0;  // its stuffed bytes: file[offset, offset + length)
+  uint8_t huff_bits[4][16];
+  uint8_t huff_vals[4][256];
+  uint16_t dq[64];                  // the component's quant table when the scan began (zigzag)
+};
+
 struct ParseState {
   const uint8_t* buf = nullptr;  // start of the file
   const uint8_t* pos = nullptr;  // current marker position
@@ -18,6 +28,8 @@ struct ParseState {
   bool have_sof = false;
   bool at_scan = false;
   int ext = 0;  // CJD_EXT_* accepted
+  int nscans = 0;  // one-component scans parsed (CJD_EXT_NONINTERLEAVED)
+  ScanInfo scans[3];
 };
 
 void parse_sof(ParseState& st);
@@ -25,6 +37,9 @@ void parse_dht(ParseState& st);
 void parse_dri(ParseState& st);
 void parse_dqt(ParseState& st);
 void parse_sos(ParseState& st);
+// CJD_EXT_NONINTERLEAVED: the SOS of a one-component scan; leaves st.pos at
+// the marker after the scan's data.
+void parse_sos_one(ParseState& st);
 int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame_info* f, int ext = 0);
 
 }  // namespace cjd

@@ -153,12 +153,30 @@ bool unstuff_segments(const uint8_t* src, size_t len, bool split, std::vector<si
 
 namespace {
 
+// One scan of an image: the interleaved scan, or (CJD_EXT_NONINTERLEAVED) a
+// one-component scan with the tables and restart interval in effect at it.
+struct ScanWork {
+  int comp = -1;                  // -1: the interleaved scan; else the scan's component
+  uint64_t off = 0, len = 0;      // its stuffed bytes in the file
+  int ri = 0;                     // restart interval (MCUs of the scan: blocks for one component)
+  int64_t units = 0;              // MCUs of the scan
+  int h = 1, v = 1, bw = 0, slot0 = 0, dc_id = 0, ac_id = 2;  // one-component scan geometry and tables
+  std::vector<size_t> bounds;     // unstuffed segment bounds
+  std::vector<UnstuffChunk> chunks;  // device unstuffing: the scan's chunks (scan-relative)
+  int64_t nseg = 0, seg_rel = 0, raw_rel = 0, chunk_rel = 0;  // offsets within the image's share
+  const uint8_t (*bits)[16] = nullptr;  // its Huffman tables (BITS, HUFFVAL by table id)
+  const uint8_t (*vals)[256] = nullptr;
+};
+
 // Sizes of one image's share of the plan (pass 1) and its offsets in the
 // batch (pass 2).
 struct ImageWork {
-  std::vector<size_t> bounds;    // unstuffed segment bounds
-  std::vector<UnstuffChunk> chunks;  // device unstuffing: the scan's chunks (image-relative)
+  std::vector<ScanWork> scans;
+  ParseState st;                  // the parse (the tables of one-component scans)
+  std::vector<size_t> bounds;     // unstuffed segment bounds of all scans, scan after scan
+  int64_t nraw = 0, nchunks = 0;  // device unstuffing: the image's raw bytes and chunks
   int64_t raw0 = 0, chunk0 = 0;  // device unstuffing: offsets in the raw buffer and the chunk list
+  int64_t tset0 = 0;             // first Huffman table set (one per scan)
   uint32_t S = 4096, C = 512, items = kWG;  // entropy parameters for this image
   int64_t nseg = 0, nitems = 0, npix = 0, nwords = 0, nent = 0;
   int64_t seg0 = 0, item0 = 0, pix0 = 0, word0 = 0, desc = 0, coef0 = 0, ent0 = 0;
@@ -204,31 +222,77 @@ inline void for_items(uint32_t nt, uint32_t per_first, uint32_t per_next, F&& f)
 
 // Pass 1 (per image, in parallel): parse, find the segments, count.
 int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, size_t size) {
-  ParseState st;
+  ParseState& st = w.st;
   int e = parse_until_scan(st, file, size, &r.f, b.ext);
   if (!e && (!st.at_scan || !st.have_sof)) e = CJD_SYNTAX_ERROR;
   if (e) return e;
   const cjd_frame_info& f = r.f;
   if (f.blocks_per_mcu > kMaxSlots) return CJD_UNSUPPORTED_ERROR;
-  if (!unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, &w.bounds, nullptr,
-                        nullptr, b.unstuff_device ? &w.chunks : nullptr))
-    return CJD_SYNTAX_ERROR;
-  const int64_t nmcu = (int64_t)f.mcus_x * f.mcus_y;
-  int64_t nseg_expected = 1;
-  if (f.restart_interval > 0) nseg_expected = (nmcu + f.restart_interval - 1) / f.restart_interval;
-  std::vector<size_t>& bd = w.bounds;
-  // a trailing RSTn after the last interval leaves an empty extra segment
-  while ((int64_t)bd.size() - 1 > nseg_expected && bd[bd.size() - 1] == bd[bd.size() - 2]) bd.pop_back();
-  if ((int64_t)bd.size() - 1 != nseg_expected) return CJD_SYNTAX_ERROR;
-  w.nseg = nseg_expected;
+  w.scans.clear();
+  if (!st.nscans) {
+    ScanWork sw;
+    sw.off = f.scan_offset;
+    sw.len = f.scan_length;
+    sw.ri = f.restart_interval;
+    sw.units = (int64_t)f.mcus_x * f.mcus_y;
+    sw.bits = f.huff_bits;
+    sw.vals = f.huff_vals;
+    w.scans.push_back(std::move(sw));
+  } else {
+    // one-component scans (T.81 A.2.2): the component's own block grid
+    for (int k = 0; k < st.nscans; ++k) {
+      const ScanInfo& sc = st.scans[k];
+      ScanWork sw;
+      sw.comp = sc.comp;
+      sw.off = sc.offset;
+      sw.len = sc.length;
+      sw.ri = sc.restart_interval;
+      sw.h = f.samples_x[sc.comp];
+      sw.v = f.samples_y[sc.comp];
+      sw.bw = (f.comp_width[sc.comp] + 7) / 8;
+      sw.units = (int64_t)sw.bw * ((f.comp_height[sc.comp] + 7) / 8);
+      for (int c = 0; c < sc.comp; ++c) sw.slot0 += f.samples_x[c] * f.samples_y[c];
+      sw.dc_id = sc.dc_id;
+      sw.ac_id = sc.ac_id;
+      sw.bits = sc.huff_bits;
+      sw.vals = sc.huff_vals;
+      w.scans.push_back(std::move(sw));
+    }
+  }
+  w.bounds.assign(1, 0);
+  w.nseg = w.nraw = w.nchunks = 0;
+  uint64_t scan_bytes = 0;
+  int64_t coded = 0;
+  for (ScanWork& sw : w.scans) {
+    if (!unstuff_segments(file + sw.off, (size_t)sw.len, sw.ri > 0, &sw.bounds, nullptr, nullptr,
+                          b.unstuff_device ? &sw.chunks : nullptr))
+      return CJD_SYNTAX_ERROR;
+    int64_t nseg_expected = 1;
+    if (sw.ri > 0) nseg_expected = (sw.units + sw.ri - 1) / sw.ri;
+    std::vector<size_t>& bd = sw.bounds;
+    // a trailing RSTn after the last interval leaves an empty extra segment
+    while ((int64_t)bd.size() - 1 > nseg_expected && bd[bd.size() - 1] == bd[bd.size() - 2]) bd.pop_back();
+    if ((int64_t)bd.size() - 1 != nseg_expected) return CJD_SYNTAX_ERROR;
+    sw.nseg = nseg_expected;
+    sw.seg_rel = w.nseg;
+    sw.raw_rel = w.nraw;
+    sw.chunk_rel = w.nchunks;
+    w.nseg += sw.nseg;
+    w.nraw += ((int64_t)sw.len + 15) & ~(int64_t)15;
+    w.nchunks += (int64_t)sw.chunks.size();
+    const size_t base = w.bounds.back();
+    for (size_t k = 1; k < bd.size(); ++k) w.bounds.push_back(base + bd[k]);
+    scan_bytes += sw.len;
+    coded += sw.comp < 0 ? f.num_blocks : sw.units;
+  }
   w.nwords = 0;
   w.nitems = 0;
   w.S = b.tile_bits;
   w.C = b.ck_bits;
   w.items = b.item_tiles;
-  const uint64_t bpb = f.num_blocks ? (uint64_t)f.scan_length * 8 / (uint64_t)f.num_blocks : 0;
+  const uint64_t bpb = coded ? scan_bytes * 8 / (uint64_t)coded : 0;
   for (const TuneEntry& t : b.tune)
-    if ((uint64_t)f.scan_length <= t.max_scan_bytes && (!t.max_bpb || bpb <= t.max_bpb)) {
+    if (scan_bytes <= t.max_scan_bytes && (!t.max_bpb || bpb <= t.max_bpb)) {
       w.S = t.tile_bits;
       w.C = t.ck_bits;
       w.items = t.item_tiles;
@@ -238,7 +302,7 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
   const uint32_t per_first = w.items, per_next = w.items - 1;
   const TileGeom g = tile_geom(w.S, w.C);
   for (int64_t s = 0; s < w.nseg; ++s) {
-    const size_t dlen = bd[s + 1] - bd[s];
+    const size_t dlen = w.bounds[s + 1] - w.bounds[s];
     w.nwords += seg_words(dlen);
     for_items(ntiles_of(dlen, w.S), per_first, per_next, [&](uint32_t t0, uint32_t n) {
       ++w.nitems;
@@ -254,7 +318,6 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
 // work items at the offsets of pass 2.
 void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint8_t* file) {
   const cjd_frame_info& f = r.f;
-  const int64_t nmcu = (int64_t)f.mcus_x * f.mcus_y;
   const int nh = f.blocks_per_mcu;
   ImageDesc d{};
   d.width = f.width;
@@ -265,7 +328,7 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
   d.nslots = f.blocks_per_mcu;
   d.mcu_w = f.mcu_width;
   d.mcu_h = f.mcu_height;
-  d.table_base = (int32_t)(w.desc * 4);
+  d.table_base = (int32_t)(w.tset0 * 4);
   d.qtab_base = (int32_t)(w.desc * 4);
   int slot = 0;
   for (int c = 0; c < f.num_channels; ++c) {
@@ -290,14 +353,19 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
     if (t == d.ntab) d.tab_dq[d.ntab++] = d.comp_dq[c];
     d.comp_tix[c] = (uint8_t)t;
   }
-  {
-    uint16_t* lut = &b->luts[(size_t)w.desc * kLutEntries];
+  // a Huffman table set per scan: the four tables in effect at it
+  for (size_t k = 0; k < w.scans.size(); ++k) {
+    const ScanWork& sw = w.scans[k];
+    const size_t ts = (size_t)w.tset0 + k;
+    uint16_t* lut = &b->luts[ts * kLutEntries];
     int pool = 0;
-    for (int t = 0; t < 4; ++t) build_lut(f.huff_bits[t], f.huff_vals[t], t >= 2, lut, lut + t * kL1Size, pool);
+    for (int t = 0; t < 4; ++t) build_lut(sw.bits[t], sw.vals[t], t >= 2, lut, lut + t * kL1Size, pool);
     for (int e = 4 * kL1Size + pool; e < kLutEntries; ++e) lut[e] = 0;
+    for (int t = 0; t < 4; ++t) build_table(sw.bits[t], sw.vals[t], b->tables[ts * 4 + t]);
   }
+  // quant tables by id (one-component scans: each component's as latched at
+  // its scan, under id c -- see parse_until_scan)
   for (int t = 0; t < 4; ++t) {
-    build_table(f.huff_bits[t], f.huff_vals[t], b->tables[(size_t)(w.desc * 4 + t)]);
     uint16_t* q = &b->qtabs[(size_t)(w.desc * 4 + t) * 64];
     for (int k = 0; k < 64; ++k) q[kZZ[k]] = f.dq_tables16[t][k];
   }
@@ -320,17 +388,19 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
     dst_off[(size_t)s] = (size_t)wo * 4;
     wo += nw;
   }
-  if (b->unstuff_device) {
-    memcpy(b->rptr + w.raw0, file + f.scan_offset, (size_t)f.scan_length);
-    for (size_t k = 0; k < w.chunks.size(); ++k) {
-      UnstuffChunk c = w.chunks[k];
-      c.src += (uint64_t)w.raw0;
-      c.seg += (uint32_t)w.seg0;
-      b->chunks[(size_t)w.chunk0 + k] = c;
+  for (const ScanWork& sw : w.scans) {
+    if (b->unstuff_device) {
+      memcpy(b->rptr + w.raw0 + sw.raw_rel, file + sw.off, (size_t)sw.len);
+      for (size_t k = 0; k < sw.chunks.size(); ++k) {
+        UnstuffChunk c = sw.chunks[k];
+        c.src += (uint64_t)(w.raw0 + sw.raw_rel);
+        c.seg += (uint32_t)(w.seg0 + sw.seg_rel);
+        b->chunks[(size_t)(w.chunk0 + sw.chunk_rel) + k] = c;
+      }
+    } else {
+      unstuff_segments(file + sw.off, (size_t)sw.len, sw.ri > 0, nullptr, reinterpret_cast<uint8_t*>(words),
+                       dst_off.data() + sw.seg_rel, nullptr);
     }
-  } else {
-    unstuff_segments(file + f.scan_offset, (size_t)f.scan_length, f.restart_interval > 0, nullptr,
-                     reinterpret_cast<uint8_t*>(words), dst_off.data(), nullptr);
   }
 
   // segments, tiles and entropy work items: one lane per tile; an item after
@@ -339,7 +409,10 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
   const TileGeom g = tile_geom(w.S, w.C);
   int64_t item = w.item0;
   int64_t eo = w.ent0;
+  size_t k = 0;
   for (int64_t s = 0; s < w.nseg; ++s) {
+    while (s >= w.scans[k].seg_rel + w.scans[k].nseg) ++k;
+    const ScanWork& sw = w.scans[k];
     const size_t dlen = w.bounds[s + 1] - w.bounds[s];
     SegmentDesc sd{};
     sd.word_base = dst_off[(size_t)s] / 4;
@@ -352,10 +425,26 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
     sd.rs = g.rs;
     sd.rd = g.rd;
     sd.ntiles = ntiles_of(dlen, w.S);
-    const int64_t mcu0 = s * (f.restart_interval > 0 ? f.restart_interval : nmcu);
-    const int64_t mcus = std::min<int64_t>(nmcu - mcu0, f.restart_interval > 0 ? f.restart_interval : nmcu);
-    sd.nblocks = (uint32_t)(mcus * nh);
-    sd.first_block = d.coef_base + mcu0 * nh;
+    // the segment's MCUs of its scan (one block each in a one-component scan)
+    const int64_t per = sw.ri > 0 ? sw.ri : sw.units;
+    const int64_t u0 = (s - sw.seg_rel) * per;
+    const int64_t units = std::min<int64_t>(sw.units - u0, per);
+    sd.tset = (uint32_t)(w.tset0 + (int64_t)k);
+    if (sw.comp < 0) {
+      sd.nblocks = (uint32_t)(units * nh);
+      sd.first_block = d.coef_base + u0 * nh;
+    } else {
+      sd.ni = 1;
+      sd.comp = (uint8_t)sw.comp;
+      sd.h = (uint8_t)sw.h;
+      sd.v = (uint8_t)sw.v;
+      sd.dc_id = (uint8_t)sw.dc_id;
+      sd.ac_id = (uint8_t)sw.ac_id;
+      sd.slot0 = (uint8_t)sw.slot0;
+      sd.bw = (uint32_t)sw.bw;
+      sd.nblocks = (uint32_t)units;
+      sd.first_block = u0;
+    }
     sd.first_item = (uint32_t)item;
     uint32_t prev = ~0u;
     for_items(sd.ntiles, per_first, per_next, [&](uint32_t t0, uint32_t n) {
@@ -466,6 +555,7 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
   });
   // pass 2: offsets, in image order
   int64_t nimg = 0, nseg = 0, nitems = 0, npix = 0, nwords = 0, nblocks = 0, nent = 0, nraw = 0, nchunks = 0;
+  int64_t ntset = 0;
   b.total_pixels = b.scan_bytes = 0;
   b.out_bytes = 0;
   for (int i = 0; i < n; ++i) {
@@ -483,9 +573,11 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     w.ent0 = nent;
     nent += w.nent;
     w.raw0 = nraw;
-    nraw += (f.scan_length + 15) & ~(int64_t)15;
+    nraw += w.nraw;
     w.chunk0 = nchunks;
-    nchunks += (int64_t)w.chunks.size();
+    nchunks += w.nchunks;
+    w.tset0 = ntset;
+    ntset += (int64_t)w.scans.size();
     nseg += w.nseg;
     nitems += w.nitems;
     npix += w.npix;
@@ -519,8 +611,8 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
   b.segs.resize((size_t)nseg);
   b.items.resize((size_t)nitems);
   b.pix.resize((size_t)npix);
-  b.tables.resize((size_t)nimg * 4);
-  b.luts.resize((size_t)nimg * kLutEntries);
+  b.tables.resize((size_t)ntset * 4);
+  b.luts.resize((size_t)ntset * kLutEntries);
   b.qtabs.resize((size_t)nimg * 4 * 64);
   b.nwords = (size_t)nwords;
   b.chunks.resize((size_t)nchunks);

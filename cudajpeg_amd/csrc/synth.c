@@ -358,10 +358,51 @@ static void put_header(bw_t *w, const synth_params *p, uint16_t q[3][64]) {
   if (p->restart_interval > 0) {
     raw(w, 0xFF); raw(w, 0xDD); raw16(w, 4); raw16(w, p->restart_interval);
   }
+  if (p->noninterleaved && C == 3) return;  /* the scans' headers come with the scans */
   /* SOS */
   raw(w, 0xFF); raw(w, 0xDA); raw16(w, 6 + 2 * C); raw(w, (uint8_t)C);
   for (int c = 0; c < C; c++) { raw(w, (uint8_t)(c + 1)); raw(w, c == 0 ? 0x00 : 0x11); }
   raw(w, 0); raw(w, 63); raw(w, 0);
+}
+
+/* One block: forward DCT, quantisation, DC clamp to an encodable difference,
+ * Huffman coding with (dc, ac); returns the block's quantised coefficients in
+ * qc and updates the predictor. */
+static void code_block(bw_t *w, const uint8_t *plane, int pw, int x0, int y0, const uint16_t qnat[64],
+                       const ehuff_t *dc, const ehuff_t *ac, int *pred, int16_t qc[64]) {
+  float blk[64];
+  for (int y = 0; y < 8; y++)
+    for (int x = 0; x < 8; x++) blk[y * 8 + x] = (float)plane[(size_t)(y0 + y) * pw + x0 + x] - 128.0f;
+  fdct_quant(blk, qnat, qc);
+  int dcd = qc[0] - *pred;
+  if (dcd > 2047 || dcd < -2047) { /* keep the DC difference encodable */
+    qc[0] = (int16_t)(*pred + (dcd > 0 ? 2047 : -2047));
+    dcd = qc[0] - *pred;
+  }
+  *pred = qc[0];
+  int s = nbits_of(dcd);
+  bw_put(w, dc->code[s], dc->len[s]);
+  bw_put(w, (uint32_t)(dcd < 0 ? dcd - 1 : dcd), s);
+  int run = 0;
+  for (int k = 1; k < 64; k++) {
+    int v = qc[k_zz[k]];
+    if (!v) { run++; continue; }
+    while (run > 15) { bw_put(w, ac->code[0xF0], ac->len[0xF0]); run -= 16; }
+    int sz = nbits_of(v);
+    int sym = (run << 4) | sz;
+    bw_put(w, ac->code[sym], ac->len[sym]);
+    bw_put(w, (uint32_t)(v < 0 ? v - 1 : v), sz);
+    run = 0;
+  }
+  if (run) bw_put(w, ac->code[0], ac->len[0]);
+}
+
+/* RSTn between two restart intervals: byte-align with 1 bits, the marker. */
+static void put_rst(bw_t *w, int *rst_idx) {
+  bw_flush_ones(w);
+  raw(w, 0xFF); raw(w, (uint8_t)(0xD0 | *rst_idx));
+  w->raw_bits += 16; /* the unstuffed stream keeps RSTn as two bytes */
+  *rst_idx = (*rst_idx + 1) & 7;
 }
 
 static inline uint8_t clampu8(double v) {
@@ -443,56 +484,90 @@ long synth_encode(const synth_params *p, const uint8_t *pix, uint8_t *out, long 
   bw_t w = {out, cap, 0, 0, 0, 0, 0};
   put_header(&w, p, q);
   int pred[3] = {0, 0, 0};
-  long blk_index = 0;
-  int rst_count = 0, rst_idx = 0;
+  int rst_idx = 0;
   const long n_mcu = (long)nbx * nby;
-  for (long m = 0; m < n_mcu; m++) {
-    const int mx = (int)(m % nbx), my = (int)(m / nbx);
-    for (int c = 0; c < C; c++) {
-      const int t = c == 0 ? 0 : 1, tq = comp_qtab(p, c);
-      for (int sy = 0; sy < vs[c]; sy++)
-        for (int sx = 0; sx < hs[c]; sx++) {
-          float blk[64];
-          const int x0 = (mx * hs[c] + sx) * 8, y0 = (my * vs[c] + sy) * 8;
-          for (int y = 0; y < 8; y++)
-            for (int x = 0; x < 8; x++)
-              blk[y * 8 + x] = (float)plane[c][(size_t)(y0 + y) * pw[c] + x0 + x] - 128.0f;
-          int16_t qc[64];
-          fdct_quant(blk, q[tq], qc);
-          int dcd = qc[0] - pred[c];
-          if (dcd > 2047 || dcd < -2047) { /* keep the DC difference encodable */
-            qc[0] = (int16_t)(pred[c] + (dcd > 0 ? 2047 : -2047));
-            dcd = qc[0] - pred[c];
-          }
-          if (coefs) memcpy(coefs + blk_index * 64, qc, sizeof(qc));
-          if (block_bits) block_bits[blk_index] = (uint32_t)w.raw_bits;
-          blk_index++;
-          pred[c] = qc[0];
-          int s = nbits_of(dcd);
-          bw_put(&w, dc_t[t].code[s], dc_t[t].len[s]);
-          bw_put(&w, (uint32_t)(dcd < 0 ? dcd - 1 : dcd), s);
-          int run = 0;
-          for (int k = 1; k < 64; k++) {
-            int v = qc[k_zz[k]];
-            if (!v) { run++; continue; }
-            while (run > 15) { bw_put(&w, ac_t[t].code[0xF0], ac_t[t].len[0xF0]); run -= 16; }
-            int sz = nbits_of(v);
-            int sym = (run << 4) | sz;
-            bw_put(&w, ac_t[t].code[sym], ac_t[t].len[sym]);
-            bw_put(&w, (uint32_t)(v < 0 ? v - 1 : v), sz);
-            run = 0;
-          }
-          if (run) bw_put(&w, ac_t[t].code[0], ac_t[t].len[0]);
+  if (p->noninterleaved && C == 3) {
+    /* one scan per component, blocks in raster order over the component's
+       own grid: ceil(comp_width / 8) x ceil(comp_height / 8) blocks, fewer
+       than the MCU layout where MCUs pad it (T.81 A.2.2) */
+    const int bpm = hs[0] * vs[0] + 2;
+    if (coefs) memset(coefs, 0, sizeof(int16_t) * 64 * (size_t)n_mcu * bpm);
+    int slot0 = 0;
+    for (int c = 0; c < 3; c++) {
+      if (c == 1) {
+        /* table 0 becomes the chroma tables for the chroma scans */
+        const uint8_t *cb[2] = {k_dc_chr_bits, p->huff == 1 ? k_ac_long_bits_chr : k_ac_chr_bits};
+        const uint8_t *cv[2] = {k_dc_vals, k_ac_chr_vals};
+        for (int t = 0; t < 2; t++) {
+          int n = 0;
+          for (int i = 0; i < 16; i++) n += cb[t][i];
+          raw(&w, 0xFF); raw(&w, 0xC4); raw16(&w, 2 + 1 + 16 + n); raw(&w, (uint8_t)(t << 4));
+          for (int i = 0; i < 16; i++) raw(&w, cb[t][i]);
+          for (int i = 0; i < n; i++) raw(&w, cv[t][i]);
         }
-    }
-    if (p->restart_interval > 0 && ++rst_count == p->restart_interval && m + 1 < n_mcu) {
+        if (p->restart_interval > 0) {
+          raw(&w, 0xFF); raw(&w, 0xDD); raw16(&w, 4); raw16(&w, p->restart_interval + 1);
+        }
+      }
+      /* mode 2: table 1 redefined before the Cr scan; Cb keeps the table it had */
+      const int redefine = c == 2 && p->noninterleaved == 2 && p->qtables == 2;
+      if (redefine) {
+        if (p->dqt16) {
+          raw(&w, 0xFF); raw(&w, 0xDB); raw16(&w, 131); raw(&w, 0x11);
+          for (int k = 0; k < 64; k++) raw16(&w, q[2][k_zz[k]]);
+        } else {
+          raw(&w, 0xFF); raw(&w, 0xDB); raw16(&w, 67); raw(&w, 1);
+          for (int k = 0; k < 64; k++) raw(&w, (uint8_t)q[2][k_zz[k]]);
+        }
+      }
+      raw(&w, 0xFF); raw(&w, 0xDA); raw16(&w, 8); raw(&w, 1);
+      raw(&w, (uint8_t)(c + 1)); raw(&w, 0x00);
+      raw(&w, 0); raw(&w, 63); raw(&w, 0);
+      const int ri = p->restart_interval > 0 ? p->restart_interval + (c > 0 ? 1 : 0) : 0;
+      const int t = c == 0 ? 0 : 1, tq = redefine ? 2 : comp_qtab(p, c);
+      const int cw = (W * hs[c] + hmax - 1) / hmax, chh = (H * vs[c] + vmax - 1) / vmax;
+      const int bw = (cw + 7) / 8, bh = (chh + 7) / 8;
+      int pr = 0, count = 0;
+      rst_idx = 0;
+      for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+          int16_t qc[64];
+          code_block(&w, plane[c], pw[c], bx * 8, by * 8, q[tq], &dc_t[t], &ac_t[t], &pr, qc);
+          const long mcu = (long)(by / vs[c]) * nbx + bx / hs[c];
+          const long idx = mcu * bpm + slot0 + (by % vs[c]) * hs[c] + bx % hs[c];
+          if (coefs) memcpy(coefs + idx * 64, qc, sizeof(qc));
+          if (block_bits) block_bits[idx] = (uint32_t)w.raw_bits;
+          if (ri > 0 && ++count == ri && (long)by * bw + bx + 1 < (long)bw * bh) {
+            put_rst(&w, &rst_idx);
+            count = 0;
+            pr = 0;
+          }
+        }
       bw_flush_ones(&w);
-      raw(&w, 0xFF); raw(&w, (uint8_t)(0xD0 | rst_idx));
-      /* the unstuffed stream keeps RSTn as two bytes */
-      w.raw_bits += 16;
-      rst_idx = (rst_idx + 1) & 7;
-      rst_count = 0;
-      pred[0] = pred[1] = pred[2] = 0;
+      slot0 += hs[c] * vs[c];
+    }
+  } else {
+    long blk_index = 0;
+    int rst_count = 0;
+    for (long m = 0; m < n_mcu; m++) {
+      const int mx = (int)(m % nbx), my = (int)(m / nbx);
+      for (int c = 0; c < C; c++) {
+        const int t = c == 0 ? 0 : 1, tq = comp_qtab(p, c);
+        for (int sy = 0; sy < vs[c]; sy++)
+          for (int sx = 0; sx < hs[c]; sx++) {
+            if (block_bits) block_bits[blk_index] = (uint32_t)w.raw_bits;
+            int16_t qc[64];
+            code_block(&w, plane[c], pw[c], (mx * hs[c] + sx) * 8, (my * vs[c] + sy) * 8, q[tq], &dc_t[t],
+                       &ac_t[t], &pred[c], qc);
+            if (coefs) memcpy(coefs + blk_index * 64, qc, sizeof(qc));
+            blk_index++;
+          }
+      }
+      if (p->restart_interval > 0 && ++rst_count == p->restart_interval && m + 1 < n_mcu) {
+        put_rst(&w, &rst_idx);
+        rst_count = 0;
+        pred[0] = pred[1] = pred[2] = 0;
+      }
     }
   }
   bw_flush_ones(&w);

@@ -35,6 +35,7 @@ class _Params(ctypes.Structure):
         ("qtables", ctypes.c_int32),
         ("huff", ctypes.c_int32),
         ("dqt16", ctypes.c_int32),
+        ("noninterleaved", ctypes.c_int32),
     ]
 
 
@@ -79,12 +80,13 @@ class ImageSpec:
     qtables: int = 2               # YCbCr: 2 (Y | CbCr) or 3 (Y | Cb | Cr) quant tables
     huff: int = 0                  # AC Huffman tables: 0 Annex K, 1 long codes (60 of 11 bits)
     dqt16: int = 0                 # 1: 16-bit quant tables (Pq = 1), entries up to 65535
+    noninterleaved: int = 0        # 1 (YCbCr): one scan per component (table 0 redefined between scans)
 
     def _params(self) -> _Params:
         ncomp, h, v = SAMPLING[self.sampling]
         return _Params(self.seed, self.width, self.height, ncomp, h, v, self.quality,
                        self.restart_interval, self.sigma, self.amp_total, self.random_tile_frac,
-                       self.qtables, self.huff, self.dqt16)
+                       self.qtables, self.huff, self.dqt16, self.noninterleaved)
 
     @property
     def ncomp(self) -> int:
@@ -99,8 +101,8 @@ class ImageSpec:
 class Encoded:
     spec: ImageSpec
     data: bytes
-    coefs: np.ndarray | None = None        # int16 [nblocks, 64], natural order, scan order
-    block_bits: np.ndarray | None = None   # uint32 [nblocks]
+    coefs: np.ndarray | None = None        # int16 [nblocks, 64], natural order, MCU order
+    block_bits: np.ndarray | None = None   # uint32 [nblocks] (interleaved scans)
     rgb: np.ndarray | None = None          # source pixels (before encoding)
 
 

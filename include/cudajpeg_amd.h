@@ -38,7 +38,8 @@ extern "C" {
 
 /* what cjd_batch_download copies */
 #define CJD_OUT_PIXELS 0        /* interleaved RGB (W*H*3) or gray (W*H) */
-#define CJD_OUT_COEFFICIENTS 1  /* int16 [num_blocks][64], quantised, natural order, scan order */
+#define CJD_OUT_COEFFICIENTS 1  /* int16 [num_blocks][64], quantised, natural order, scan order
+                                   (non-interleaved files: the interleaved scan's order) */
 #define CJD_OUT_PLANE0 2        /* planar component c = CJD_OUT_PLANE0 + c, stride x plane height */
 
 /* Frame description produced by the host parser: what decodeSOF / decodeDHT /
@@ -62,7 +63,8 @@ typedef struct {
   uint64_t scan_offset;                 /* entropy-coded segment: file[scan_offset, +scan_length) */
   uint64_t scan_length;
   int32_t error_line;                   /* source line of the failed check, 0 if none */
-  int32_t pad_;
+  int32_t num_scans;                    /* 1, or with CJD_EXT_NONINTERLEAVED the one-component scans
+                                           (scan_offset / scan_length: the first) */
   uint16_t dq_tables16[4][64];          /* the same tables with 16-bit entries (CJD_EXT_DQT16), zigzag */
 } cjd_frame_info;
 
@@ -73,6 +75,16 @@ typedef struct {
  *                  then holds the entries clamped to 255, dq_tables16 the
  *                  values. */
 #define CJD_EXT_DQT16 1
+/*   CJD_EXT_NONINTERLEAVED  non-interleaved baseline scans: one scan per
+ *                  component (T.81 A.2.2-A.2.3), which src/decodeScanGPU.cu
+ *                  rejects (:124-131).  Scans may be separated by DHT, DQT
+ *                  and DRI; each uses the Huffman tables and restart
+ *                  interval in effect at its SOS, and each component the
+ *                  quant table in effect when its scan begins.  Blocks and
+ *                  coefficients are laid out as the interleaved scan would
+ *                  order them, with zero blocks where MCUs pad a component's
+ *                  own block grid (outside the image). */
+#define CJD_EXT_NONINTERLEAVED 2
 
 typedef struct cjd_context cjd_context;
 typedef struct cjd_batch cjd_batch;

@@ -40,6 +40,12 @@ class _Channel(ctypes.Structure):
                  "width", "height", "stride", "plane_height")]
 
 
+class _Scan(ctypes.Structure):
+    _fields_ = [("comp", ctypes.c_int), ("dc_id", ctypes.c_int), ("ac_id", ctypes.c_int),
+                ("restart_interval", ctypes.c_int), ("scan", ctypes.c_void_p), ("scan_len", ctypes.c_size_t),
+                ("bits", (ctypes.c_uint8 * 16) * 4), ("vals", (ctypes.c_uint8 * 256) * 4)]
+
+
 class _Jpeg(ctypes.Structure):
     _fields_ = [
         ("error", ctypes.c_int), ("error_line", ctypes.c_int),
@@ -55,6 +61,7 @@ class _Jpeg(ctypes.Structure):
         ("dht_present", ctypes.c_int * 4),
         ("scan", ctypes.c_void_p), ("scan_len", ctypes.c_size_t), ("have_scan", ctypes.c_int),
         ("ext", ctypes.c_int), ("dq16", (ctypes.c_uint16 * 64) * 4),
+        ("num_scans", ctypes.c_int), ("scans", _Scan * 3), ("comp_dq16", (ctypes.c_uint16 * 64) * 3),
     ]
 
 
@@ -113,6 +120,7 @@ class Parsed:
 
 
 EXT_DQT16 = 1  # ORC_EXT_DQT16: 16-bit quantisation tables (beyond the reference)
+EXT_NONINTERLEAVED = 2  # ORC_EXT_NONINTERLEAVED: one scan per component (beyond the reference)
 
 
 def parse(data: bytes, ext: int = 0) -> Parsed:

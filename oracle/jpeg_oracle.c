@@ -169,6 +169,40 @@ static int parse_sos(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const ui
   return 0;
 }
 
+/* ORC_EXT_NONINTERLEAVED: SOS of a one-component scan (T.81 B.2.3 with
+ * Ns = 1), which src/decodeScanGPU.cu:124-131 rejects.  Records the scan
+ * with the tables and restart interval in effect, latches the component's
+ * quant table, and finds the scan's end: the first 0xFF followed by neither
+ * 0x00 nor RSTn (T.81 B.1.1.2 and B.1.1.5), at the latest the EOI. */
+static int parse_sos_ni(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const uint8_t **next) {
+  unsigned header_len = read16(p);
+  if (header_len != 8 || p + header_len > end) FAIL(ORC_SYNTAX_ERROR);
+  const uint8_t *pos = p + 3;
+  int c = 0;
+  while (c < j->num_channels && j->ch[c].id != pos[0]) c++;
+  if (c == j->num_channels) FAIL(ORC_SYNTAX_ERROR);
+  for (int k = 0; k < j->num_scans; k++)
+    if (j->scans[k].comp == c) FAIL(ORC_SYNTAX_ERROR); /* each component in one scan */
+  if (j->num_scans >= j->num_channels) FAIL(ORC_SYNTAX_ERROR);
+  if (pos[1] & 0xEE) FAIL(ORC_SYNTAX_ERROR);
+  if (pos[2] || pos[3] != 63 || pos[4]) FAIL(ORC_UNSUPPORTED_ERROR);
+  const int k = j->num_scans++;
+  j->scans[k].comp = c;
+  j->scans[k].dc_id = pos[1] >> 4;
+  j->scans[k].ac_id = (pos[1] & 1) | 2;
+  j->scans[k].restart_interval = j->restart_interval;
+  memcpy(j->scans[k].bits, j->dht_bits, sizeof(j->dht_bits));
+  memcpy(j->scans[k].vals, j->dht_vals, sizeof(j->dht_vals));
+  memcpy(j->comp_dq16[c], j->dq16[j->ch[c].dq_id], sizeof(j->comp_dq16[c]));
+  const uint8_t *d = p + header_len, *e = d;
+  while (e < end && !(e[0] == 0xFF && e + 1 < end + 2 && e[1] != 0x00 && (e[1] & 0xF8) != 0xD0)) e++;
+  j->scans[k].scan = d;
+  j->scans[k].scan_len = (size_t)(e - d);
+  j->have_scan = 1;
+  *next = e;
+  return 0;
+}
+
 /* src/format.cu:113-215 (file read replaced by an in-memory buffer) */
 int orc_parse(const uint8_t *buf, size_t size, orc_jpeg *j) { return orc_parse_ext(buf, size, 0, j); }
 
@@ -210,6 +244,12 @@ int orc_parse_ext(const uint8_t *buf, size_t size, int ext, orc_jpeg *j) {
       }
       case 0xDA: {
         const uint8_t *after;
+        if ((j->ext & ORC_EXT_NONINTERLEAVED) && j->num_channels > 1 && read16(pos) >= 3 && pos[2] == 1) {
+          e = parse_sos_ni(j, pos, end, &after);
+          if (!e) pos = after;
+          break;
+        }
+        if (j->num_scans) FAIL(ORC_SYNTAX_ERROR); /* an interleaved scan after one-component scans */
         e = parse_sos(j, pos, end, &after);
         if (!e) {
           /* the scan runs to the EOI (src/decodeScanGPU.cu:13 unstuffs up to jpg->end) */
@@ -232,6 +272,7 @@ int orc_parse_ext(const uint8_t *buf, size_t size, int ext, orc_jpeg *j) {
     if (e) return e;
     if (m == 0xD9) break;
   }
+  if (j->num_scans && j->num_scans != j->num_channels) FAIL(ORC_SYNTAX_ERROR); /* a component never coded */
   return 0;
 }
 
@@ -304,9 +345,85 @@ static int get_vlc(reader_t *r, const vlc_t *tab, uint8_t *code) {
   return value;
 }
 
+/* One block (src/decodeScanGPU.cu:78-100): DC difference and AC run/levels
+ * into q (natural order) and, when w, the dequantised values. */
+static int decode_block(reader_t *r, const vlc_t *dct, const vlc_t *act, const uint16_t *dq, int *pred,
+                        int16_t q[64], int32_t *w) {
+  memset(q, 0, 64 * sizeof(int16_t));
+  *pred += get_vlc(r, dct, NULL);
+  q[0] = (int16_t)*pred;
+  if (w) { memset(w, 0, 64 * sizeof(int32_t)); w[0] = *pred * dq[0]; }
+  int coef = 0;
+  uint8_t code = 0;
+  do {
+    int value = get_vlc(r, act, &code);
+    if (!code) break;
+    if (!(code & 0x0F) && code != 0xF0) return ORC_SYNTAX_ERROR;
+    coef += (code >> 4) + 1;
+    if (coef > 63) return ORC_SYNTAX_ERROR;
+    q[kDeZigZag[coef]] = (int16_t)value;
+    if (w) w[kDeZigZag[coef]] = value * dq[coef];
+  } while (coef < 63);
+  return r->error;
+}
+
+/* ORC_EXT_NONINTERLEAVED: the one-component scans in file order.  Scan k
+ * codes component c's own grid of ceil(width_c / 8) x ceil(height_c / 8)
+ * blocks in raster order (T.81 A.2.2), a restart interval counting blocks;
+ * block (bx, by) is block (by % v) h + bx % h of component c in MCU
+ * (bx / h, by / v), and goes where the interleaved scan decodes it. */
+static int decode_scans_ni(orc_jpeg *j, int16_t *qcoef, int32_t *work[3]) {
+  const long bpm = j->blocks_per_mcu;
+  if (qcoef) memset(qcoef, 0, sizeof(int16_t) * 64 * (size_t)j->num_blocks);
+  for (int c = 0; c < j->num_channels; c++)
+    if (work && work[c]) memset(work[c], 0, sizeof(int32_t) * (size_t)j->ch[c].stride * j->ch[c].plane_height);
+  vlc_t *lut = (vlc_t *)malloc(sizeof(vlc_t) * 65536 * 2);
+  if (!lut) FAIL(ORC_OOM_ERROR);
+  int err = 0;
+  for (int k = 0; k < j->num_scans && !err; k++) {
+    const int c = j->scans[k].comp;
+    const orc_channel *ch = &j->ch[c];
+    int slot0 = 0;
+    for (int i = 0; i < c; i++) slot0 += j->ch[i].samples_x * j->ch[i].samples_y;
+    build_lut(j->scans[k].bits[j->scans[k].dc_id], j->scans[k].vals[j->scans[k].dc_id], lut);
+    build_lut(j->scans[k].bits[j->scans[k].ac_id], j->scans[k].vals[j->scans[k].ac_id], lut + 65536);
+    uint8_t *ub = (uint8_t *)malloc(j->scans[k].scan_len + 2);
+    if (!ub) { err = ORC_OOM_ERROR; break; }
+    size_t ulen = orc_unstuff(j->scans[k].scan, j->scans[k].scan_len, ub);
+    reader_t r = {ub, ub, ub + ulen, 0, 0, 0};
+    const int h = ch->samples_x, v = ch->samples_y;
+    const int bw = (ch->width + 7) / 8, bh = (ch->height + 7) / 8;
+    const int ri = j->scans[k].restart_interval;
+    int pred = 0, rcount = ri, next_rst = 0;
+    for (int by = 0; by < bh && !err; by++)
+      for (int bx = 0; bx < bw && !err; bx++) {
+        const long mcu = (long)(by / v) * j->num_blocks_x + bx / h;
+        const long local = (long)(by % v) * h + bx % h;
+        int16_t q[64];
+        int32_t *w = work && work[c] ? work[c] + (mcu * h * v + local) * 64 : NULL;
+        err = decode_block(&r, lut, lut + 65536, j->comp_dq16[c], &pred, q, w);
+        if (!err && qcoef) memcpy(qcoef + (mcu * bpm + slot0 + local) * 64, q, sizeof(q));
+        /* src/decodeScanGPU.cu:476-486, per block */
+        if (!err && ri && !(--rcount) && r.pos < r.end) {
+          r.num_bufbits &= 0xF8;
+          int m = get_bits(&r, 16);
+          if ((m & 0xFFF8) != 0xFFD0 || (m & 7) != next_rst) err = ORC_SYNTAX_ERROR;
+          next_rst = (next_rst + 1) & 7;
+          rcount = ri;
+          pred = 0;
+        }
+      }
+    free(ub);
+  }
+  free(lut);
+  if (err) FAIL(err);
+  return 0;
+}
+
 int orc_decode_scan(const orc_jpeg *jc, int16_t *qcoef, int32_t *work[3], uint32_t *bitpos) {
   orc_jpeg *j = (orc_jpeg *)jc; /* only error fields are written */
   if (!j->have_scan) FAIL(ORC_SYNTAX_ERROR);
+  if (j->num_scans) return decode_scans_ni(j, qcoef, work);
   vlc_t *lut = (vlc_t *)malloc(sizeof(vlc_t) * 65536 * 4);
   uint8_t *ub = (uint8_t *)malloc(j->scan_len + 2);
   if (!lut || !ub) { free(lut); free(ub); FAIL(ORC_OOM_ERROR); }
@@ -488,7 +605,7 @@ int orc_planes(const orc_jpeg *j, const int16_t *qcoef, uint8_t *planes[3], int 
     for (int bx = 0; bx < j->num_blocks_x; bx++)
       for (int c = 0; c < j->num_channels; c++) {
         const orc_channel *ch = &j->ch[c];
-        const uint16_t *dq = j->dq16[ch->dq_id];
+        const uint16_t *dq = j->num_scans ? j->comp_dq16[c] : j->dq16[ch->dq_id];
         for (int sy = 0; sy < ch->samples_y; sy++)
           for (int sx = 0; sx < ch->samples_x; sx++, blk++) {
             int32_t deq[64];

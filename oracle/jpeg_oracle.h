@@ -45,11 +45,28 @@ typedef struct {
   int have_scan;
   int ext;                          /* extensions accepted (ORC_EXT_*), set by orc_parse_ext */
   uint16_t dq16[4][64];             /* the quant tables with 16-bit entries, zigzag order */
+  /* ORC_EXT_NONINTERLEAVED: one scan per component (T.81 A.2.2); 0 for the
+   * single interleaved scan above */
+  int num_scans;
+  struct {
+    int comp, dc_id, ac_id, restart_interval;  /* the scan's component and the state at its SOS */
+    const uint8_t *scan;                       /* stuffed, from after its SOS header to the next marker */
+    size_t scan_len;
+    uint8_t bits[4][16], vals[4][256];         /* Huffman tables in effect at the scan */
+  } scans[3];
+  uint16_t comp_dq16[3][64];        /* each component's quant table when its scan began (zigzag) */
 } orc_jpeg;
 
 /* Extensions beyond the reference: 16-bit quantisation tables (DQT Pq = 1,
  * T.81 B.2.4.1), which src/format.cu:432 rejects. */
 #define ORC_EXT_DQT16 1
+/* Non-interleaved scans (one component per scan, T.81 A.2.2-A.2.3), which
+ * src/decodeScanGPU.cu:131 rejects: the scans may be separated by DHT, DQT
+ * and DRI, and each uses the tables and restart interval in effect at its
+ * SOS; each component's quant table is the one in effect when its scan
+ * begins.  Coefficients come out in the interleaved (MCU) layout, with zero
+ * blocks where MCUs pad a component's own block grid. */
+#define ORC_EXT_NONINTERLEAVED 2
 
 /* Parse markers, SOF, DHT, DQT, DRI and the SOS header of an in-memory file
  * (src/format.cu:113-215, :242-442; src/decodeScanGPU.cu:124-141).
@@ -62,10 +79,13 @@ size_t orc_unstuff(const uint8_t *scan, size_t len, uint8_t *out);
 
 /* Serial scan decode (src/decodeScanGPU.cu:28-100,435-490 ==
  * src/decodeScanCPU.cu:10-152).  All outputs optional:
- *  qcoef  int16 [num_blocks*64]  quantised coefficients, natural order, scan order
+ *  qcoef  int16 [num_blocks*64]  quantised coefficients, natural order, MCU order
  *  work   int32 per component    dequantised working_space in the reference layout
  *         (work[c] has stride*plane_height entries, 64 per block in decode order)
  *  bitpos uint32 [num_blocks]    start bit of each block in the unstuffed segment
+ *                                (interleaved scans)
+ * Non-interleaved scans (num_scans > 0) are decoded one after the other, each
+ * block written where the interleaved scan holds it.
  * Returns the reference error code. */
 int orc_decode_scan(const orc_jpeg *j, int16_t *qcoef, int32_t *work[3], uint32_t *bitpos);
 

@@ -217,12 +217,12 @@ void run_entropy(Emu& E, int order) {
     sh.look_ok = 1;
     EntCtx& c = R.ctx[i];
     c.init(p.wptr, p.images.data(), p.segs.data(), p.tables.data(), E.ebuf.data(), p.items[i]);
-    memcpy(sh.lut, &p.luts[(size_t)c.sg.image * kLutEntries], sizeof(sh.lut));
+    memcpy(sh.lut, &p.luts[(size_t)c.sg.tset * kLutEntries], sizeof(sh.lut));
     memcpy(sh.dz, kDeZigZag, 64);
     for (int k = 0; k < 4; ++k) canon_fill(sh.canon[k], c.tabs[k]);
     for (int k = 0; k < c.nh; ++k) {
-      sh.tsel[2 * k] = (uint16_t)(c.im->slot_ac[k] * kL1Size);
-      sh.tsel[2 * k + 1] = (uint16_t)(c.im->slot_dc[k] * kL1Size);
+      sh.tsel[2 * k] = (uint16_t)(seg_ac(c.sg, *c.im, k) * kL1Size);
+      sh.tsel[2 * k + 1] = (uint16_t)(seg_dc(c.sg, *c.im, k) * kL1Size);
     }
     for (int t = 0; t < kWG; ++t) ent_spec(sh, t, c);
     if (g_plant_ck)  // failure injection: checkpoint words this pass did not write
@@ -368,6 +368,14 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
   E->coefs.assign((size_t)E->plan.total_blocks * 64, (int16_t)0x5555);
   BlockRec junk{0, 0, 0x5555};  // eidx, nw, dc
   E->recs.assign((size_t)E->plan.total_blocks, junk);
+  // as staging does: images with one-component scans start with zero records
+  // (the blocks no scan codes keep them)
+  for (const ImageRec& r : E->plan.recs)
+    if (!r.error && r.f.num_scans > 1) {
+      std::fill(E->recs.begin() + r.coef_base, E->recs.begin() + r.coef_base + r.f.num_blocks, BlockRec{0, 0, 0});
+      std::fill(E->coefs.begin() + r.coef_base * 64, E->coefs.begin() + (r.coef_base + r.f.num_blocks) * 64,
+                (int16_t)0);
+    }
   E->out.assign(E->plan.out_bytes, 0xAB);
   // status words of an older launch with every bit set: this launch must
   // not read them as its own

@@ -105,13 +105,13 @@ void lane_main(Group& g, EntShared& sh, const Launch& X, uint32_t item_id, int t
   if (tid == 0) {
     sh.lovf = 0;
     sh.look_ok = 1;
-    memcpy(sh.lut, &P.luts[(size_t)c.sg.image * kLutEntries], sizeof(sh.lut));
+    memcpy(sh.lut, &P.luts[(size_t)c.sg.tset * kLutEntries], sizeof(sh.lut));
     memcpy(sh.dz, kDeZigZag, 64);
   }
   if (tid < 4) canon_fill(sh.canon[tid], c.tabs[tid]);
   if (tid < c.nh) {
-    sh.tsel[2 * tid] = (uint16_t)(c.im->slot_ac[tid] * kL1Size);
-    sh.tsel[2 * tid + 1] = (uint16_t)(c.im->slot_dc[tid] * kL1Size);
+    sh.tsel[2 * tid] = (uint16_t)(seg_ac(c.sg, *c.im, tid) * kL1Size);
+    sh.tsel[2 * tid + 1] = (uint16_t)(seg_dc(c.sg, *c.im, tid) * kL1Size);
   }
   g.sync();
   ent_spec(sh, tid, c);
