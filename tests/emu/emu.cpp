@@ -245,10 +245,26 @@ void run_entropy(Emu& E, int order) {
   // order (predecessors inclusive) or backwards (predecessors aggregate-only).
   // order 2: backwards with no waiting and the segment-first items last, so
   // every item after a segment's first gives up (a failed lookback)
-  if (order != 2)
-    for (size_t i = 0; i < ni; ++i)
-      if (!R.ctx[i].lead) resolve(R, i);
-  for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
+  // order >= 3: a random order (seeded by `order`), so that walks meet a mix
+  // of inclusive and aggregate-only predecessors, as uneven progress on the
+  // GPU gives them
+  if (order >= 3) {
+    std::vector<size_t> perm(ni);
+    for (size_t k = 0; k < ni; ++k) perm[k] = k;
+    uint64_t x = (uint64_t)order * 0x9E3779B97F4A7C15ull;
+    for (size_t k = ni; k > 1; --k) {
+      x ^= x << 13;
+      x ^= x >> 7;
+      x ^= x << 17;
+      std::swap(perm[k - 1], perm[(size_t)(x % k)]);
+    }
+    for (size_t k = 0; k < ni; ++k) resolve(R, perm[k]);
+  } else {
+    if (order != 2)
+      for (size_t i = 0; i < ni; ++i)
+        if (!R.ctx[i].lead) resolve(R, i);
+    for (size_t k = 0; k < ni; ++k) resolve(R, order ? ni - 1 - k : k);
+  }
   E.fixes = R.fixes;
   if (g_carry_on) g_carry = E.ebuf;
   for (size_t i = 0; i < ni; ++i) {

@@ -101,6 +101,22 @@ def test_emulated_small_items_chain_fixups(item_tiles, tile_bits, ck_bits, order
         assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
 
 
+@pytest.mark.parametrize("seed", [3, 17, 91, 1234])
+def test_emulated_lookback_in_random_orders(seed):
+    """The lookback resolved in a random order of the work items (emu order
+    >= 3, seeded): every walk meets its own mix of inclusive and
+    aggregate-only predecessors, as uneven progress of the workgroups on the
+    GPU gives it; 3-tile items of 512-bit tiles make chains tens of items
+    long.  Coefficients stay exactly the encoder's."""
+    specs = [s for s in synth.config_mixed(24, 777) if s.pixels < 3_000_000][:8]
+    encs = [synth.encode(s, truth=True) for s in specs]
+    r = EmuResult([e.data for e in encs], 0, 512, False, seed, False, 3)
+    assert r.items > 1000
+    for i, (s, e) in enumerate(zip(specs, encs)):
+        assert r.errors[i] == 0, s
+        assert np.array_equal(r.get(i, 1, e.coefs.shape, np.int16), e.coefs), s
+
+
 def test_emulated_corrupt_scan_reports_syntax():
     enc = synth.encode(synth.ImageSpec(64, 64, "420", 75, 6.0, 21))
     data = bytearray(enc.data)
