@@ -433,17 +433,58 @@ CJD_HD inline void pix_planes(const SH& sh, int tid, const PixCtx& c, uint8_t* o
 // adjacent clip-and-shift results into one packed 16-bit operation whose
 // upper half it then assumes to be zero (on gfx950 it is not: the bits come
 // back from the register's previous value).
-CJD_HD inline uint32_t clip_shr8(int v) { return (uint32_t)(v < 0 ? 0 : (v > 65535 ? 65535 : v)) >> 8; }
+// clip8 as a 32-bit value.  The colour transform clips ((Y << 8) + t) >> 8
+// (CPUonly/src/pixelTransformCPU.c:160-165) as clamp255(Y + (t >> 8)): equal
+// for all integers, and no shift follows the clamp (a clamp-then-shift of two
+// adjacent results was fused by the compiler into v_ashr_pk_u8_i32, whose
+// register was then OR-ed into a word as if its upper half were zero).
+CJD_HD inline uint32_t clamp255(int v) { return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v)); }
 
-// Aligned 16-byte store (device: one dwordx4 store; host: memcpy).
-CJD_HD inline void store_u32x4(uint8_t* p, uint32_t a, uint32_t b, uint32_t d, uint32_t e) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  *reinterpret_cast<uint4*>(p) = make_uint4(a, b, d, e);
-#else
+// 16 bytes at a 4-byte-aligned address (device: one dwordx4 store, which
+// needs only dword alignment; host: memcpy).
+CJD_HD inline void store_16_a4(uint8_t* p, uint32_t a, uint32_t b, uint32_t d, uint32_t e) {
   const uint32_t v[4] = {a, b, d, e};
-  memcpy(p, v, 16);
-#endif
+  __builtin_memcpy(reinterpret_cast<uint32_t*>(p), v, 16);
 }
+
+// Bytes [lo, hi) (0 <= lo < hi <= 16) of the 16 bytes o at a 4-byte-aligned
+// address p: the dwords wholly inside as dword stores, the 1-3 bytes at
+// either edge as byte and short stores.  At most nine stores, where a byte
+// loop takes sixteen: a wave issues every store any of its lanes needs.
+CJD_HD inline void store_part16(uint8_t* p, const uint32_t o[4], int lo, int hi) {
+  uint32_t* p4 = reinterpret_cast<uint32_t*>(p);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (4 * q >= lo && 4 * q + 4 <= hi) p4[q] = o[q];
+  // the dword holding lo, when lo is inside one: bytes lo .. min(hi, its end)
+  if (lo & 3) {
+    const int q = lo >> 2, e = hi < 4 * q + 4 ? hi : 4 * q + 4;
+    const uint32_t v = q == 0 ? o[0] : q == 1 ? o[1] : q == 2 ? o[2] : o[3];
+    int k = lo;
+    if (k & 1) {
+      p[k] = (uint8_t)(v >> (8 * (k & 3)));
+      ++k;
+    }
+    if (k + 2 <= e) {
+      *reinterpret_cast<uint16_t*>(p + k) = (uint16_t)(v >> (8 * (k & 3)));
+      k += 2;
+    }
+    if (k < e) p[k] = (uint8_t)(v >> (8 * (k & 3)));
+  }
+  // the dword holding hi, when hi is inside one the head above did not
+  // write: its bytes below hi (lo is at or below its start)
+  if ((hi & 3) && !((lo & 3) && (hi >> 2) == (lo >> 2))) {
+    const int q = hi >> 2;
+    const uint32_t v = q == 0 ? o[0] : q == 1 ? o[1] : q == 2 ? o[2] : o[3];
+    int k = 4 * q;
+    if (k + 2 <= hi) {
+      *reinterpret_cast<uint16_t*>(p + k) = (uint16_t)v;
+      k += 2;
+    }
+    if (k < hi) p[k] = (uint8_t)(v >> (8 * (k & 3)));
+  }
+}
+
 
 // Bytes s .. s+3 of the 8-byte value hi:lo (s = 0..3).
 CJD_HD inline uint32_t align_bytes(uint32_t hi, uint32_t lo, uint32_t s) {
@@ -477,14 +518,16 @@ CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* o
   const int xs2 = im.comp_xshift[2], ys2 = im.comp_yshift[2];
   for (int yy = wave; yy < rows; yy += kPixWaves) {
     const uint64_t a0 = im.out_offset + ((uint64_t)(y0 + yy) * (uint64_t)im.width + (uint64_t)x0) * (uint64_t)bpp;
-    const int mis = (int)(a0 & 15u);
+    // 16-byte chunks on the row's 4-byte grid: only the first and last
+    // chunks are partial, in their first and last dword
+    const int mis = (int)(a0 & 3u);
     const int nch = (mis + n + 15) >> 4;
     uint8_t* base = out + (a0 - (uint64_t)mis);
     const uint8_t* srow0 = &sh.samp[c.toff0 + (yy >> ys0) * c.tw0];
     const uint8_t* srow1 = &sh.samp[c.toff1 + (yy >> ys1) * c.tw1];
     const uint8_t* srow2 = &sh.samp[c.toff2 + (yy >> ys2) * c.tw2];
     for (int w = lane; w < nch; w += 64) {
-      const int pos = 16 * w - mis;  // segment byte of the chunk's first byte (-15..)
+      const int pos = 16 * w - mis;  // segment byte of the chunk's first byte (-3..)
       uint32_t o[4];
       if (bpp == 1) {
         for (int k = 0; k < 4; ++k) {
@@ -498,17 +541,28 @@ CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* o
         }
       } else {
         const int q0 = (pos + 15) / 3 - 5;  // pixel of byte pos (floor, pos >= -15)
-        uint32_t px[6];
+        // the six pixels' samples first (their LDS reads in flight together),
+        // then (Y << 8) + t >> 8 computed exactly as Y + (t >> 8), with the
+        // chroma offsets of 128 folded into the constants:
+        //   359 (cr - 128) + 128 = 359 cr - 45824, 454 (cb - 128) + 128 =
+        //   454 cb - 57984, 128 - 88 (cb - 128) - 183 (cr - 128) = 34816 -
+        //   88 cb - 183 cr
+        int sy[6], sb[6], sr[6];
+#pragma unroll
         for (int e = 0; e < 6; ++e) {
           int xx = q0 + e;
           xx = xx < 0 ? 0 : (xx >= rw ? rw - 1 : xx);
-          const int Y = (int)srow0[xx >> xs0] << 8;
-          const int cb = (int)srow1[xx >> xs1] - 128;
-          const int cr = (int)srow2[xx >> xs2] - 128;
-          const uint32_t r = clip_shr8(Y + 359 * cr + 128);
-          const uint32_t g = clip_shr8(Y - 88 * cb - 183 * cr + 128);
-          const uint32_t b = clip_shr8(Y + 454 * cb + 128);
-          px[e] = r | (g << 8) | (b << 16);
+          sy[e] = srow0[xx >> xs0];
+          sb[e] = srow1[xx >> xs1];
+          sr[e] = srow2[xx >> xs2];
+        }
+        uint32_t px[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+          const int r = sy[e] + ((359 * sr[e] - 45824) >> 8);
+          const int g = sy[e] + ((34816 - 88 * sb[e] - 183 * sr[e]) >> 8);
+          const int b = sy[e] + ((454 * sb[e] - 57984) >> 8);
+          px[e] = clamp255(r) | (clamp255(g) << 8) | (clamp255(b) << 16);
         }
         // the six pixels' 18 bytes as dwords, then the 16 from byte pos - 3 q0
         const uint32_t d0 = px[0] | (px[1] << 24), d1 = (px[1] >> 8) | (px[2] << 16);
@@ -520,12 +574,10 @@ CJD_HD inline void pix_colour(const SH& sh, int tid, const PixCtx& c, uint8_t* o
         o[3] = align_bytes(d4, d3, sft);
       }
       uint8_t* dst = base + 16 * w;
-      if (pos >= 0 && pos + 16 <= n) {
-        store_u32x4(dst, o[0], o[1], o[2], o[3]);
-      } else {
-        for (int j = 0; j < 16; ++j)
-          if (pos + j >= 0 && pos + j < n) dst[j] = (uint8_t)(o[j >> 2] >> (8 * (j & 3)));
-      }
+      if (pos >= 0 && pos + 16 <= n)
+        store_16_a4(dst, o[0], o[1], o[2], o[3]);
+      else
+        store_part16(dst, o, pos < 0 ? -pos : 0, n - pos < 16 ? n - pos : 16);
     }
   }
 }
