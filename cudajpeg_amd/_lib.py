@@ -18,6 +18,7 @@ ENTROPY_TILED, ENTROPY_SERIAL = 0, 1
 UNSTUFF_HOST, UNSTUFF_DEVICE = 0, 1
 EXT_DQT16 = 1
 EXT_NONINTERLEAVED = 2
+EXT_PROGRESSIVE = 4
 OUT_PIXELS, OUT_COEFFICIENTS, OUT_PLANE0 = 0, 1, 2
 
 # Every symbol include/cudajpeg_amd.h and include/format.h declare.
@@ -55,6 +56,7 @@ class FrameInfo(ctypes.Structure):
         ("scan_offset", ctypes.c_uint64), ("scan_length", ctypes.c_uint64),
         ("error_line", ctypes.c_int32), ("num_scans", ctypes.c_int32),
         ("dq_tables16", (ctypes.c_uint16 * 64) * 4),
+        ("progressive", ctypes.c_int32), ("reserved_", ctypes.c_int32),
     ]
 
 

@@ -14,6 +14,10 @@ namespace cjd {
 
 static inline unsigned read16(const uint8_t* p) { return (unsigned)((p[0] << 8) | p[1]); }
 
+// most scans of a progressive file (T.81 sets no limit; libjpeg's scripts use
+// 6 to 10)
+constexpr int kMaxScans = 64;
+
 #define PFAIL(e)              \
   do {                        \
     st.error = (e);           \
@@ -171,6 +175,30 @@ void parse_sos(ParseState& st) {
   f.scan_length = (uint64_t)(st.end - st.pos);
 }
 
+// The data of a scan that starts at d: to the first 0xFF followed by neither
+// 0x00 nor an RSTn (T.81 B.1.1.2, B.1.1.5), at the latest to the EOI; st.pos
+// goes there.  The first scan's bounds are the frame's scan_offset/length.
+static void scan_data(ParseState& st, ScanInfo& sc, const uint8_t* d) {
+  const uint8_t* e = d;
+  for (;;) {
+    const void* ff = e < st.end ? memchr(e, 0xFF, (size_t)(st.end - e)) : nullptr;
+    if (!ff) {
+      e = st.end;
+      break;
+    }
+    e = static_cast<const uint8_t*>(ff);  // e < st.end: e[1] is in the file
+    if (e[1] != 0x00 && (e[1] & 0xF8) != 0xD0) break;
+    e += 2;
+  }
+  sc.offset = (uint64_t)(d - st.buf);
+  sc.length = (uint64_t)(e - d);
+  if (st.nscans == 1) {
+    st.f->scan_offset = sc.offset;
+    st.f->scan_length = sc.length;
+  }
+  st.pos = e;
+}
+
 // T.81 B.2.3 with Ns = 1 (src/decodeScanGPU.cu:124-131 rejects it): the
 // scan's component, its tables and the restart interval in effect, the
 // component's quant table as it is now; the scan's data runs to the first
@@ -189,7 +217,8 @@ void parse_sos_one(ParseState& st) {
     if (st.scans[k].comp == c) PFAIL(CJD_SYNTAX_ERROR);  // each component in one scan
   if (pos[1] & 0xEE) PFAIL(CJD_SYNTAX_ERROR);
   if (pos[2] || pos[3] != 63 || pos[4]) PFAIL(CJD_UNSUPPORTED_ERROR);
-  ScanInfo& sc = st.scans[st.nscans++];
+  st.scans.emplace_back();
+  ScanInfo& sc = st.scans[(size_t)st.nscans++];
   sc.comp = c;
   sc.dc_id = pos[1] >> 4;
   sc.ac_id = (pos[1] & 1) | 2;
@@ -197,25 +226,68 @@ void parse_sos_one(ParseState& st) {
   memcpy(sc.huff_bits, f.huff_bits, sizeof(sc.huff_bits));
   memcpy(sc.huff_vals, f.huff_vals, sizeof(sc.huff_vals));
   memcpy(sc.dq, f.dq_tables16[f.dq_id[c]], sizeof(sc.dq));
-  const uint8_t* d = st.pos + header_len;
-  const uint8_t* e = d;
-  for (;;) {
-    const void* ff = e < st.end ? memchr(e, 0xFF, (size_t)(st.end - e)) : nullptr;
-    if (!ff) {
-      e = st.end;
-      break;
+  scan_data(st, sc, st.pos + header_len);
+}
+
+// T.81 B.2.3 in a progressive frame (G.1.1.1), which the reference never
+// reaches (src/format.cu:198-200 rejects SOF2): Ns components in frame
+// order; a DC scan (Ss = 0) has Se = 0, an AC scan 1 <= Ss <= Se <= 63 and
+// one component; Al <= 13 and a refinement (Ah > 0) has Al = Ah - 1.  The
+// band's previous scan must have ended at Ah (cbits), a first scan must be
+// the band's first, and an AC scan needs the component's DC begun.  Each
+// component's quant table is latched at its first scan.
+void parse_sos_prog(ParseState& st) {
+  cjd_frame_info& f = *st.f;
+  if (!st.have_sof) PFAIL(CJD_SYNTAX_ERROR);
+  const uint8_t* pos = st.pos;
+  const unsigned header_len = read16(pos);
+  if (header_len < 3 || pos + header_len > st.end) PFAIL(CJD_SYNTAX_ERROR);
+  const int ns = pos[2];
+  if (ns < 1 || ns > f.num_channels || header_len != 6u + 2u * (unsigned)ns) PFAIL(CJD_SYNTAX_ERROR);
+  if (st.nscans >= kMaxScans) PFAIL(CJD_UNSUPPORTED_ERROR);
+  ScanInfo sc;
+  sc.ncomp = ns;
+  pos += 3;
+  int last = -1;
+  for (int i = 0; i < ns; ++i, pos += 2) {
+    int c = last + 1;
+    while (c < f.num_channels && f.comp_id[c] != pos[0]) ++c;
+    if (c == f.num_channels) PFAIL(CJD_SYNTAX_ERROR);  // unknown, repeated or out of frame order
+    if (pos[1] & 0xEE) PFAIL(CJD_SYNTAX_ERROR);
+    sc.comps[i] = c;
+    sc.dcs[i] = pos[1] >> 4;
+    sc.acs[i] = (pos[1] & 1) | 2;
+    last = c;
+  }
+  sc.ss = pos[0];
+  sc.se = pos[1];
+  sc.ah = pos[2] >> 4;
+  sc.al = pos[2] & 15;
+  if (sc.ss == 0 ? sc.se != 0 : (sc.se < sc.ss || sc.se > 63 || ns != 1)) PFAIL(CJD_SYNTAX_ERROR);
+  if (sc.al > 13 || (sc.ah && sc.al != sc.ah - 1)) PFAIL(CJD_SYNTAX_ERROR);
+  for (int i = 0; i < ns; ++i) {
+    const int8_t* cb = st.cbits[sc.comps[i]];
+    if (sc.ss > 0 && cb[0] < 0) PFAIL(CJD_SYNTAX_ERROR);
+    for (int k = sc.ss; k <= sc.se; ++k)
+      if (cb[k] != (sc.ah ? sc.ah : -1)) PFAIL(CJD_SYNTAX_ERROR);
+  }
+  for (int i = 0; i < ns; ++i) {
+    const int c = sc.comps[i];
+    for (int k = sc.ss; k <= sc.se; ++k) st.cbits[c][k] = (int8_t)sc.al;
+    if (!st.latched[c]) {
+      st.latched[c] = true;
+      memcpy(st.comp_dq[c], f.dq_tables16[f.dq_id[c]], sizeof(st.comp_dq[c]));
     }
-    e = static_cast<const uint8_t*>(ff);  // e < st.end: e[1] is in the file
-    if (e[1] != 0x00 && (e[1] & 0xF8) != 0xD0) break;
-    e += 2;
   }
-  sc.offset = (uint64_t)(d - st.buf);
-  sc.length = (uint64_t)(e - d);
-  if (st.nscans == 1) {
-    f.scan_offset = sc.offset;
-    f.scan_length = sc.length;
-  }
-  st.pos = e;
+  sc.comp = sc.comps[0];
+  sc.dc_id = sc.dcs[0];
+  sc.ac_id = sc.acs[0];
+  sc.restart_interval = f.restart_interval;
+  memcpy(sc.huff_bits, f.huff_bits, sizeof(sc.huff_bits));
+  memcpy(sc.huff_vals, f.huff_vals, sizeof(sc.huff_vals));
+  st.scans.push_back(sc);
+  ++st.nscans;
+  scan_data(st, st.scans.back(), st.pos + header_len);
 }
 
 // src/format.cu:113-215 without the file read; stops after the SOS header
@@ -232,6 +304,10 @@ int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame
   st.have_sof = false;
   st.at_scan = false;
   st.nscans = 0;
+  st.scans.clear();
+  st.progressive = false;
+  memset(st.cbits, -1, sizeof(st.cbits));
+  for (bool& l : st.latched) l = false;
   if (size < 4 || data[0] != 0xFF || data[1] != 0xD8 || data[size - 2] != 0xFF || data[size - 1] != 0xD9) {
     st.error = CJD_FILE_ERROR;
     st.error_line = __LINE__;
@@ -254,10 +330,23 @@ int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame
     const uint8_t m = st.pos[-1];
     switch (m) {
       case 0xC0: parse_sof(st); break;
+      case 0xC2:  // src/format.cu:198-200 rejects it as an unknown marker
+        if (!(st.ext & CJD_EXT_PROGRESSIVE)) {
+          st.error = CJD_SYNTAX_ERROR;
+          st.error_line = __LINE__;
+          break;
+        }
+        parse_sof(st);
+        st.progressive = true;
+        break;
       case 0xC4: parse_dht(st); break;
       case 0xDB: parse_dqt(st); break;
       case 0xDD: parse_dri(st); break;
       case 0xDA:
+        if (st.progressive) {
+          parse_sos_prog(st);
+          break;
+        }
         if ((st.ext & CJD_EXT_NONINTERLEAVED) && f->num_channels > 1 && st.pos + 3 <= st.end && st.pos[2] == 1) {
           parse_sos_one(st);
           break;
@@ -281,7 +370,22 @@ int parse_until_scan(ParseState& st, const uint8_t* data, size_t size, cjd_frame
     }
     if (st.error || st.at_scan || m == 0xD9) break;
   }
-  if (!st.error && st.nscans) {
+  if (!st.error && st.progressive) {
+    // each component's quant table as latched at its first scan, listed
+    // under table id c (a component no scan codes keeps zero coefficients)
+    st.at_scan = st.nscans > 0;
+    for (int c = 0; c < f->num_channels; ++c) {
+      const uint16_t* q = st.latched[c] ? st.comp_dq[c] : f->dq_tables16[f->dq_id[c]];
+      uint16_t t[64];
+      memcpy(t, q, sizeof(t));
+      f->dq_id[c] = c;
+      for (int i = 0; i < 64; ++i) {
+        f->dq_tables16[c][i] = t[i];
+        f->dq_tables[c][i] = (uint8_t)(t[i] > 255 ? 255 : t[i]);
+      }
+    }
+    f->progressive = 1;
+  } else if (!st.error && st.nscans) {
     if (st.nscans != f->num_channels) {  // a component no scan codes
       st.error = CJD_SYNTAX_ERROR;
       st.error_line = __LINE__;

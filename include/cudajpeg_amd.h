@@ -63,9 +63,12 @@ typedef struct {
   uint64_t scan_offset;                 /* entropy-coded segment: file[scan_offset, +scan_length) */
   uint64_t scan_length;
   int32_t error_line;                   /* source line of the failed check, 0 if none */
-  int32_t num_scans;                    /* 1, or with CJD_EXT_NONINTERLEAVED the one-component scans
+  int32_t num_scans;                    /* 1, or with CJD_EXT_NONINTERLEAVED the one-component scans,
+                                           with CJD_EXT_PROGRESSIVE the progressive scans
                                            (scan_offset / scan_length: the first) */
   uint16_t dq_tables16[4][64];          /* the same tables with 16-bit entries (CJD_EXT_DQT16), zigzag */
+  int32_t progressive;                  /* 1 for a progressive frame (CJD_EXT_PROGRESSIVE) */
+  int32_t reserved_;
 } cjd_frame_info;
 
 /* Extensions beyond the reference decoder (off by default: the reference's
@@ -85,6 +88,22 @@ typedef struct {
  *                  order them, with zero blocks where MCUs pad a component's
  *                  own block grid (outside the image). */
 #define CJD_EXT_NONINTERLEAVED 2
+/*   CJD_EXT_PROGRESSIVE  progressive frames (SOF2, T.81 G.1), which
+ *                  src/format.cu:198-200 rejects as an unknown marker with
+ *                  SYNTAX_ERROR: DC first and refinement scans (interleaved
+ *                  or not), AC first scans with end-of-band runs and AC
+ *                  refinement scans, each with the Huffman tables and restart
+ *                  interval in effect at its SOS; each component is
+ *                  dequantised with the quant table in effect at its first
+ *                  scan.  The progression must follow T.81 G.1.1.1 (a band's
+ *                  first scan has Ah = 0, each later one Ah = the previous
+ *                  Al and Al = Ah - 1, Al <= 13, a component's AC scans after
+ *                  its DC first scan); anything else is SYNTAX_ERROR.  The
+ *                  scans are decoded on the GPU one lane per restart interval
+ *                  of a scan, scans that code disjoint coefficients side by
+ *                  side; coefficients come out in the interleaved layout as
+ *                  for CJD_EXT_NONINTERLEAVED. */
+#define CJD_EXT_PROGRESSIVE 4
 
 typedef struct cjd_context cjd_context;
 typedef struct cjd_batch cjd_batch;

@@ -11,7 +11,10 @@ import this package, and only as the checker.  The product decoder in
 
 Parity status of each function (how it is pinned):
   parse / decode_scan   encoder ground truth (coefficients, block bit offsets)
-                        + reference decodeScanCPU.cu on the golden files
+                        + reference decodeScanCPU.cu on the golden files;
+                        progressive scans: libjpeg (Pillow) progressive and
+                        baseline files of the same pixels decode to the same
+                        coefficients (tests/golden/progressive.npz)
   idct_int / colour     reference CPUonly pixels on the golden files
   idct_int_noshortcut   shortcut algebra (equal to idct_int on random blocks)
   idct_f64              T.81 A.3.3 definition; checked against numpy's matrix
@@ -43,7 +46,13 @@ class _Channel(ctypes.Structure):
 class _Scan(ctypes.Structure):
     _fields_ = [("comp", ctypes.c_int), ("dc_id", ctypes.c_int), ("ac_id", ctypes.c_int),
                 ("restart_interval", ctypes.c_int), ("scan", ctypes.c_void_p), ("scan_len", ctypes.c_size_t),
-                ("bits", (ctypes.c_uint8 * 16) * 4), ("vals", (ctypes.c_uint8 * 256) * 4)]
+                ("bits", (ctypes.c_uint8 * 16) * 4), ("vals", (ctypes.c_uint8 * 256) * 4),
+                ("ncomp", ctypes.c_int), ("comps", ctypes.c_int * 3), ("dcs", ctypes.c_int * 3),
+                ("acs", ctypes.c_int * 3), ("ss", ctypes.c_int), ("se", ctypes.c_int), ("ah", ctypes.c_int),
+                ("al", ctypes.c_int)]
+
+
+MAX_SCANS = 64  # ORC_MAX_SCANS
 
 
 class _Jpeg(ctypes.Structure):
@@ -61,7 +70,8 @@ class _Jpeg(ctypes.Structure):
         ("dht_present", ctypes.c_int * 4),
         ("scan", ctypes.c_void_p), ("scan_len", ctypes.c_size_t), ("have_scan", ctypes.c_int),
         ("ext", ctypes.c_int), ("dq16", (ctypes.c_uint16 * 64) * 4),
-        ("num_scans", ctypes.c_int), ("scans", _Scan * 3), ("comp_dq16", (ctypes.c_uint16 * 64) * 3),
+        ("num_scans", ctypes.c_int), ("scans", _Scan * MAX_SCANS), ("comp_dq16", (ctypes.c_uint16 * 64) * 3),
+        ("progressive", ctypes.c_int),
     ]
 
 
@@ -121,6 +131,7 @@ class Parsed:
 
 EXT_DQT16 = 1  # ORC_EXT_DQT16: 16-bit quantisation tables (beyond the reference)
 EXT_NONINTERLEAVED = 2  # ORC_EXT_NONINTERLEAVED: one scan per component (beyond the reference)
+EXT_PROGRESSIVE = 4  # ORC_EXT_PROGRESSIVE: progressive frames, T.81 G.1 (beyond the reference)
 
 
 def parse(data: bytes, ext: int = 0) -> Parsed:

@@ -169,6 +169,8 @@ static int parse_sos(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const ui
   return 0;
 }
 
+static const uint8_t *scan_end(const uint8_t *d, const uint8_t *end);
+
 /* ORC_EXT_NONINTERLEAVED: SOS of a one-component scan (T.81 B.2.3 with
  * Ns = 1), which src/decodeScanGPU.cu:124-131 rejects.  Records the scan
  * with the tables and restart interval in effect, latches the component's
@@ -194,8 +196,82 @@ static int parse_sos_ni(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const
   memcpy(j->scans[k].bits, j->dht_bits, sizeof(j->dht_bits));
   memcpy(j->scans[k].vals, j->dht_vals, sizeof(j->dht_vals));
   memcpy(j->comp_dq16[c], j->dq16[j->ch[c].dq_id], sizeof(j->comp_dq16[c]));
-  const uint8_t *d = p + header_len, *e = d;
-  while (e < end && !(e[0] == 0xFF && e + 1 < end + 2 && e[1] != 0x00 && (e[1] & 0xF8) != 0xD0)) e++;
+  const uint8_t *d = p + header_len, *e = scan_end(d, end);
+  j->scans[k].scan = d;
+  j->scans[k].scan_len = (size_t)(e - d);
+  j->have_scan = 1;
+  *next = e;
+  return 0;
+}
+
+/* End of a scan's entropy-coded data: the first 0xFF followed by neither
+ * 0x00 nor RSTn (T.81 B.1.1.2 and B.1.1.5), at the latest the EOI. */
+static const uint8_t *scan_end(const uint8_t *d, const uint8_t *end) {
+  const uint8_t *e = d;
+  while (e < end && !(e[0] == 0xFF && e[1] != 0x00 && (e[1] & 0xF8) != 0xD0)) e++;
+  return e;
+}
+
+/* ORC_EXT_PROGRESSIVE: SOS of a progressive frame (T.81 B.2.3, G.1.1.1).
+ * Ns components in frame order; a DC scan (Ss = 0) has Se = 0, an AC scan
+ * 1 <= Ss <= Se <= 63 and one component; Al <= 13 and a refinement scan
+ * (Ah > 0) has Al = Ah - 1.  cbits[c][k] is the Al of the last scan that
+ * coded coefficient k of component c (-1: none yet): a first scan needs -1,
+ * a refinement Ah; and an AC scan needs the component's DC begun.  Records
+ * the scan with the tables and restart interval in effect and latches the
+ * quant table of each component at its first scan. */
+static int parse_sos_prog(orc_jpeg *j, const uint8_t *p, const uint8_t *end, const uint8_t **next,
+                          signed char cbits[3][64]) {
+  if (!j->num_channels) FAIL(ORC_SYNTAX_ERROR);
+  unsigned header_len = read16(p);
+  if (p + header_len > end || header_len < 3) FAIL(ORC_SYNTAX_ERROR);
+  const int ns = p[2];
+  if (ns < 1 || ns > j->num_channels || header_len != 6u + 2u * (unsigned)ns) FAIL(ORC_SYNTAX_ERROR);
+  if (j->num_scans >= ORC_MAX_SCANS) FAIL(ORC_UNSUPPORTED_ERROR);
+  const int k = j->num_scans;
+  const uint8_t *pos = p + 3;
+  int last = -1;
+  for (int i = 0; i < ns; i++, pos += 2) {
+    int c = last + 1;
+    while (c < j->num_channels && j->ch[c].id != pos[0]) c++;
+    if (c == j->num_channels) FAIL(ORC_SYNTAX_ERROR); /* unknown, repeated or out of frame order */
+    if (pos[1] & 0xEE) FAIL(ORC_SYNTAX_ERROR);
+    j->scans[k].comps[i] = c;
+    j->scans[k].dcs[i] = pos[1] >> 4;
+    j->scans[k].acs[i] = (pos[1] & 1) | 2;
+    last = c;
+  }
+  const int ss = pos[0], se = pos[1], ah = pos[2] >> 4, al = pos[2] & 15;
+  if (ss == 0 ? se != 0 : (se < ss || se > 63 || ns != 1)) FAIL(ORC_SYNTAX_ERROR);
+  if (al > 13 || (ah && al != ah - 1)) FAIL(ORC_SYNTAX_ERROR);
+  for (int i = 0; i < ns; i++) {
+    signed char *cb = cbits[j->scans[k].comps[i]];
+    if (ss > 0 && cb[0] < 0) FAIL(ORC_SYNTAX_ERROR);
+    for (int q = ss; q <= se; q++) {
+      if (cb[q] != (ah ? ah : -1)) FAIL(ORC_SYNTAX_ERROR);
+      cb[q] = (signed char)al;
+    }
+  }
+  for (int i = 0; i < ns; i++) {
+    const int c = j->scans[k].comps[i];
+    int seen = 0;
+    for (int t = 0; t < k; t++)
+      for (int q = 0; q < j->scans[t].ncomp; q++) seen |= j->scans[t].comps[q] == c;
+    if (!seen) memcpy(j->comp_dq16[c], j->dq16[j->ch[c].dq_id], sizeof(j->comp_dq16[c]));
+  }
+  j->num_scans++;
+  j->scans[k].ncomp = ns;
+  j->scans[k].comp = j->scans[k].comps[0];
+  j->scans[k].dc_id = j->scans[k].dcs[0];
+  j->scans[k].ac_id = j->scans[k].acs[0];
+  j->scans[k].ss = ss;
+  j->scans[k].se = se;
+  j->scans[k].ah = ah;
+  j->scans[k].al = al;
+  j->scans[k].restart_interval = j->restart_interval;
+  memcpy(j->scans[k].bits, j->dht_bits, sizeof(j->dht_bits));
+  memcpy(j->scans[k].vals, j->dht_vals, sizeof(j->dht_vals));
+  const uint8_t *d = p + header_len, *e = scan_end(d, end);
   j->scans[k].scan = d;
   j->scans[k].scan_len = (size_t)(e - d);
   j->have_scan = 1;
@@ -214,6 +290,8 @@ int orc_parse_ext(const uint8_t *buf, size_t size, int ext, orc_jpeg *j) {
   if (size < 6) FAIL(ORC_SYNTAX_ERROR);
   const uint8_t *end = buf + size - 2;
   const uint8_t *pos = buf + 2;
+  signed char cbits[3][64];
+  memset(cbits, -1, sizeof(cbits));
   for (;;) {
     if (pos > end) FAIL(ORC_SYNTAX_ERROR);
     if (pos[0] != 0xFF) FAIL(ORC_SYNTAX_ERROR);
@@ -224,6 +302,14 @@ int orc_parse_ext(const uint8_t *buf, size_t size, int ext, orc_jpeg *j) {
       case 0xC0:
         e = parse_sof(j, pos, end);
         if (!e) pos += read16(pos);
+        break;
+      case 0xC2: /* src/format.cu:198-200 rejects it as an unknown marker */
+        if (!(j->ext & ORC_EXT_PROGRESSIVE)) FAIL(ORC_SYNTAX_ERROR);
+        e = parse_sof(j, pos, end);
+        if (!e) {
+          j->progressive = 1;
+          pos += read16(pos);
+        }
         break;
       case 0xC4:
         e = parse_dht(j, pos, end);
@@ -244,6 +330,11 @@ int orc_parse_ext(const uint8_t *buf, size_t size, int ext, orc_jpeg *j) {
       }
       case 0xDA: {
         const uint8_t *after;
+        if (j->progressive) {
+          e = parse_sos_prog(j, pos, end, &after, cbits);
+          if (!e) pos = after;
+          break;
+        }
         if ((j->ext & ORC_EXT_NONINTERLEAVED) && j->num_channels > 1 && read16(pos) >= 3 && pos[2] == 1) {
           e = parse_sos_ni(j, pos, end, &after);
           if (!e) pos = after;
@@ -272,7 +363,12 @@ int orc_parse_ext(const uint8_t *buf, size_t size, int ext, orc_jpeg *j) {
     if (e) return e;
     if (m == 0xD9) break;
   }
-  if (j->num_scans && j->num_scans != j->num_channels) FAIL(ORC_SYNTAX_ERROR); /* a component never coded */
+  if (!j->progressive && j->num_scans && j->num_scans != j->num_channels)
+    FAIL(ORC_SYNTAX_ERROR); /* a component never coded */
+  if (j->progressive) { /* a component without scans keeps zero coefficients: any table will do */
+    for (int c = 0; c < j->num_channels; c++)
+      if (cbits[c][0] < 0) memcpy(j->comp_dq16[c], j->dq16[j->ch[c].dq_id], sizeof(j->comp_dq16[c]));
+  }
   return 0;
 }
 
@@ -420,9 +516,203 @@ static int decode_scans_ni(orc_jpeg *j, int16_t *qcoef, int32_t *work[3]) {
   return 0;
 }
 
+/* ORC_EXT_PROGRESSIVE (T.81 G.1.2).  Every scan adds to one coefficient
+ * array in the interleaved layout, zero at the start.  A scan of several
+ * components runs over the frame's MCUs (each component's h x v blocks, as in
+ * the interleaved baseline scan), a one-component scan over the component's
+ * own grid of ceil(width_c / 8) x ceil(height_c / 8) blocks (T.81 A.2.2);
+ * the restart interval counts those units.  The four procedures:
+ *  DC first   (Ss = 0, Ah = 0): DC difference as in baseline, stored << Al;
+ *  DC refine  (Ss = 0, Ah > 0): one raw bit per block, ORed in at bit Al;
+ *  AC first   (Ss > 0, Ah = 0): run/size symbols over Ss..Se, values << Al;
+ *             EOBn (size 0, run r < 15) ends this block and the next
+ *             2^r + (r extra bits) - 1 blocks of the band (G.1.2.2);
+ *  AC refine  (Ss > 0, Ah > 0): symbols (run, size 0 or 1); a size-1 symbol
+ *             brings a sign bit for a new coefficient +-2^Al placed at the
+ *             (run + 1)-th coefficient of the band still zero; every nonzero
+ *             coefficient passed on the way takes a correction bit, and a 1
+ *             adds 2^Al to its magnitude; ZRL passes 16 zero coefficients;
+ *             EOBn ends the symbols of this block and the next blocks of the
+ *             run, whose nonzero coefficients still take correction bits
+ *             (G.1.2.3).
+ * A restart resets the DC predictors and the EOB run.  An invalid code, a
+ * refinement size other than 0 or 1, or a run past Se is SYNTAX_ERROR. */
+typedef struct {
+  reader_t r;
+  const vlc_t *dct[3], *act;
+  int pred[3], eobrun;
+  int ss, se, ah, al;
+} prog_t;
+
+static int prog_block(prog_t *P, int ci, int16_t *q) {
+  reader_t *r = &P->r;
+  uint8_t code = 0;
+  if (P->ss == 0) {
+    if (P->ah == 0) {
+      P->pred[ci] += get_vlc(r, P->dct[ci], NULL);
+      q[0] = (int16_t)(P->pred[ci] * (1 << P->al));
+    } else if (get_bits(r, 1)) {
+      q[0] = (int16_t)(q[0] | (1 << P->al));
+    }
+    return r->error;
+  }
+  const int p1 = 1 << P->al;
+  int k = P->ss;
+  if (P->ah == 0) {
+    if (P->eobrun > 0) {
+      P->eobrun--;
+      return 0;
+    }
+    while (k <= P->se) {
+      int v = get_vlc(r, P->act, &code);
+      if (r->error) return r->error;
+      int run = code >> 4, size = code & 15;
+      if (size) {
+        k += run;
+        if (k > P->se) return ORC_SYNTAX_ERROR;
+        q[kDeZigZag[k++]] = (int16_t)(v * p1);
+      } else if (run == 15) {
+        k += 16;
+        if (k > P->se + 1) return ORC_SYNTAX_ERROR;
+      } else {
+        P->eobrun = (1 << run) - 1 + (run ? get_bits(r, run) : 0);
+        break;
+      }
+    }
+    return r->error;
+  }
+  /* refinement: the correction bit of a nonzero coefficient */
+#define REFINE(x)                                                   \
+  do {                                                              \
+    if (get_bits(r, 1) && !((x) & p1)) (x) = (int16_t)((x) + ((x) >= 0 ? p1 : -p1)); \
+  } while (0)
+  if (P->eobrun == 0) {
+    while (k <= P->se) {
+      /* a size-1 symbol's one extra bit is the new coefficient's sign, which
+       * EXTEND turns into +1 or -1 */
+      const int v = get_vlc(r, P->act, &code);
+      if (r->error) return r->error;
+      const int run = code >> 4, size = code & 15;
+      if (size > 1) return ORC_SYNTAX_ERROR;
+      if (!size && run != 15) {
+        P->eobrun = (1 << run) + (run ? get_bits(r, run) : 0);
+        break;
+      }
+      int left = run; /* zero coefficients to pass before the target one */
+      while (k <= P->se) {
+        int16_t *x = &q[kDeZigZag[k]];
+        if (*x) REFINE(*x);
+        else if (left-- == 0) break;
+        k++;
+      }
+      if (k > P->se) {
+        if (size) return ORC_SYNTAX_ERROR;
+        break;
+      }
+      if (size) q[kDeZigZag[k]] = (int16_t)(v * p1);
+      k++;
+    }
+  }
+  if (P->eobrun > 0) {
+    for (; k <= P->se; k++) {
+      int16_t *x = &q[kDeZigZag[k]];
+      if (*x) REFINE(*x);
+    }
+    P->eobrun--;
+  }
+#undef REFINE
+  return r->error;
+}
+
+static int decode_scans_prog(orc_jpeg *j, int16_t *qcoef, int32_t *work[3]) {
+  const long bpm = j->blocks_per_mcu;
+  int16_t *q = qcoef ? qcoef : (int16_t *)malloc(sizeof(int16_t) * 64 * (size_t)j->num_blocks);
+  vlc_t *lut = (vlc_t *)malloc(sizeof(vlc_t) * 65536 * 4);
+  if (!q || !lut) {
+    if (!qcoef) free(q);
+    free(lut);
+    FAIL(ORC_OOM_ERROR);
+  }
+  memset(q, 0, sizeof(int16_t) * 64 * (size_t)j->num_blocks);
+  int slot0[3] = {0, 0, 0};
+  for (int c = 1; c < j->num_channels; c++) slot0[c] = slot0[c - 1] + j->ch[c - 1].samples_x * j->ch[c - 1].samples_y;
+  int err = 0;
+  for (int k = 0; k < j->num_scans && !err; k++) {
+    const int ns = j->scans[k].ncomp;
+    prog_t P;
+    memset(&P, 0, sizeof(P));
+    P.ss = j->scans[k].ss;
+    P.se = j->scans[k].se;
+    P.ah = j->scans[k].ah;
+    P.al = j->scans[k].al;
+    /* lut: DC tables 0 and 1, then the scan's AC table */
+    build_lut(j->scans[k].bits[0], j->scans[k].vals[0], lut);
+    build_lut(j->scans[k].bits[1], j->scans[k].vals[1], lut + 65536);
+    build_lut(j->scans[k].bits[j->scans[k].ac_id], j->scans[k].vals[j->scans[k].ac_id], lut + 2 * 65536);
+    for (int i = 0; i < ns; i++) P.dct[i] = lut + 65536 * j->scans[k].dcs[i];
+    P.act = lut + 2 * 65536;
+    uint8_t *ub = (uint8_t *)malloc(j->scans[k].scan_len + 2);
+    if (!ub) { err = ORC_OOM_ERROR; break; }
+    size_t ulen = orc_unstuff(j->scans[k].scan, j->scans[k].scan_len, ub);
+    reader_t rd = {ub, ub, ub + ulen, 0, 0, 0};
+    P.r = rd;
+    const int ri = j->scans[k].restart_interval;
+    int rcount = ri, next_rst = 0;
+    long units, bw = 0;
+    const orc_channel *c0 = &j->ch[j->scans[k].comps[0]];
+    if (ns > 1) {
+      units = (long)j->num_blocks_x * j->num_blocks_y;
+    } else {
+      bw = (c0->width + 7) / 8;
+      units = bw * ((c0->height + 7) / 8);
+    }
+    for (long u = 0; u < units && !err; u++) {
+      if (ns > 1) {
+        for (int i = 0; i < ns && !err; i++) {
+          const int c = j->scans[k].comps[i];
+          const int n = j->ch[c].samples_x * j->ch[c].samples_y;
+          for (int b = 0; b < n && !err; b++) err = prog_block(&P, i, q + (u * bpm + slot0[c] + b) * 64);
+        }
+      } else {
+        const int c = j->scans[k].comps[0], h = c0->samples_x, v = c0->samples_y;
+        const long by = u / bw, bx = u % bw;
+        const long mcu = (by / v) * j->num_blocks_x + bx / h;
+        err = prog_block(&P, 0, q + (mcu * bpm + slot0[c] + (by % v) * h + bx % h) * 64);
+      }
+      /* src/decodeScanGPU.cu:476-486, per unit of the scan */
+      if (!err && ri && !(--rcount) && P.r.pos < P.r.end) {
+        P.r.num_bufbits &= 0xF8;
+        int m = get_bits(&P.r, 16);
+        if ((m & 0xFFF8) != 0xFFD0 || (m & 7) != next_rst) err = ORC_SYNTAX_ERROR;
+        next_rst = (next_rst + 1) & 7;
+        rcount = ri;
+        P.pred[0] = P.pred[1] = P.pred[2] = 0;
+        P.eobrun = 0;
+      }
+    }
+    free(ub);
+  }
+  free(lut);
+  /* working space: each component's blocks in decode order, dequantised */
+  for (int c = 0; c < j->num_channels && !err && work; c++) {
+    if (!work[c]) continue;
+    const int n = j->ch[c].samples_x * j->ch[c].samples_y;
+    int32_t *w = work[c];
+    memset(w, 0, sizeof(int32_t) * (size_t)j->ch[c].stride * j->ch[c].plane_height);
+    for (long m = 0; m < (long)j->num_blocks_x * j->num_blocks_y; m++)
+      for (int b = 0; b < n; b++, w += 64)
+        for (int z = 0; z < 64; z++)
+          w[kDeZigZag[z]] = q[(m * bpm + slot0[c] + b) * 64 + kDeZigZag[z]] * j->comp_dq16[c][z];
+  }
+  if (!qcoef) free(q);
+  if (err) FAIL(err);
+  return 0;
+}
+
 int orc_decode_scan(const orc_jpeg *jc, int16_t *qcoef, int32_t *work[3], uint32_t *bitpos) {
   orc_jpeg *j = (orc_jpeg *)jc; /* only error fields are written */
   if (!j->have_scan) FAIL(ORC_SYNTAX_ERROR);
+  if (j->progressive) return decode_scans_prog(j, qcoef, work);
   if (j->num_scans) return decode_scans_ni(j, qcoef, work);
   vlc_t *lut = (vlc_t *)malloc(sizeof(vlc_t) * 65536 * 4);
   uint8_t *ub = (uint8_t *)malloc(j->scan_len + 2);
@@ -605,7 +895,7 @@ int orc_planes(const orc_jpeg *j, const int16_t *qcoef, uint8_t *planes[3], int 
     for (int bx = 0; bx < j->num_blocks_x; bx++)
       for (int c = 0; c < j->num_channels; c++) {
         const orc_channel *ch = &j->ch[c];
-        const uint16_t *dq = j->num_scans ? j->comp_dq16[c] : j->dq16[ch->dq_id];
+        const uint16_t *dq = j->num_scans || j->progressive ? j->comp_dq16[c] : j->dq16[ch->dq_id];
         for (int sy = 0; sy < ch->samples_y; sy++)
           for (int sx = 0; sx < ch->samples_x; sx++, blk++) {
             int32_t deq[64];

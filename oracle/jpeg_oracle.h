@@ -21,6 +21,10 @@ extern "C" {
 #define ORC_OOM_ERROR 3
 #define ORC_FILE_ERROR 5
 
+/* most scans of a file with several (progressive files: T.81 sets no limit;
+ * libjpeg's scripts use 6 to 10) */
+#define ORC_MAX_SCANS 64
+
 typedef struct {
   int id, samples_x, samples_y, dq_id, dc_id, ac_id;
   int width, height, stride, plane_height;
@@ -45,16 +49,21 @@ typedef struct {
   int have_scan;
   int ext;                          /* extensions accepted (ORC_EXT_*), set by orc_parse_ext */
   uint16_t dq16[4][64];             /* the quant tables with 16-bit entries, zigzag order */
-  /* ORC_EXT_NONINTERLEAVED: one scan per component (T.81 A.2.2); 0 for the
-   * single interleaved scan above */
+  /* ORC_EXT_NONINTERLEAVED: one scan per component (T.81 A.2.2); with
+   * ORC_EXT_PROGRESSIVE the scans of a progressive frame; 0 for the single
+   * interleaved scan above */
   int num_scans;
   struct {
-    int comp, dc_id, ac_id, restart_interval;  /* the scan's component and the state at its SOS */
+    int comp, dc_id, ac_id, restart_interval;  /* the scan's (first) component and the state at its SOS */
     const uint8_t *scan;                       /* stuffed, from after its SOS header to the next marker */
     size_t scan_len;
     uint8_t bits[4][16], vals[4][256];         /* Huffman tables in effect at the scan */
-  } scans[3];
-  uint16_t comp_dq16[3][64];        /* each component's quant table when its scan began (zigzag) */
+    /* progressive scans: components (frame indexes, frame order) with their
+     * DC / AC table ids, spectral band Ss..Se and successive approximation */
+    int ncomp, comps[3], dcs[3], acs[3], ss, se, ah, al;
+  } scans[ORC_MAX_SCANS];
+  uint16_t comp_dq16[3][64];        /* each component's quant table when its first scan began (zigzag) */
+  int progressive;                  /* SOF2 frame (ORC_EXT_PROGRESSIVE) */
 } orc_jpeg;
 
 /* Extensions beyond the reference: 16-bit quantisation tables (DQT Pq = 1,
@@ -67,6 +76,16 @@ typedef struct {
  * begins.  Coefficients come out in the interleaved (MCU) layout, with zero
  * blocks where MCUs pad a component's own block grid. */
 #define ORC_EXT_NONINTERLEAVED 2
+/* Progressive frames (SOF2, T.81 G.1), which src/format.cu:198-200 rejects:
+ * DC first and refinement scans (interleaved or not), AC first scans with
+ * end-of-band runs and AC refinement scans with correction bits, each with the
+ * tables and restart interval in effect at its SOS.  The progression must
+ * follow T.81 G.1.1.1: a band's first scan has Ah = 0, each later scan of it
+ * Ah = the previous Al and Al = Ah - 1, a component's AC scans come after its
+ * DC first scan; anything else is SYNTAX_ERROR.  Coefficients come out in the
+ * interleaved (MCU) layout; each component is dequantised with the table in
+ * effect at its first scan. */
+#define ORC_EXT_PROGRESSIVE 4
 
 /* Parse markers, SOF, DHT, DQT, DRI and the SOS header of an in-memory file
  * (src/format.cu:113-215, :242-442; src/decodeScanGPU.cu:124-141).
@@ -85,7 +104,8 @@ size_t orc_unstuff(const uint8_t *scan, size_t len, uint8_t *out);
  *  bitpos uint32 [num_blocks]    start bit of each block in the unstuffed segment
  *                                (interleaved scans)
  * Non-interleaved scans (num_scans > 0) are decoded one after the other, each
- * block written where the interleaved scan holds it.
+ * block written where the interleaved scan holds it; progressive scans
+ * likewise, into coefficients that start at zero (bitpos is left untouched).
  * Returns the reference error code. */
 int orc_decode_scan(const orc_jpeg *j, int16_t *qcoef, int32_t *work[3], uint32_t *bitpos);
 
