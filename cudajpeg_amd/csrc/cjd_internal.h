@@ -93,8 +93,32 @@ struct SegmentDesc {
   // block per MCU of the scan; its block (bx, by) is the interleaved layout's
   // slot slot0 + (by % v) h + bx % h of MCU (bx / h, by / v).
   uint8_t ni, comp, h, v;
-  uint8_t dc_id, ac_id, slot0, pad8_;  // the scan's DC (0..1) and AC (2..3) table ids
+  uint8_t dc_id, ac_id, slot0;  // the scan's DC (0..1) and AC (2..3) table ids
+  uint8_t prog;                 // a progressive scan's interval: decoded by k_prog, no entropy work items
   uint32_t bw;
+};
+
+// One lane of the progressive decoder (CJD_EXT_PROGRESSIVE): one restart
+// interval of one scan (T.81 G.1.2).  A scan of several components runs over
+// the frame's MCUs, bpu blocks each (each component's h x v blocks in frame
+// order, as the interleaved scan); a one-component scan over the component's
+// own grid of bw blocks a row, one block per unit (T.81 A.2.2).
+struct ProgTask {
+  uint32_t seg;      // SegmentDesc of its unstuffed words, image and Huffman table set
+  uint32_t u0;       // first unit (MCU, or block of a one-component scan) of the interval
+  uint32_t nunits;
+  uint32_t bw;       // one-component scans: blocks a row of the component's grid
+  uint8_t ncomp, ss, se, ah;  // components, spectral band Ss..Se (zigzag), successive approximation
+  uint8_t al, bpu, h, v;      // h, v: a one-component scan's sampling factors
+  uint8_t comps[3], last;     // components (frame indexes); last: the scan's final interval
+  uint8_t dcs[3], ac;         // DC table ids (0..1) of the components, the AC table id (2..3)
+};
+
+// A progressive image's coefficient blocks: [coef_base, coef_base + nblocks)
+// of the coefficient buffer, block0 = the blocks of the progressive images
+// before it (k_prog_prepare clears them and writes their records).
+struct ProgImage {
+  int64_t coef_base, nblocks, block0;
 };
 
 // One entropy workgroup: tiles [t0, t0 + ntiles) of a segment.  When t0 > 0 the

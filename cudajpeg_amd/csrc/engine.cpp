@@ -95,6 +95,7 @@ struct cjd_batch {
 
   DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_luts, d_qtabs, d_coefs, d_out, d_scratch;
   DevBuf d_btab;  // MFMA table operands per image (k_mfma_tables at staging)
+  DevBuf d_ptasks, d_pimgs;  // progressive images (CJD_EXT_PROGRESSIVE): k_prog tasks and coefficient ranges
   // Pinned staging: the planner writes the scan words into h_words, the
   // descriptors and tables are packed into h_meta, and every upload is a DMA
   // from them.  up_ev follows the last stage's uploads: the next stage waits
@@ -315,7 +316,7 @@ static void plan_split(cjd_batch* b) {
 }
 
 extern "C" int cjd_batch_set_extensions(cjd_batch* b, int ext) {
-  if (!b || (ext & ~(CJD_EXT_DQT16 | CJD_EXT_NONINTERLEAVED))) return CJD_PROGRAMMER_ERROR;
+  if (!b || (ext & ~(CJD_EXT_DQT16 | CJD_EXT_NONINTERLEAVED | CJD_EXT_PROGRESSIVE))) return CJD_PROGRAMMER_ERROR;
   b->plan.ext = ext;
   return 0;
 }
@@ -361,6 +362,8 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   mp.count(P.tables);
   mp.count(P.luts);
   mp.count(P.qtabs);
+  mp.count(P.ptasks);
+  mp.count(P.pimgs);
   const bool pinned = b->h_meta.ensure(mp.total);
   int e = 0;
   if (P.unstuff_device) {
@@ -380,6 +383,8 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   if (!e) e = mp.put(b->d_pix, P.pix, pinned, s);
   if (!e) e = mp.put(b->d_tables, P.tables, pinned, s);
   if (!e) e = mp.put(b->d_luts, P.luts, pinned, s);
+  if (!e) e = mp.put(b->d_ptasks, P.ptasks, pinned, s);
+  if (!e) e = mp.put(b->d_pimgs, P.pimgs, pinned, s);
   if (!e && b->plan.entry_words >= ((int64_t)1 << 32)) e = CJD_OOM_ERROR;  // records hold 32-bit entry indexes
   if (!e) e = b->d_ebuf.ensure((size_t)b->plan.entry_words * 4);
   if (!e) e = b->d_recs.ensure((size_t)b->plan.total_blocks * sizeof(BlockRec));
@@ -387,7 +392,7 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   // with (outside the image), and decodes write only coded blocks' records:
   // those images' records start as zero blocks
   for (const ImageRec& r : P.recs)
-    if (!e && !r.error && r.f.num_scans > 1 &&
+    if (!e && !r.error && r.f.num_scans > 1 && !r.f.progressive &&
         launch_clear(reinterpret_cast<uint32_t*>(b->d_recs.as<BlockRec>() + r.coef_base),
                      (uint64_t)r.f.num_blocks * (sizeof(BlockRec) / 4), s) != hipSuccess)
       e = CJD_KERNEL_LAUNCH_ERROR;
@@ -404,7 +409,7 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
   // images with one-component scans: zero coefficient blocks as well (the
   // serial decoder writes the coded blocks only)
   for (const ImageRec& r : P.recs)
-    if (!e && !r.error && r.f.num_scans > 1 &&
+    if (!e && !r.error && r.f.num_scans > 1 && !r.f.progressive &&
         launch_clear(reinterpret_cast<uint32_t*>(b->d_coefs.as<int16_t>() + r.coef_base * 64),
                      (uint64_t)r.f.num_blocks * 32, s) != hipSuccess)
       e = CJD_KERNEL_LAUNCH_ERROR;
@@ -476,6 +481,19 @@ static int decode_epoch(cjd_batch* b, hipStream_t s, bool captured, uint32_t* ep
              : CJD_KERNEL_LAUNCH_ERROR;
 }
 
+// Progressive images: their scans (k_prog_prepare, then k_prog level by
+// level) with the decode's epoch, before its entropy launches.
+static int enqueue_prog(cjd_batch* b, hipStream_t s, uint32_t epoch) {
+  const Plan& P = b->plan;
+  if (!P.prog_blocks) return 0;
+  return launch_prog(b->d_pimgs.as<ProgImage>(), (uint32_t)P.pimgs.size(), P.prog_blocks, b->d_ptasks.as<ProgTask>(),
+                     P.plevel.data(), (uint32_t)(P.plevel.size() - 1), b->d_words.as<uint32_t>(),
+                     b->d_images.as<ImageDesc>(), b->d_segs.as<SegmentDesc>(), b->d_tables.as<HuffTable>(),
+                     b->d_coefs.as<int16_t>(), b->d_recs.as<BlockRec>(), status_of(b), epoch, s) == hipSuccess
+             ? 0
+             : CJD_KERNEL_LAUNCH_ERROR;
+}
+
 static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed, bool captured = false) {
   uint32_t* status = status_of(b);
   LookbackRec* lb = reinterpret_cast<LookbackRec*>(status + b->status_words);
@@ -483,6 +501,7 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed, bool capture
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   uint32_t epoch = 0;
   if (int e = decode_epoch(b, s, captured, &epoch)) return e;
+  if (int e = enqueue_prog(b, s, epoch)) return e;
   if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   hipError_t he;
   if (b->entropy_mode == CJD_ENTROPY_SERIAL)
@@ -546,6 +565,7 @@ static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed, bool captured 
   if (timed && hipEventRecord(b->ev[0], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   uint32_t epoch = 0;
   if (int e = decode_epoch(b, s, captured, &epoch)) return e;
+  if (int e = enqueue_prog(b, s, epoch)) return e;
   if (timed && hipEventRecord(b->ev[3], s) != hipSuccess) return CJD_KERNEL_LAUNCH_ERROR;
   b->coefs_stale = true;
   b->last_stream = s;
@@ -597,7 +617,9 @@ static std::vector<uint64_t> graph_signature(const cjd_batch* b) {
       (uint64_t)b->d_coefs.p,  (uint64_t)b->d_out.p,    (uint64_t)b->d_scratch.p, (uint64_t)b->d_btab.p,
       (uint64_t)b->d_ebuf.p,   (uint64_t)b->d_recs.p,   (uint64_t)b->aux,         b->status_words,
       b->plan.items.size(),    b->plan.pix.size(),      b->plan.segs.size(),      (uint64_t)b->plan.entry_words,
-      (uint64_t)b->idct_mode,  (uint64_t)b->entropy_mode, b->wait_ticks,          (uint64_t)b->nsplit};
+      (uint64_t)b->idct_mode,  (uint64_t)b->entropy_mode, b->wait_ticks,          (uint64_t)b->nsplit,
+      (uint64_t)b->d_ptasks.p, (uint64_t)b->d_pimgs.p,  b->plan.pimgs.size(),     (uint64_t)b->plan.prog_blocks};
+  for (uint32_t l : b->plan.plevel) v.push_back(l);
   for (int j = 0; j <= kMaxParts; ++j) {
     v.push_back(b->split_item[j]);
     v.push_back(b->split_pix[j]);
@@ -688,6 +710,10 @@ static int check_entropy(cjd_batch* b, hipStream_t s, std::vector<size_t>* redo)
     ImageRec& r = b->plan.recs[i];
     r.decode_error = 0;
     if (r.error) continue;
+    if (r.f.progressive) {  // k_prog's verdict is final (no redo)
+      if (words[i] & (kStatusSyntax | kStatusTruncated)) r.decode_error = CJD_SYNTAX_ERROR;
+      continue;
+    }
     const uint32_t st = words[i] | (redo_all ? kStatusUnresolved : 0u);
     if (st & redo_bits) {
       const uint32_t first = (st & 0xFFu) << kStatusFirstShift;

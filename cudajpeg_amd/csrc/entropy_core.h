@@ -1127,6 +1127,7 @@ CJD_HD inline void ent_records(const EntShared& sh, int tid, const EntCtx& c, Bl
 CJD_HD inline void ent_serial_segment(const uint32_t* words, const ImageDesc* images, const SegmentDesc& sg,
                                       const HuffTable* tables, int16_t* coefs, BlockRec* recs, uint32_t* status,
                                       uint32_t epoch) {
+  if (sg.prog) return;  // a progressive scan's interval: k_prog decodes it
   const ImageDesc& im = images[sg.image];
   const HuffTable* tab = tables + (size_t)sg.tset * 4;
   const int nh = seg_nh(sg, im);

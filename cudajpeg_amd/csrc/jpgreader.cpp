@@ -181,10 +181,11 @@ void decodeDRI(JPGReader* jpg) {
 static void decode_scan(JPGReader* jpg, int entropy_mode, bool coefficients) {
   const clock_t t0 = clock();
   ReaderEngine* e = eng(jpg);
-  // CJD_EXT_NONINTERLEAVED: a one-component first scan; the batch's planner
-  // parses the file's scans
-  const bool one = (jpg->extensions & CJD_EXT_NONINTERLEAVED) && jpg->num_channels > 1 && jpg->pos + 3 <= jpg->end &&
-                   jpg->pos[2] == 1;
+  // CJD_EXT_NONINTERLEAVED: a one-component first scan; CJD_EXT_PROGRESSIVE:
+  // a progressive frame's first scan.  The batch's planner parses the file's
+  // scans.
+  const bool one = e->f.progressive || ((jpg->extensions & CJD_EXT_NONINTERLEAVED) && jpg->num_channels > 1 &&
+                                        jpg->pos + 3 <= jpg->end && jpg->pos[2] == 1);
   if (!one) {
     run_parser(jpg, cjd::parse_sos, "decodeScan");
     if (jpg->error) return;
@@ -299,6 +300,14 @@ static int run_marker_loop(JPGReader* jpg, size_t size) {
     jpg->pos += 2;
     switch (jpg->pos[-1]) {
       case 0xC0: decodeSOF(jpg); break;
+      case 0xC2:  // CJD_EXT_PROGRESSIVE; src/format.cu:198-200 rejects it as an unknown marker
+        if (!(jpg->extensions & CJD_EXT_PROGRESSIVE)) {
+          set_error(jpg, SYNTAX_ERROR, __LINE__, "jpgreader.cpp", "openJPG");
+          break;
+        }
+        decodeSOF(jpg);
+        e->f.progressive = 1;
+        break;
       case 0xC4: decodeDHT(jpg); break;
       case 0xDB: decodeDQT(jpg); break;
       case 0xDD: decodeDRI(jpg); break;

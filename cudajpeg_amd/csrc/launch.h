@@ -59,6 +59,13 @@ hipError_t launch_entropy_serial(const uint32_t* words, const ImageDesc* images,
 // MFMA table operands of every image (kMfmaTabVecs uint4 per image).
 hipError_t launch_mfma_tables(const ImageDesc* images, uint32_t nimages, const uint16_t* qtabs, uint4* btab,
                               hipStream_t s);
+// Progressive images (CJD_EXT_PROGRESSIVE): k_prog_prepare over their
+// prog_blocks blocks, then one k_prog launch per level (tasks
+// [plevel[l], plevel[l + 1]) of the host array plevel, nlevels + 1 entries).
+hipError_t launch_prog(const ProgImage* pimgs, uint32_t npimgs, int64_t prog_blocks, const ProgTask* tasks,
+                       const uint32_t* plevel, uint32_t nlevels, const uint32_t* words, const ImageDesc* images,
+                       const SegmentDesc* segs, const HuffTable* tables, int16_t* coefs, BlockRec* recs,
+                       uint32_t* status, uint32_t epoch, hipStream_t s);
 hipError_t launch_pixel(int mode, const ImageDesc* images, const PixItem* items, uint32_t nitems,
                         const BlockRec* recs, const uint32_t* ebuf, uint64_t ewords, const int16_t* coefs,
                         const uint16_t* qtabs, const uint4* btab, uint8_t* out, hipStream_t s);

@@ -156,7 +156,8 @@ namespace {
 // One scan of an image: the interleaved scan, or (CJD_EXT_NONINTERLEAVED) a
 // one-component scan with the tables and restart interval in effect at it.
 struct ScanWork {
-  int comp = -1;                  // -1: the interleaved scan; else the scan's component
+  int comp = -1;                  // -1: the interleaved scan; else the scan's (first) component
+  int prog = -1;                  // a progressive scan: its index in ParseState::scans
   uint64_t off = 0, len = 0;      // its stuffed bytes in the file
   int ri = 0;                     // restart interval (MCUs of the scan: blocks for one component)
   int64_t units = 0;              // MCUs of the scan
@@ -177,6 +178,8 @@ struct ImageWork {
   int64_t nraw = 0, nchunks = 0;  // device unstuffing: the image's raw bytes and chunks
   int64_t raw0 = 0, chunk0 = 0;  // device unstuffing: offsets in the raw buffer and the chunk list
   int64_t tset0 = 0;             // first Huffman table set (one per scan)
+  std::vector<int> level;         // progressive: each scan's level
+  std::vector<int64_t> lev_count, lev_off;  // progressive: tasks per level, first task of each level
   uint32_t S = 4096, C = 512, items = kWG;  // entropy parameters for this image
   int64_t nseg = 0, nitems = 0, npix = 0, nwords = 0, nent = 0;
   int64_t seg0 = 0, item0 = 0, pix0 = 0, word0 = 0, desc = 0, coef0 = 0, ent0 = 0;
@@ -229,7 +232,43 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
   const cjd_frame_info& f = r.f;
   if (f.blocks_per_mcu > kMaxSlots) return CJD_UNSUPPORTED_ERROR;
   w.scans.clear();
-  if (!st.nscans) {
+  w.level.clear();
+  w.lev_count.clear();
+  if (st.progressive) {
+    // T.81 G.1.2: a scan of several components runs over the frame's MCUs,
+    // a one-component scan over the component's own block grid; a scan's
+    // level is one more than the last earlier scan's that codes one of its
+    // coefficients
+    for (int k = 0; k < st.nscans; ++k) {
+      const ScanInfo& sc = st.scans[(size_t)k];
+      ScanWork sw;
+      sw.prog = k;
+      sw.comp = sc.comp;
+      sw.off = sc.offset;
+      sw.len = sc.length;
+      sw.ri = sc.restart_interval;
+      if (sc.ncomp > 1) {
+        sw.units = (int64_t)f.mcus_x * f.mcus_y;
+      } else {
+        sw.h = f.samples_x[sc.comp];
+        sw.v = f.samples_y[sc.comp];
+        sw.bw = (f.comp_width[sc.comp] + 7) / 8;
+        sw.units = (int64_t)sw.bw * ((f.comp_height[sc.comp] + 7) / 8);
+      }
+      sw.bits = sc.huff_bits;
+      sw.vals = sc.huff_vals;
+      int lev = 0;
+      for (int t = 0; t < k; ++t) {
+        const ScanInfo& o = st.scans[(size_t)t];
+        bool shared = false;
+        for (int i = 0; i < sc.ncomp; ++i)
+          for (int q = 0; q < o.ncomp; ++q) shared |= sc.comps[i] == o.comps[q];
+        if (shared && o.ss <= sc.se && sc.ss <= o.se) lev = std::max(lev, w.level[(size_t)t] + 1);
+      }
+      w.level.push_back(lev);
+      w.scans.push_back(std::move(sw));
+    }
+  } else if (!st.nscans) {
     ScanWork sw;
     sw.off = f.scan_offset;
     sw.len = f.scan_length;
@@ -284,6 +323,11 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
     for (size_t k = 1; k < bd.size(); ++k) w.bounds.push_back(base + bd[k]);
     scan_bytes += sw.len;
     coded += sw.comp < 0 ? f.num_blocks : sw.units;
+    if (sw.prog >= 0) {
+      const size_t lev = (size_t)w.level[(size_t)sw.prog];
+      if (w.lev_count.size() <= lev) w.lev_count.resize(lev + 1, 0);
+      w.lev_count[lev] += sw.nseg;
+    }
   }
   w.nwords = 0;
   w.nitems = 0;
@@ -304,6 +348,7 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
   for (int64_t s = 0; s < w.nseg; ++s) {
     const size_t dlen = w.bounds[s + 1] - w.bounds[s];
     w.nwords += seg_words(dlen);
+    if (st.progressive) continue;  // decoded by k_prog: no entropy work items
     for_items(ntiles_of(dlen, w.S), per_first, per_next, [&](uint32_t t0, uint32_t n) {
       ++w.nitems;
       w.nent += (int64_t)item_words(n + (t0 > 0 ? 1u : 0u), g.rs, g.rd, g.nck);
@@ -316,7 +361,7 @@ int image_pass1(const Plan& b, ImageRec& r, ImageWork& w, const uint8_t* file, s
 
 // Pass 3 (per image, in parallel): write the image's words, descriptors and
 // work items at the offsets of pass 2.
-void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint8_t* file) {
+void image_pass3(Plan* b, int index, ImageRec& r, ImageWork& w, const uint8_t* file) {
   const cjd_frame_info& f = r.f;
   const int nh = f.blocks_per_mcu;
   ImageDesc d{};
@@ -430,6 +475,38 @@ void image_pass3(Plan* b, int index, ImageRec& r, const ImageWork& w, const uint
     const int64_t u0 = (s - sw.seg_rel) * per;
     const int64_t units = std::min<int64_t>(sw.units - u0, per);
     sd.tset = (uint32_t)(w.tset0 + (int64_t)k);
+    if (sw.prog >= 0) {
+      const ScanInfo& sc = w.st.scans[(size_t)sw.prog];
+      sd.prog = 1;
+      sd.nblocks = (uint32_t)units;
+      sd.first_block = u0;
+      sd.ntiles = 0;
+      ProgTask t{};
+      t.seg = (uint32_t)(w.seg0 + s);
+      t.u0 = (uint32_t)u0;
+      t.nunits = (uint32_t)units;
+      t.bw = (uint32_t)sw.bw;
+      t.ncomp = (uint8_t)sc.ncomp;
+      t.ss = (uint8_t)sc.ss;
+      t.se = (uint8_t)sc.se;
+      t.ah = (uint8_t)sc.ah;
+      t.al = (uint8_t)sc.al;
+      int bpu = 0;
+      for (int i = 0; i < sc.ncomp; ++i) {
+        t.comps[i] = (uint8_t)sc.comps[i];
+        t.dcs[i] = (uint8_t)sc.dcs[i];
+        bpu += sc.ncomp > 1 ? f.samples_x[sc.comps[i]] * f.samples_y[sc.comps[i]] : 1;
+      }
+      t.bpu = (uint8_t)bpu;
+      t.h = (uint8_t)sw.h;
+      t.v = (uint8_t)sw.v;
+      t.ac = (uint8_t)sc.ac_id;
+      t.last = s + 1 == sw.seg_rel + sw.nseg;
+      const size_t lev = (size_t)w.level[(size_t)sw.prog];
+      b->ptasks[(size_t)w.lev_off[lev]++] = t;
+      b->segs[(size_t)(w.seg0 + s)] = sd;
+      continue;
+    }
     if (sw.comp < 0) {
       sd.nblocks = (uint32_t)(units * nh);
       sd.first_block = d.coef_base + u0 * nh;
@@ -605,6 +682,35 @@ void plan_batch(Plan& b, int n, const uint8_t* const* files, const size_t* sizes
     b.total_pixels += r.pixels;
     b.scan_bytes += (int64_t)w.bounds.back();
   }
+  // progressive tasks, level by level across the batch; each image's tasks
+  // of a level follow those of the images before it
+  size_t nlev = 0;
+  for (int i = 0; i < n; ++i)
+    if (!b.recs[(size_t)i].error) nlev = std::max(nlev, work[(size_t)i].lev_count.size());
+  b.plevel.assign(nlev + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    if (b.recs[(size_t)i].error) continue;
+    const ImageWork& w = work[(size_t)i];
+    for (size_t l = 0; l < w.lev_count.size(); ++l) b.plevel[l + 1] += (uint32_t)w.lev_count[l];
+  }
+  for (size_t l = 0; l < nlev; ++l) b.plevel[l + 1] += b.plevel[l];
+  std::vector<int64_t> cursor(b.plevel.begin(), b.plevel.end());
+  b.pimgs.clear();
+  b.prog_blocks = 0;
+  for (int i = 0; i < n; ++i) {
+    const ImageRec& r = b.recs[(size_t)i];
+    if (r.error || !r.f.progressive) continue;
+    ImageWork& w = work[(size_t)i];
+    w.lev_off.assign(w.lev_count.size(), 0);
+    for (size_t l = 0; l < w.lev_count.size(); ++l) {
+      w.lev_off[l] = cursor[l];
+      cursor[l] += w.lev_count[l];
+    }
+    b.pimgs.push_back(ProgImage{r.coef_base, r.f.num_blocks, b.prog_blocks});
+    b.prog_blocks += r.f.num_blocks;
+  }
+  b.ptasks.resize(b.plevel.back());
+  if (nlev == 0) b.plevel.clear();
   b.total_blocks = nblocks;
   b.entry_words = nent;
   b.images.resize((size_t)nimg);

@@ -51,6 +51,13 @@ struct Plan {
   std::vector<SegmentDesc> segs;
   std::vector<WorkItem> items;
   std::vector<PixItem> pix;
+  // Progressive images (CJD_EXT_PROGRESSIVE): one task per restart interval
+  // of a scan, level by level (level l: tasks [plevel[l], plevel[l + 1])),
+  // and the images' coefficient ranges.
+  std::vector<ProgTask> ptasks;
+  std::vector<uint32_t> plevel;
+  std::vector<ProgImage> pimgs;
+  int64_t prog_blocks = 0;
   std::vector<HuffTable> tables;
   std::vector<uint16_t> luts;    // kLutEntries per image
   std::vector<uint16_t> qtabs;   // natural order, 64 per table

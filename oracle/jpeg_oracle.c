@@ -679,8 +679,10 @@ static int decode_scans_prog(orc_jpeg *j, int16_t *qcoef, int32_t *work[3]) {
         const long mcu = (by / v) * j->num_blocks_x + bx / h;
         err = prog_block(&P, 0, q + (mcu * bpm + slot0[c] + (by % v) * h + bx % h) * 64);
       }
-      /* src/decodeScanGPU.cu:476-486, per unit of the scan */
-      if (!err && ri && !(--rcount) && P.r.pos < P.r.end) {
+      /* between restart intervals (src/decodeScanGPU.cu:476-486, per unit of
+       * the scan): byte-align and read RSTn; after the scan's last interval
+       * nothing is read (data left there is ignored) */
+      if (!err && ri && !(--rcount) && u + 1 < units) {
         P.r.num_bufbits &= 0xF8;
         int m = get_bits(&P.r, 16);
         if ((m & 0xFFF8) != 0xFFD0 || (m & 7) != next_rst) err = ORC_SYNTAX_ERROR;

@@ -17,6 +17,7 @@
 #include "pixel_core.h"
 #include "plan.h"
 #include "unstuff_core.h"
+#include "prog_core.h"
 
 using namespace cjd;
 
@@ -396,6 +397,17 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
   // status words of an older launch with every bit set: this launch must
   // not read them as its own
   E->status.assign((size_t)n, (kEpoch - 1) << 8 | 0xFFu);
+  // progressive images (k_prog_prepare, then k_prog level by level; within a
+  // level in reverse task order, which must not matter)
+  {
+    const Plan& P = E->plan;
+    for (int64_t i = 0; i < P.prog_blocks; ++i)
+      prog_prepare_block(P.pimgs.data(), (uint32_t)P.pimgs.size(), i, E->coefs.data(), E->recs.data());
+    for (size_t l = 0; l + 1 < P.plevel.size(); ++l)
+      for (uint32_t t = P.plevel[l + 1]; t-- > P.plevel[l];)
+        prog_task(P.wptr, P.images.data(), P.segs.data(), P.ptasks[t], P.tables.data(), E->coefs.data(),
+                  E->status.data(), kEpoch);
+  }
   if (serial) {
     for (auto& s : E->plan.segs)
       ent_serial_segment(E->plan.wptr, E->plan.images.data(), s, E->plan.tables.data(), E->coefs.data(),
@@ -411,7 +423,8 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
     st = status_bits(E->status[(size_t)i], kEpoch);
     // as the engine: unresolved images and tiled syntax / truncation errors
     // are decoded again serially, and the serial verdict is reported
-    if (!serial && (st & (kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated))) {
+    if (!serial && !r.f.progressive &&
+        (st & (kStatusUnresolved | kStatusInconsistent | kStatusSyntax | kStatusTruncated))) {
       // the serial redo is a launch of its own (next epoch)
       ++E->fallbacks;
       for (uint32_t s = 0; s < r.seg_count; ++s)

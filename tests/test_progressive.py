@@ -188,3 +188,88 @@ def test_product_parser(name):
     # each component's quant table is the one in effect at its first scan
     for c in range(p.num_channels):
         assert list(info.dq_tables16[info.dq_id[c]]) == list(p._raw.comp_dq16[c])
+
+
+def pillow_progressive(w, h, seed, gray=False, q=85, **opts):
+    """A progressive file written by libjpeg through Pillow (imported here
+    only: the tests that need larger files than the committed fixtures)."""
+    Image = pytest.importorskip("PIL.Image")
+    import io
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_progressive", os.path.join(HERE, "golden",
+                                                                                    "make_progressive.py"))
+    mp = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mp)
+    b = io.BytesIO()
+    Image.fromarray(mp.pixels(w, h, seed, gray)).save(b, "JPEG", quality=q, progressive=True, **opts)
+    return b.getvalue()
+
+
+@pytest.mark.parametrize("serial", [False, True])
+def test_emulator_matches_oracle(serial):
+    """The planner (scan levels, restart intervals as tasks), k_prog_prepare
+    and k_prog (prog_core.h), lane by lane on the CPU, beside baseline files
+    in the same batch: coefficients equal the oracle's, integer-iDCT pixels
+    and planes too."""
+    from tests.emu import EmuResult
+    from tests.emu import lib as emulib
+    from cudajpeg_amd import synth
+    files = [pair(n)[1] for n in NAMES]
+    files += [pillow_progressive(203, 150, 21, restart_marker_rows=2),
+              pillow_progressive(120, 64, 22, gray=True, q=95)]
+    base = [synth.encode(synth.ImageSpec(96, 64, "420", 80, 6.0, 23)).data]
+    files = files[:3] + base + files[3:]
+    emulib().emu_ext(L.EXT_PROGRESSIVE)
+    try:
+        r = EmuResult(files, 0, 512, True, 0, serial)
+    finally:
+        emulib().emu_ext(0)
+    for i, data in enumerate(files):
+        assert r.errors[i] == 0, i
+        p = oracle.parse(data, PR)
+        q = oracle.decode_scan(p).qcoef
+        assert np.array_equal(r.get(i, 1, (p.num_blocks, 64), np.int16), q), i
+        pl = oracle.planes(p, q, 0)
+        for c in range(p.num_channels):
+            assert np.array_equal(r.get(i, 2 + c, pl[c].shape, np.uint8), pl[c]), (i, c)
+        ref = oracle.colour(p, pl)
+        assert np.array_equal(r.get(i, 0, ref.shape, np.uint8), ref), i
+
+
+def test_emulator_matches_oracle_on_corrupt_streams():
+    """Bytes of one scan's data overwritten at random (never making or
+    breaking a marker): the product's decoder (emulated) and the oracle give
+    the same verdict, and the same coefficients when both decode the file.
+    Between restart intervals both require the interval's data to end in its
+    last byte before the RSTn; data after a scan's last interval is
+    ignored."""
+    from tests.emu import EmuResult
+    from tests.emu import lib as emulib
+    rs = np.random.RandomState(5)
+    files = []
+    for it in range(330):
+        prog = bytearray(pair(NAMES[it % len(NAMES)])[1])
+        pp = oracle.parse(bytes(prog), PR)
+        sc = scans(pp)
+        a, b = scan_bounds(pp, sc[rs.randint(len(sc))])
+        for _ in range(rs.randint(1, 4)):
+            pos = rs.randint(a, b) if b > a else a
+            if pos >= b or prog[pos] == 0xFF or prog[pos - 1] == 0xFF:
+                continue
+            prog[pos] = min(rs.randint(256), 0xFE)
+        files.append(bytes(prog))
+    emulib().emu_ext(L.EXT_PROGRESSIVE)
+    try:
+        r = EmuResult(files, 0, 512, True, 0, False)
+    finally:
+        emulib().emu_ext(0)
+    verdicts = {0: 0, 1: 0}
+    for i, d in enumerate(files):
+        p = oracle.parse(d, PR)
+        assert p.error == 0
+        o = oracle.decode_scan(p)
+        assert r.errors[i] == o.error, i
+        verdicts[o.error] += 1
+        if not o.error:
+            assert np.array_equal(r.get(i, 1, (p.num_blocks, 64), np.int16), o.qcoef), i
+    assert verdicts[0] > 30 and verdicts[1] > 30  # both outcomes are exercised
