@@ -112,6 +112,10 @@ struct ProgTask {
   uint8_t al, bpu, h, v;      // h, v: a one-component scan's sampling factors
   uint8_t comps[3], last;     // components (frame indexes); last: the scan's final interval
   uint8_t dcs[3], ac;         // DC table ids (0..1) of the components, the AC table id (2..3)
+  // a scan of several components: the MCU's blocks in decode order, each
+  // block's slot (4 bits each) and its component's index in the scan (2 bits each)
+  uint64_t slots;
+  uint32_t cis, pad_;
 };
 
 // A progressive image's coefficient blocks: [coef_base, coef_base + nblocks)

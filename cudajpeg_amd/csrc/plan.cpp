@@ -493,9 +493,15 @@ void image_pass3(Plan* b, int index, ImageRec& r, ImageWork& w, const uint8_t* f
       t.al = (uint8_t)sc.al;
       int bpu = 0;
       for (int i = 0; i < sc.ncomp; ++i) {
-        t.comps[i] = (uint8_t)sc.comps[i];
+        const int c = sc.comps[i];
+        t.comps[i] = (uint8_t)c;
         t.dcs[i] = (uint8_t)sc.dcs[i];
-        bpu += sc.ncomp > 1 ? f.samples_x[sc.comps[i]] * f.samples_y[sc.comps[i]] : 1;
+        const int n = sc.ncomp > 1 ? f.samples_x[c] * f.samples_y[c] : 1;
+        for (int q = 0; q < n && sc.ncomp > 1; ++q) {
+          t.slots |= (uint64_t)(d.comp_first_slot[c] + q) << (4 * (bpu + q));
+          t.cis |= (uint32_t)i << (2 * (bpu + q));
+        }
+        bpu += n;
       }
       t.bpu = (uint8_t)bpu;
       t.h = (uint8_t)sw.h;

@@ -16,16 +16,21 @@ __global__ void __launch_bounds__(256) k_prog_prepare(const ProgImage* __restric
   if (i < total) prog_prepare_block(pimgs, n, i, coefs, recs);
 }
 
-// One lane per task of a level.
-__global__ void __launch_bounds__(64) k_prog(const ProgTask* __restrict__ tasks, uint32_t first, uint32_t count,
+// One wave64 per task of a level (prog_core.h: the lanes decode in lockstep,
+// lane z owning the coefficient at zigzag position z).
+__global__ void __launch_bounds__(64) k_prog(const ProgTask* __restrict__ tasks, uint32_t first,
                                              const uint32_t* __restrict__ words, const ImageDesc* __restrict__ images,
                                              const SegmentDesc* __restrict__ segs,
-                                             const HuffTable* __restrict__ tables, int16_t* coefs,
-                                             uint32_t* status, uint32_t epoch) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const ProgTask t = tasks[first + i];
-  prog_task(words, images, segs, t, tables, coefs, status, epoch);
+                                             const HuffTable* __restrict__ tables, int16_t* __restrict__ coefs,
+                                             uint32_t* __restrict__ status, uint32_t epoch) {
+  __shared__ ProgShared sh;
+  const ProgTask t = tasks[first + blockIdx.x];
+  const SegmentDesc sg = segs[t.seg];
+  const uint4* src = reinterpret_cast<const uint4*>(tables + (size_t)sg.tset * 4);
+  uint4* dst = reinterpret_cast<uint4*>(sh.tab);
+  for (uint32_t i = threadIdx.x; i < sizeof(sh.tab) / 16; i += 64) dst[i] = src[i];
+  __syncthreads();
+  prog_task(sh, words, images, sg, t, coefs, status, epoch);
 }
 
 hipError_t launch_prog(const ProgImage* pimgs, uint32_t npimgs, int64_t prog_blocks, const ProgTask* tasks,
@@ -40,8 +45,8 @@ hipError_t launch_prog(const ProgImage* pimgs, uint32_t npimgs, int64_t prog_blo
   for (uint32_t l = 0; l < nlevels && e == hipSuccess; ++l) {
     const uint32_t count = plevel[l + 1] - plevel[l];
     if (!count) continue;
-    hipLaunchKernelGGL(k_prog, dim3((count + 63) / 64), dim3(64), 0, s, tasks, plevel[l], count, words, images, segs,
-                       tables, coefs, status, epoch);
+    hipLaunchKernelGGL(k_prog, dim3(count), dim3(64), 0, s, tasks, plevel[l], words, images, segs, tables, coefs,
+                       status, epoch);
     e = hipGetLastError();
   }
   return e;

@@ -405,8 +405,12 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
       prog_prepare_block(P.pimgs.data(), (uint32_t)P.pimgs.size(), i, E->coefs.data(), E->recs.data());
     for (size_t l = 0; l + 1 < P.plevel.size(); ++l)
       for (uint32_t t = P.plevel[l + 1]; t-- > P.plevel[l];)
-        prog_task(P.wptr, P.images.data(), P.segs.data(), P.ptasks[t], P.tables.data(), E->coefs.data(),
-                  E->status.data(), kEpoch);
+      {
+        const SegmentDesc& sg = P.segs[P.ptasks[t].seg];
+        static ProgShared sh;  // the workgroup's LDS
+        memcpy(sh.tab, P.tables.data() + (size_t)sg.tset * 4, sizeof(sh.tab));
+        prog_task(sh, P.wptr, P.images.data(), sg, P.ptasks[t], E->coefs.data(), E->status.data(), kEpoch);
+      }
   }
   if (serial) {
     for (auto& s : E->plan.segs)

@@ -216,7 +216,8 @@ def test_emulator_matches_oracle(serial):
     from cudajpeg_amd import synth
     files = [pair(n)[1] for n in NAMES]
     files += [pillow_progressive(203, 150, 21, restart_marker_rows=2),
-              pillow_progressive(120, 64, 22, gray=True, q=95)]
+              pillow_progressive(120, 64, 22, gray=True, q=95),
+              pillow_progressive(640, 480, 24, q=95)]  # scans of many stream refills, blocks of many batches
     base = [synth.encode(synth.ImageSpec(96, 64, "420", 80, 6.0, 23)).data]
     files = files[:3] + base + files[3:]
     emulib().emu_ext(L.EXT_PROGRESSIVE)
