@@ -226,19 +226,28 @@ struct ProgMasks {
   uint64_t nz, neg, has;
 };
 
+// The set bits of `mask` whose rank (the number of set bits below) passes
+// keep(rank): on the device each lane l ranks its own bit with one popcount
+// and a ballot collects the answers; on the host a loop does the same.
+template <class F>
+CJD_HD inline uint64_t prog_rank_select(uint64_t mask, F&& keep) {
+  return prog_ballot([&](int l) {
+    return ((mask >> l) & 1) && keep(__builtin_popcountll(mask & ((1ull << l) - 1)));
+  });
+}
+
 // Correction bits of the nonzero coefficients in `span`, read in position
-// order; returns the positions whose bit is 1 and that lack bit Al (those
-// move 2^Al away from zero).
+// order (up to 32 at a time, the i-th bit read going to the i-th position);
+// returns the positions whose bit is 1 and that lack bit Al (those move 2^Al
+// away from zero).
 CJD_HD inline uint64_t prog_corrections(ProgBits& br, const ProgMasks& m, uint64_t span) {
   uint64_t todo = m.nz & span, bump = 0;
   while (todo) {
     const int c = __builtin_popcountll(todo), n = c < 32 ? c : 32;
-    uint32_t bits = br.get(n) << (32 - n);  // MSB first
-    for (int i = 0; i < n; ++i, bits <<= 1) {
-      const uint64_t low = todo & (~todo + 1);
-      todo ^= low;
-      if (bits & 0x80000000u) bump |= low;
-    }
+    const uint64_t part = n == c ? todo : prog_rank_select(todo, [](int r) { return r < 32; });
+    const uint32_t bits = br.get(n);  // bit n-1-i belongs to the i-th position of part
+    bump |= prog_rank_select(part, [&](int r) { return ((bits >> (n - 1 - r)) & 1u) != 0; });
+    todo &= ~part;
   }
   return bump & ~m.has;
 }
@@ -429,8 +438,7 @@ CJD_HD inline void prog_task(ProgShared& sh, const uint32_t* words, const ImageD
             }
             // the target: the (run + 1)-th coefficient from k on that is
             // zero; the nonzero ones before it take correction bits
-            uint64_t zeros = ~m.nz & prog_span(k, t.se);
-            for (int r = 0; r < run && zeros; ++r) zeros &= zeros - 1;
+            const uint64_t zeros = prog_rank_select(~m.nz & prog_span(k, t.se), [&](int r) { return r == run; });
             if (!zeros) {  // the band ends first
               bump |= prog_corrections(br, m, prog_span(k, t.se));
               k = t.se + 1;
