@@ -34,6 +34,9 @@ Reported (all by default; --no-plain, --no-serve, --no-cpu skip sections):
                      (a worker process pinned to one CPU); all_cores: the
                      whole batch over --cpu-workers (16) processes started
                      together; the host CPU model is stated
+
+--progressive times progressive files instead (progressive_bench) and prints
+one JSON line of its own.
 """
 from __future__ import annotations
 
@@ -331,6 +334,61 @@ def serve_child(args):
     return res
 
 
+def progressive_bench(args) -> dict:
+    """Progressive files (CJD_EXT_PROGRESSIVE), written by libjpeg through
+    Pillow with its standard ten-scan script: a 4096x3072 4:2:0 q85 file
+    without restart markers and with a restart interval per MCU row, and 64
+    restart-free 1024x768 files in one batch.  Decode time (stage excluded,
+    decode + finish, best of 3) beside libjpeg's own decode of the same
+    files through Pillow on one host core."""
+    import io
+    import importlib.util
+
+    import torch
+    from PIL import Image
+
+    from cudajpeg_amd import IDCT_INT, BatchDecoder
+    from cudajpeg_amd._lib import EXT_PROGRESSIVE
+
+    spec = importlib.util.spec_from_file_location("make_progressive",
+                                                  os.path.join(ROOT, "tests", "golden", "make_progressive.py"))
+    mp = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mp)
+
+    def encode(w, h, seed, **opts):
+        b = io.BytesIO()
+        Image.fromarray(mp.pixels(w, h, seed)).save(b, "JPEG", quality=85, progressive=True, **opts)
+        return b.getvalue()
+
+    cases = [("4096x3072, no restarts", [encode(4096, 3072, 41)], 4096 * 3072),
+             ("4096x3072, restart per MCU row", [encode(4096, 3072, 41, restart_marker_rows=1)], 4096 * 3072),
+             ("64 x 1024x768, no restarts", [encode(1024, 768, 100 + i) for i in range(64)], 64 * 1024 * 768)]
+    out = {}
+    for name, files, px in cases:
+        b = BatchDecoder(IDCT_INT)
+        b.set_extensions(EXT_PROGRESSIVE)
+        check(b.stage(files), name)
+        b.decode()
+        check(b.finish(), name)
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.decode()
+            check(b.finish(), name)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        b.close()
+        t0 = time.perf_counter()
+        for f in files:
+            Image.open(io.BytesIO(f)).load()
+        cpu = time.perf_counter() - t0
+        out[name] = {"gpu_ms": round(min(ts) * 1e3, 2), "gpu_mpix_s": round(px / min(ts) / 1e6, 1),
+                     "libjpeg_1core_ms": round(cpu * 1e3, 1), "libjpeg_1core_mpix_s": round(px / cpu / 1e6, 1)}
+        log(f"progressive {name}: {out[name]}")
+    return out
+
+
 def measured_traffic(kernel: str, args) -> int | None:
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this same workload (profiles/hbm_traffic_mixed256.md, counters
@@ -384,6 +442,8 @@ def main():
                     help="seconds without a completed request (some outstanding) that end the serving run")
     ap.add_argument("--serve-only", metavar="PATH", default=None,
                     help="only run the serving section and write its result (JSON) to PATH")
+    ap.add_argument("--progressive", action="store_true",
+                    help="only time progressive files (CJD_EXT_PROGRESSIVE; needs Pillow) and print them as JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (reads MIB, writes MIB) "
                          "three times: the reference point for scaling FETCH_SIZE/WRITE_SIZE counters")
@@ -404,6 +464,9 @@ def main():
         res = serve_bench(args)
         with open(args.serve_only, "w") as f:
             json.dump(res, f)
+        return
+    if args.progressive:
+        print(json.dumps({"progressive": progressive_bench(args)}), flush=True)
         return
     if args.copy_calibration:
         n = args.copy_calibration << 18  # int32 elements
