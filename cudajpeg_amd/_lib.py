@@ -76,7 +76,8 @@ class ServerCounters(ctypes.Structure):
     """cjd_server_counters"""
     _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "pending", "inflight", "redone", "batches",
                                               "stage_ns", "launch_ns", "finish_ns", "slot_wait_ns", "device_ns",
-                                              "device_max_ns", "turnaround_ns")]
+                                              "device_max_ns", "turnaround_ns", "finish_max_ns",
+                                              "protocol_redos")]
 
 
 class Completion(ctypes.Structure):

@@ -40,10 +40,13 @@ struct DevBuf {
   size_t cap = 0;
   int ensure(size_t bytes) {
     if (bytes <= cap && p) return 0;
+    // a buffer that has to grow again takes half as much again: hipFree waits
+    // for the whole device, so a serving loop should seldom pay for a growth
+    size_t n = std::max<size_t>(bytes, 256);
+    if (p) n = std::max(n, cap + cap / 2);
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    size_t n = std::max<size_t>(bytes, 256);
     if (hipMalloc(&p, n) != hipSuccess) {
       p = nullptr;
       return CJD_DEVICE_MEM_ERROR;
@@ -69,10 +72,11 @@ struct PinnedBuf {
   size_t cap = 0;
   bool ensure(size_t bytes) {
     if (bytes <= cap && p) return true;
+    // room to grow, and half as much again as before when growing again
+    const size_t n = std::max<size_t>(std::max(bytes + bytes / 8, p ? cap + cap / 2 : 0), 4096);
     if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
-    const size_t n = std::max<size_t>(bytes + bytes / 8, 4096);  // room to grow
     if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
       p = nullptr;
       return false;

@@ -5,8 +5,8 @@
 // Threads: callers submit / poll / release; a batcher thread forms batches
 // (flush on max_batch waiting requests or on the oldest request's deadline,
 // admission by decoded-frame bytes), stages them with the parallel host
-// planner and launches them on the stream of a free batch slot; a completion
-// thread waits for the slots in launch order and publishes their frames.  A
+// planner and launches them on the stream of a free batch slot; each slot's
+// completion thread waits for that slot's decode and publishes its frames.  A
 // slot is reused once every frame it holds has been released, so a frame's
 // device memory is stable from completion to release.
 #include <hip/hip_runtime.h>
@@ -50,7 +50,9 @@ struct Slot {
   std::vector<Request*> reqs;
   int held = 0;       // completed requests of this slot not yet released
   bool busy = false;  // staged / in flight / holding frames
+  bool launched = false;  // a decode is in flight for this slot's completer
   int64_t redone_seen = 0;  // cjd_batch_fallbacks of b after its last finish
+  int64_t protocol_seen = 0;  // cjd_batch_protocol_redos of b after its last finish
 };
 
 }  // namespace
@@ -60,20 +62,22 @@ struct cjd_server {
   cjd_server_options opt{};
   std::mutex mu;
   std::condition_variable cv_work;      // batcher: requests, a freed slot or budget, stop
-  std::condition_variable cv_inflight;  // completer: a launched slot, stop
+  std::condition_variable cv_inflight;  // completers: a launched slot, stop
   std::condition_variable cv_done;      // pollers: completions
   std::deque<Request*> pending;
-  std::deque<int> inflight;
+  int inflight = 0;  // slots launched and not yet finished
   std::deque<Request*> completed;
   std::unordered_map<uint64_t, Request*> live;  // submitted and not released
   std::vector<Slot> slots;
   uint64_t next_id = 1;
   uint64_t admitted_bytes = 0;
   int64_t n_submitted = 0, n_completed = 0, n_redone = 0;
-  int64_t n_batches = 0, t_stage = 0, t_launch = 0, t_finish = 0, t_slot_wait = 0;
+  int64_t n_batches = 0, t_stage = 0, t_launch = 0, t_finish = 0, t_slot_wait = 0, t_finish_max = 0,
+          n_protocol = 0;
   int64_t t_device = 0, t_device_max = 0, t_turnaround = 0;
   bool stop = false;
-  std::thread batcher, completer;
+  std::thread batcher;
+  std::vector<std::thread> completers;  // one per slot
 
   void complete_locked(Request* r) {
     ++n_completed;
@@ -160,20 +164,22 @@ struct cjd_server {
       ++n_batches;
       t_stage += tl - ts;
       t_launch += te - tl;
-      inflight.push_back(si);
-      cv_inflight.notify_one();
+      sl.launched = true;
+      ++inflight;
+      cv_inflight.notify_all();
     }
   }
 
-  void completer_loop() {
+  // Slot si's completer: finishes each decode launched on the slot.  Every
+  // slot has its own, so a batch whose finish redoes an image delays only its
+  // own requests, not the batches launched after it on other slots.
+  void completer_loop(int si) {
     (void)hipSetDevice(ctx->device);
+    Slot& sl = slots[(size_t)si];
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-      cv_inflight.wait(lk, [&] { return stop || !inflight.empty(); });
-      if (inflight.empty()) return;  // stopping and drained
-      const int si = inflight.front();
-      inflight.pop_front();
-      Slot& sl = slots[(size_t)si];
+      cv_inflight.wait(lk, [&] { return stop || sl.launched; });
+      if (!sl.launched) return;  // stopping and drained
       lk.unlock();
       const int n = (int)sl.reqs.size();
       std::vector<int> errs((size_t)n, 0);
@@ -181,6 +187,7 @@ struct cjd_server {
       const int e = cjd_batch_finish(sl.b, sl.stream, errs.data());
       const int64_t tr = mono_ns();
       const int64_t redone = cjd_batch_fallbacks(sl.b);  // read by the thread that owns the batch now
+      const int64_t protocol = cjd_batch_protocol_redos(sl.b);
       float ms_ent = 0.f, ms_pix = 0.f, ms_all = 0.f;
       if (!e) (void)cjd_batch_timings(sl.b, &ms_ent, &ms_pix, &ms_all);
       const int64_t dev_ns = (int64_t)((double)ms_all * 1e6);
@@ -196,12 +203,17 @@ struct cjd_server {
       }
       lk.lock();
       t_finish += tr - tf;
+      t_finish_max = std::max(t_finish_max, tr - tf);
+      n_protocol += protocol - sl.protocol_seen;
+      sl.protocol_seen = protocol;
       t_device += dev_ns;
       t_device_max = std::max(t_device_max, dev_ns);
       t_turnaround += tr - t_launched;
       n_redone += redone - sl.redone_seen;
       sl.redone_seen = redone;
       sl.held = n;
+      sl.launched = false;
+      --inflight;
       for (Request* r : sl.reqs) complete_locked(r);
     }
   }
@@ -235,7 +247,7 @@ extern "C" int cjd_server_create(cjd_context* ctx, const cjd_server_options* opt
     }
   }
   s->batcher = std::thread([s] { s->batcher_loop(); });
-  s->completer = std::thread([s] { s->completer_loop(); });
+  for (int k = 0; k < s->opt.slots; ++k) s->completers.emplace_back([s, k] { s->completer_loop(k); });
   *out = s;
   return 0;
 }
@@ -294,7 +306,7 @@ extern "C" int cjd_server_stats(cjd_server* s, cjd_server_counters* out) {
   out->submitted = s->n_submitted;
   out->completed = s->n_completed;
   out->pending = (int64_t)s->pending.size();
-  out->inflight = (int64_t)s->inflight.size();
+  out->inflight = (int64_t)s->inflight;
   out->redone = s->n_redone;
   out->batches = s->n_batches;
   out->stage_ns = s->t_stage;
@@ -304,6 +316,8 @@ extern "C" int cjd_server_stats(cjd_server* s, cjd_server_counters* out) {
   out->device_ns = s->t_device;
   out->device_max_ns = s->t_device_max;
   out->turnaround_ns = s->t_turnaround;
+  out->finish_max_ns = s->t_finish_max;
+  out->protocol_redos = s->n_protocol;
   return 0;
 }
 
@@ -336,7 +350,8 @@ extern "C" void cjd_server_destroy(cjd_server* s) {
   s->cv_work.notify_all();
   if (s->batcher.joinable()) s->batcher.join();
   s->cv_inflight.notify_all();
-  if (s->completer.joinable()) s->completer.join();
+  for (std::thread& t : s->completers)
+    if (t.joinable()) t.join();
   for (Slot& sl : s->slots) {
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.b) cjd_batch_destroy(sl.b);
