@@ -26,11 +26,7 @@ __global__ void __launch_bounds__(64) k_prog(const ProgTask* __restrict__ tasks,
   __shared__ ProgShared sh;
   const ProgTask t = tasks[first + blockIdx.x];
   const SegmentDesc sg = segs[t.seg];
-  const uint4* src = reinterpret_cast<const uint4*>(tables + (size_t)sg.tset * 4);
-  uint4* dst = reinterpret_cast<uint4*>(sh.tab);
-  for (uint32_t i = threadIdx.x; i < sizeof(sh.tab) / 16; i += 64) dst[i] = src[i];
-  __syncthreads();
-  prog_task(sh, words, images, sg, t, coefs, status, epoch);
+  prog_task(sh, words, images, sg, t, tables + (size_t)sg.tset * 4, coefs, status, epoch);
 }
 
 hipError_t launch_prog(const ProgImage* pimgs, uint32_t npimgs, int64_t prog_blocks, const ProgTask* tasks,

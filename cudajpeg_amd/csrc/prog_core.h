@@ -35,7 +35,6 @@ constexpr int kProgBatch = 64;        // refinement blocks fetched at a time
 
 // LDS of one k_prog workgroup (a local variable of the emulator's).
 struct ProgShared {
-  HuffTable tab[4];                // the scan's Huffman tables
   uint32_t ring[kProgRingWords];   // stream words as stored (big-endian bytes)
   alignas(16) int16_t blk[kProgBatch][64];  // a batch of refinement blocks, natural order
 };
@@ -257,8 +256,9 @@ struct ProgRow {
   uint32_t w[32];
 };
 
-// Decode one task (a restart interval of a scan) into coefs; sh.tab holds the
-// scan's four Huffman tables.  DC first: the DC difference as in baseline,
+// Decode one task (a restart interval of a scan) into coefs; tab holds the
+// scan's four Huffman tables (the table set sg.tset; on the device their
+// lookups, at wave-uniform addresses, are scalar loads).  DC first: the DC difference as in baseline,
 // the predictor stored << Al; DC refine: one raw bit per block ORed in at bit
 // Al; AC first: run/size symbols over Ss..Se with values << Al, EOBn (size
 // 0, run r < 15) ending this block and the next 2^r + (r extra bits) - 1
@@ -271,9 +271,9 @@ struct ProgRow {
 // (for an interval followed by an RSTn) data that does not end in the
 // interval's last byte sets kStatusSyntax.
 CJD_HD inline void prog_task(ProgShared& sh, const uint32_t* words, const ImageDesc* images, const SegmentDesc& sg,
-                             const ProgTask& t, int16_t* coefs, uint32_t* status, uint32_t epoch) {
+                             const ProgTask& t, const HuffTable* tab, int16_t* coefs, uint32_t* status,
+                             uint32_t epoch) {
   const ImageDesc& im = images[sg.image];
-  const HuffTable* tab = sh.tab;
   ProgBits br;
   br.init(words + sg.word_base, sg.nwords, sh.ring);
   const uint32_t nb = t.nunits * t.bpu;

@@ -408,8 +408,8 @@ void* emu_run(int n, const uint8_t* const* files, const size_t* sizes, int idct_
       {
         const SegmentDesc& sg = P.segs[P.ptasks[t].seg];
         static ProgShared sh;  // the workgroup's LDS
-        memcpy(sh.tab, P.tables.data() + (size_t)sg.tset * 4, sizeof(sh.tab));
-        prog_task(sh, P.wptr, P.images.data(), sg, P.ptasks[t], E->coefs.data(), E->status.data(), kEpoch);
+        prog_task(sh, P.wptr, P.images.data(), sg, P.ptasks[t], P.tables.data() + (size_t)sg.tset * 4,
+                  E->coefs.data(), E->status.data(), kEpoch);
       }
   }
   if (serial) {
