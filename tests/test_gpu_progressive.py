@@ -10,10 +10,8 @@ Pillow) beside baseline files in the same batch:
 * integer-iDCT pixels equal the oracle's; FP32 and MFMA planes are within 1
   of the oracle's FP64 planes;
 * a second decode of the staged batch, graph replays, the serial entropy
-  mode and the reader API give the same; corrupt streams get the oracle's
-  verdict."""
-import io
-import os
+  mode, the decode in parts and the reader API give the same; corrupt
+  streams get the oracle's verdict."""
 import time
 
 import numpy as np
@@ -171,4 +169,27 @@ def test_progressive_corrupt_streams(gpu):
         assert errs[i] == o.error, i
         if not o.error:
             assert np.array_equal(b.coefficients(i), o.qcoef), i
+    b.close()
+
+
+@pytest.mark.parametrize("parts", [1, 2, 3])
+def test_progressive_in_decode_parts(gpu, parts):
+    """Progressive images between baseline images that give the batch
+    hundreds of entropy work items (512-bit tiles, 16 per item), so the
+    decode runs in parts: the progressive scans are decoded before the first
+    run's entropy launch, whichever run their pixels fall in."""
+    base = [synth.encode(synth.ImageSpec(1024, 1024, "420", 90, 8.0, 70 + i)).data for i in range(6)]
+    prog = [pair(n)[1] for n in NAMES[:5]] + [pillow_progressive(640, 480, 31, q=90)]
+    files = base[:3] + prog[:3] + base[3:] + prog[3:]
+    b = BatchDecoder(IDCT_INT, tile_bits=512)
+    b.set_tuning(0, 16)
+    b.set_overlap(parts)
+    b.set_extensions(EXT_PROGRESSIVE)
+    assert not b.stage(files).any()
+    for rep in range(2):
+        b.decode()
+        assert not b.finish().any()
+        for i, d in enumerate(files):
+            print("case progressive parts", parts, rep, i, flush=True)
+            _check(b, i, d, IDCT_INT)
     b.close()
