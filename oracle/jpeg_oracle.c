@@ -9,7 +9,10 @@
  * (2) the reference sources compiled from /root/reference into oracle/_ref
  * (CPUonly pixels; src/decodeScanCPU.cu dequantised coefficients) on the
  * golden files, (3) closed forms (iDCT shortcut algebra, FP64 iDCT vs the
- * integer iDCT bound).
+ * integer iDCT bound), (4) for progressive scans (T.81 G.1.2, which the
+ * reference does not decode): libjpeg's baseline and progressive files of the
+ * same pixels, whose coefficients must agree, and every scan prefix of a
+ * progressive file against T.81's point transform of the final coefficients.
  */
 #include "jpeg_oracle.h"
 
