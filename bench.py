@@ -5,7 +5,8 @@ in 256x256..4096x3072, aspect 1:1/4:3/16:9, 70% 4:2:0 / 20% 4:4:4 / 10% gray,
 quality U{60..95}, noise U[2,12] -- with the scans already resident in HBM
 and the RGB/gray frames left in HBM.
 
-Reported (all by default; --no-plain, --no-serve, --no-cpu skip sections):
+Reported (all by default; --no-plain, --no-serve, --no-progressive, --no-cpu
+skip sections):
   value              batched form, Mpixel/s (W*H): each decode runs in
                      --parts runs of images (2): each run's pixel kernel on a
                      second stream beside the next runs' entropy decodes,
@@ -35,8 +36,13 @@ Reported (all by default; --no-plain, --no-serve, --no-cpu skip sections):
                      whole batch over --cpu-workers (16) processes started
                      together; the host CPU model is stated
 
---progressive times progressive files instead (progressive_bench) and prints
-one JSON line of its own.
+  progressive        progressive files (CJD_EXT_PROGRESSIVE, written by
+                     libjpeg through Pillow; progressive_bench): one
+                     4096x3072 file without and with restart intervals, 64
+                     restart-free 1024x768 files, beside libjpeg's one-core
+                     decode of the same files
+
+--progressive runs only that section and prints one JSON line of its own.
 """
 from __future__ import annotations
 
@@ -432,6 +438,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain form (autotune, then one image per launch) and its latencies")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop load test")
+    ap.add_argument("--no-progressive", action="store_true", help="skip the progressive-file timings")
     ap.add_argument("--no-graph", action="store_true", help="skip the timing of graph replays of the decode")
     ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
                     default=[500.0, 1000.0, 2000.0, 4000.0, 8000.0], help="Poisson rates of the sweep, req/s")
@@ -692,6 +699,13 @@ def main():
         for d in per:
             d.close()
 
+    progressive = None
+    if not args.no_progressive:
+        try:
+            progressive = progressive_bench(args)
+        except ImportError as e:  # Pillow writes the progressive files
+            log(f"progressive files skipped: {e}")
+
     cpu = None if args.no_cpu else cpu_baseline(encs, args.cpu_seconds)
     if cpu is not None and args.cpu_workers > 1:
         cpu["all_cores"] = cpu_all_cores(encs, args.cpu_workers, 4)
@@ -743,6 +757,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args),
                      "algorithmic_bytes": int(dom_bytes)},
         "serving": serving,
+        "progressive": progressive,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
