@@ -69,7 +69,8 @@ class TuneEntry(ctypes.Structure):
 class ServerOptions(ctypes.Structure):
     """cjd_server_options"""
     _fields_ = [("max_batch", ctypes.c_int), ("max_wait_us", ctypes.c_int), ("frame_budget", ctypes.c_uint64),
-                ("slots", ctypes.c_int), ("idct_mode", ctypes.c_int), ("tile_bits", ctypes.c_int)]
+                ("slots", ctypes.c_int), ("idct_mode", ctypes.c_int), ("tile_bits", ctypes.c_int),
+                ("extensions", ctypes.c_int)]
 
 
 class ServerCounters(ctypes.Structure):

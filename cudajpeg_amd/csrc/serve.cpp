@@ -236,6 +236,7 @@ extern "C" int cjd_server_create(cjd_context* ctx, const cjd_server_options* opt
   for (Slot& sl : s->slots) {
     int e = cjd_batch_create(ctx, &sl.b);
     if (!e) e = cjd_batch_set_options(sl.b, s->opt.idct_mode, CJD_ENTROPY_TILED, s->opt.tile_bits, 0);
+    if (!e) e = cjd_batch_set_extensions(sl.b, s->opt.extensions);
     if (!e && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) e = CJD_DEVICE_MEM_ERROR;
     if (e) {
       for (Slot& t : s->slots) {
@@ -255,7 +256,7 @@ extern "C" int cjd_server_create(cjd_context* ctx, const cjd_server_options* opt
 extern "C" int cjd_server_submit(cjd_server* s, const uint8_t* data, size_t size, uint64_t* id) {
   if (!s || !data || !id) return CJD_PROGRAMMER_ERROR;
   cjd_frame_info f{};
-  const int e = cjd_parse(data, size, &f);
+  const int e = cjd_parse_ext(data, size, s->opt.extensions, &f);
   if (e) return e;
   auto* r = new (std::nothrow) Request;
   if (!r) return CJD_OOM_ERROR;

@@ -38,9 +38,9 @@ class Completion:
 
 class Server:
     def __init__(self, max_batch: int = 64, max_wait_us: int = 2000, frame_budget: int = 64 << 30, slots: int = 4,
-                 idct: int = L.IDCT_MFMA, tile_bits: int = 0, ctx: Context | None = None):
+                 idct: int = L.IDCT_MFMA, tile_bits: int = 0, ctx: Context | None = None, extensions: int = 0):
         self.ctx = ctx or default_context()
-        opt = L.ServerOptions(max_batch, max_wait_us, frame_budget, slots, idct, tile_bits)
+        opt = L.ServerOptions(max_batch, max_wait_us, frame_budget, slots, idct, tile_bits, extensions)
         self.h = ctypes.c_void_p()
         _check(L.lib().cjd_server_create(self.ctx.h, ctypes.byref(opt), ctypes.byref(self.h)), "server: ")
         self._buf = (L.Completion * 256)()

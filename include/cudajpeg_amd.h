@@ -332,6 +332,7 @@ typedef struct {
   int slots;              /* batches in flight or holding frames (0 = 4) */
   int idct_mode;          /* CJD_IDCT_* */
   int tile_bits;          /* entropy tile length (0 = default) */
+  int extensions;         /* CJD_EXT_* the loop accepts (0 = refuse those files as the reference does) */
 } cjd_server_options;
 
 typedef struct {

@@ -193,3 +193,32 @@ def test_progressive_in_decode_parts(gpu, parts):
             print("case progressive parts", parts, rep, i, flush=True)
             _check(b, i, d, IDCT_INT)
     b.close()
+
+
+def test_progressive_through_serving_loop(gpu):
+    """The serving loop with CJD_EXT_PROGRESSIVE in its options decodes
+    progressive requests beside baseline ones; without it, submitting a
+    progressive file fails with the reference's SYNTAX_ERROR."""
+    from cudajpeg_amd.serve import Server
+    files = [pair(n)[1] for n in NAMES] + [synth.encode(synth.ImageSpec(96, 64, "420", 80, 6.0, 80)).data]
+    srv = Server(max_batch=8, idct=IDCT_INT)
+    with pytest.raises(Exception) as exc:
+        srv.submit(files[0])
+    assert getattr(exc.value, "code", None) == 1
+    srv.close()
+    srv = Server(max_batch=8, idct=IDCT_INT, extensions=EXT_PROGRESSIVE)
+    ids = {srv.submit(d): i for i, d in enumerate(files)}
+    got = {}
+    deadline = time.time() + 60
+    while len(got) < len(files) and time.time() < deadline:
+        for c in srv.poll(wait_us=100000):
+            i = ids[c.id]
+            assert c.error == 0, i
+            got[i] = np.asarray(srv.download(c)).copy()
+            srv.release(c.id)
+    srv.close()
+    assert len(got) == len(files)
+    for i, d in enumerate(files):
+        p = oracle.parse(d, PR)
+        ref = oracle.colour(p, oracle.planes(p, oracle.decode_scan(p).qcoef, 0))
+        assert np.array_equal(got[i].reshape(-1), ref.reshape(-1)), i
