@@ -12,16 +12,19 @@
 // codes a coefficient it codes (same component, overlapping band), so the
 // scans of one level touch disjoint coefficients and run side by side, one
 // launch per level.  A task writes only the coefficients of its own band, as
-// 16-bit stores, so tasks of one level never write the same bytes.
+// 16-bit stores (the DC refinement as an atomic OR on the DC's word), so
+// tasks of one level never write the same bytes.
 //
 // The wave decodes in lockstep: the Huffman decode is the same in every lane,
 // and lane z owns the coefficient at zigzag position z of the current block
 // (its nonzero / sign / bit-Al masks are three ballots; the coefficient's new
 // value is kept in the lane and stored once per block).  Memory waits are
-// what a lone wave cannot hide, so none is left in the per-block loop: the
-// stream is read from a ring in LDS refilled 4 KB at a time, and a
-// refinement scan's blocks are fetched 64 at a time, one batch ahead, into
-// registers and then LDS.  On the host (the emulator) one thread plays all 64
+// what a lone wave cannot hide, and on CDNA the vector-memory counter also
+// counts stores, so no vector-memory wait is left in the per-block loop: the
+// stream is read from a ring in LDS refilled 4 KB at a time, a refinement
+// scan's blocks are fetched 64 at a time, one batch ahead, into registers and
+// then LDS, and the Huffman lookups (wave-uniform addresses) are scalar
+// loads, which wait on a counter of their own.  On the host (the emulator) one thread plays all 64
 // lanes: the per-lane values are arrays and the lane loops run in full.
 #pragma once
 #include "entropy_core.h"
