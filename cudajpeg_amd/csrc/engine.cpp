@@ -98,8 +98,7 @@ struct cjd_batch {
   Plan plan;
 
   DevBuf d_words, d_images, d_segs, d_items, d_pix, d_tables, d_luts, d_qtabs, d_coefs, d_out, d_scratch;
-  DevBuf d_btab;  // MFMA table operands per image (k_mfma_tables at the first MFMA decode after a stage)
-  bool btab_stale = true;  // d_btab does not yet hold the staged images' tables
+  DevBuf d_btab;  // MFMA table operands per image (k_mfma_tables at staging)
   DevBuf d_ptasks, d_pimgs;  // progressive images (CJD_EXT_PROGRESSIVE): k_prog tasks and coefficient ranges
   // Pinned staging: the planner writes the scan words into h_words, the
   // descriptors and tables are packed into h_meta, and every upload is a DMA
@@ -407,7 +406,9 @@ extern "C" int cjd_batch_stage_async(cjd_batch* b, int n, const uint8_t* const* 
     else b->up_pending = true;
   }
   if (!e) e = b->d_btab.ensure(b->plan.images.size() * (size_t)kMfmaTabVecs * 16);
-  b->btab_stale = true;
+  if (!e && launch_mfma_tables(b->d_images.as<ImageDesc>(), (uint32_t)b->plan.images.size(), b->d_qtabs.as<uint16_t>(),
+                               b->d_btab.as<uint4>(), s) != hipSuccess)
+    e = CJD_KERNEL_LAUNCH_ERROR;
   if (!e) e = b->d_coefs.ensure((size_t)b->plan.total_blocks * 128);
   // images with one-component scans: zero coefficient blocks as well (the
   // serial decoder writes the coded blocks only)
@@ -531,19 +532,6 @@ static int enqueue_entropy(cjd_batch* b, hipStream_t s, bool timed, bool capture
   return 0;
 }
 
-// MFMA table operands of the staged images, computed on s at the first MFMA
-// decode after a stage (the other iDCT forms never read them).  A captured
-// launch runs at every replay and leaves the flag set, so a direct decode
-// after a capture still computes them itself.
-static int ensure_mfma_tables(cjd_batch* b, hipStream_t s, bool captured) {
-  if (b->idct_mode != kIdctMfma || !b->btab_stale) return 0;
-  if (launch_mfma_tables(b->d_images.as<ImageDesc>(), (uint32_t)b->plan.images.size(), b->d_qtabs.as<uint16_t>(),
-                         b->d_btab.as<uint4>(), s) != hipSuccess)
-    return CJD_KERNEL_LAUNCH_ERROR;
-  if (!captured) b->btab_stale = false;
-  return 0;
-}
-
 static hipError_t pixel_range(cjd_batch* b, uint32_t first, uint32_t count, hipStream_t s) {
   return launch_pixel(b->idct_mode, b->d_images.as<ImageDesc>(), b->d_pix.as<PixItem>() + first, count,
                       b->d_recs.as<BlockRec>(), b->d_ebuf.as<uint32_t>(), (uint64_t)b->plan.entry_words,
@@ -619,7 +607,6 @@ static int enqueue_split(cjd_batch* b, hipStream_t s, bool timed, bool captured 
 }
 
 static int enqueue(cjd_batch* b, hipStream_t s, bool timed, bool captured = false) {
-  if (int e = ensure_mfma_tables(b, s, captured)) return e;
   if (b->nsplit > 1 && b->entropy_mode == CJD_ENTROPY_TILED) return enqueue_split(b, s, timed, captured);
   int e = enqueue_entropy(b, s, timed, captured);
   return e ? e : enqueue_pixel(b, s, timed);
@@ -788,7 +775,6 @@ extern "C" int cjd_batch_decode_scan(cjd_batch* b, void* stream) {
 
 extern "C" int cjd_batch_transform(cjd_batch* b, void* stream) {
   if (!b) return CJD_PROGRAMMER_ERROR;
-  if (int e = ensure_mfma_tables(b, (hipStream_t)stream, false)) return e;
   return enqueue_pixel(b, (hipStream_t)stream, false);
 }
 
