@@ -5,8 +5,8 @@
 // Threads: callers submit / poll / release; a batcher thread forms batches
 // (flush on max_batch waiting requests or on the oldest request's deadline,
 // admission by decoded-frame bytes), stages them with the parallel host
-// planner and launches them on the stream of a free batch slot; each slot's
-// completion thread waits for that slot's decode and publishes its frames.  A
+// planner and launches them on the stream of a free batch slot; a completion
+// thread waits for the slots in launch order and publishes their frames.  A
 // slot is reused once every frame it holds has been released, so a frame's
 // device memory is stable from completion to release.
 #include <hip/hip_runtime.h>
@@ -50,7 +50,6 @@ struct Slot {
   std::vector<Request*> reqs;
   int held = 0;       // completed requests of this slot not yet released
   bool busy = false;  // staged / in flight / holding frames
-  bool launched = false;  // a decode is in flight for this slot's completer
   int64_t redone_seen = 0;  // cjd_batch_fallbacks of b after its last finish
   int64_t protocol_seen = 0;  // cjd_batch_protocol_redos of b after its last finish
 };
@@ -62,10 +61,10 @@ struct cjd_server {
   cjd_server_options opt{};
   std::mutex mu;
   std::condition_variable cv_work;      // batcher: requests, a freed slot or budget, stop
-  std::condition_variable cv_inflight;  // completers: a launched slot, stop
+  std::condition_variable cv_inflight;  // completer: a launched slot, stop
   std::condition_variable cv_done;      // pollers: completions
   std::deque<Request*> pending;
-  int inflight = 0;  // slots launched and not yet finished
+  std::deque<int> inflight;  // launched slots, in launch order
   std::deque<Request*> completed;
   std::unordered_map<uint64_t, Request*> live;  // submitted and not released
   std::vector<Slot> slots;
@@ -76,8 +75,7 @@ struct cjd_server {
           n_protocol = 0;
   int64_t t_device = 0, t_device_max = 0, t_turnaround = 0;
   bool stop = false;
-  std::thread batcher;
-  std::vector<std::thread> completers;  // one per slot
+  std::thread batcher, completer;
 
   void complete_locked(Request* r) {
     ++n_completed;
@@ -164,22 +162,21 @@ struct cjd_server {
       ++n_batches;
       t_stage += tl - ts;
       t_launch += te - tl;
-      sl.launched = true;
-      ++inflight;
-      cv_inflight.notify_all();
+      inflight.push_back(si);
+      cv_inflight.notify_one();
     }
   }
 
-  // Slot si's completer: finishes each decode launched on the slot.  Every
-  // slot has its own, so a batch whose finish redoes an image delays only its
-  // own requests, not the batches launched after it on other slots.
-  void completer_loop(int si) {
+  // Finishes the launched slots in launch order.
+  void completer_loop() {
     (void)hipSetDevice(ctx->device);
-    Slot& sl = slots[(size_t)si];
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-      cv_inflight.wait(lk, [&] { return stop || sl.launched; });
-      if (!sl.launched) return;  // stopping and drained
+      cv_inflight.wait(lk, [&] { return stop || !inflight.empty(); });
+      if (inflight.empty()) return;  // stopping and drained
+      const int si = inflight.front();
+      inflight.pop_front();
+      Slot& sl = slots[(size_t)si];
       lk.unlock();
       const int n = (int)sl.reqs.size();
       std::vector<int> errs((size_t)n, 0);
@@ -212,8 +209,6 @@ struct cjd_server {
       n_redone += redone - sl.redone_seen;
       sl.redone_seen = redone;
       sl.held = n;
-      sl.launched = false;
-      --inflight;
       for (Request* r : sl.reqs) complete_locked(r);
     }
   }
@@ -248,7 +243,7 @@ extern "C" int cjd_server_create(cjd_context* ctx, const cjd_server_options* opt
     }
   }
   s->batcher = std::thread([s] { s->batcher_loop(); });
-  for (int k = 0; k < s->opt.slots; ++k) s->completers.emplace_back([s, k] { s->completer_loop(k); });
+  s->completer = std::thread([s] { s->completer_loop(); });
   *out = s;
   return 0;
 }
@@ -307,7 +302,7 @@ extern "C" int cjd_server_stats(cjd_server* s, cjd_server_counters* out) {
   out->submitted = s->n_submitted;
   out->completed = s->n_completed;
   out->pending = (int64_t)s->pending.size();
-  out->inflight = (int64_t)s->inflight;
+  out->inflight = (int64_t)s->inflight.size();
   out->redone = s->n_redone;
   out->batches = s->n_batches;
   out->stage_ns = s->t_stage;
@@ -351,8 +346,7 @@ extern "C" void cjd_server_destroy(cjd_server* s) {
   s->cv_work.notify_all();
   if (s->batcher.joinable()) s->batcher.join();
   s->cv_inflight.notify_all();
-  for (std::thread& t : s->completers)
-    if (t.joinable()) t.join();
+  if (s->completer.joinable()) s->completer.join();
   for (Slot& sl : s->slots) {
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.b) cjd_batch_destroy(sl.b);

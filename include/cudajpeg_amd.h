@@ -318,9 +318,8 @@ const char *cjd_error_name(int code);
  * arrive; a batcher thread flushes when max_batch requests wait or the oldest
  * has waited max_wait_us, admits requests while the decoded frames held by
  * the loop fit frame_budget bytes, stages each batch while earlier ones
- * decode (`slots` batches, one stream each), and each slot's completion
- * thread hands its batch's finished frames to cjd_server_poll (a batch whose
- * finish decodes an image again delays only its own requests).  Frames stay in device memory until
+ * decode (`slots` batches, one stream each), and a completion thread hands
+ * finished frames to cjd_server_poll, batch by batch in launch order.  Frames stay in device memory until
  * cjd_server_release; every completed request (also one with an error) must
  * be released. */
 typedef struct cjd_server cjd_server;
@@ -358,8 +357,8 @@ int cjd_server_release(cjd_server *s, uint64_t id);
  * submitted and completed, requests waiting for a batch, batches launched and
  * not yet finished, images the slots' batches decoded again
  * (cjd_batch_fallbacks summed), batches launched, and host time (ns) the
- * batcher spent staging and launching batches, the completion threads spent
- * in cjd_batch_finish (summed), and the batcher waited for a free slot; then the
+ * batcher spent staging and launching batches, the completion thread spent
+ * in cjd_batch_finish, and the batcher waited for a free slot; then the
  * batches' device time (ns, cjd_batch_timings of each decode summed, and its
  * largest value) and the time from a batch's launch to its completion
  * (summed). */
